@@ -1,0 +1,144 @@
+/*
+ * tfim.h -- C ABI of the MI355X support-counting library (libtfim.so).
+ *
+ * The reference has no library API: its counting path sits behind internal
+ * C++ functions reading process globals.  Each entry point below replaces one
+ * of them and is cited.  Conventions:
+ *   - every call returns an int status (TFIM_OK == 0) and never exits;
+ *     tfim_last_error() holds the message of the last failing call;
+ *   - no hidden globals: all state lives in a tfim_ctx (one per engine; one
+ *     HIP stream per ctx, default priority);
+ *   - plain pointers and sizes only.  Host pointers are host memory unless a
+ *     parameter is a tfim_buf (device memory owned by the library).
+ *   - counts are exact int32 supports: out = popcount(AND of rows).
+ *
+ * Bitmap layout (identical to the reference's buildBitmap, kernel_bmma.cu:
+ * 388-426): one row per dense item id, row-major, LSB-first (transaction t is
+ * bit t&63 of u64 word t>>6), zero padding beyond n_bits.
+ */
+#ifndef TFIM_H
+#define TFIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tfim_ctx tfim_ctx;
+typedef struct tfim_buf tfim_buf;
+
+enum {
+    TFIM_OK = 0,
+    TFIM_E_INVALID = -1,   /* bad argument */
+    TFIM_E_HIP = -2,       /* HIP runtime error */
+    TFIM_E_NOMEM = -3,     /* device allocation failed */
+    TFIM_E_STATE = -4,     /* call made before the state it needs */
+    TFIM_E_RANGE = -5      /* size beyond a documented limit */
+};
+
+/* Counting paths. */
+enum {
+    TFIM_PATH_AUTO = 0,     /* library chooses (autotuned per shape) */
+    TFIM_PATH_PACKED = 1,   /* 128-bit loads, AND, v_bcnt, wave reductions */
+    TFIM_PATH_MFMA_I8 = 2   /* bits unpacked to 0/1 int8 in LDS, v_mfma_i32_*_i8 */
+};
+
+/* ---- context ---------------------------------------------------------- */
+
+/* Opens device `device` (HIP ordinal).  Replaces the implicit CUDA context and
+ * the thread_local streams of kernel_bitmap.cu:447-449. */
+int tfim_open(int device, tfim_ctx **out);
+void tfim_close(tfim_ctx *ctx);
+/* Message of the last failing call on this ctx ("" if none). */
+const char *tfim_last_error(const tfim_ctx *ctx);
+/* Device facts for the Results header (replaces displayWorkingEnvironment,
+ * kernel_bmma.cu:166-206). */
+int tfim_device_info(tfim_ctx *ctx, char *name, int name_len, int32_t *compute_units,
+                     int64_t *hbm_bytes, int32_t *clock_mhz);
+
+/* ---- resident item bitmaps -------------------------------------------- */
+
+/* Uploads the vertical bitmaps once; replaces uploadBitmapToGPU
+ * (kernel_bmma.cu:428-444, kernel_bitmap.cu:382-388).  `rows` is host memory,
+ * n_items rows of words_per_row u64 words.  The first n_cols rows are the
+ * survivors counted by every count call (freqItems, kernel_bmma.cu:394-418);
+ * rows n_cols..n_items-1 (item 0's row, quirk A.3) may appear in prefixes
+ * only.  The device copy is padded to a 2048-bit multiple per row. */
+int tfim_load_bitmaps(tfim_ctx *ctx, const uint64_t *rows, int32_t n_items,
+                      int64_t n_bits, int64_t words_per_row, int32_t n_cols);
+
+/* Like tfim_load_bitmaps, for a database that is `replicate` concatenated
+ * copies of a base of `n_bits` transactions (data/make_replica.py): row bits
+ * are repeated on the device, so the host holds only the base. */
+int tfim_load_bitmaps_replicated(tfim_ctx *ctx, const uint64_t *rows, int32_t n_items,
+                                 int64_t n_bits, int64_t words_per_row, int32_t n_cols,
+                                 int32_t replicate);
+
+/* F1 on the device: out[i] = popcount(row i) for i < n_items
+ * (batch_proto.cu:57-65 item_count_kernel). */
+int tfim_item_supports(tfim_ctx *ctx, int32_t *out);
+
+/* ---- counting ---------------------------------------------------------- */
+
+/* Plain per-node path; replaces getResFrequencyCuda (kernel_bitmap.cu:451-536)
+ * and its kernel (:412-442).  out[e] = popcount(AND_i row[items[i]] & row[e])
+ * for every survivor e < n_cols; synchronous, `out` is host memory. */
+int tfim_count_node(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t *out);
+
+/* Batched path; replaces batchCountBMMA (kernel_bmma.cu:772-872), including
+ * its mask build (:619-634), count (:636-758) and split-k reduce (:761-768),
+ * as one fused launch.  Row r has prefix items item_ids[item_off[r] ..
+ * item_off[r+1]) (CSR; uneven lengths allowed, k >= 1).  If col_ids is NULL,
+ * row r is counted against all survivors and out is [n_rows][n_cols];
+ * otherwise row r is counted against its tail col_ids[col_off[r] ..
+ * col_off[r+1]) and out is CSR-aligned with col_ids.  Host memory in and out;
+ * synchronous. */
+int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item_off,
+                     int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
+                     int32_t *out, int32_t path);
+
+/* Statistics of the batched path since tfim_open (launches, graph replays,
+ * rows, summed kernel milliseconds measured with HIP events). */
+int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *graph_replays,
+                    int64_t *rows, double *kernel_ms);
+
+/* ---- device buffers and the isolated count (microbench) ---------------- */
+
+int tfim_buf_alloc(tfim_ctx *ctx, int64_t bytes, tfim_buf **out);
+void tfim_buf_free(tfim_ctx *ctx, tfim_buf *buf);
+int tfim_buf_write(tfim_ctx *ctx, tfim_buf *buf, int64_t offset, const void *src, int64_t bytes);
+int tfim_buf_read(tfim_ctx *ctx, const tfim_buf *buf, int64_t offset, void *dst, int64_t bytes);
+
+/* Row pitch in u64 words the matrix kernels expect for n_bits-bit rows:
+ * ceil(n_bits / 2048) * 32 (the reference's 2048-bit padding,
+ * kernel_bmma.cu:389). */
+int64_t tfim_pitch_words(int64_t n_bits);
+
+/* Fills `rows` rows of `pitch_words` u64 words with i.i.d. Bernoulli bits:
+ * bit = (u32 draw < thr32), draws from SplitMix64 keyed by (seed, matrix,
+ * row, bit pair); bits >= n_bits are zero. */
+int tfim_fill_bernoulli(tfim_ctx *ctx, tfim_buf *dst, int64_t rows, int64_t n_bits,
+                        int64_t pitch_words, uint64_t seed, uint32_t matrix, uint32_t thr32);
+
+/* C[i][j] = popcount(A_i AND B_j) over pitch_words words, int32, row-major
+ * [b][d] in C.  Replaces bmma_kernel / bmma_smem_kernel / bitmap_kernel of
+ * code/microbench/bmma_bench.cu:50-174.  split_k >= 1 splits the word axis;
+ * 0 lets the library choose.  Asynchronous on the ctx stream. */
+int tfim_popcount_matrix(tfim_ctx *ctx, const tfim_buf *A, const tfim_buf *B, tfim_buf *C,
+                         int64_t b, int64_t d, int64_t pitch_words, int32_t path,
+                         int32_t split_k);
+
+/* Times `reps` back-to-back tfim_popcount_matrix launches after `warmup`
+ * untimed ones with HIP events on the ctx stream; avg_ms = mean per launch. */
+int tfim_time_popcount_matrix(tfim_ctx *ctx, const tfim_buf *A, const tfim_buf *B, tfim_buf *C,
+                              int64_t b, int64_t d, int64_t pitch_words, int32_t path,
+                              int32_t split_k, int32_t warmup, int32_t reps, double *avg_ms);
+
+/* Waits for all work on the ctx stream. */
+int tfim_sync(tfim_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

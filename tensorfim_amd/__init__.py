@@ -1,0 +1,292 @@
+"""tensorfim_amd -- MI355X-native exact maximal-frequent-itemset engine.
+
+Python side of the package: a ctypes binding of the C ABI in include/tfim.h
+(libtfim.so, hand-written gfx950 HIP kernels) and helpers that drive the C++
+engine binary (`tfim_mine`, the reference CLI `exe <file> <thr>`).
+
+The binding fails loudly if the native library is missing: there is no CPU
+fallback anywhere in the product path.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libtfim.so")
+BIN_DIR = os.path.join(_PKG, "bin")
+
+TFIM_OK = 0
+PATH_AUTO, PATH_PACKED, PATH_MFMA_I8 = 0, 1, 2
+_PATHS = {"auto": PATH_AUTO, "packed": PATH_PACKED, "mfma": PATH_MFMA_I8, "mfma_i8": PATH_MFMA_I8}
+
+_lib = None
+
+
+class TfimError(RuntimeError):
+    pass
+
+
+def _p(t):
+    return ctypes.POINTER(t)
+
+
+_SIGS = {
+    "tfim_open": (ctypes.c_int, [ctypes.c_int, _p(ctypes.c_void_p)]),
+    "tfim_close": (None, [ctypes.c_void_p]),
+    "tfim_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "tfim_device_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _p(ctypes.c_int32),
+                                         _p(ctypes.c_int64), _p(ctypes.c_int32)]),
+    "tfim_load_bitmaps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
+                                          ctypes.c_int64, ctypes.c_int32]),
+    "tfim_load_bitmaps_replicated": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                     ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                                     ctypes.c_int32]),
+    "tfim_item_supports": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tfim_count_node": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "tfim_count_nodes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "tfim_wave_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64),
+                                        _p(ctypes.c_int64), _p(ctypes.c_double)]),
+    "tfim_buf_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _p(ctypes.c_void_p)]),
+    "tfim_buf_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tfim_buf_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_int64]),
+    "tfim_buf_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                      ctypes.c_int64]),
+    "tfim_pitch_words": (ctypes.c_int64, [ctypes.c_int64]),
+    "tfim_fill_bernoulli": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]),
+    "tfim_popcount_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                             ctypes.c_int32]),
+    "tfim_time_popcount_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                                  ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                  _p(ctypes.c_double)]),
+    "tfim_sync": (ctypes.c_int, [ctypes.c_void_p]),
+}
+
+EXPORTED_SYMBOLS = sorted(_SIGS)
+
+
+def load_library(path=LIB_PATH):
+    """Load libtfim.so; raises TfimError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise TfimError("libtfim.so not built at %s (run python -m tensorfim_amd.build)" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def pitch_words(n_bits):
+    return int(load_library().tfim_pitch_words(int(n_bits)))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class DeviceBuffer:
+    """Device memory owned by the library (tfim_buf)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        h = ctypes.c_void_p()
+        ctx._check(ctx._lib.tfim_buf_alloc(ctx._h, self.nbytes, ctypes.byref(h)))
+        self._h = h
+
+    def write(self, arr, offset=0):
+        a = np.ascontiguousarray(arr)
+        self.ctx._check(self.ctx._lib.tfim_buf_write(self.ctx._h, self._h, int(offset), _ptr(a), a.nbytes))
+
+    def read(self, dtype, count, offset=0):
+        out = np.empty(int(count), dtype=dtype)
+        self.ctx._check(self.ctx._lib.tfim_buf_read(self.ctx._h, self._h, int(offset), _ptr(out), out.nbytes))
+        return out
+
+    def free(self):
+        if self._h:
+            self.ctx._lib.tfim_buf_free(self.ctx._h, self._h)
+            self._h = None
+
+
+class Context:
+    """One tfim_ctx: a device, a stream, resident item bitmaps."""
+
+    def __init__(self, device=0):
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self._lib.tfim_open(int(device), ctypes.byref(h))
+        if rc != TFIM_OK:
+            raise TfimError("tfim_open(%d) failed with status %d" % (device, rc))
+        self._h = h
+        self.n_cols = 0
+        self.n_items = 0
+
+    def close(self):
+        if self._h:
+            self._lib.tfim_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != TFIM_OK:
+            raise TfimError("tfim status %d: %s" % (rc, self._lib.tfim_last_error(self._h).decode()))
+
+    def device_info(self):
+        name = ctypes.create_string_buffer(256)
+        cus, hbm, clk = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int32()
+        self._check(self._lib.tfim_device_info(self._h, name, 256, ctypes.byref(cus), ctypes.byref(hbm),
+                                               ctypes.byref(clk)))
+        return {"name": name.value.decode(), "compute_units": cus.value, "hbm_bytes": hbm.value,
+                "clock_mhz": clk.value}
+
+    def load_bitmaps(self, rows, n_bits, n_cols=None, replicate=1):
+        """rows: uint64 array [n_items, words]; the first n_cols rows are counted."""
+        rows = np.ascontiguousarray(rows, dtype=np.uint64)
+        n_items, words = rows.shape
+        n_cols = n_items if n_cols is None else int(n_cols)
+        if replicate == 1:
+            self._check(self._lib.tfim_load_bitmaps(self._h, _ptr(rows), n_items, int(n_bits), words, n_cols))
+        else:
+            self._check(self._lib.tfim_load_bitmaps_replicated(self._h, _ptr(rows), n_items, int(n_bits), words,
+                                                               n_cols, int(replicate)))
+        self.n_items, self.n_cols = n_items, n_cols
+
+    def item_supports(self):
+        out = np.empty(self.n_items, dtype=np.int32)
+        self._check(self._lib.tfim_item_supports(self._h, _ptr(out)))
+        return out
+
+    def count_node(self, items):
+        """Plain path: supports of items+{e} for every survivor e (getResFrequencyCuda)."""
+        it = np.ascontiguousarray(items, dtype=np.int32)
+        out = np.empty(self.n_cols, dtype=np.int32)
+        self._check(self._lib.tfim_count_node(self._h, _ptr(it), len(it), _ptr(out)))
+        return out
+
+    def count_nodes(self, prefixes, tails=None, path="auto"):
+        """Batched path.  prefixes: list of item lists; tails: None (all
+        survivors) or list of column lists.  Returns a list of int32 arrays."""
+        item_off = np.zeros(len(prefixes) + 1, dtype=np.int32)
+        item_off[1:] = np.cumsum([len(p) for p in prefixes])
+        item_ids = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.int32) for p in prefixes])
+                                        if prefixes else np.zeros(0, np.int32), dtype=np.int32)
+        n = len(prefixes)
+        if tails is None:
+            out = np.empty(max(1, n * self.n_cols), dtype=np.int32)
+            self._check(self._lib.tfim_count_nodes(self._h, _ptr(item_ids), _ptr(item_off), n, None, None,
+                                                   _ptr(out), _PATHS[path]))
+            return [out[i * self.n_cols:(i + 1) * self.n_cols] for i in range(n)]
+        col_off = np.zeros(n + 1, dtype=np.int32)
+        col_off[1:] = np.cumsum([len(t) for t in tails])
+        col_ids = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.int32) for t in tails])
+                                       if n else np.zeros(0, np.int32), dtype=np.int32)
+        out = np.empty(max(1, int(col_off[-1])), dtype=np.int32)
+        self._check(self._lib.tfim_count_nodes(self._h, _ptr(item_ids), _ptr(item_off), n,
+                                               _ptr(col_ids) if len(col_ids) else _ptr(np.zeros(1, np.int32)),
+                                               _ptr(col_off), _ptr(out), _PATHS[path]))
+        return [out[col_off[i]:col_off[i + 1]] for i in range(n)]
+
+    def wave_stats(self):
+        a, b, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        self._check(self._lib.tfim_wave_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                              ctypes.byref(d)))
+        return {"launches": a.value, "graph_replays": b.value, "rows": c.value, "kernel_ms": d.value}
+
+    def buffer(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def fill_bernoulli(self, buf, rows, n_bits, pitch, seed, matrix, p):
+        thr32 = int(np.floor(p * 2.0 ** 32))
+        self._check(self._lib.tfim_fill_bernoulli(self._h, buf._h, int(rows), int(n_bits), int(pitch),
+                                                  ctypes.c_uint64(seed), int(matrix), thr32))
+
+    def popcount_matrix(self, A, B, C, b, d, pitch, path="auto", split_k=1):
+        self._check(self._lib.tfim_popcount_matrix(self._h, A._h, B._h, C._h, int(b), int(d), int(pitch),
+                                                   _PATHS[path], int(split_k)))
+
+    def time_popcount_matrix(self, A, B, C, b, d, pitch, path="auto", split_k=1, warmup=1, reps=3):
+        ms = ctypes.c_double()
+        self._check(self._lib.tfim_time_popcount_matrix(self._h, A._h, B._h, C._h, int(b), int(d), int(pitch),
+                                                        _PATHS[path], int(split_k), int(warmup), int(reps),
+                                                        ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        self._check(self._lib.tfim_sync(self._h))
+
+
+# ---------------------------------------------------------------------------
+# Engine binary (reference CLI) and Results files.
+
+def binary(name):
+    p = os.path.join(BIN_DIR, name)
+    if not os.path.exists(p):
+        raise TfimError("%s not built (run python -m tensorfim_amd.build)" % p)
+    return p
+
+
+def results_path(trans_file, support):
+    """<file>-<%f>=Results.txt, as kernel_bmma.cu:1291-1292 names it."""
+    return "%s-%f=Results.txt" % (trans_file, support)
+
+
+def mine(trans_file, support, engine="mafia", out=None, extra=(), timeout=None):
+    """Run the C++ engine; returns (results path, parsed stdout key/values)."""
+    cmd = [binary("tfim_mine"), trans_file, repr(float(support)), "--engine", engine]
+    if out:
+        cmd += ["--out", out]
+    cmd += list(extra)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise TfimError("tfim_mine failed (%d):\n%s\n%s" % (r.returncode, r.stdout[-4000:], r.stderr[-4000:]))
+    stats = {}
+    for ln in r.stdout.splitlines():
+        if ":" in ln:
+            k, v = ln.split(":", 1)
+            stats[k.strip()] = v.strip()
+    return (out or results_path(trans_file, support)), stats
+
+
+_LINE = re.compile(r"^\d+th:\s+\d+\s+\{\s*([^}]*)\}\s+support:\s+\S+\s+frequency:\s+(\d+)")
+
+
+def load_results(path):
+    """{frozenset(items): frequency}, the comparison of scripts/verify_all.py:16-23."""
+    sets = {}
+    with open(path, encoding="utf-8", errors="replace") as f:
+        for ln in f:
+            m = _LINE.match(ln.strip())
+            if m:
+                sets[frozenset(int(x) for x in m.group(1).split())] = int(m.group(2))
+    return sets
+
+
+def load_header(path):
+    """Header fields of a Results file (name -> string value)."""
+    h = {}
+    with open(path, encoding="utf-8", errors="replace") as f:
+        for ln in f:
+            if ln.startswith("=====") and "List of MFIs" in ln:
+                break
+            if ":" in ln:
+                k, v = ln.split(":", 1)
+                h.setdefault(k.strip(), v.strip())
+    return h

@@ -1,0 +1,144 @@
+// Bits-as-int8 GEMM on CDNA4 matrix cores.
+//
+// CDNA4 has no 1-bit matrix op (the reference uses b1 wmma with AND/POPC,
+// kernel_bmma.cu:636-758, bmma_bench.cu:50-152).  Here each bit becomes a 0/1
+// int8, so C[i][j] = sum_t a_it * b_jt is exactly popcount(A_i AND B_j), and
+// v_mfma_i32_32x32x32_i8 accumulates it in int32 without rounding.
+//
+// Tile: 128 rows x 128 columns per 256-thread workgroup; waves in a 2x2 grid
+// each own 64x64 = 2x2 MFMA tiles (64 accumulator registers).  K is staged
+// 256 bits per step: the packed words are loaded with 128-bit loads and
+// unpacked to bytes straight into LDS (32 KB for A, 32 KB for B), with a
+// 16-byte-slot XOR swizzle so the ds_read_b128 of a 32-row fragment hits
+// distinct banks.
+//
+// Operand layout.  For v_mfma_i32_32x32x32_i8, lane l supplies 16 bytes of
+// row (l % 32) of A and the same 16 k-positions of column (l % 32) of B; the
+// half l / 32 selects which 16 of the 32 k.  The instruction sums over all 64
+// (half, byte) pairs, so any consistent assignment of k to (half, byte) is
+// exact; here byte b of half h is k = 16h + b.  The accumulator layout is the
+// 32x32 one of the CDNA3/CDNA4 ISA: register q of lane l holds
+// C[8*(q/4) + 4*(l/32) + (q%4)][l % 32].
+#include "tfim_internal.h"
+
+namespace tfim {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// 4 bits -> 4 bytes of 0/1: bit j of the nibble lands on bit 8j.
+__device__ __forceinline__ uint32_t nib2bytes(uint32_t x) {
+    return ((x & 0xFu) * 0x00204081u) & 0x01010101u;
+}
+
+// 16 bits -> 16 bytes (one LDS slot).
+__device__ __forceinline__ uint4 unpack16(uint32_t x16) {
+    return make_uint4(nib2bytes(x16), nib2bytes(x16 >> 4), nib2bytes(x16 >> 8), nib2bytes(x16 >> 12));
+}
+
+constexpr int MM_T = 128;     // rows and columns per workgroup tile
+constexpr int MM_SLOTS = 16;  // 16-byte slots per row per 256-bit K step
+
+__global__ __launch_bounds__(256) void k_popmat_mfma(const uint64_t *__restrict__ A,
+                                                     const uint64_t *__restrict__ B,
+                                                     int32_t *__restrict__ C, int64_t b, int64_t d,
+                                                     int64_t pitch, int64_t k_per_split, int atomic,
+                                                     int tiles_n) {
+    __shared__ uint4 As[MM_T * MM_SLOTS];
+    __shared__ uint4 Bs[MM_T * MM_SLOTS];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    // Consecutive tiles of one row band go to one XCD (blocks are dealt to the
+    // 8 XCDs round-robin), so the band's A rows are shared through its L2.
+    const int64_t nblk = (int64_t)gridDim.x;
+    const int64_t bid = blockIdx.x;
+    const int64_t per_xcd = nblk / 8;
+    const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * per_xcd + bid / 8 : bid;
+    const int64_t tile_m = lin / tiles_n, tile_n = lin % tiles_n;
+    const int64_t row0 = tile_m * MM_T, col0 = tile_n * MM_T;
+    const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
+    const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
+
+    // loader role: thread -> (row, half of the 256-bit step)
+    const int lr = tid >> 1, lh = tid & 1;
+    const bool a_ok = row0 + lr < b, b_ok = col0 + lr < d;
+    const uint4 *Arow = reinterpret_cast<const uint4 *>(A + (row0 + lr) * pitch);
+    const uint4 *Brow = reinterpret_cast<const uint4 *>(B + (col0 + lr) * pitch);
+
+    const int wr = wave >> 1, wc = wave & 1;
+    v16i acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[i][j][q] = 0;
+
+    for (int64_t kw = kbeg; kw < kend; kw += 4) {
+        const int64_t u = (kw >> 1) + lh;   // uint4 index of this thread's 128 bits
+        const uint4 av = a_ok ? Arow[u] : make_uint4(0, 0, 0, 0);
+        const uint4 bv = b_ok ? Brow[u] : make_uint4(0, 0, 0, 0);
+        const uint32_t aw[4] = {av.x, av.y, av.z, av.w};
+        const uint32_t bw[4] = {bv.x, bv.y, bv.z, bv.w};
+        __syncthreads();   // previous step's fragment reads are done
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int slot = lh * 8 + j;
+            const uint32_t xa = aw[j >> 1] >> ((j & 1) * 16);
+            const uint32_t xb = bw[j >> 1] >> ((j & 1) * 16);
+            As[lr * MM_SLOTS + (slot ^ (lr & 15))] = unpack16(xa);
+            Bs[lr * MM_SLOTS + (slot ^ (lr & 15))] = unpack16(xb);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 8; kk++) {
+            const int slot = kk * 2 + (lane >> 5);
+            v4i af[2], bf[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int r = wr * 64 + i * 32 + (lane & 31);
+                const uint4 x = As[r * MM_SLOTS + (slot ^ (r & 15))];
+                af[i] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int c = wc * 64 + j * 32 + (lane & 31);
+                const uint4 x = Bs[c * MM_SLOTS + (slot ^ (c & 15))];
+                bf[j] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int64_t r = row0 + wr * 64 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                const int64_t c = col0 + wc * 64 + j * 32 + (lane & 31);
+                if (r < b && c < d) {
+                    if (atomic) atomicAdd(&C[r * d + c], acc[i][j][q]);
+                    else C[r * d + c] = acc[i][j][q];
+                }
+            }
+}
+
+void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                        int64_t pitch_words, int split_k, hipStream_t s) {
+    if (split_k < 1) split_k = 1;
+    int64_t kps = (pitch_words + split_k - 1) / split_k;
+    kps = (kps + 3) / 4 * 4;
+    const int64_t zs = (pitch_words + kps - 1) / kps;
+    const int64_t tm = (b + MM_T - 1) / MM_T, tn = (d + MM_T - 1) / MM_T;
+    dim3 grid((unsigned)(tm * tn), (unsigned)zs);
+    hipLaunchKernelGGL(k_popmat_mfma, grid, dim3(256), 0, s, A, B, C, b, d, pitch_words, kps,
+                       zs > 1 ? 1 : 0, (int)tn);
+}
+
+}  // namespace tfim

@@ -1,0 +1,414 @@
+// Packed-bitmap kernels for gfx950: 128-bit loads, AND, v_bcnt_u32_b32,
+// wave-level reductions into int32 supports.
+//
+//   k_count_node  -- plain per-node path (replaces getResFrequencyCudaKernel,
+//                    kernel_bitmap.cu:412-442).  The reference re-ANDs the
+//                    prefix rows once per column block; here a block ANDs the
+//                    prefix once for its word chunk and reuses it for up to 64
+//                    columns, so per node the traffic is (k + n_cols) rows.
+//   k_count_wave  -- fused, persistent wave kernel (replaces maskBuildKernelFast
+//                    + bmmaKernel* + bmmaSplitReduceKernel, kernel_bmma.cu:
+//                    619-768): prefix AND, per-row tail counts and the split
+//                    over word chunks accumulated with int32 atomics.
+//   k_popmat_packed -- isolated C = A AND-popcount B^T (bitmap_kernel of
+//                    code/microbench/bmma_bench.cu:159-174), LDS tiled 64x64,
+//                    4x4 register micro-tiles.
+#include "tfim_internal.h"
+
+namespace tfim {
+
+__device__ __forceinline__ uint32_t popc_and4(const uint4 a, const uint4 b) {
+    return __popc(a.x & b.x) + __popc(a.y & b.y) + __popc(a.z & b.z) + __popc(a.w & b.w);
+}
+
+__device__ __forceinline__ uint4 and4(const uint4 a, const uint4 b) {
+    return make_uint4(a.x & b.x, a.y & b.y, a.z & b.z, a.w & b.w);
+}
+
+// Reduce-scatter of 8 per-lane values over a 64-lane wave in 10 shuffles:
+// afterwards lane L with (L & 7) == 0 holds the wave total of v[(L >> 3) & 7].
+__device__ __forceinline__ int wave_reduce_scatter8(const int v[8]) {
+    const int lane = threadIdx.x & 63;
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    int w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int send = b5 ? v[j] : v[j + 4];
+        const int keep = b5 ? v[j + 4] : v[j];
+        w[j] = keep + __shfl_xor(send, 32);
+    }
+    int x[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int send = b4 ? w[j] : w[j + 2];
+        const int keep = b4 ? w[j + 2] : w[j];
+        x[j] = keep + __shfl_xor(send, 16);
+    }
+    const int send = b3 ? x[0] : x[1];
+    const int keep = b3 ? x[1] : x[0];
+    int y = keep + __shfl_xor(send, 8);
+    y += __shfl_xor(y, 4);
+    y += __shfl_xor(y, 2);
+    y += __shfl_xor(y, 1);
+    return y;
+}
+
+// ---------------------------------------------------------------------------
+// Plain per-node count.  grid.x = word chunks of 256*R uint4, grid.y = column
+// blocks of <= 64 columns.  Every block ANDs the k prefix rows over its chunk
+// once (registers), then streams its columns 8 at a time.
+template <int R>
+__global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                    const int32_t *__restrict__ items, int k,
+                                                    int n_cols, int32_t *__restrict__ out) {
+    __shared__ int red[4][64];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t base = (int64_t)blockIdx.x * (256 * R) + tid;
+    const int c0 = blockIdx.y * 64;
+    const int c1 = min(n_cols, c0 + 64);
+    uint4 m[R];
+    {
+        const uint4 *row = bm + (int64_t)items[0] * pitch4;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t idx = base + r * 256;
+            m[r] = idx < pitch4 ? row[idx] : make_uint4(0, 0, 0, 0);
+        }
+    }
+    for (int i = 1; i < k; i++) {
+        const uint4 *row = bm + (int64_t)items[i] * pitch4;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t idx = base + r * 256;
+            if (idx < pitch4) m[r] = and4(m[r], row[idx]);
+        }
+    }
+    for (int c = c0; c < c1; c += 8) {
+        int v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            uint32_t s = 0;
+            if (c + j < c1) {
+                const uint4 *row = bm + (int64_t)(c + j) * pitch4;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int64_t idx = base + r * 256;
+                    if (idx < pitch4) s += popc_and4(m[r], row[idx]);
+                }
+            }
+            v[j] = (int)s;
+        }
+        const int y = wave_reduce_scatter8(v);
+        if ((lane & 7) == 0) {
+            const int cj = (lane >> 3) & 7;
+            if (c + cj < c1) red[wave][c - c0 + cj] = y;
+        }
+    }
+    __syncthreads();
+    if (tid < c1 - c0) {
+        const int s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+        if (s) atomicAdd(&out[c0 + tid], s);
+    }
+}
+
+void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
+                       int32_t n_cols, int32_t *out, int n_cu, hipStream_t s) {
+    const int64_t pitch4 = pitch_words / 2;
+    const int ycols = (n_cols + 63) / 64;
+    auto chunks = [&](int R) { return (pitch4 + 256 * R - 1) / (256 * R); };
+    const int64_t want = 4LL * n_cu;
+    const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
+    if (chunks(4) * ycols >= want) {
+        hipLaunchKernelGGL(k_count_node<4>, dim3((unsigned)chunks(4), ycols), dim3(256), 0, s, b4, pitch4, items, k, n_cols, out);
+    } else if (chunks(2) * ycols >= want) {
+        hipLaunchKernelGGL(k_count_node<2>, dim3((unsigned)chunks(2), ycols), dim3(256), 0, s, b4, pitch4, items, k, n_cols, out);
+    } else {
+        hipLaunchKernelGGL(k_count_node<1>, dim3((unsigned)chunks(1), ycols), dim3(256), 0, s, b4, pitch4, items, k, n_cols, out);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused persistent wave kernel.  A unit of work is (word chunk s, tile t): one
+// 64-lane wave ANDs the prefix of tile t's row over chunk s into registers and
+// counts the tile's columns against it.  Units are spread so that every chunk
+// s is handled by blocks of one XCD (blockIdx % 8 == s % 8): the chunk of each
+// item row is then fetched into that XCD's L2 once and reused by every row of
+// the wave.
+template <int RW>
+__global__ __launch_bounds__(256) void k_count_wave(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                    WaveDesc d) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int xcd = blockIdx.x & 7;
+    const int64_t gw = (int64_t)(blockIdx.x >> 3) * 4 + wave;
+    const int64_t GW = (int64_t)(gridDim.x >> 3) * 4;
+    const int S = d.n_chunks;
+    const int nchunks_x = xcd < S ? (S - xcd + 7) / 8 : 0;
+    const int64_t U = (int64_t)nchunks_x * d.n_tiles;
+    for (int64_t v = gw; v < U; v += GW) {
+        const int s = (int)(v / d.n_tiles) * 8 + xcd;
+        const int t = (int)(v % d.n_tiles);
+        const WaveTile tile = d.tiles[t];
+        const int r = tile.row;
+        const int64_t base = (int64_t)s * (64 * RW) + lane;
+        const int ib = d.item_off[r], ie = d.item_off[r + 1];
+        uint4 m[RW];
+        {
+            const uint4 *row = bm + (int64_t)d.item_ids[ib] * pitch4;
+#pragma unroll
+            for (int q = 0; q < RW; q++) {
+                const int64_t idx = base + q * 64;
+                m[q] = idx < pitch4 ? row[idx] : make_uint4(0, 0, 0, 0);
+            }
+        }
+        for (int i = ib + 1; i < ie; i++) {
+            const uint4 *row = bm + (int64_t)d.item_ids[i] * pitch4;
+#pragma unroll
+            for (int q = 0; q < RW; q++) {
+                const int64_t idx = base + q * 64;
+                if (idx < pitch4) m[q] = and4(m[q], row[idx]);
+            }
+        }
+        const int ob = d.col_off[r];
+        for (int j = tile.col_begin; j < tile.col_end; j += 8) {
+            int v8[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                uint32_t acc = 0;
+                if (j + q < tile.col_end) {
+                    const int col = d.col_ids ? d.col_ids[ob + j + q] : j + q;
+                    const uint4 *row = bm + (int64_t)col * pitch4;
+#pragma unroll
+                    for (int qq = 0; qq < RW; qq++) {
+                        const int64_t idx = base + qq * 64;
+                        if (idx < pitch4) acc += popc_and4(m[qq], row[idx]);
+                    }
+                }
+                v8[q] = (int)acc;
+            }
+            const int y = wave_reduce_scatter8(v8);
+            if ((lane & 7) == 0) {
+                const int q = (lane >> 3) & 7;
+                if (j + q < tile.col_end && y) atomicAdd(&d.out[ob + j + q], y);
+            }
+        }
+    }
+}
+
+int wave_chunk_words(int rw) { return 64 * rw * 2; }
+
+void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d, int grid_blocks,
+                       int rw, hipStream_t s) {
+    const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
+    const int64_t pitch4 = pitch_words / 2;
+    switch (rw) {
+        case 1: hipLaunchKernelGGL(k_count_wave<1>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
+        case 2: hipLaunchKernelGGL(k_count_wave<2>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
+        case 8: hipLaunchKernelGGL(k_count_wave<8>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
+        default: hipLaunchKernelGGL(k_count_wave<4>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_zero_i32(int32_t *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+}
+
+// ---------------------------------------------------------------------------
+// Packed matrix count: 64x64 output tile per 256-thread block; thread (tr, tc)
+// owns rows tr+16i and columns tc+16j (i, j < 4), a layout whose LDS reads are
+// conflict-free with the 17-word row stride.  K is staged 16 u64 words at a
+// time; split_k blocks along grid.z accumulate with atomics.
+constexpr int PK_KW = 16;
+__global__ __launch_bounds__(256) void k_popmat_packed(const uint64_t *__restrict__ A,
+                                                       const uint64_t *__restrict__ B,
+                                                       int32_t *__restrict__ C, int64_t b, int64_t d,
+                                                       int64_t pitch, int64_t k_per_split, int atomic) {
+    __shared__ uint64_t As[64][PK_KW + 1];
+    __shared__ uint64_t Bs[64][PK_KW + 1];
+    const int tid = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.y * 64, col0 = (int64_t)blockIdx.x * 64;
+    const int64_t kbeg = (int64_t)blockIdx.z * k_per_split;
+    const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
+    const int tr = tid >> 4, tc = tid & 15;
+    uint32_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0;
+    for (int64_t kw = kbeg; kw < kend; kw += PK_KW) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + q * 256;
+            const int r = e / PK_KW, w = e % PK_KW;
+            const bool kin = kw + w < kend;
+            As[r][w] = (row0 + r < b && kin) ? A[(row0 + r) * pitch + kw + w] : 0ull;
+            Bs[r][w] = (col0 + r < d && kin) ? B[(col0 + r) * pitch + kw + w] : 0ull;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int w = 0; w < PK_KW; w++) {
+            uint64_t a[4], bb[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = As[tr + 16 * i][w];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bb[j] = Bs[tc + 16 * j][w];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] += __popcll(a[i] & bb[j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int64_t r = row0 + tr + 16 * i;
+        if (r >= b) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int64_t c = col0 + tc + 16 * j;
+            if (c >= d) continue;
+            if (atomic) atomicAdd(&C[r * d + c], (int)acc[i][j]);
+            else C[r * d + c] = (int)acc[i][j];
+        }
+    }
+}
+
+void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                          int64_t pitch_words, int split_k, hipStream_t s) {
+    if (split_k < 1) split_k = 1;
+    int64_t kps = (pitch_words + split_k - 1) / split_k;
+    kps = (kps + PK_KW - 1) / PK_KW * PK_KW;
+    const int64_t zs = (pitch_words + kps - 1) / kps;
+    dim3 grid((unsigned)((d + 63) / 64), (unsigned)((b + 63) / 64), (unsigned)zs);
+    hipLaunchKernelGGL(k_popmat_packed, grid, dim3(256), 0, s, A, B, C, b, d, pitch_words, kps, zs > 1 ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// SplitMix64 (Steele, Lea & Flood 2014).  One thread per u64 word: 32 draws,
+// two 32-bit halves per draw, bit = draw < thr32.
+__device__ __forceinline__ uint64_t sm64_mix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+constexpr uint64_t kGamma = 0x9E3779B97F4A7C15ull;
+
+__global__ void k_fill_bernoulli(uint64_t *dst, int64_t rows, int64_t n_bits, int64_t pitch,
+                                 uint64_t seed, uint32_t matrix, uint32_t thr32) {
+    const uint64_t km = sm64_mix(seed + kGamma * ((uint64_t)matrix + 1));
+    const int64_t total = rows * pitch;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / pitch, w = i % pitch;
+        uint64_t word = 0;
+        const int64_t base = w * 64;
+        if (base < n_bits) {
+            const uint64_t kr = sm64_mix(km + kGamma * ((uint64_t)row + 1));
+#pragma unroll 4
+            for (int p = 0; p < 32; p++) {
+                const uint64_t pair = (uint64_t)w * 32 + (uint64_t)p;
+                const uint64_t x = sm64_mix(kr + kGamma * (pair + 1));
+                word |= (uint64_t)((uint32_t)x < thr32) << (2 * p);
+                word |= (uint64_t)((uint32_t)(x >> 32) < thr32) << (2 * p + 1);
+            }
+            const int64_t keep = n_bits - base;
+            if (keep < 64) word &= (1ull << keep) - 1;
+        }
+        dst[i] = word;
+    }
+}
+
+void launch_fill_bernoulli(uint64_t *dst, int64_t rows, int64_t n_bits, int64_t pitch_words,
+                           uint64_t seed, uint32_t matrix, uint32_t thr32, hipStream_t s) {
+    const int64_t total = rows * pitch_words;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_bernoulli, dim3((unsigned)blocks), dim3(256), 0, s, dst, rows, n_bits,
+                       pitch_words, seed, matrix, thr32);
+}
+
+// ---------------------------------------------------------------------------
+// Replication (data/make_replica.py on the device): dst bit t of a row is base
+// bit (t mod base_bits) for t < replicate*base_bits, zero beyond.
+__device__ __forceinline__ uint64_t get_bits64(const uint64_t *row, int64_t pitch, int64_t p) {
+    const int64_t w = p >> 6;
+    const int sh = (int)(p & 63);
+    const uint64_t lo = w < pitch ? row[w] : 0ull;
+    if (sh == 0) return lo;
+    const uint64_t hi = (w + 1) < pitch ? row[w + 1] : 0ull;
+    return (lo >> sh) | (hi << (64 - sh));
+}
+
+__global__ void k_replicate(const uint64_t *base, int64_t base_pitch, int64_t base_bits,
+                            uint64_t *dst, int64_t dst_pitch, int32_t rows, int32_t replicate) {
+    const int64_t total = (int64_t)rows * dst_pitch;
+    const int64_t dst_bits = base_bits * replicate;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / dst_pitch, w = i % dst_pitch;
+        const uint64_t *src = base + row * base_pitch;
+        const int64_t t0 = w * 64;
+        uint64_t word = 0;
+        if (t0 < dst_bits) {
+            if (base_bits >= 64) {
+                const int64_t p = t0 % base_bits;
+                const int64_t n1 = base_bits - p;   // bits before the wrap
+                uint64_t x = get_bits64(src, base_pitch, p);
+                if (n1 < 64) {
+                    x &= (1ull << n1) - 1;
+                    x |= get_bits64(src, base_pitch, 0) << n1;
+                }
+                word = x;
+            } else {
+                for (int j = 0; j < 64; j++) {
+                    const int64_t p = (t0 + j) % base_bits;
+                    word |= ((src[p >> 6] >> (p & 63)) & 1ull) << j;
+                }
+            }
+            const int64_t keep = dst_bits - t0;
+            if (keep < 64) word &= (1ull << keep) - 1;
+        }
+        dst[i] = word;
+    }
+}
+
+void launch_replicate_rows(const uint64_t *base, int64_t base_pitch, int64_t base_bits,
+                           uint64_t *dst, int64_t dst_pitch, int32_t rows, int32_t replicate,
+                           hipStream_t s) {
+    const int64_t total = (int64_t)rows * dst_pitch;
+    const int64_t blocks = std::min<int64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_replicate, dim3((unsigned)blocks), dim3(256), 0, s, base, base_pitch,
+                       base_bits, dst, dst_pitch, rows, replicate);
+}
+
+// F1: one block per row (batch_proto.cu:57-65).
+__global__ __launch_bounds__(256) void k_row_popcount(const uint4 *bm, int64_t pitch4, int32_t *out) {
+    __shared__ int red[4];
+    const uint4 *row = bm + (int64_t)blockIdx.x * pitch4;
+    uint32_t s = 0;
+    for (int64_t i = threadIdx.x; i < pitch4; i += 256) {
+        const uint4 x = row[i];
+        s += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    }
+    int v = (int)s;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+void launch_row_popcount(const uint64_t *bm, int64_t pitch_words, int32_t rows, int32_t *out,
+                         hipStream_t s) {
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_row_popcount, dim3(rows), dim3(256), 0, s,
+                       reinterpret_cast<const uint4 *>(bm), pitch_words / 2, out);
+}
+
+}  // namespace tfim

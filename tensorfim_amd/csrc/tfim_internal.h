@@ -1,0 +1,73 @@
+// Internal declarations shared by the libtfim HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "tfim.h"
+
+namespace tfim {
+
+// Row pitch granularity: 2048 bits = 32 u64 words (kernel_bmma.cu:389).
+constexpr int64_t kPitchWords = 32;
+
+inline int64_t pitch_for_bits(int64_t n_bits) {
+    return ((n_bits + 2047) / 2048) * kPitchWords;
+}
+
+// ---- launchers (defined in kernels_*.hip) --------------------------------
+
+// Plain per-node count: out[c] += popcount(AND rows[items] & rows[c]) for
+// c < n_cols; `out` must be zeroed.  items are device ids.
+void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
+                       int32_t n_cols, int32_t *out, int n_cu, hipStream_t s);
+
+// One tile of a wave: a row's prefix, and a run of its columns.
+struct WaveTile {
+    int32_t row;        // node index
+    int32_t col_begin;  // index into the row's column list
+    int32_t col_end;
+    int32_t pad;
+};
+
+// Wave descriptor consumed by the fused wave kernel (device pointers).
+struct WaveDesc {
+    const int32_t *item_ids;   // prefix items, CSR by item_off
+    const int32_t *item_off;   // [n_rows+1]
+    const int32_t *col_ids;    // columns, CSR by col_off; nullptr = identity over [0,n_cols)
+    const int32_t *col_off;    // [n_rows+1] output offsets (also when col_ids is null)
+    const WaveTile *tiles;     // [n_tiles]
+    int32_t *out;              // counts, indexed by col_off[row] + j
+    int32_t n_tiles;
+    int32_t n_chunks;          // word chunks of the wave kernel's chunk size
+};
+
+// Fused wave kernel: prefix AND + column counts + split-k accumulation
+// (atomics).  `out` must be zeroed.  grid_blocks must be a multiple of 8.
+void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d,
+                       int grid_blocks, int chunk_uint4_per_lane, hipStream_t s);
+int wave_chunk_words(int chunk_uint4_per_lane);
+
+// Zero `n` int32 with a fill kernel (graph-capture safe).
+void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
+
+// Matrix AND-popcount, packed (VALU) path.  C must be zeroed when split_k>1.
+void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                          int64_t pitch_words, int split_k, hipStream_t s);
+// Matrix AND-popcount, int8 MFMA path.  C must be zeroed when split_k>1.
+void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                        int64_t pitch_words, int split_k, hipStream_t s);
+
+void launch_fill_bernoulli(uint64_t *dst, int64_t rows, int64_t n_bits, int64_t pitch_words,
+                           uint64_t seed, uint32_t matrix, uint32_t thr32, hipStream_t s);
+
+void launch_replicate_rows(const uint64_t *base, int64_t base_pitch, int64_t base_bits,
+                           uint64_t *dst, int64_t dst_pitch, int32_t rows, int32_t replicate,
+                           hipStream_t s);
+
+void launch_row_popcount(const uint64_t *bm, int64_t pitch_words, int32_t rows, int32_t *out,
+                         hipStream_t s);
+
+}  // namespace tfim

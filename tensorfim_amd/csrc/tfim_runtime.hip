@@ -1,0 +1,467 @@
+// libtfim runtime: the C ABI of include/tfim.h over HIP.
+//
+// One tfim_ctx per engine holds the device, one default-priority stream, the
+// resident item bitmaps and grow-only scratch (device and pinned host), so a
+// count call does no allocation once warm.  The reference keeps the same state
+// in process globals and thread_locals (kernel_bitmap.cu:334-340,447-449;
+// kernel_bmma.cu:345-350,780-797).
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "tfim_internal.h"
+
+using namespace tfim;
+
+struct tfim_buf {
+    void *ptr = nullptr;
+    int64_t bytes = 0;
+};
+
+struct tfim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int n_cu = 256;
+    char name[256] = {0};
+    int64_t hbm = 0;
+    int clock_mhz = 0;
+    std::string err;
+
+    // resident bitmaps
+    uint64_t *d_bm = nullptr;
+    int64_t pitch = 0;     // u64 words per row (multiple of 32)
+    int64_t n_bits = 0;
+    int32_t n_items = 0, n_cols = 0;
+
+    // per-node scratch
+    int32_t *d_node_items = nullptr, *d_node_out = nullptr;
+    int32_t *h_node_items = nullptr, *h_node_out = nullptr;   // pinned
+    int64_t node_items_cap = 0, node_out_cap = 0;
+
+    // wave scratch: one descriptor block + one output block
+    char *d_desc = nullptr, *h_desc = nullptr;   // h_desc pinned
+    int64_t desc_cap = 0;
+    int32_t *d_wave_out = nullptr, *h_wave_out = nullptr;
+    int64_t wave_out_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    int64_t wave_launches = 0, wave_graph_replays = 0, wave_rows = 0;
+    double wave_ms = 0;
+};
+
+namespace {
+
+int fail(tfim_ctx *c, int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, TFIM_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+// grow-only device + pinned host pair
+template <typename T>
+int ensure_pair(tfim_ctx *c, T **d, T **h, int64_t *cap, int64_t need) {
+    if (need <= *cap) return TFIM_OK;
+    int64_t n = std::max<int64_t>(need, std::max<int64_t>(*cap * 2, 1024));
+    if (*d) hipFree(*d);
+    if (h && *h) hipHostFree(*h);
+    *d = nullptr;
+    if (h) *h = nullptr;
+    *cap = 0;
+    if (hipMalloc((void **)d, (size_t)n * sizeof(T)) != hipSuccess)
+        return fail(c, TFIM_E_NOMEM, "device scratch allocation of %lld bytes failed", (long long)(n * sizeof(T)));
+    if (h && hipHostMalloc((void **)h, (size_t)n * sizeof(T), hipHostMallocDefault) != hipSuccess)
+        return fail(c, TFIM_E_NOMEM, "pinned scratch allocation of %lld bytes failed", (long long)(n * sizeof(T)));
+    *cap = n;
+    return TFIM_OK;
+}
+
+int set_device(tfim_ctx *c) {
+    HIPCHK(c, hipSetDevice(c->device));
+    return TFIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfim_open(int device, tfim_ctx **out) {
+    if (!out) return TFIM_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return TFIM_E_HIP;
+    if (device < 0 || device >= n) return TFIM_E_INVALID;
+    tfim_ctx *c = new tfim_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return TFIM_E_HIP; }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return TFIM_E_HIP; }
+    c->n_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+    snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
+    c->hbm = (int64_t)p.totalGlobalMem;
+    c->clock_mhz = p.clockRate / 1000;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TFIM_E_HIP; }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return TFIM_E_HIP; }
+    *out = c;
+    return TFIM_OK;
+}
+
+void tfim_close(tfim_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    hipFree(c->d_bm);
+    hipFree(c->d_node_items); hipFree(c->d_node_out);
+    hipHostFree(c->h_node_items); hipHostFree(c->h_node_out);
+    hipFree(c->d_desc); hipHostFree(c->h_desc);
+    hipFree(c->d_wave_out); hipHostFree(c->h_wave_out);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *tfim_last_error(const tfim_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int tfim_device_info(tfim_ctx *c, char *name, int name_len, int32_t *cus, int64_t *hbm, int32_t *clk) {
+    if (!c) return TFIM_E_INVALID;
+    if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", c->name);
+    if (cus) *cus = c->n_cu;
+    if (hbm) *hbm = c->hbm;
+    if (clk) *clk = c->clock_mhz;
+    return TFIM_OK;
+}
+
+int64_t tfim_pitch_words(int64_t n_bits) { return pitch_for_bits(n_bits); }
+
+static int load_common(tfim_ctx *c, int32_t n_items, int64_t n_bits, int32_t n_cols) {
+    if (n_items < 1 || n_bits < 1 || n_cols < 0 || n_cols > n_items)
+        return fail(c, TFIM_E_INVALID, "load_bitmaps: n_items=%d n_bits=%lld n_cols=%d", n_items,
+                    (long long)n_bits, n_cols);
+    if (n_bits > (int64_t)INT32_MAX)
+        return fail(c, TFIM_E_RANGE, "load_bitmaps: %lld transactions exceed int32 supports", (long long)n_bits);
+    if (int r = set_device(c)) return r;
+    hipFree(c->d_bm);
+    c->d_bm = nullptr;
+    c->pitch = pitch_for_bits(n_bits);
+    const size_t bytes = (size_t)n_items * (size_t)c->pitch * 8;
+    if (hipMalloc(&c->d_bm, bytes) != hipSuccess)
+        return fail(c, TFIM_E_NOMEM, "bitmap allocation of %zu bytes failed", bytes);
+    c->n_bits = n_bits;
+    c->n_items = n_items;
+    c->n_cols = n_cols;
+    return TFIM_OK;
+}
+
+int tfim_load_bitmaps(tfim_ctx *c, const uint64_t *rows, int32_t n_items, int64_t n_bits,
+                      int64_t words_per_row, int32_t n_cols) {
+    if (!c || !rows) return fail(c, TFIM_E_INVALID, "load_bitmaps: null argument");
+    if (words_per_row < (n_bits + 63) / 64)
+        return fail(c, TFIM_E_INVALID, "load_bitmaps: words_per_row %lld < ceil(n_bits/64)", (long long)words_per_row);
+    if (int r = load_common(c, n_items, n_bits, n_cols)) return r;
+    const int64_t used = (n_bits + 63) / 64;
+    HIPCHK(c, hipMemsetAsync(c->d_bm, 0, (size_t)n_items * c->pitch * 8, c->stream));
+    HIPCHK(c, hipMemcpy2DAsync(c->d_bm, (size_t)c->pitch * 8, rows, (size_t)words_per_row * 8,
+                               (size_t)used * 8, (size_t)n_items, hipMemcpyHostToDevice, c->stream));
+    // bits beyond n_bits in the last used word must be zero for exact counts
+    if (n_bits % 64) {
+        std::vector<uint64_t> tail((size_t)n_items);
+        const uint64_t keep = (1ull << (n_bits % 64)) - 1;
+        for (int32_t i = 0; i < n_items; i++) tail[i] = rows[(size_t)i * words_per_row + used - 1] & keep;
+        HIPCHK(c, hipMemcpy2DAsync(c->d_bm + used - 1, (size_t)c->pitch * 8, tail.data(), 8, 8,
+                                   (size_t)n_items, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TFIM_OK;
+}
+
+int tfim_load_bitmaps_replicated(tfim_ctx *c, const uint64_t *rows, int32_t n_items, int64_t n_bits,
+                                 int64_t words_per_row, int32_t n_cols, int32_t replicate) {
+    if (!c || !rows || replicate < 1) return fail(c, TFIM_E_INVALID, "load_bitmaps_replicated: bad argument");
+    if (words_per_row < (n_bits + 63) / 64)
+        return fail(c, TFIM_E_INVALID, "load_bitmaps_replicated: words_per_row too small");
+    const int64_t total_bits = n_bits * (int64_t)replicate;
+    if (int r = load_common(c, n_items, total_bits, n_cols)) return r;
+    // stage the base rows (zero beyond n_bits) on the device, then expand
+    const int64_t bp = pitch_for_bits(n_bits);
+    std::vector<uint64_t> base((size_t)n_items * bp, 0ull);
+    const int64_t used = (n_bits + 63) / 64;
+    for (int32_t i = 0; i < n_items; i++) {
+        memcpy(&base[(size_t)i * bp], rows + (size_t)i * words_per_row, (size_t)used * 8);
+        if (n_bits % 64) base[(size_t)i * bp + used - 1] &= (1ull << (n_bits % 64)) - 1;
+    }
+    uint64_t *d_base = nullptr;
+    if (hipMalloc(&d_base, base.size() * 8) != hipSuccess) return fail(c, TFIM_E_NOMEM, "base staging allocation failed");
+    hipError_t e = hipMemcpyAsync(d_base, base.data(), base.size() * 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_replicate_rows(d_base, bp, n_bits, c->d_bm, c->pitch, n_items, replicate, c->stream);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(d_base);
+    if (e != hipSuccess) return fail(c, TFIM_E_HIP, "replicate: %s", hipGetErrorString(e));
+    return TFIM_OK;
+}
+
+int tfim_item_supports(tfim_ctx *c, int32_t *out) {
+    if (!c || !out) return fail(c, TFIM_E_INVALID, "item_supports: null argument");
+    if (!c->d_bm) return fail(c, TFIM_E_STATE, "item_supports: no bitmaps loaded");
+    if (int r = set_device(c)) return r;
+    if (int r = ensure_pair(c, &c->d_node_out, &c->h_node_out, &c->node_out_cap, c->n_items)) return r;
+    launch_row_popcount(c->d_bm, c->pitch, c->n_items, c->d_node_out, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_node_out, c->d_node_out, (size_t)c->n_items * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memcpy(out, c->h_node_out, (size_t)c->n_items * 4);
+    return TFIM_OK;
+}
+
+int tfim_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t *out) {
+    if (!c || !items || !out || k < 1) return fail(c, TFIM_E_INVALID, "count_node: null argument or k < 1");
+    if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_node: no bitmaps loaded");
+    for (int32_t i = 0; i < k; i++)
+        if (items[i] < 0 || items[i] >= c->n_items)
+            return fail(c, TFIM_E_INVALID, "count_node: item %d out of range [0,%d)", items[i], c->n_items);
+    if (c->n_cols == 0) return TFIM_OK;
+    if (int r = set_device(c)) return r;
+    if (int r = ensure_pair(c, &c->d_node_items, &c->h_node_items, &c->node_items_cap, k)) return r;
+    if (int r = ensure_pair(c, &c->d_node_out, &c->h_node_out, &c->node_out_cap, c->n_cols)) return r;
+    memcpy(c->h_node_items, items, (size_t)k * 4);
+    HIPCHK(c, hipMemcpyAsync(c->d_node_items, c->h_node_items, (size_t)k * 4, hipMemcpyHostToDevice, c->stream));
+    launch_zero_i32(c->d_node_out, c->n_cols, c->stream);
+    launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_node_out, c->d_node_out, (size_t)c->n_cols * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memcpy(out, c->h_node_out, (size_t)c->n_cols * 4);
+    return TFIM_OK;
+}
+
+int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
+                     const int32_t *col_ids, const int32_t *col_off, int32_t *out, int32_t path) {
+    if (!c || !item_ids || !item_off || !out || n_rows < 0)
+        return fail(c, TFIM_E_INVALID, "count_nodes: null argument");
+    if (col_ids && !col_off) return fail(c, TFIM_E_INVALID, "count_nodes: col_ids without col_off");
+    if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_nodes: no bitmaps loaded");
+    if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED)
+        return fail(c, TFIM_E_INVALID, "count_nodes: path %d not available for node waves", path);
+    if (n_rows == 0) return TFIM_OK;
+    if (item_off[0] != 0) return fail(c, TFIM_E_INVALID, "count_nodes: item_off[0] != 0");
+    for (int32_t r = 0; r < n_rows; r++)
+        if (item_off[r + 1] <= item_off[r]) return fail(c, TFIM_E_INVALID, "count_nodes: row %d has no prefix item", r);
+    const int64_t n_item_ids = item_off[n_rows];
+    for (int64_t i = 0; i < n_item_ids; i++)
+        if (item_ids[i] < 0 || item_ids[i] >= c->n_items)
+            return fail(c, TFIM_E_INVALID, "count_nodes: item %d out of range", item_ids[i]);
+    int64_t n_out;
+    if (col_ids) {
+        if (col_off[0] != 0) return fail(c, TFIM_E_INVALID, "count_nodes: col_off[0] != 0");
+        for (int32_t r = 0; r < n_rows; r++)
+            if (col_off[r + 1] < col_off[r]) return fail(c, TFIM_E_INVALID, "count_nodes: col_off decreasing at %d", r);
+        n_out = col_off[n_rows];
+        for (int64_t i = 0; i < n_out; i++)
+            if (col_ids[i] < 0 || col_ids[i] >= c->n_cols)
+                return fail(c, TFIM_E_INVALID, "count_nodes: column %d out of range [0,%d)", col_ids[i], c->n_cols);
+    } else {
+        n_out = (int64_t)n_rows * c->n_cols;
+    }
+    if (n_out > INT32_MAX) return fail(c, TFIM_E_RANGE, "count_nodes: output too large");
+    if (int r = set_device(c)) return r;
+
+    // tiles of <= 32 columns per row
+    constexpr int kTileCols = 32;
+    std::vector<WaveTile> tiles;
+    tiles.reserve((size_t)n_rows * 2);
+    for (int32_t r = 0; r < n_rows; r++) {
+        const int32_t nc = col_ids ? col_off[r + 1] - col_off[r] : c->n_cols;
+        for (int32_t b = 0; b < nc; b += kTileCols) tiles.push_back({r, b, std::min(nc, b + kTileCols), 0});
+    }
+    if (tiles.empty()) return TFIM_OK;
+
+    // pack the descriptor block: item_ids | item_off | col_off | col_ids | tiles
+    auto align16 = [](int64_t x) { return (x + 15) / 16 * 16; };
+    const int64_t o_items = 0;
+    const int64_t o_ioff = align16(o_items + n_item_ids * 4);
+    const int64_t o_coff = align16(o_ioff + (int64_t)(n_rows + 1) * 4);
+    const int64_t o_cids = align16(o_coff + (int64_t)(n_rows + 1) * 4);
+    const int64_t o_tiles = align16(o_cids + (col_ids ? n_out * 4 : 0));
+    const int64_t total = o_tiles + (int64_t)tiles.size() * (int64_t)sizeof(WaveTile);
+    if (int r = ensure_pair(c, &c->d_desc, &c->h_desc, &c->desc_cap, total)) return r;
+    if (int r = ensure_pair(c, &c->d_wave_out, &c->h_wave_out, &c->wave_out_cap, n_out)) return r;
+    char *h = c->h_desc;
+    memcpy(h + o_items, item_ids, (size_t)n_item_ids * 4);
+    memcpy(h + o_ioff, item_off, (size_t)(n_rows + 1) * 4);
+    int32_t *coff = reinterpret_cast<int32_t *>(h + o_coff);
+    if (col_ids) {
+        memcpy(coff, col_off, (size_t)(n_rows + 1) * 4);
+        memcpy(h + o_cids, col_ids, (size_t)n_out * 4);
+    } else {
+        for (int32_t r = 0; r <= n_rows; r++) coff[r] = r * c->n_cols;
+    }
+    memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
+    HIPCHK(c, hipMemcpyAsync(c->d_desc, c->h_desc, (size_t)total, hipMemcpyHostToDevice, c->stream));
+
+    WaveDesc d;
+    d.item_ids = reinterpret_cast<const int32_t *>(c->d_desc + o_items);
+    d.item_off = reinterpret_cast<const int32_t *>(c->d_desc + o_ioff);
+    d.col_off = reinterpret_cast<const int32_t *>(c->d_desc + o_coff);
+    d.col_ids = col_ids ? reinterpret_cast<const int32_t *>(c->d_desc + o_cids) : nullptr;
+    d.tiles = reinterpret_cast<const WaveTile *>(c->d_desc + o_tiles);
+    d.out = c->d_wave_out;
+    d.n_tiles = (int32_t)tiles.size();
+    const int rw = c->pitch >= 2048 ? 4 : (c->pitch >= 512 ? 2 : 1);
+    const int64_t cw = wave_chunk_words(rw);
+    d.n_chunks = (int32_t)((c->pitch + cw - 1) / cw);
+    const int64_t units = (int64_t)d.n_chunks * d.n_tiles;
+    int64_t blocks = (units + 3) / 4;
+    blocks = std::min<int64_t>(blocks, (int64_t)c->n_cu * 8);
+    blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+
+    launch_zero_i32(c->d_wave_out, n_out, c->stream);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    launch_count_wave(c->d_bm, c->pitch, d, (int)blocks, rw, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_wave_out, c->d_wave_out, (size_t)n_out * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->wave_ms += ms;
+    c->wave_launches++;
+    c->wave_rows += n_rows;
+    memcpy(out, c->h_wave_out, (size_t)n_out * 4);
+    return TFIM_OK;
+}
+
+int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *replays, int64_t *rows, double *ms) {
+    if (!c) return TFIM_E_INVALID;
+    if (launches) *launches = c->wave_launches;
+    if (replays) *replays = c->wave_graph_replays;
+    if (rows) *rows = c->wave_rows;
+    if (ms) *ms = c->wave_ms;
+    return TFIM_OK;
+}
+
+int tfim_buf_alloc(tfim_ctx *c, int64_t bytes, tfim_buf **out) {
+    if (!c || !out || bytes <= 0) return fail(c, TFIM_E_INVALID, "buf_alloc: bad argument");
+    if (int r = set_device(c)) return r;
+    tfim_buf *b = new tfim_buf();
+    if (hipMalloc(&b->ptr, (size_t)bytes) != hipSuccess) {
+        delete b;
+        return fail(c, TFIM_E_NOMEM, "buf_alloc: %lld bytes failed", (long long)bytes);
+    }
+    b->bytes = bytes;
+    *out = b;
+    return TFIM_OK;
+}
+
+void tfim_buf_free(tfim_ctx *c, tfim_buf *b) {
+    if (!b) return;
+    if (c) {
+        hipSetDevice(c->device);
+        hipStreamSynchronize(c->stream);
+    }
+    hipFree(b->ptr);
+    delete b;
+}
+
+int tfim_buf_write(tfim_ctx *c, tfim_buf *b, int64_t off, const void *src, int64_t bytes) {
+    if (!c || !b || !src || off < 0 || bytes < 0 || off + bytes > b->bytes)
+        return fail(c, TFIM_E_INVALID, "buf_write: range");
+    if (int r = set_device(c)) return r;
+    HIPCHK(c, hipMemcpyAsync((char *)b->ptr + off, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TFIM_OK;
+}
+
+int tfim_buf_read(tfim_ctx *c, const tfim_buf *b, int64_t off, void *dst, int64_t bytes) {
+    if (!c || !b || !dst || off < 0 || bytes < 0 || off + bytes > b->bytes)
+        return fail(c, TFIM_E_INVALID, "buf_read: range");
+    if (int r = set_device(c)) return r;
+    HIPCHK(c, hipMemcpyAsync(dst, (const char *)b->ptr + off, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TFIM_OK;
+}
+
+int tfim_fill_bernoulli(tfim_ctx *c, tfim_buf *dst, int64_t rows, int64_t n_bits, int64_t pitch,
+                        uint64_t seed, uint32_t matrix, uint32_t thr32) {
+    if (!c || !dst || rows < 0 || n_bits < 0 || pitch < (n_bits + 63) / 64)
+        return fail(c, TFIM_E_INVALID, "fill_bernoulli: bad argument");
+    if (rows * pitch * 8 > dst->bytes) return fail(c, TFIM_E_INVALID, "fill_bernoulli: buffer too small");
+    if (int r = set_device(c)) return r;
+    launch_fill_bernoulli((uint64_t *)dst->ptr, rows, n_bits, pitch, seed, matrix, thr32, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return TFIM_OK;
+}
+
+static int popmat_check(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
+                        int64_t d, int64_t pitch, int32_t path) {
+    if (!c || !A || !B || !C || b < 1 || d < 1 || pitch < 1) return fail(c, TFIM_E_INVALID, "popcount_matrix: bad argument");
+    if (pitch % kPitchWords) return fail(c, TFIM_E_INVALID, "popcount_matrix: pitch %lld not a multiple of 32 words", (long long)pitch);
+    if (b * pitch * 8 > A->bytes || d * pitch * 8 > B->bytes || b * d * 4 > C->bytes)
+        return fail(c, TFIM_E_INVALID, "popcount_matrix: buffer too small");
+    if (pitch * 64 > (int64_t)INT32_MAX) return fail(c, TFIM_E_RANGE, "popcount_matrix: rows too long for int32 counts");
+    if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED && path != TFIM_PATH_MFMA_I8)
+        return fail(c, TFIM_E_INVALID, "popcount_matrix: unknown path %d", path);
+    return TFIM_OK;
+}
+
+static void popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
+                          int64_t d, int64_t pitch, int32_t path, int32_t split_k) {
+    if (split_k <= 0) split_k = 1;
+    if (split_k > 1) launch_zero_i32((int32_t *)C->ptr, b * d, c->stream);
+    if (path == TFIM_PATH_PACKED)
+        launch_popmat_packed((const uint64_t *)A->ptr, (const uint64_t *)B->ptr, (int32_t *)C->ptr, b, d, pitch, split_k, c->stream);
+    else
+        launch_popmat_mfma((const uint64_t *)A->ptr, (const uint64_t *)B->ptr, (int32_t *)C->ptr, b, d, pitch, split_k, c->stream);
+}
+
+int tfim_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
+                         int64_t d, int64_t pitch, int32_t path, int32_t split_k) {
+    if (int r = popmat_check(c, A, B, C, b, d, pitch, path)) return r;
+    if (int r = set_device(c)) return r;
+    popmat_launch(c, A, B, C, b, d, pitch, path, split_k);
+    HIPCHK(c, hipGetLastError());
+    return TFIM_OK;
+}
+
+int tfim_time_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
+                              int64_t d, int64_t pitch, int32_t path, int32_t split_k, int32_t warmup,
+                              int32_t reps, double *avg_ms) {
+    if (int r = popmat_check(c, A, B, C, b, d, pitch, path)) return r;
+    if (reps < 1 || !avg_ms) return fail(c, TFIM_E_INVALID, "time_popcount_matrix: reps < 1");
+    if (int r = set_device(c)) return r;
+    for (int i = 0; i < warmup; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split_k);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < reps; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split_k);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = (double)ms / reps;
+    return TFIM_OK;
+}
+
+int tfim_sync(tfim_ctx *c) {
+    if (!c) return TFIM_E_INVALID;
+    if (int r = set_device(c)) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return TFIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""GPU parity of the counting kernels against the CPU oracle (bit-exact).
+
+Shapes cover the edge cases the reference's own checks target
+(bmma_bench.cu:326-358, RESULTS.md:45): sparse random data (never all-ones),
+transaction counts off multiples of 64 and 2048, column counts off multiples
+of 8/64/128, prefixes of depth 1..31, an extra prefix-only row (item 0,
+kernel_bmma.cu:402-403), empty tails.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+import tensorfim_amd as tf
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_rows(rng, n_items, n_bits, density=0.3):
+    words = (n_bits + 63) // 64
+    bits = rng.random((n_items, words * 64)) < density
+    bits[:, n_bits:] = False
+    return np.packbits(bits, axis=1, bitorder="little").view(np.uint64).reshape(n_items, words)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = tf.Context(0)
+    print("device:", c.device_info(), flush=True)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n_bits,n_items", [(70, 9), (3200, 20), (2048 * 5 + 17, 131), (1_000_003, 70)])
+def test_count_node_matches_oracle(ctx, n_bits, n_items):
+    rng = np.random.default_rng(n_bits)
+    rows = rand_rows(rng, n_items, n_bits, 0.55)
+    n_cols = n_items - 1          # last row: prefix-only (item 0 quirk)
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    for k in (1, 2, 3, 7):
+        prefix = rng.choice(n_items, size=k, replace=False)
+        got = ctx.count_node(prefix)
+        exp = oracle_lib.node_counts(rows, prefix, n_cols)
+        np.testing.assert_array_equal(got, exp, err_msg="k=%d prefix=%s" % (k, prefix))
+
+
+def test_item_supports(ctx):
+    rng = np.random.default_rng(5)
+    rows = rand_rows(rng, 37, 100_001, 0.2)
+    ctx.load_bitmaps(rows, 100_001, 30)
+    exp = np.array([int(np.unpackbits(r.view(np.uint8)).sum()) for r in rows], dtype=np.int32)
+    np.testing.assert_array_equal(ctx.item_supports(), exp)
+
+
+@pytest.mark.parametrize("n_bits,n_items", [(1100, 300), (3200, 75), (65_537, 40), (2_000_000, 33)])
+def test_count_nodes_full_columns(ctx, n_bits, n_items):
+    rng = np.random.default_rng(n_items)
+    rows = rand_rows(rng, n_items, n_bits, 0.7)
+    n_cols = n_items
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    prefixes = [rng.choice(n_items, size=int(rng.integers(1, min(31, n_items) + 1)), replace=False)
+                for _ in range(77)]
+    got = ctx.count_nodes(prefixes)
+    for p, g in zip(prefixes, got):
+        np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols))
+
+
+@pytest.mark.parametrize("n_bits,n_items", [(3200, 75), (300_001, 61)])
+def test_count_nodes_tails(ctx, n_bits, n_items):
+    rng = np.random.default_rng(7 + n_items)
+    rows = rand_rows(rng, n_items, n_bits, 0.6)
+    n_cols = n_items - 1
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    prefixes, tails = [], []
+    for i in range(150):
+        k = int(rng.integers(1, 12))
+        prefixes.append(rng.choice(n_items, size=k, replace=False))
+        t = int(rng.integers(0, n_cols + 1)) if i % 10 else 0      # some empty tails
+        tails.append(np.sort(rng.choice(n_cols, size=t, replace=False)))
+    got = ctx.count_nodes(prefixes, tails)
+    for p, t, g in zip(prefixes, tails, got):
+        exp = oracle_lib.node_counts(rows, p, n_cols)[t]
+        np.testing.assert_array_equal(g, exp)
+
+
+def test_replicated_bitmaps(ctx):
+    """make_replica.py semantics on the device: every support scales by k."""
+    rng = np.random.default_rng(11)
+    base_bits, k, n_items = 49_046 // 16, 5, 12
+    rows = rand_rows(rng, n_items, base_bits, 0.5)
+    # host-side replication, bit by bit
+    bits = np.unpackbits(rows.view(np.uint8), axis=1, bitorder="little")[:, :base_bits]
+    rep = np.tile(bits, (1, k))
+    words = (base_bits * k + 63) // 64
+    pad = np.zeros((n_items, words * 64), dtype=np.uint8)
+    pad[:, :base_bits * k] = rep
+    rep_rows = np.packbits(pad, axis=1, bitorder="little").view(np.uint64).reshape(n_items, words)
+    ctx.load_bitmaps(rows, base_bits, n_items, replicate=k)
+    got_rep = [ctx.count_node([i, j]) for i, j in [(0, 1), (3, 7), (11, 2)]]
+    ctx.load_bitmaps(rep_rows, base_bits * k, n_items)
+    for (i, j), g in zip([(0, 1), (3, 7), (11, 2)], got_rep):
+        np.testing.assert_array_equal(g, ctx.count_node([i, j]))
+        np.testing.assert_array_equal(g, k * oracle_lib.node_counts(rows, [i, j], n_items))
+
+
+def test_fill_bernoulli_matches_oracle(ctx):
+    n_bits, rows = 100_000 + 13, 9
+    pitch = tf.pitch_words(n_bits)
+    buf = ctx.buffer(rows * pitch * 8)
+    thr32 = int(np.floor(0.3 * 2 ** 32))
+    ctx.fill_bernoulli(buf, rows, n_bits, pitch, seed=1, matrix=0, p=0.3)
+    got = buf.read(np.uint64, rows * pitch).reshape(rows, pitch)
+    for r in range(rows):
+        exp = oracle_lib.bernoulli_row(1, 0, r, n_bits, pitch, thr32)
+        np.testing.assert_array_equal(got[r], exp)
+    dens = np.unpackbits(got.view(np.uint8)).sum() / (rows * n_bits)
+    assert abs(dens - 0.3) < 0.01
+    buf.free()
+
+
+@pytest.mark.parametrize("path", ["packed", "mfma"])
+@pytest.mark.parametrize("b,d,n_bits,split", [(130, 70, 4096 + 65, 1), (256, 256, 65_536, 1),
+                                               (200, 513, 40_000, 3), (64, 33, 2048 * 9, 8),
+                                               (1, 1, 64, 1)])
+def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
+    rng = np.random.default_rng(b * 1000 + d)
+    pitch = tf.pitch_words(n_bits)
+    A = np.zeros((b, pitch), np.uint64)
+    B = np.zeros((d, pitch), np.uint64)
+    w = (n_bits + 63) // 64
+    A[:, :w] = rand_rows(rng, b, n_bits, 0.3)
+    B[:, :w] = rand_rows(rng, d, n_bits, 0.3)
+    dA, dB, dC = ctx.buffer(A.nbytes), ctx.buffer(B.nbytes), ctx.buffer(b * d * 4)
+    dA.write(A)
+    dB.write(B)
+    ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path=path, split_k=split)
+    got = dC.read(np.int32, b * d).reshape(b, d)
+    np.testing.assert_array_equal(got, oracle_lib.popcount_matrix(A, B))
+    for x in (dA, dB, dC):
+        x.free()
