@@ -1,0 +1,196 @@
+"""Benchmark: exact support counting on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--paths mfma,packed]
+
+A step is one pass of the hot path over one batch: the isolated count of
+configuration 2, C[i][j] = popcount(A_i AND B_j) for 65,536 prefix bitmaps x
+512 item bitmaps over 2^22 transactions (i.i.d. Bernoulli(0.3) bits, seed 1),
+resident in HBM before timing.  `value` is support-count Tbit-op/s of the
+int8-MFMA path (bit-ops = rows x items x transactions).  Also reported in the
+same run: the packed AND+v_bcnt path on the same data, the CPU baseline (the
+oracle's popcount on a 512-row sample, host threads stated), and unless
+--no-e2e the end-to-end mining time of configuration 4 (49,046-row base x256
+= 12,555,776 transactions, support 0.85) for the plain per-node GPU engine and
+the batched MAFIA engine with the learned policy, with their MFI sets compared.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import tensorfim_amd as tf  # noqa: E402
+
+METRIC = "support-count Tbit-op/s; end-to-end MFI seconds, batched+learned vs plain GPU"
+HBM_PEAK_GBS = 8000.0
+I8_PEAK_TOPS = 5000.0      # dense int8 MFMA, MI355X
+VALU_PEAK_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz int32 ops
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(A_rows, B_rows, n_bits, threads):
+    """The oracle's popcount (bmma_bench.cu cpu_ref semantics) on a sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    t0 = time.perf_counter()
+    C = oracle_lib.popcount_matrix(A_rows, B_rows, threads=threads)
+    dt = time.perf_counter() - t0
+    bitops = A_rows.shape[0] * B_rows.shape[0] * n_bits
+    return C, bitops / dt / 1e12, dt
+
+
+def bench_isolated(ctx, args):
+    b, d, n_bits = 65536, 512, 1 << 22
+    pitch = tf.pitch_words(n_bits)
+    log("config 2: A %d x %d bits (%.1f GiB), B %d x %d bits" % (b, n_bits, b * pitch * 8 / 2 ** 30, d, n_bits))
+    A = ctx.buffer(b * pitch * 8)
+    B = ctx.buffer(d * pitch * 8)
+    C = ctx.buffer(b * d * 4)
+    t0 = time.perf_counter()
+    ctx.fill_bernoulli(A, b, n_bits, pitch, seed=1, matrix=0, p=0.3)
+    ctx.fill_bernoulli(B, d, n_bits, pitch, seed=1, matrix=1, p=0.3)
+    ctx.sync()
+    log("generated on device in %.2f s" % (time.perf_counter() - t0))
+    bitops = float(b) * d * n_bits
+    alg_bytes = (b + d) * n_bits / 8 + 4.0 * b * d
+    res = {}
+    sample_rows = np.array([0, 1, 4097, 33333, b - 1])
+    B_host = B.read(np.uint64, d * pitch).reshape(d, pitch)
+    A_sample = np.stack([A.read(np.uint64, pitch, offset=int(r) * pitch * 8) for r in sample_rows])
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    C_exp = oracle_lib.popcount_matrix(A_sample, B_host, threads=8)
+    for path in args.paths:
+        split = 8 if path == "mfma" else 1
+        ms = ctx.time_popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=split, warmup=args.warmup,
+                                      reps=args.steps)
+        # exactness on sampled rows (all 512 columns) + full edge tile
+        got = np.stack([C.read(np.int32, d, offset=int(r) * d * 4) for r in sample_rows])
+        ok = bool((got == C_exp).all())
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=split)
+        ctx.sync()
+        wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        tbit = bitops / (ms * 1e-3) / 1e12
+        res[path] = {"kernel_ms": ms, "wall_ms_per_step": wall_ms, "tbitops": tbit, "exact_on_sample": ok,
+                     "split_k": split}
+        log("%-6s %.3f ms/launch  %.1f Tbit-op/s  exact=%s" % (path, ms, tbit, ok))
+        if not ok:
+            raise SystemExit("support counts differ from the oracle on sampled rows (%s path)" % path)
+    for x in (A, B, C):
+        x.free()
+    return res, bitops, alg_bytes, B_host, A_sample, n_bits
+
+
+def run_engine(data, engine, extra, out):
+    t0 = time.perf_counter()
+    path, st = tf.mine(data, 0.85, engine=engine, out=out, extra=["--replicate", "256"] + extra, timeout=1800)
+    wall = time.perf_counter() - t0
+    return path, st, wall
+
+
+def bench_e2e(args):
+    tmp = tempfile.mkdtemp(prefix="tfim_bench_")
+    data = os.path.join(tmp, "c4_base.txt")
+    subprocess.run([tf.binary("tfim_gen"), "4", data], check=True)
+    out = {}
+    sets = {}
+    for name, engine, extra in [("batched_learned", "mafia", ["--learned"]), ("plain", "plain", [])]:
+        log("config 4 e2e: %s ..." % name)
+        path, st, wall = run_engine(data, engine, extra, os.path.join(tmp, name + ".res"))
+        sets[name] = tf.load_results(path)
+        out[name] = {"wall_s": wall, "computation_s": float(st["computation_time(s)"]),
+                     "rows": int(st["rows"]), "waves": int(st["waves"]), "mfis": int(st["MFIsNumber"]),
+                     "kernel_ms": float(st.get("kernel_ms", 0)), "counted_bitops": float(st["counted_bitops"])}
+        log("  %s: %.2f s wall, %.2f s computation, %d rows" % (name, wall, out[name]["computation_s"],
+                                                               out[name]["rows"]))
+    out["identical_mfi_sets"] = sets["plain"] == sets["batched_learned"]
+    out["speedup_computation"] = out["plain"]["computation_s"] / out["batched_learned"]["computation_s"]
+    out["speedup_wall"] = out["plain"]["wall_s"] / out["batched_learned"]["wall_s"]
+    if not out["identical_mfi_sets"]:
+        raise SystemExit("batched+learned MFI set differs from the plain engine")
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-e2e", action="store_true")
+    ap.add_argument("--paths", default="mfma,packed")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    args.paths = [p for p in args.paths.split(",") if p]
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (transaction or subtree "
+              "sharding is a design note); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    ctx = tf.Context(0)
+    info = ctx.device_info()
+    log("device:", info)
+    res, bitops, alg_bytes, B_host, A_sample, n_bits = bench_isolated(ctx, args)
+    ctx.close()
+    head = res.get("mfma") or res[args.paths[0]]
+    head_path = "mfma" if "mfma" in res else args.paths[0]
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    log("cpu baseline: oracle popcount, %d threads, 512 rows x 512 items x 2^22 ..." % threads)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    rng_rows = np.repeat(A_sample, 103, axis=0)[:512]   # 512 device-generated rows (5 distinct)
+    _, cpu_tbit, cpu_s = cpu_baseline(rng_rows, B_host, n_bits, threads)
+    log("cpu: %.3f Tbit-op/s (%.1f s)" % (cpu_tbit, cpu_s))
+
+    e2e = None if args.no_e2e else bench_e2e(args)
+
+    kernel_s = head["kernel_ms"] * 1e-3
+    if head_path == "mfma":
+        achieved = 2.0 * bitops / kernel_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": I8_PEAK_TOPS, "unit": "TFLOP/s",
+                "frac": achieved / I8_PEAK_TOPS, "traffic": None}
+    else:
+        achieved = alg_bytes / kernel_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    line = {
+        "metric": METRIC,
+        "value": head["tbitops"],
+        "unit": "Tbit-op/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall_ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i8" if head_path == "mfma" else "u32",
+        "data": "synthetic (Bernoulli(0.3) bits generated on the device, seed 1)",
+        "config": {"workload": "isolated support count: 65,536 prefix bitmaps x 512 item bitmaps x 2^22 "
+                               "transactions, int8-MFMA path, split-k 8",
+                   "rows": 65536, "items": 512, "transactions": 1 << 22},
+        "roofline": roof,
+        "cpu_baseline": {"value": cpu_tbit, "unit": "Tbit-op/s", "cores": threads, "kind": "port",
+                         "sample": "oracle AND-popcount of 512 rows x 512 items x 2^22 transactions "
+                                   "(%.1f s)" % cpu_s},
+        "paths": res,
+        "device": info,
+    }
+    if e2e is not None:
+        line["e2e_config4"] = e2e
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

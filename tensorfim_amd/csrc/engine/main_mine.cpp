@@ -1,0 +1,139 @@
+// tfim_mine -- the reference CLI (kernel_bmma.cu:1179-1296) on MI355X.
+//
+//   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
+//             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
+//             [--learned] [--model PATH] [--trace PATH] [--verbose]
+//
+// Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the
+// reference layout and prints run statistics as "key: value" lines.  Exit
+// status is 0 on success (the reference always returns 1, kernel_bmma.cu:
+// 1295; callers here need a real status) and 1 on any failure, including an
+// unreadable input (no Results file is written then, as in the reference).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "dataset.h"
+#include "results.h"
+#include "search.h"
+#include "tfim.h"
+
+using namespace tfim;
+
+static double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int main(int argc, char **argv) {
+    SearchOptions opt;
+    std::string out_path;
+    int replicate = 0, device = 0;
+    std::vector<std::string> pos;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        auto next = [&]() -> std::string {
+            if (i + 1 >= argc) { fprintf(stderr, "missing value for %s\n", a.c_str()); exit(1); }
+            return argv[++i];
+        };
+        if (a == "--engine") opt.engine = next();
+        else if (a == "--replicate") replicate = atoi(next().c_str());
+        else if (a == "--out") out_path = next();
+        else if (a == "--wave-cap") opt.wave_cap = atoi(next().c_str());
+        else if (a == "--device") device = atoi(next().c_str());
+        else if (a == "--learned") opt.learned = true;
+        else if (a == "--model") opt.model_path = next();
+        else if (a == "--trace") opt.trace_path = next();
+        else if (a == "--verbose") opt.verbose = true;
+        else pos.push_back(a);
+    }
+    if (pos.size() < 2) {
+        fprintf(stderr, "usage: tfim_mine <file> <support> [--engine plain|wave|mafia] [--replicate K] [--out PATH]\n");
+        return 1;
+    }
+    const std::string file = pos[0];
+    const double support = atof(pos[1].c_str());
+    const double t0 = now();
+
+    tfim_ctx *ctx = nullptr;
+    if (tfim_open(device, &ctx) != TFIM_OK) {
+        fprintf(stderr, "tfim_open(%d) failed: no usable GPU\n", device);
+        return 1;
+    }
+    Database db;
+    std::string err;
+    if (!load_database(file, support, replicate, db, err)) {
+        fprintf(stderr, "%s\n", err.c_str());
+        tfim_close(ctx);
+        return 1;
+    }
+    const double t_up0 = now();
+    int rc = TFIM_OK;
+    if (db.d_count > 0 && db.base_bits > 0) {
+        if (db.replicate > 1)
+            rc = tfim_load_bitmaps_replicated(ctx, db.bitmap.data(), db.d_count, db.base_bits, db.words, db.n_surv,
+                                              db.replicate);
+        else
+            rc = tfim_load_bitmaps(ctx, db.bitmap.data(), db.d_count, db.base_bits, db.words, db.n_surv);
+    }
+    if (rc != TFIM_OK) {
+        fprintf(stderr, "upload failed: %s\n", tfim_last_error(ctx));
+        tfim_close(ctx);
+        return 1;
+    }
+    const double t_up1 = now();
+
+    std::vector<Mfi> mfis;
+    SearchStats st;
+    const double c0 = now();
+    if (db.d_count > 0 && db.base_bits > 0) rc = run_search(ctx, db, opt, mfis, st, err);
+    const double c1 = now();
+    if (rc != TFIM_OK) {
+        fprintf(stderr, "search failed: %s\n", err.c_str());
+        tfim_close(ctx);
+        return 1;
+    }
+    RunInfo info;
+    char name[256];
+    int32_t cus = 0, clk = 0;
+    int64_t hbm = 0;
+    tfim_device_info(ctx, name, sizeof name, &cus, &hbm, &clk);
+    info.device = name;
+    info.compute_units = cus;
+    info.hbm_bytes = hbm;
+    info.engine = opt.engine + (opt.learned ? " (learned ranker/gate)" : "");
+    info.preprocessing_time = db.preprocessing_seconds;
+    info.computation_time = c1 - c0;
+    char path[4096];
+    if (!out_path.empty()) snprintf(path, sizeof path, "%s", out_path.c_str());
+    else snprintf(path, sizeof path, "%s-%f=Results.txt", file.c_str(), support);
+    info.wall_time = now() - t0;
+    if (!write_results(path, file, db, mfis, info)) {
+        fprintf(stderr, "cannot write %s\n", path);
+        tfim_close(ctx);
+        return 1;
+    }
+    const double wall = now() - t0;
+    printf("results: %s\n", path);
+    printf("engine: %s\n", info.engine.c_str());
+    printf("device: %s\n", name);
+    printf("transNum: %d\ntransNumReduced: %d\nsurvivors: %d\nroots: %d\n", db.trans_num, db.trans_num_reduced,
+           db.n_surv, db.n_roots);
+    printf("MFIsNumber: %zu\n", mfis.size());
+    printf("mfi_total: %lld\n", (long long)st.mfi_total);
+    printf("parse_time(s): %g\npreprocessing_time(s): %g\nupload_time(s): %g\n", db.parse_seconds,
+           db.preprocessing_seconds, t_up1 - t_up0);
+    printf("computation_time(s): %g\nwall_time(s): %g\n", info.computation_time, wall);
+    printf("count_time(s): %g\nkernel_ms: %g\n", st.count_seconds, st.kernel_ms);
+    printf("rows: %lld\nwaves: %lld\n", (long long)st.nodes, (long long)st.waves);
+    printf("expand_rows: %lld\nhut_rows: %lld\nleaf_rows: %lld\nref_rows: %lld\n", (long long)st.expand_rows,
+           (long long)st.hut_rows, (long long)st.leaf_rows, (long long)st.ref_rows);
+    printf("hut_success: %lld\nhut_skipped: %lld\npep_moves: %lld\nhutmfi_pruned: %lld\nleaf_subsumed: %lld\n",
+           (long long)st.hut_success, (long long)st.hut_skipped, (long long)st.pep_moves,
+           (long long)st.hutmfi_pruned, (long long)st.leaf_subsumed);
+    printf("counted_bitops: %.6g\nlearn_time(s): %g\nmlp_train_steps: %lld\n", st.counted_bitops, st.learn_seconds,
+           (long long)st.mlp_train_steps);
+    tfim_close(ctx);
+    return 0;
+}

@@ -1,0 +1,50 @@
+// Search policy of the MAFIA engine: the order of each node's tail (which
+// child gets the largest tail) and whether to spend a head-union-tail
+// lookahead row on it.  Neither choice can change the output: the set
+// enumeration is complete for any tail order, and a skipped lookahead only
+// forgoes a pruning; every pruning that is taken is confirmed by exact counts.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "dataset.h"
+#include "tfim.h"
+
+namespace tfim {
+
+struct SearchOptions;
+struct SearchStats;
+
+// A node after tail counting and parent-equivalence pruning.
+struct Decision {
+    std::vector<int32_t> head;      // dense ids
+    int32_t sup = 0;                // sup(head)
+    std::vector<int32_t> tail;      // frequent, non-equivalent tail items
+    std::vector<int32_t> tail_sup;  // sup(head + tail[i])
+    int32_t depth = 0;
+    std::vector<int32_t> order;     // out: permutation of tail indices
+    bool lookahead = false;         // out: count head+tail before expanding
+    int32_t record = -1;            // out: policy record id for observe()
+};
+
+class Policy {
+   public:
+    virtual ~Policy() {}
+    virtual void decide(std::vector<Decision *> &ds, const Database &db) = 0;
+    // outcome: 1 = HUT frequent, 0 = not frequent, -1 = lookahead not taken
+    virtual void observe(int32_t record, int outcome) { (void)record; (void)outcome; }
+    virtual void finish(SearchStats &st) { (void)st; }
+};
+
+// MAFIA's plain policy: tail by increasing support (ties: ascending id),
+// lookahead whenever the tail holds two or more items.
+class PlainPolicy : public Policy {
+   public:
+    void decide(std::vector<Decision *> &ds, const Database &db) override;
+};
+
+std::unique_ptr<Policy> make_policy(tfim_ctx *ctx, const SearchOptions &opt);
+
+}  // namespace tfim

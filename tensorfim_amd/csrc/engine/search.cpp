@@ -1,0 +1,521 @@
+#include "search.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <thread>
+#include <unordered_set>
+
+#include "policy.h"
+
+namespace tfim {
+
+namespace {
+
+double now() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// One wave of count rows in CSR form, handed to tfim_count_nodes.
+class WaveBatch {
+   public:
+    void clear() {
+        item_ids_.clear();
+        col_ids_.clear();
+        item_off_.assign(1, 0);
+        col_off_.assign(1, 0);
+    }
+    int add(const int32_t *prefix, int k, const int32_t *cols, int nc) {
+        item_ids_.insert(item_ids_.end(), prefix, prefix + k);
+        item_off_.push_back((int32_t)item_ids_.size());
+        col_ids_.insert(col_ids_.end(), cols, cols + nc);
+        col_off_.push_back((int32_t)col_ids_.size());
+        return rows() - 1;
+    }
+    int rows() const { return (int)item_off_.size() - 1; }
+    int64_t cols_total() const { return (int64_t)col_ids_.size(); }
+    int run(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, std::string &err) {
+        out_.resize(std::max<size_t>(1, col_ids_.size()));
+        st.nodes += rows();
+        if (col_ids_.empty()) return TFIM_OK;   // nothing to count (e.g. a leaf holding every survivor)
+        const double t0 = now();
+        const int rc = tfim_count_nodes(ctx, item_ids_.data(), item_off_.data(), rows(),
+                                        col_ids_.empty() ? nullptr : col_ids_.data(),
+                                        col_ids_.empty() ? nullptr : col_off_.data(), out_.data(), TFIM_PATH_AUTO);
+        st.count_seconds += now() - t0;
+        if (rc != TFIM_OK) {
+            err = std::string("tfim_count_nodes: ") + tfim_last_error(ctx);
+            return rc;
+        }
+        st.waves++;
+        st.counted_bitops += (double)cols_total() * (double)n_bits;
+        return TFIM_OK;
+    }
+    const int32_t *result(int r) const { return out_.data() + col_off_[r]; }
+    int ncols(int r) const { return col_off_[r + 1] - col_off_[r]; }
+
+   private:
+    std::vector<int32_t> item_ids_, item_off_{0}, col_ids_, col_off_{0}, out_;
+};
+
+bool frequent(const Database &db, int32_t s) { return !((double)s < db.thr_abs); }
+
+// ---------------------------------------------------------------------------
+// Reference enumeration (plain and wave engines).  A node is (items in
+// insertion order, frequency, progress); counting covers every survivor;
+// maximal iff #{e : sup(X+e) >= thr} == |X|; children X+i for i < progress
+// with sup(X+i) >= thr, i not in X (kernel_bmma.cu:874-890).
+struct RefNode {
+    std::vector<int32_t> items;
+    int32_t freq = 0;
+    int32_t progress = 0;
+};
+
+void ref_expand(const Database &db, RefNode &nd, const int32_t *res, std::vector<RefNode> &stack,
+                std::vector<Mfi> &out) {
+    int32_t cnt = 0;
+    for (int32_t e = 0; e < db.n_surv; e++) cnt += frequent(db, res[e]);
+    if (cnt == (int32_t)nd.items.size()) {
+        out.push_back({nd.items, nd.freq});
+        return;
+    }
+    for (int32_t i = 0; i <= nd.progress - 1 && i < db.n_surv; i++) {
+        if (!frequent(db, res[i])) continue;
+        if (std::find(nd.items.begin(), nd.items.end(), i) != nd.items.end()) continue;
+        RefNode c;
+        c.items = nd.items;
+        c.items.push_back(i);
+        c.freq = res[i];
+        c.progress = i;
+        stack.push_back(std::move(c));
+    }
+}
+
+std::vector<RefNode> ref_roots(const Database &db, bool only_item0) {
+    std::vector<RefNode> st;
+    for (int32_t i = 0; i < db.n_roots; i++) {
+        const int32_t it = db.sorted_items[i];
+        if (only_item0 && it != db.n_surv) continue;
+        RefNode r;
+        r.items = {it};
+        r.freq = db.freq_dense[it];
+        r.progress = it;
+        st.push_back(std::move(r));
+    }
+    return st;
+}
+
+int run_plain(tfim_ctx *ctx, const Database &db, std::vector<Mfi> &out, SearchStats &st, std::string &err) {
+    std::vector<RefNode> stack = ref_roots(db, false);
+    std::vector<int32_t> res((size_t)db.n_surv + 1, 0);
+    while (!stack.empty()) {
+        RefNode nd = std::move(stack.back());
+        stack.pop_back();
+        const double t0 = now();
+        const int rc = db.n_surv > 0 ? tfim_count_node(ctx, nd.items.data(), (int32_t)nd.items.size(), res.data())
+                                     : TFIM_OK;
+        st.count_seconds += now() - t0;
+        if (rc != TFIM_OK) {
+            err = std::string("tfim_count_node: ") + tfim_last_error(ctx);
+            return rc;
+        }
+        st.nodes++;
+        st.waves++;
+        st.ref_rows++;
+        st.counted_bitops += (double)db.n_surv * (double)db.n_bits();
+        ref_expand(db, nd, res.data(), stack, out);
+    }
+    return TFIM_OK;
+}
+
+int32_t auto_wave_cap(const Database &db) {
+    // Keep one wave's column traffic near 4 GiB of bitmap words so a wave is
+    // long enough to amortise the launch and short enough to keep the DFS
+    // frontier small.
+    const double row_bytes = (double)db.n_bits() / 8.0 * std::max(1, db.n_surv);
+    const double cap = 4.0 * (1ull << 30) / std::max(1.0, row_bytes);
+    return (int32_t)std::min(2048.0, std::max(64.0, cap));
+}
+
+int run_wave(tfim_ctx *ctx, const Database &db, int32_t cap, std::vector<Mfi> &out, SearchStats &st,
+             std::string &err) {
+    std::vector<RefNode> stack = ref_roots(db, false);
+    std::vector<int32_t> all(db.n_surv);
+    std::iota(all.begin(), all.end(), 0);
+    WaveBatch wb;
+    std::vector<RefNode> wave;
+    while (!stack.empty()) {
+        wave.clear();
+        while ((int)wave.size() < cap && !stack.empty()) {
+            wave.push_back(std::move(stack.back()));
+            stack.pop_back();
+        }
+        wb.clear();
+        for (auto &nd : wave) wb.add(nd.items.data(), (int)nd.items.size(), all.data(), db.n_surv);
+        if (int rc = wb.run(ctx, st, db.n_bits(), err)) return rc;
+        st.ref_rows += (int64_t)wave.size();
+        for (size_t i = 0; i < wave.size(); i++) ref_expand(db, wave[i], wb.result((int)i), stack, out);
+    }
+    return TFIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MAFIA-style engine.
+//
+// Node kinds, each one count row (prefix items, column list):
+//   EXPAND (head, tail)          cols = tail      -> PEP, children or lookahead
+//   HUT    (head' + tail'')      cols = {h0} + (U \ HUT)
+//                                 -> sup(HUT) in col 0; HUT maximal iff the
+//                                    other columns are all below threshold
+//   LEAF   (H)                   cols = U \ H     -> H maximal iff all below
+//   REF    (reference node)      cols = U         -> only in the subtree of a
+//                                                    frequent item 0 (quirk A.3)
+// U is the survivor set [0, F).  The reference reports the MFIs of U whose
+// largest dense id is a root (strict-threshold roots, quirk A.2), plus its
+// own enumeration under a frequent item-0 root; both are reproduced.
+struct MNode {
+    enum Kind : uint8_t { EXPAND, HUT, LEAF, REF } kind = EXPAND;
+    std::vector<int32_t> head;       // EXPAND/HUT/LEAF head; REF items
+    int32_t sup = 0;                 // sup(head) (REF: node frequency)
+    std::vector<int32_t> tail;       // EXPAND: candidates; HUT: ordered tail''
+    std::vector<int32_t> tail_sup;   // HUT: sup(head'+t) for the fallback children
+    int32_t progress = 0;            // REF
+    int32_t depth = 0;
+    int32_t decision = -1;           // HUT: policy record index for its outcome
+};
+
+// Found MFIs, sliced by item: bits_[i] has bit m set when MFI m contains
+// item i.  X is inside some found MFI iff the AND of its items' slices is
+// non-zero, scanned word by word with an early exit.
+class MfiIndex {
+   public:
+    explicit MfiIndex(int32_t n) : bits_((size_t)std::max(1, n)) {}
+    bool subsumed(const std::vector<int32_t> &x) const {
+        if (x.empty()) return n_ > 0;
+        for (size_t w = 0; w < words_; w++) {
+            uint64_t acc = ~0ull;
+            for (int32_t i : x) {
+                acc &= bits_[i][w];
+                if (!acc) break;
+            }
+            if (acc) return true;
+        }
+        return false;
+    }
+    void add(const std::vector<int32_t> &x) {
+        if (n_ % 64 == 0) {
+            words_++;
+            for (auto &b : bits_) b.push_back(0ull);
+        }
+        for (int32_t i : x) bits_[i][n_ >> 6] |= 1ull << (n_ & 63);
+        n_++;
+    }
+
+   private:
+    std::vector<std::vector<uint64_t>> bits_;
+    size_t words_ = 0;
+    int64_t n_ = 0;
+};
+
+struct VecHash {
+    size_t operator()(const std::vector<int32_t> &v) const {
+        uint64_t h = 1469598103934665603ull;
+        for (int32_t x : v) h = (h ^ (uint32_t)x) * 1099511628211ull;
+        return (size_t)h;
+    }
+};
+
+class Mafia {
+   public:
+    Mafia(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, Policy &pol, SearchStats &st)
+        : ctx_(ctx), db_(db), opt_(opt), pol_(pol), st_(st), index_(db.n_surv) {}
+
+    int run(std::vector<Mfi> &out, std::string &err) {
+        const int32_t F = db_.n_surv;
+        cap_ = opt_.wave_cap > 0 ? opt_.wave_cap : auto_wave_cap(db_);
+        // item-0 subtree: the reference enumeration, reproduced node for node
+        if (db_.d_count > F && db_.is_root[F]) {
+            for (auto &r : ref_roots(db_, true)) {
+                MNode n;
+                n.kind = MNode::REF;
+                n.head = r.items;
+                n.sup = r.freq;
+                n.progress = r.progress;
+                stack_.push_back(std::move(n));
+            }
+        }
+        // root: empty head, tail = U; children are the single items
+        if (F > 0) {
+            Decision root;
+            root.head = {};
+            root.sup = db_.trans_num_reduced;
+            root.depth = 0;
+            for (int32_t i = 0; i < F; i++) {
+                root.tail.push_back(i);
+                root.tail_sup.push_back(db_.freq_dense[i]);
+            }
+            root.lookahead = false;
+            std::vector<Decision *> ds{&root};
+            pol_.decide(ds, db_);
+            root.lookahead = false;   // the empty head has no HUT row
+            push_children(root);
+        }
+        while (!stack_.empty()) {
+            wave_.clear();
+            while ((int)wave_.size() < cap_ && !stack_.empty()) {
+                wave_.push_back(std::move(stack_.back()));
+                stack_.pop_back();
+            }
+            if (int rc = count_wave(err)) return rc;
+            if (int rc = process_wave(out)) return rc;
+        }
+        return TFIM_OK;
+    }
+
+   private:
+    void complement(const std::vector<int32_t> &x, std::vector<int32_t> &cols, bool lead) {
+        mark_.assign(db_.n_surv, 0);
+        for (int32_t i : x) mark_[i] = 1;
+        cols.clear();
+        if (lead) cols.push_back(x[0]);
+        for (int32_t i = 0; i < db_.n_surv; i++)
+            if (!mark_[i]) cols.push_back(i);
+    }
+
+    int count_wave(std::string &err) {
+        wb_.clear();
+        std::vector<int32_t> cols;
+        for (auto &n : wave_) {
+            switch (n.kind) {
+                case MNode::EXPAND:
+                    wb_.add(n.head.data(), (int)n.head.size(), n.tail.data(), (int)n.tail.size());
+                    st_.expand_rows++;
+                    break;
+                case MNode::HUT: {
+                    std::vector<int32_t> hut = n.head;
+                    hut.insert(hut.end(), n.tail.begin(), n.tail.end());
+                    complement(hut, cols, true);
+                    wb_.add(hut.data(), (int)hut.size(), cols.data(), (int)cols.size());
+                    st_.hut_rows++;
+                    break;
+                }
+                case MNode::LEAF:
+                    complement(n.head, cols, false);
+                    wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
+                    st_.leaf_rows++;
+                    break;
+                case MNode::REF:
+                    cols.resize(db_.n_surv);
+                    std::iota(cols.begin(), cols.end(), 0);
+                    wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
+                    st_.ref_rows++;
+                    break;
+            }
+        }
+        return wb_.run(ctx_, st_, db_.n_bits(), err);
+    }
+
+    void emit(std::vector<int32_t> items, int32_t sup, std::vector<Mfi> &out) {
+        std::sort(items.begin(), items.end(), std::greater<int32_t>());   // insertion order: root first
+        if (!seen_.insert(items).second) return;
+        index_.add(items);
+        st_.mfi_total++;
+        if (db_.is_root[items[0]]) {
+            // a single-item MFI carries the root's occurrence count, as the
+            // reference reports roots with freqPerItem (kernel_bmma.cu:1227)
+            const int32_t f = items.size() == 1 ? db_.freq_dense[items[0]] : sup;
+            out.push_back({items, f});
+        }
+    }
+
+    void push_children(const Decision &d) {
+        // child i: head + t_i with tail t_{i+1..}; pushed last-first so the
+        // first child (largest tail) is counted first
+        const size_t k = d.order.size();
+        for (size_t q = k; q-- > 0;) {
+            MNode c;
+            c.kind = MNode::EXPAND;
+            c.head = d.head;
+            c.head.push_back(d.tail[d.order[q]]);
+            c.sup = d.tail_sup[d.order[q]];
+            c.depth = d.depth + 1;
+            for (size_t r = q + 1; r < k; r++) c.tail.push_back(d.tail[d.order[r]]);
+            if (c.tail.empty()) {
+                // no candidates left: the child is a leaf
+                if (index_.subsumed(c.head)) { st_.leaf_subsumed++; continue; }
+                c.kind = MNode::LEAF;
+            }
+            stack_.push_back(std::move(c));
+        }
+    }
+
+    int process_wave(std::vector<Mfi> &out) {
+        std::vector<Decision> decisions;
+        decisions.reserve(wave_.size());
+        for (int r = 0; r < (int)wave_.size(); r++) {
+            MNode &n = wave_[r];
+            const int32_t *res = wb_.result(r);
+            const int nc = wb_.ncols(r);
+            if (n.kind == MNode::EXPAND) {
+                Decision d;
+                d.head = n.head;
+                d.sup = n.sup;
+                d.depth = n.depth;
+                for (int j = 0; j < nc; j++) {
+                    const int32_t t = n.tail[j], s = res[j];
+                    if (!frequent(db_, s)) continue;
+                    if (s == n.sup) {   // PEP: every MFI above head contains t
+                        d.head.push_back(t);
+                        st_.pep_moves++;
+                    } else {
+                        d.tail.push_back(t);
+                        d.tail_sup.push_back(s);
+                    }
+                }
+                if (d.tail.empty()) {
+                    if (index_.subsumed(d.head)) {
+                        st_.leaf_subsumed++;
+                    } else {
+                        MNode leaf;
+                        leaf.kind = MNode::LEAF;
+                        leaf.head = d.head;
+                        leaf.sup = d.sup;
+                        pending_.push_back(std::move(leaf));
+                    }
+                    continue;
+                }
+                std::vector<int32_t> hut = d.head;
+                hut.insert(hut.end(), d.tail.begin(), d.tail.end());
+                if (index_.subsumed(hut)) {
+                    st_.hutmfi_pruned++;
+                    continue;
+                }
+                decisions.push_back(std::move(d));
+            } else if (n.kind == MNode::HUT) {
+                const bool hut_freq = frequent(db_, res[0]);
+                pol_.observe(n.decision, hut_freq);
+                if (hut_freq) {
+                    st_.hut_success++;
+                    bool maximal = true;
+                    for (int j = 1; j < nc && maximal; j++) maximal = !frequent(db_, res[j]);
+                    std::vector<int32_t> hut = n.head;
+                    hut.insert(hut.end(), n.tail.begin(), n.tail.end());
+                    if (maximal) emit(hut, res[0], out);
+                } else {
+                    Decision d;
+                    d.head = n.head;
+                    d.sup = n.sup;
+                    d.depth = n.depth;
+                    d.tail = n.tail;
+                    d.tail_sup = n.tail_sup;
+                    d.order.resize(d.tail.size());
+                    std::iota(d.order.begin(), d.order.end(), 0);   // tail kept in decided order
+                    hut_children_.push_back(std::move(d));
+                }
+            } else if (n.kind == MNode::LEAF) {
+                bool maximal = true;
+                for (int j = 0; j < nc && maximal; j++) maximal = !frequent(db_, res[j]);
+                if (maximal) emit(n.head, n.sup, out);
+            } else {   // REF: reference rule, own output path
+                int32_t cnt = 0;
+                for (int j = 0; j < nc; j++) cnt += frequent(db_, res[j]);
+                if (cnt == (int32_t)n.head.size()) {
+                    out.push_back({n.head, n.sup});
+                } else {
+                    for (int32_t i = 0; i <= n.progress - 1 && i < db_.n_surv; i++) {
+                        if (!frequent(db_, res[i])) continue;
+                        if (std::find(n.head.begin(), n.head.end(), i) != n.head.end()) continue;
+                        MNode c;
+                        c.kind = MNode::REF;
+                        c.head = n.head;
+                        c.head.push_back(i);
+                        c.sup = res[i];
+                        c.progress = i;
+                        pending_.push_back(std::move(c));
+                    }
+                }
+            }
+        }
+        // policy: order tails, choose lookaheads (batched over the wave)
+        std::vector<Decision *> ptrs;
+        for (auto &d : decisions) ptrs.push_back(&d);
+        if (!ptrs.empty()) {
+            const double t0 = now();
+            pol_.decide(ptrs, db_);
+            st_.learn_seconds += now() - t0;
+        }
+        // push in reverse so the first decision's work is on top
+        for (auto &n : pending_) stack_.push_back(std::move(n));
+        pending_.clear();
+        for (auto &d : hut_children_) push_children(d);
+        hut_children_.clear();
+        for (size_t q = decisions.size(); q-- > 0;) {
+            Decision &d = decisions[q];
+            if (d.lookahead && d.tail.size() >= 2) {
+                MNode h;
+                h.kind = MNode::HUT;
+                h.head = d.head;
+                h.sup = d.sup;
+                h.depth = d.depth;
+                for (int32_t o : d.order) {
+                    h.tail.push_back(d.tail[o]);
+                    h.tail_sup.push_back(d.tail_sup[o]);
+                }
+                h.decision = d.record;
+                stack_.push_back(std::move(h));
+            } else {
+                if (d.tail.size() >= 2) st_.hut_skipped++;
+                pol_.observe(d.record, -1);
+                push_children(d);
+            }
+        }
+        return TFIM_OK;
+    }
+
+    tfim_ctx *ctx_;
+    const Database &db_;
+    const SearchOptions &opt_;
+    Policy &pol_;
+    SearchStats &st_;
+    MfiIndex index_;
+    std::unordered_set<std::vector<int32_t>, VecHash> seen_;
+    std::vector<MNode> stack_, wave_, pending_;
+    std::vector<Decision> hut_children_;
+    std::vector<uint8_t> mark_;
+    WaveBatch wb_;
+    int32_t cap_ = 256;
+};
+
+}  // namespace
+
+int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std::vector<Mfi> &out,
+               SearchStats &st, std::string &err) {
+    if (opt.engine == "plain") return run_plain(ctx, db, out, st, err);
+    if (opt.engine == "wave") {
+        const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
+        const int rc = run_wave(ctx, db, cap, out, st, err);
+        tfim_wave_stats(ctx, nullptr, nullptr, nullptr, &st.kernel_ms);
+        return rc;
+    }
+    if (opt.engine == "mafia") {
+        if (db.duplicates) {
+            // Repeated ids inside one transaction make occurrence counts differ
+            // from bitmap supports, which the reference's own rules mix; its
+            // enumeration is reproduced as is in that case.
+            const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
+            return run_wave(ctx, db, cap, out, st, err);
+        }
+        std::unique_ptr<Policy> pol = make_policy(ctx, opt);
+        Mafia m(ctx, db, opt, *pol, st);
+        const int rc = m.run(out, err);
+        pol->finish(st);
+        tfim_wave_stats(ctx, nullptr, nullptr, nullptr, &st.kernel_ms);
+        return rc;
+    }
+    err = "unknown engine " + opt.engine;
+    return TFIM_E_INVALID;
+}
+
+}  // namespace tfim

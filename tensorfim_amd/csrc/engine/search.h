@@ -1,0 +1,58 @@
+// Depth-first MFI search engines above the C ABI of libtfim.
+//
+//   plain  -- the reference baseline engine in GPU-only mode (RUN_MODE 1,
+//             kernel_bitmap.cu:853-894): one count launch per DFS node, all
+//             survivors counted, reference maximality and extension rule.
+//   wave   -- the reference TensorFIM engine (runWaveEngine, kernel_bmma.cu:
+//             896-910): the same enumeration, nodes popped in waves and each
+//             wave counted in one fused launch.
+//   mafia  -- batched MAFIA-style search (Burdick, Calimlim & Gehrke, ICDE
+//             2001): tail counting, parent-equivalence pruning, head-union-tail
+//             lookahead, HUT-subset pruning against found MFIs, maximality
+//             decided by an exact all-survivor count.  Optional learned tail
+//             ranking and lookahead gating (every decision confirmed by exact
+//             counts).  Output is set-identical to the reference enumeration.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "dataset.h"
+#include "results.h"
+#include "tfim.h"
+
+namespace tfim {
+
+struct SearchOptions {
+    std::string engine = "mafia";
+    int32_t wave_cap = 0;         // rows per wave; 0 = automatic
+    int32_t workers = 1;          // plain engine: host workers (one stream each)
+    bool learned = false;         // mafia: use the MLP ranker/gate
+    std::string model_path;       // mafia: load MLP weights (else trained online)
+    std::string trace_path;       // mafia: write (features, outcome) records
+    bool verbose = false;
+};
+
+struct SearchStats {
+    int64_t nodes = 0;            // count rows issued (all kinds)
+    int64_t waves = 0;            // count launches
+    int64_t expand_rows = 0, hut_rows = 0, leaf_rows = 0, ref_rows = 0;
+    int64_t hut_success = 0;      // HUT lookaheads that found HUT frequent
+    int64_t hut_skipped = 0;      // lookaheads the gate declined
+    int64_t pep_moves = 0;        // items moved into heads by PEP
+    int64_t hutmfi_pruned = 0;    // subtrees cut because HUT is inside a found MFI
+    int64_t leaf_subsumed = 0;    // leaves discarded by a found superset
+    int64_t mfi_total = 0;        // MFIs of the survivor universe found
+    double counted_bitops = 0;    // sum over rows of columns x transactions
+    double count_seconds = 0;     // host time inside count calls
+    double kernel_ms = 0;         // device time of count kernels (HIP events)
+    double learn_seconds = 0;     // host time in the learned part
+    int64_t mlp_train_steps = 0;
+};
+
+// Runs the engine; returns MFIs (dense ids, insertion order) with frequencies.
+int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std::vector<Mfi> &out,
+               SearchStats &stats, std::string &err);
+
+}  // namespace tfim

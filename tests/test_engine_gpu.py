@@ -1,0 +1,81 @@
+"""End-to-end engine parity on the GPU.
+
+Every engine (plain per-node, wave, batched MAFIA with and without the learned
+policy) must reproduce the reference's output set exactly: the gold Results
+files for the six FIMI datasets in the checkout (scripts/verify_all.py
+comparison: {frozenset(items): frequency}), and the oracle's reference DFS on
+the synthetic configurations.
+"""
+import os
+import subprocess
+
+import pytest
+
+import oracle_lib
+import tensorfim_amd as tf
+from conftest import GOLD_PAIRS, gold_path
+
+pytestmark = pytest.mark.gpu
+
+ENGINES = [("plain", []), ("wave", []), ("mafia", []), ("mafia", ["--learned"])]
+HEADER_KEYS = ["transNum", "dimension", "itemIDmax", "dimensionReduced", "transNumReduced", "MFIsNumber",
+               "transLengthMean", "transLengthReducedMean"]
+
+
+def _id(e):
+    return e[0] + ("-learned" if e[1] else "")
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=_id)
+@pytest.mark.parametrize("ds,thr", GOLD_PAIRS)
+def test_gold_identity(fimi_dir, tmp_path, ds, thr, engine):
+    name, extra = engine
+    if name == "plain" and ds == "connect.txt" and thr == 0.85:
+        pytest.skip("142k single-node launches; covered by the wave and mafia engines")
+    out = str(tmp_path / ("%s-%s.res" % (ds, name)))
+    path, stats = tf.mine(str(fimi_dir / ds), thr, engine=name, out=out, extra=extra, timeout=300)
+    print(ds, thr, name, extra, {k: stats.get(k) for k in ("MFIsNumber", "rows", "waves", "computation_time(s)")},
+          flush=True)
+    got, gold = tf.load_results(path), tf.load_results(gold_path(ds, thr))
+    assert got == gold
+    hg, hr = tf.load_header(path), tf.load_header(gold_path(ds, thr))
+    for k in HEADER_KEYS:
+        assert hg[k] == hr[k], k
+
+
+def _gen(cfg, path, rows=None):
+    cmd = [tf.binary("tfim_gen"), str(cfg), str(path)]
+    if rows:
+        cmd += ["--rows", str(rows)]
+    subprocess.run(cmd, check=True)
+
+
+@pytest.mark.parametrize("cfg,thr,rows", [(1, 0.6, None), (5, 0.12, None), (4, 0.9, 6000), (3, 0.02, 100000)])
+def test_synthetic_configs_match_oracle(tmp_path, cfg, thr, rows):
+    data = tmp_path / ("c%d.txt" % cfg)
+    _gen(cfg, data, rows)
+    ref = tmp_path / "oracle.res"
+    oracle_lib.mine(data, thr, threads=16, out=ref)
+    exp = tf.load_results(str(ref))
+    assert exp, "oracle found no MFI"
+    for name, extra in ENGINES:
+        if name == "plain" and cfg == 5:
+            continue   # 19k launches; the wave engine runs the same enumeration
+        out = tmp_path / ("%s%s.res" % (name, "-l" if extra else ""))
+        path, stats = tf.mine(str(data), thr, engine=name, out=str(out), extra=extra, timeout=600)
+        print(cfg, name, extra, {k: stats.get(k) for k in ("MFIsNumber", "rows", "computation_time(s)")}, flush=True)
+        assert tf.load_results(path) == exp, (name, extra)
+
+
+def test_replicated_config4_scales_exactly(tmp_path):
+    """x k replication scales every support by k; thresholds 0.9*N are
+    non-integers for both sizes, so the MFI set is unchanged."""
+    data = tmp_path / "c4.txt"
+    _gen(4, data, 3001)
+    base = tmp_path / "base.res"
+    oracle_lib.mine(data, 0.9, threads=16, out=base)
+    exp = {s: 16 * f for s, f in tf.load_results(str(base)).items()}
+    path, _ = tf.mine(str(data), 0.9, engine="mafia", out=str(tmp_path / "rep.res"), extra=["--replicate", "16"])
+    got = tf.load_results(path)
+    # single-item MFIs report occurrence counts, which also scale by k
+    assert got == exp
