@@ -50,6 +50,11 @@ enum {
  * the thread_local streams of kernel_bitmap.cu:447-449. */
 int tfim_open(int device, tfim_ctx **out);
 void tfim_close(tfim_ctx *ctx);
+/* A second context on the same device with its own stream and scratch that
+ * shares `parent`'s resident bitmaps (read-only; parent must outlive it).
+ * One per host worker replaces the per-thread streams and buffers of the
+ * reference's worker pool (kernel_bitmap.cu:447-449,853-894). */
+int tfim_open_shared(tfim_ctx *parent, tfim_ctx **out);
 /* Message of the last failing call on this ctx ("" if none). */
 const char *tfim_last_error(const tfim_ctx *ctx);
 /* Device facts for the Results header (replaces displayWorkingEnvironment,
@@ -134,6 +139,31 @@ int tfim_popcount_matrix(tfim_ctx *ctx, const tfim_buf *A, const tfim_buf *B, tf
 int tfim_time_popcount_matrix(tfim_ctx *ctx, const tfim_buf *A, const tfim_buf *B, tfim_buf *C,
                               int64_t b, int64_t d, int64_t pitch_words, int32_t path,
                               int32_t split_k, int32_t warmup, int32_t reps, double *avg_ms);
+
+/* ---- learned search policy (MLP 14 -> 64 -> 64 -> 2) -------------------- */
+/* No counterpart in the reference (its search learns nothing).  The MAFIA
+ * engine's tail ranker (head 1) and head-union-tail lookahead gate (head 0).
+ * Parameters: W1[64][14] b1[64] W2[64][64] b2[64] W3[2][64] b3[2], fp32;
+ * inference may read a bf16 copy.  Loss: mean over masked outputs of binary
+ * cross-entropy of sigmoid(logit) against a 0/1 label; Adam updates. */
+#define TFIM_MLP_INPUTS 14
+#define TFIM_MLP_OUTPUTS 2
+#define TFIM_MLP_PARAMS 5250
+typedef struct tfim_mlp tfim_mlp;
+
+/* He-uniform weights from SplitMix64(seed), zero biases. */
+int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out);
+void tfim_mlp_destroy(tfim_mlp *mlp);
+int tfim_mlp_get_params(tfim_mlp *mlp, float *params);
+int tfim_mlp_set_params(tfim_mlp *mlp, const float *params);
+/* logits[n][2] for x[n][14]; bf16 != 0 uses the bf16 weight copy. */
+int tfim_mlp_forward(tfim_mlp *mlp, const float *x, int32_t n, float *logits, int32_t bf16);
+/* Gradient of the loss w.r.t. all parameters (no update). */
+int tfim_mlp_gradients(tfim_mlp *mlp, const float *x, const float *labels, const float *mask, int32_t n,
+                       float *grads, float *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch. */
+int tfim_mlp_train_step(tfim_mlp *mlp, const float *x, const float *labels, const float *mask, int32_t n,
+                        float lr, float *loss);
 
 /* Waits for all work on the ctx stream. */
 int tfim_sync(tfim_ctx *ctx);

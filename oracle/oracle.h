@@ -128,6 +128,11 @@ void oracle_proto_gen(uint64_t *B, int32_t *pat_items, int32_t *pat_len);
  * level_found[0..7] (index = level-1; level 1 = F1) and returns the total. */
 int64_t oracle_proto_levelwise(const uint64_t *B, int threads, int64_t *level_found);
 
+/* Search-policy MLP (mlp_oracle.c): loss and gradients in double. */
+int oracle_mlp_nparam(void);
+double oracle_mlp_loss_grad(const double *P, const float *X, const float *Y, const float *M, int n,
+                            double *logits, double *grads);
+
 #ifdef __cplusplus
 }
 #endif

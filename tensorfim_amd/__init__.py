@@ -36,6 +36,7 @@ def _p(t):
 _SIGS = {
     "tfim_open": (ctypes.c_int, [ctypes.c_int, _p(ctypes.c_void_p)]),
     "tfim_close": (None, [ctypes.c_void_p]),
+    "tfim_open_shared": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_void_p)]),
     "tfim_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "tfim_device_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, _p(ctypes.c_int32),
                                          _p(ctypes.c_int64), _p(ctypes.c_int32)]),
@@ -67,6 +68,16 @@ _SIGS = {
                                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                   _p(ctypes.c_double)]),
     "tfim_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "tfim_mlp_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _p(ctypes.c_void_p)]),
+    "tfim_mlp_destroy": (None, [ctypes.c_void_p]),
+    "tfim_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tfim_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "tfim_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_int32]),
+    "tfim_mlp_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_int32, ctypes.c_void_p, _p(ctypes.c_float)]),
+    "tfim_mlp_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int32, ctypes.c_float, _p(ctypes.c_float)]),
 }
 
 EXPORTED_SYMBOLS = sorted(_SIGS)
@@ -231,6 +242,54 @@ class Context:
 
     def sync(self):
         self._check(self._lib.tfim_sync(self._h))
+
+
+MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 14, 2, 5250
+
+
+class Mlp:
+    """The search policy's MLP (14 -> 64 -> 64 -> 2) on the device."""
+
+    def __init__(self, ctx, seed=0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx._check(ctx._lib.tfim_mlp_create(ctx._h, ctypes.c_uint64(seed), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self.ctx._lib.tfim_mlp_destroy(self._h)
+            self._h = None
+
+    def params(self):
+        p = np.empty(MLP_PARAMS, dtype=np.float32)
+        self.ctx._check(self.ctx._lib.tfim_mlp_get_params(self._h, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self.ctx._check(self.ctx._lib.tfim_mlp_set_params(self._h, _ptr(p)))
+
+    def forward(self, x, bf16=False):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.empty((x.shape[0], MLP_OUTPUTS), dtype=np.float32)
+        self.ctx._check(self.ctx._lib.tfim_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(out), int(bf16)))
+        return out
+
+    def gradients(self, x, y, m):
+        x, y, m = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, m))
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        self.ctx._check(self.ctx._lib.tfim_mlp_gradients(self._h, _ptr(x), _ptr(y), _ptr(m), x.shape[0], _ptr(g),
+                                                         ctypes.byref(loss)))
+        return g, loss.value
+
+    def train_step(self, x, y, m, lr=1e-3):
+        x, y, m = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, m))
+        loss = ctypes.c_float()
+        self.ctx._check(self.ctx._lib.tfim_mlp_train_step(self._h, _ptr(x), _ptr(y), _ptr(m), x.shape[0],
+                                                          ctypes.c_float(lr), ctypes.byref(loss)))
+        return loss.value
 
 
 # ---------------------------------------------------------------------------

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
-LIB_SOURCES = ["tfim_runtime.hip", "kernels_packed.hip", "kernels_mfma.hip"]
+LIB_SOURCES = ["tfim_runtime.hip", "tfim_mlp.hip", "kernels_packed.hip", "kernels_mfma.hip", "kernels_mlp.hip"]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O3", "-std=c++17", "-march=x86-64-v2", "-Wall", "-Wno-unused-result"]
 

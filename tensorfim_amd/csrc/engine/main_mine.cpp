@@ -2,7 +2,8 @@
 //
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
-//             [--learned] [--model PATH] [--trace PATH] [--verbose]
+//             [--workers N] [--learned] [--gate indep|always] [--model PATH]
+//             [--trace PATH] [--verbose]
 //
 // Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the
 // reference layout and prints run statistics as "key: value" lines.  Exit
@@ -43,6 +44,8 @@ int main(int argc, char **argv) {
         else if (a == "--wave-cap") opt.wave_cap = atoi(next().c_str());
         else if (a == "--device") device = atoi(next().c_str());
         else if (a == "--learned") opt.learned = true;
+        else if (a == "--gate") opt.gate = next();
+        else if (a == "--workers") opt.workers = atoi(next().c_str());
         else if (a == "--model") opt.model_path = next();
         else if (a == "--trace") opt.trace_path = next();
         else if (a == "--verbose") opt.verbose = true;
@@ -102,7 +105,8 @@ int main(int argc, char **argv) {
     info.device = name;
     info.compute_units = cus;
     info.hbm_bytes = hbm;
-    info.engine = opt.engine + (opt.learned ? " (learned ranker/gate)" : "");
+    info.engine = opt.engine + (opt.learned ? " (learned ranker/gate)" : "") +
+                  (opt.engine == "plain" ? " (" + std::to_string(std::max(1, opt.workers)) + " host workers)" : "");
     info.preprocessing_time = db.preprocessing_seconds;
     info.computation_time = c1 - c0;
     char path[4096];

@@ -1,6 +1,8 @@
 #include "policy.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
 #include <numeric>
 
 #include "search.h"
@@ -8,7 +10,7 @@
 namespace tfim {
 
 void PlainPolicy::decide(std::vector<Decision *> &ds, const Database &db) {
-    (void)db;
+    const double log_thr = std::log(std::max(1e-300, db.thr_abs));
     for (Decision *d : ds) {
         d->order.resize(d->tail.size());
         std::iota(d->order.begin(), d->order.end(), 0);
@@ -17,13 +19,272 @@ void PlainPolicy::decide(std::vector<Decision *> &ds, const Database &db) {
             return d->tail[a] < d->tail[b];
         });
         d->lookahead = d->tail.size() >= 2;
+        if (d->lookahead && !always_) {
+            const double ls = std::log(std::max(1, d->sup));
+            double est = ls;
+            for (int32_t t : d->tail_sup) est += std::log(std::max(1, t)) - ls;
+            d->lookahead = est >= log_thr;
+        }
     }
 }
 
+namespace {
+
+constexpr int NF = TFIM_MLP_INPUTS;
+
+// Learned policy.  One MLP, two heads, trained online on the GPU from what the
+// exact engine observes:
+//   head 0 (gate)   P(head+tail is frequent) for a node; label = the exact
+//                   outcome of a lookahead that was taken (1/0);
+//   head 1 (ranker) for each tail item t, the fraction of the child
+//                   (head+t)'s tail that stays frequent when that child is
+//                   counted; the tail is ordered by increasing prediction
+//                   (items that cut the tail down go first, so later children's
+//                   head-union-tail sets exclude them, MAFIA's rationale for
+//                   ordering by support).
+// Until the first training round the plain policy decides.  Every decision is
+// confirmed by exact counts: a skipped lookahead only forgoes a pruning, and
+// any tail order gives the same set enumeration.
+class LearnedPolicy : public Policy {
+   public:
+    LearnedPolicy(tfim_ctx *ctx, const SearchOptions &opt) : ctx_(ctx), opt_(opt), plain_(opt.gate == "always") {
+        if (tfim_mlp_create(ctx, 0x5EEDull, &mlp_) != TFIM_OK) mlp_ = nullptr;
+        if (mlp_ && !opt.model_path.empty()) {
+            std::vector<float> p(TFIM_MLP_PARAMS);
+            FILE *f = fopen(opt.model_path.c_str(), "rb");
+            if (f && fread(p.data(), 4, p.size(), f) == p.size()) {
+                tfim_mlp_set_params(mlp_, p.data());
+                trained_ = true;
+            }
+            if (f) fclose(f);
+        }
+        if (!opt.trace_path.empty()) trace_ = fopen(opt.trace_path.c_str(), "wb");
+    }
+    ~LearnedPolicy() override {
+        if (mlp_) tfim_mlp_destroy(mlp_);
+        if (trace_) fclose(trace_);
+    }
+
+    void decide(std::vector<Decision *> &ds, const Database &db) override {
+        N_ = (double)std::max(1, db.trans_num_reduced);
+        thr_ = std::max(1.0, db.thr_abs);
+        freq_ = &db.freq_dense;
+        plain_.decide(ds, db);                       // fallback order and gate
+        // features for every decision and tail item (inference batch)
+        feats_.clear();
+        std::vector<int64_t> node_row(ds.size(), -1), item_row0(ds.size(), -1);
+        for (size_t q = 0; q < ds.size(); q++) {
+            Decision *d = ds[q];
+            if (d->head.empty() || d->tail.empty()) continue;
+            node_features(*d);
+            node_row[q] = (int64_t)(feats_.size() / NF) - 1;
+            if (d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail) {
+                // rank of each item in the plain (increasing support) order
+                rank_.assign(d->tail.size(), 0);
+                for (size_t j = 0; j < d->order.size(); j++) rank_[d->order[j]] = (int32_t)j;
+                item_row0[q] = node_row[q] + 1;
+                for (size_t i = 0; i < d->tail.size(); i++) item_features(*d, i);
+            }
+        }
+        const int64_t n = (int64_t)(feats_.size() / NF);
+        if (n == 0) return;
+        if (trained_ && mlp_) {
+            logits_.resize((size_t)n * 2);
+            if (tfim_mlp_forward(mlp_, feats_.data(), (int32_t)n, logits_.data(), 1) != TFIM_OK) trained_ = false;
+        }
+        for (size_t q = 0; q < ds.size(); q++) {
+            Decision *d = ds[q];
+            if (node_row[q] < 0) continue;
+            if (trained_) {
+                const float pg = sigmoid(logits_[node_row[q] * 2 + 0]);
+                const bool explore = rand01() < 0.1;
+                d->lookahead = d->tail.size() >= 2 && (pg >= 0.25f || explore);
+                if (item_row0[q] >= 0) {
+                    std::vector<float> sc(d->tail.size());
+                    for (size_t i = 0; i < d->tail.size(); i++) sc[i] = logits_[(item_row0[q] + i) * 2 + 1];
+                    std::stable_sort(d->order.begin(), d->order.end(), [&](int32_t a, int32_t b) {
+                        if (sc[a] != sc[b]) return sc[a] < sc[b];
+                        return d->tail_sup[a] < d->tail_sup[b];
+                    });
+                }
+            }
+            // records: the gate record is labelled only if the lookahead runs
+            d->record = d->lookahead ? add_record(&feats_[node_row[q] * NF], 0) : -1;
+            if (item_row0[q] >= 0) {
+                d->item_records.resize(d->tail.size());
+                for (size_t i = 0; i < d->tail.size(); i++)
+                    d->item_records[i] = add_record(&feats_[(item_row0[q] + i) * NF], 1);
+            }
+        }
+    }
+
+    void observe(int32_t record, int outcome) override {
+        if (record < 0 || outcome < 0) return;
+        label(record, 0, (float)outcome);
+    }
+    void observe_value(int32_t record, float value) override {
+        if (record < 0) return;
+        label(record, 1, value);
+    }
+
+    void finish(SearchStats &st) override {
+        st.mlp_train_steps = steps_;
+        st.learn_seconds += 0;
+        if (mlp_ && !opt_.model_path.empty() && opt_.learned && trained_) {
+            // keep what was learned for the next run on the same data
+            std::vector<float> p(TFIM_MLP_PARAMS);
+            if (tfim_mlp_get_params(mlp_, p.data()) == TFIM_OK) {
+                FILE *f = fopen(opt_.model_path.c_str(), "wb");
+                if (f) {
+                    fwrite(p.data(), 4, p.size(), f);
+                    fclose(f);
+                }
+            }
+        }
+    }
+
+   private:
+    static float sigmoid(float x) { return 1.0f / (1.0f + std::exp(-x)); }
+    float rand01() {
+        uint64_t z = (rng_ += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (float)((z >> 40) * (1.0 / 16777216.0));
+    }
+
+    struct NodeAgg {
+        double s, sum_log_r, min_r, max_r, mean_r, frac_hi, mean_freq, log_min_s;
+    };
+    NodeAgg agg(const Decision &d) const {
+        NodeAgg a{};
+        a.s = std::max(1, d.sup);
+        a.min_r = 1e9;
+        a.max_r = 0;
+        double min_s = 1e18;
+        for (size_t i = 0; i < d.tail.size(); i++) {
+            const double r = std::max(1, d.tail_sup[i]) / a.s;
+            a.sum_log_r += std::log(r);
+            a.min_r = std::min(a.min_r, r);
+            a.max_r = std::max(a.max_r, r);
+            a.mean_r += r;
+            a.frac_hi += r >= 0.98;
+            a.mean_freq += (double)(*freq_)[d.tail[i]] / N_;
+            min_s = std::min(min_s, (double)d.tail_sup[i]);
+        }
+        const double k = (double)d.tail.size();
+        a.mean_r /= k;
+        a.frac_hi /= k;
+        a.mean_freq /= k;
+        a.log_min_s = std::log(std::max(1.0, min_s) / thr_);
+        return a;
+    }
+    void node_features(const Decision &d) {
+        const NodeAgg a = agg(d);
+        const double margin = std::log(a.s / thr_);
+        const float f[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
+                             (float)(d.tail.size() / 16.0), (float)margin, (float)a.sum_log_r,
+                             (float)(margin + a.sum_log_r), (float)a.min_r, (float)a.mean_r, (float)a.max_r,
+                             (float)a.frac_hi, (float)(d.depth / 16.0), (float)a.mean_freq, 0.0f};
+        feats_.insert(feats_.end(), f, f + NF);
+        cur_ = a;
+    }
+    void item_features(const Decision &d, size_t i) {
+        const NodeAgg &a = cur_;
+        const double margin = std::log(a.s / thr_);
+        const double r = std::max(1, d.tail_sup[i]) / a.s;
+        const int rank = rank_[i];
+        const float f[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
+                             (float)(d.tail.size() / 16.0), (float)margin, (float)std::log(r),
+                             (float)std::log(std::max(1, d.tail_sup[i]) / thr_),
+                             (float)(rank / std::max<double>(1, d.tail.size() - 1)),
+                             (float)((double)(*freq_)[d.tail[i]] / N_), (float)(r - a.mean_r),
+                             (float)(margin + a.sum_log_r), (float)(d.depth / 16.0),
+                             (float)(margin + a.sum_log_r - std::log(r)), 1.0f};
+        feats_.insert(feats_.end(), f, f + NF);
+    }
+
+    // Records live in a ring of kRing slots; a label that arrives after its
+    // slot was reused is dropped.
+    int32_t add_record(const float *f, int head) {
+        (void)head;
+        const int64_t id = next_id_++;
+        const size_t slot = (size_t)(id % kRing);
+        std::copy_n(f, NF, &rec_x_[slot * NF]);
+        rec_id_[slot] = id;
+        rec_y_[slot * 2] = rec_y_[slot * 2 + 1] = 0.0f;
+        rec_m_[slot * 2] = rec_m_[slot * 2 + 1] = 0.0f;
+        return (int32_t)(id & 0x7FFFFFFF);
+    }
+    void label(int32_t id32, int head, float v) {
+        const size_t slot = (size_t)(id32 % kRing);
+        if ((rec_id_[slot] & 0x7FFFFFFF) != id32) return;
+        rec_y_[slot * 2 + head] = v;
+        rec_m_[slot * 2 + head] = 1.0f;
+        if (trace_) {
+            fwrite(&rec_x_[slot * NF], 4, NF, trace_);
+            const float yh[2] = {(float)head, v};
+            fwrite(yh, 4, 2, trace_);
+        }
+        // the batch keeps the most recent labelled records
+        const size_t b = (size_t)(n_labelled_ % kBatch);
+        if (bx_.empty()) {
+            bx_.assign((size_t)kBatch * NF, 0.0f);
+            by_.assign((size_t)kBatch * 2, 0.0f);
+            bm_.assign((size_t)kBatch * 2, 0.0f);
+        }
+        std::copy_n(&rec_x_[slot * NF], NF, &bx_[b * NF]);
+        by_[b * 2] = head == 0 ? v : 0.0f;
+        by_[b * 2 + 1] = head == 1 ? v : 0.0f;
+        bm_[b * 2] = head == 0 ? 1.0f : 0.0f;
+        bm_[b * 2 + 1] = head == 1 ? 1.0f : 0.0f;
+        n_labelled_++;
+        if (n_labelled_ - last_train_ >= kTrainEvery) train();
+    }
+    void train() {
+        if (!mlp_) return;
+        last_train_ = n_labelled_;
+        const int64_t take = std::min<int64_t>(n_labelled_, kBatch);
+        float loss = 0;
+        for (int e = 0; e < kEpochs; e++) {
+            if (tfim_mlp_train_step(mlp_, bx_.data(), by_.data(), bm_.data(), (int32_t)take, 2e-3f, &loss) != TFIM_OK)
+                return;
+            steps_++;
+        }
+        if (opt_.verbose) fprintf(stderr, "mlp train: %lld records, loss %.4f\n", (long long)take, loss);
+        trained_ = true;
+    }
+
+    static constexpr int64_t kTrainEvery = 2048;
+    static constexpr int64_t kBatch = 8192;
+    static constexpr int64_t kRing = 1 << 20;
+    static constexpr int kEpochs = 8;
+    static constexpr size_t kMaxRankTail = 128;   // longer tails keep the plain order
+
+    tfim_ctx *ctx_;
+    const SearchOptions &opt_;
+    tfim_mlp *mlp_ = nullptr;
+    FILE *trace_ = nullptr;
+    PlainPolicy plain_;
+    bool trained_ = false;
+    double N_ = 1, thr_ = 1;
+    const std::vector<int32_t> *freq_ = nullptr;
+    NodeAgg cur_{};
+    std::vector<int32_t> rank_;
+    std::vector<float> feats_, logits_;
+    std::vector<float> rec_x_ = std::vector<float>((size_t)kRing * NF);
+    std::vector<float> rec_y_ = std::vector<float>((size_t)kRing * 2), rec_m_ = std::vector<float>((size_t)kRing * 2);
+    std::vector<int64_t> rec_id_ = std::vector<int64_t>((size_t)kRing, -1);
+    std::vector<float> bx_, by_, bm_;
+    int64_t next_id_ = 0, n_labelled_ = 0, last_train_ = 0, steps_ = 0;
+    uint64_t rng_ = 0xC0FFEEull;
+};
+
+}  // namespace
+
 std::unique_ptr<Policy> make_policy(tfim_ctx *ctx, const SearchOptions &opt) {
-    (void)ctx;
-    (void)opt;
-    return std::unique_ptr<Policy>(new PlainPolicy());
+    if (opt.learned) return std::unique_ptr<Policy>(new LearnedPolicy(ctx, opt));
+    return std::unique_ptr<Policy>(new PlainPolicy(opt.gate == "always"));
 }
 
 }  // namespace tfim

@@ -26,7 +26,8 @@ struct Decision {
     int32_t depth = 0;
     std::vector<int32_t> order;     // out: permutation of tail indices
     bool lookahead = false;         // out: count head+tail before expanding
-    int32_t record = -1;            // out: policy record id for observe()
+    int32_t record = -1;            // out: gate record id for observe()
+    std::vector<int32_t> item_records;  // out: ranker record id per tail index
 };
 
 class Policy {
@@ -35,14 +36,23 @@ class Policy {
     virtual void decide(std::vector<Decision *> &ds, const Database &db) = 0;
     // outcome: 1 = HUT frequent, 0 = not frequent, -1 = lookahead not taken
     virtual void observe(int32_t record, int outcome) { (void)record; (void)outcome; }
+    // ranker label of a child: fraction of its tail that stayed frequent
+    virtual void observe_value(int32_t record, float value) { (void)record; (void)value; }
     virtual void finish(SearchStats &st) { (void)st; }
 };
 
-// MAFIA's plain policy: tail by increasing support (ties: ascending id),
-// lookahead whenever the tail holds two or more items.
+// MAFIA's plain policy: tail by increasing support (ties: ascending id).
+// Lookahead rule ("indep", default): take it when the support head+tail would
+// have if its items were independent given the head, sup(H) * prod_t
+// sup(H+t)/sup(H), reaches the threshold; "always": whenever the tail holds
+// two or more items.
 class PlainPolicy : public Policy {
    public:
+    explicit PlainPolicy(bool always = false) : always_(always) {}
     void decide(std::vector<Decision *> &ds, const Database &db) override;
+
+   private:
+    bool always_;
 };
 
 std::unique_ptr<Policy> make_policy(tfim_ctx *ctx, const SearchOptions &opt);

@@ -2,6 +2,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -107,27 +109,73 @@ std::vector<RefNode> ref_roots(const Database &db, bool only_item0) {
     return st;
 }
 
-int run_plain(tfim_ctx *ctx, const Database &db, std::vector<Mfi> &out, SearchStats &st, std::string &err) {
+// Worker pool over one LIFO stack (kernel_bitmap.cu:853-894): each worker has
+// its own context (stream + scratch) sharing the resident bitmaps, pops a
+// node, counts it with one launch and pushes its children.
+int run_plain(tfim_ctx *ctx, const Database &db, int workers, std::vector<Mfi> &out, SearchStats &st,
+              std::string &err) {
     std::vector<RefNode> stack = ref_roots(db, false);
-    std::vector<int32_t> res((size_t)db.n_surv + 1, 0);
-    while (!stack.empty()) {
-        RefNode nd = std::move(stack.back());
-        stack.pop_back();
-        const double t0 = now();
-        const int rc = db.n_surv > 0 ? tfim_count_node(ctx, nd.items.data(), (int32_t)nd.items.size(), res.data())
-                                     : TFIM_OK;
-        st.count_seconds += now() - t0;
-        if (rc != TFIM_OK) {
-            err = std::string("tfim_count_node: ") + tfim_last_error(ctx);
-            return rc;
+    if (db.n_surv == 0) return TFIM_OK;
+    workers = std::max(1, workers);
+    std::vector<tfim_ctx *> ctxs(workers, nullptr);
+    ctxs[0] = ctx;
+    for (int w = 1; w < workers; w++)
+        if (tfim_open_shared(ctx, &ctxs[w]) != TFIM_OK) {
+            err = std::string("tfim_open_shared: ") + tfim_last_error(ctx);
+            for (int v = 1; v < w; v++) tfim_close(ctxs[v]);
+            return TFIM_E_HIP;
         }
-        st.nodes++;
-        st.waves++;
-        st.ref_rows++;
-        st.counted_bitops += (double)db.n_surv * (double)db.n_bits();
-        ref_expand(db, nd, res.data(), stack, out);
+    std::mutex mu;
+    std::condition_variable cv;
+    int active = 0, rc_all = TFIM_OK;
+    std::vector<SearchStats> wst(workers);
+    auto worker = [&](int w) {
+        std::vector<int32_t> res((size_t)db.n_surv + 1, 0);
+        std::vector<RefNode> kids;
+        std::vector<Mfi> found;
+        for (;;) {
+            RefNode nd;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !stack.empty() || active == 0 || rc_all != TFIM_OK; });
+                if (stack.empty() || rc_all != TFIM_OK) break;
+                nd = std::move(stack.back());
+                stack.pop_back();
+                active++;
+            }
+            const double t0 = now();
+            const int rc = tfim_count_node(ctxs[w], nd.items.data(), (int32_t)nd.items.size(), res.data());
+            wst[w].count_seconds += now() - t0;
+            kids.clear();
+            found.clear();
+            if (rc == TFIM_OK) ref_expand(db, nd, res.data(), kids, found);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (rc != TFIM_OK && rc_all == TFIM_OK) {
+                    rc_all = rc;
+                    err = std::string("tfim_count_node: ") + tfim_last_error(ctxs[w]);
+                }
+                for (auto &k : kids) stack.push_back(std::move(k));
+                for (auto &m : found) out.push_back(std::move(m));
+                active--;
+            }
+            cv.notify_all();
+            wst[w].nodes++;
+        }
+        cv.notify_all();
+    };
+    std::vector<std::thread> th;
+    for (int w = 0; w < workers; w++) th.emplace_back(worker, w);
+    for (auto &t : th) t.join();
+    for (int w = 1; w < workers; w++) tfim_close(ctxs[w]);
+    for (auto &x : wst) {
+        st.nodes += x.nodes;
+        st.count_seconds += x.count_seconds;
     }
-    return TFIM_OK;
+    st.waves = st.nodes;
+    st.ref_rows = st.nodes;
+    st.counted_bitops = (double)st.nodes * (double)db.n_surv * (double)db.n_bits();
+    return rc_all;
 }
 
 int32_t auto_wave_cap(const Database &db) {
@@ -184,6 +232,7 @@ struct MNode {
     int32_t progress = 0;            // REF
     int32_t depth = 0;
     int32_t decision = -1;           // HUT: policy record index for its outcome
+    int32_t item_record = -1;        // EXPAND: ranker record of the item that made it
 };
 
 // Found MFIs, sliced by item: bits_[i] has bit m set when MFI m contains
@@ -341,6 +390,7 @@ class Mafia {
             c.head.push_back(d.tail[d.order[q]]);
             c.sup = d.tail_sup[d.order[q]];
             c.depth = d.depth + 1;
+            if (!d.item_records.empty()) c.item_record = d.item_records[d.order[q]];
             for (size_t r = q + 1; r < k; r++) c.tail.push_back(d.tail[d.order[r]]);
             if (c.tail.empty()) {
                 // no candidates left: the child is a leaf
@@ -363,6 +413,9 @@ class Mafia {
                 d.head = n.head;
                 d.sup = n.sup;
                 d.depth = n.depth;
+                int kept = 0;
+                for (int j = 0; j < nc; j++) kept += frequent(db_, res[j]);
+                if (n.item_record >= 0) pol_.observe_value(n.item_record, nc > 0 ? (float)kept / nc : 0.0f);
                 for (int j = 0; j < nc; j++) {
                     const int32_t t = n.tail[j], s = res[j];
                     if (!frequent(db_, s)) continue;
@@ -492,7 +545,7 @@ class Mafia {
 
 int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std::vector<Mfi> &out,
                SearchStats &st, std::string &err) {
-    if (opt.engine == "plain") return run_plain(ctx, db, out, st, err);
+    if (opt.engine == "plain") return run_plain(ctx, db, opt.workers, out, st, err);
     if (opt.engine == "wave") {
         const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
         const int rc = run_wave(ctx, db, cap, out, st, err);

@@ -29,6 +29,7 @@ struct SearchOptions {
     int32_t wave_cap = 0;         // rows per wave; 0 = automatic
     int32_t workers = 1;          // plain engine: host workers (one stream each)
     bool learned = false;         // mafia: use the MLP ranker/gate
+    std::string gate = "indep";   // mafia plain lookahead rule: indep | always
     std::string model_path;       // mafia: load MLP weights (else trained online)
     std::string trace_path;       // mafia: write (features, outcome) records
     bool verbose = false;

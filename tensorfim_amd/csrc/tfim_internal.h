@@ -70,4 +70,20 @@ void launch_replicate_rows(const uint64_t *base, int64_t base_pitch, int64_t bas
 void launch_row_popcount(const uint64_t *bm, int64_t pitch_words, int32_t rows, int32_t *out,
                          hipStream_t s);
 
+// MLP kernels (kernels_mlp.hip)
+int mlp_nparam();
+void launch_mlp_fwd_bwd(const float *P, const float *X, const float *Y, const float *M, int n, float inv_count,
+                        float *H1, float *H2, float *D1, float *D2, float *DY, float *lossv, hipStream_t s);
+void launch_mlp_grad(const float *X, const float *H1, const float *H2, const float *D1, const float *D2,
+                     const float *DY, int n, float *G, hipStream_t s);
+void launch_mlp_adam(float *P, const float *G, float *Mo, float *Vo, uint16_t *Pb, float lr, float b1, float b2,
+                     float eps, float c1, float c2, hipStream_t s);
+void launch_mlp_to_bf16(const float *P, uint16_t *Pb, hipStream_t s);
+void launch_mlp_infer(const float *P, const uint16_t *Pb, const float *X, int n, float *out, hipStream_t s);
+
+// Context accessors for translation units other than the runtime.
+hipStream_t ctx_stream(tfim_ctx *c);
+int ctx_device(tfim_ctx *c);
+int ctx_fail(tfim_ctx *c, int code, const char *msg);
+
 }  // namespace tfim

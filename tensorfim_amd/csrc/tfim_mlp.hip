@@ -1,0 +1,190 @@
+// Host side of the MLP entry points (include/tfim.h): device buffers, Adam
+// state, He-uniform initialisation.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "tfim_internal.h"
+
+using namespace tfim;
+
+struct tfim_mlp {
+    tfim_ctx *ctx = nullptr;
+    float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
+    uint16_t *Pb = nullptr;
+    int64_t step = 0;
+    // batch scratch (grow-only): X, Y, M, H1, H2, D1, D2, DY, loss, out
+    float *buf = nullptr;
+    int64_t cap = 0;
+};
+
+namespace {
+
+constexpr int IN = 14, H = 64, OUT = 2;
+
+uint64_t sm64(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct Scratch {
+    float *X, *Y, *M, *H1, *H2, *D1, *D2, *DY, *L, *O;
+};
+
+int ensure(tfim_mlp *m, int64_t n, Scratch &sc) {
+    const int64_t per = IN + OUT + OUT + 4 * H + OUT + 1 + OUT;
+    if (n > m->cap) {
+        hipFree(m->buf);
+        m->buf = nullptr;
+        m->cap = 0;
+        const int64_t c = std::max<int64_t>(n, 4096);
+        if (hipMalloc(&m->buf, (size_t)(c * per) * sizeof(float)) != hipSuccess)
+            return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp scratch allocation failed");
+        m->cap = c;
+    }
+    float *p = m->buf;
+    const int64_t c = m->cap;
+    sc.X = p; p += c * IN;
+    sc.Y = p; p += c * OUT;
+    sc.M = p; p += c * OUT;
+    sc.H1 = p; p += c * H;
+    sc.H2 = p; p += c * H;
+    sc.D1 = p; p += c * H;
+    sc.D2 = p; p += c * H;
+    sc.DY = p; p += c * OUT;
+    sc.L = p; p += c;
+    sc.O = p;
+    return TFIM_OK;
+}
+
+#define MCHK(m, expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return ctx_fail((m)->ctx, TFIM_E_HIP, hipGetErrorString(e_)); \
+    } while (0)
+
+// forward+backward into scratch; returns the loss and leaves G filled
+int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float *loss, Scratch &sc) {
+    if (int r = ensure(m, n, sc)) return r;
+    hipStream_t s = ctx_stream(m->ctx);
+    double cnt = 0;
+    for (int64_t i = 0; i < (int64_t)n * OUT; i++) cnt += mask[i];
+    const float inv = (float)(1.0 / std::max(1.0, cnt));
+    MCHK(m, hipMemcpyAsync(sc.X, x, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
+    MCHK(m, hipMemcpyAsync(sc.Y, y, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
+    MCHK(m, hipMemcpyAsync(sc.M, mask, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
+    launch_mlp_fwd_bwd(m->P, sc.X, sc.Y, sc.M, n, inv, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, sc.L, s);
+    launch_mlp_grad(sc.X, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, n, m->G, s);
+    MCHK(m, hipGetLastError());
+    if (loss) {
+        std::vector<float> lv((size_t)n);
+        MCHK(m, hipMemcpyAsync(lv.data(), sc.L, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+        MCHK(m, hipStreamSynchronize(s));
+        double t = 0;
+        for (float v : lv) t += v;
+        *loss = (float)t;
+    }
+    return TFIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out) {
+    if (!ctx || !out) return TFIM_E_INVALID;
+    tfim_mlp *m = new tfim_mlp();
+    m->ctx = ctx;
+    const int np = mlp_nparam();
+    if (hipSetDevice(ctx_device(ctx)) != hipSuccess || hipMalloc(&m->P, np * 4) != hipSuccess ||
+        hipMalloc(&m->G, np * 4) != hipSuccess || hipMalloc(&m->Mo, np * 4) != hipSuccess ||
+        hipMalloc(&m->Vo, np * 4) != hipSuccess || hipMalloc(&m->Pb, np * 2) != hipSuccess) {
+        tfim_mlp_destroy(m);
+        return ctx_fail(ctx, TFIM_E_NOMEM, "mlp allocation failed");
+    }
+    std::vector<float> p((size_t)np, 0.0f);
+    uint64_t s = seed;
+    auto init = [&](int off, int rows, int cols) {
+        const double a = std::sqrt(6.0 / cols);
+        for (int i = 0; i < rows * cols; i++)
+            p[off + i] = (float)((2.0 * ((double)(sm64(s) >> 11) / 9007199254740992.0) - 1.0) * a);
+    };
+    init(0, H, IN);                       // W1
+    init(H * IN + H, H, H);               // W2
+    init(H * IN + H + H * H + H, OUT, H); // W3
+    *out = m;
+    return tfim_mlp_set_params(m, p.data());
+}
+
+void tfim_mlp_destroy(tfim_mlp *m) {
+    if (!m) return;
+    hipFree(m->P); hipFree(m->G); hipFree(m->Mo); hipFree(m->Vo); hipFree(m->Pb); hipFree(m->buf);
+    delete m;
+}
+
+int tfim_mlp_get_params(tfim_mlp *m, float *params) {
+    if (!m || !params) return TFIM_E_INVALID;
+    hipStream_t s = ctx_stream(m->ctx);
+    MCHK(m, hipMemcpyAsync(params, m->P, (size_t)mlp_nparam() * 4, hipMemcpyDeviceToHost, s));
+    MCHK(m, hipStreamSynchronize(s));
+    return TFIM_OK;
+}
+
+int tfim_mlp_set_params(tfim_mlp *m, const float *params) {
+    if (!m || !params) return TFIM_E_INVALID;
+    hipStream_t s = ctx_stream(m->ctx);
+    const size_t b = (size_t)mlp_nparam() * 4;
+    MCHK(m, hipMemcpyAsync(m->P, params, b, hipMemcpyHostToDevice, s));
+    MCHK(m, hipMemsetAsync(m->Mo, 0, b, s));
+    MCHK(m, hipMemsetAsync(m->Vo, 0, b, s));
+    launch_mlp_to_bf16(m->P, m->Pb, s);
+    MCHK(m, hipGetLastError());
+    MCHK(m, hipStreamSynchronize(s));
+    m->step = 0;
+    return TFIM_OK;
+}
+
+int tfim_mlp_forward(tfim_mlp *m, const float *x, int32_t n, float *logits, int32_t bf16) {
+    if (!m || !x || !logits || n < 0) return TFIM_E_INVALID;
+    if (n == 0) return TFIM_OK;
+    Scratch sc;
+    if (int r = ensure(m, n, sc)) return r;
+    hipStream_t s = ctx_stream(m->ctx);
+    MCHK(m, hipMemcpyAsync(sc.X, x, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
+    launch_mlp_infer(m->P, bf16 ? m->Pb : nullptr, sc.X, n, sc.O, s);
+    MCHK(m, hipGetLastError());
+    MCHK(m, hipMemcpyAsync(logits, sc.O, (size_t)n * OUT * 4, hipMemcpyDeviceToHost, s));
+    MCHK(m, hipStreamSynchronize(s));
+    return TFIM_OK;
+}
+
+int tfim_mlp_gradients(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float *grads,
+                       float *loss) {
+    if (!m || !x || !y || !mask || !grads || n < 1) return TFIM_E_INVALID;
+    Scratch sc;
+    if (int r = fwd_bwd(m, x, y, mask, n, loss, sc)) return r;
+    hipStream_t s = ctx_stream(m->ctx);
+    MCHK(m, hipMemcpyAsync(grads, m->G, (size_t)mlp_nparam() * 4, hipMemcpyDeviceToHost, s));
+    MCHK(m, hipStreamSynchronize(s));
+    return TFIM_OK;
+}
+
+int tfim_mlp_train_step(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float lr,
+                        float *loss) {
+    if (!m || !x || !y || !mask || n < 1) return TFIM_E_INVALID;
+    Scratch sc;
+    if (int r = fwd_bwd(m, x, y, mask, n, loss, sc)) return r;
+    m->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = (float)(1.0 - std::pow(0.9, (double)m->step));
+    const float c2 = (float)(1.0 - std::pow(0.999, (double)m->step));
+    hipStream_t s = ctx_stream(m->ctx);
+    launch_mlp_adam(m->P, m->G, m->Mo, m->Vo, m->Pb, lr, b1, b2, 1e-8f, c1, c2, s);
+    MCHK(m, hipGetLastError());
+    MCHK(m, hipStreamSynchronize(s));
+    return TFIM_OK;
+}
+
+}  // extern "C"

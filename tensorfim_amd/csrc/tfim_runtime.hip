@@ -31,6 +31,7 @@ struct tfim_ctx {
 
     // resident bitmaps
     uint64_t *d_bm = nullptr;
+    bool owns_bm = true;   // false for contexts opened with tfim_open_shared
     int64_t pitch = 0;     // u64 words per row (multiple of 32)
     int64_t n_bits = 0;
     int32_t n_items = 0, n_cols = 0;
@@ -96,6 +97,12 @@ int set_device(tfim_ctx *c) {
 
 }  // namespace
 
+namespace tfim {
+hipStream_t ctx_stream(tfim_ctx *c) { return c->stream; }
+int ctx_device(tfim_ctx *c) { return c->device; }
+int ctx_fail(tfim_ctx *c, int code, const char *msg) { return fail(c, code, "%s", msg); }
+}  // namespace tfim
+
 extern "C" {
 
 int tfim_open(int device, tfim_ctx **out) {
@@ -123,7 +130,7 @@ void tfim_close(tfim_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    hipFree(c->d_bm);
+    if (c->owns_bm) hipFree(c->d_bm);
     hipFree(c->d_node_items); hipFree(c->d_node_out);
     hipHostFree(c->h_node_items); hipHostFree(c->h_node_out);
     hipFree(c->d_desc); hipHostFree(c->h_desc);
@@ -132,6 +139,21 @@ void tfim_close(tfim_ctx *c) {
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+int tfim_open_shared(tfim_ctx *parent, tfim_ctx **out) {
+    if (!parent || !out) return TFIM_E_INVALID;
+    if (!parent->d_bm) return fail(parent, TFIM_E_STATE, "open_shared: parent has no bitmaps loaded");
+    tfim_ctx *c = nullptr;
+    if (int r = tfim_open(parent->device, &c)) return r;
+    c->d_bm = parent->d_bm;
+    c->owns_bm = false;
+    c->pitch = parent->pitch;
+    c->n_bits = parent->n_bits;
+    c->n_items = parent->n_items;
+    c->n_cols = parent->n_cols;
+    *out = c;
+    return TFIM_OK;
 }
 
 const char *tfim_last_error(const tfim_ctx *c) { return c ? c->err.c_str() : "null context"; }
@@ -153,6 +175,7 @@ static int load_common(tfim_ctx *c, int32_t n_items, int64_t n_bits, int32_t n_c
                     (long long)n_bits, n_cols);
     if (n_bits > (int64_t)INT32_MAX)
         return fail(c, TFIM_E_RANGE, "load_bitmaps: %lld transactions exceed int32 supports", (long long)n_bits);
+    if (!c->owns_bm) return fail(c, TFIM_E_STATE, "load_bitmaps: context shares its parent's bitmaps");
     if (int r = set_device(c)) return r;
     hipFree(c->d_bm);
     c->d_bm = nullptr;

@@ -34,6 +34,10 @@ def lib():
         L.oracle_bernoulli_row.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64,
                                            ctypes.c_int64, ctypes.c_uint32, vp]
         L.oracle_bernoulli_row.restype = None
+        L.oracle_proto_gen.argtypes = [vp, vp, vp]
+        L.oracle_proto_gen.restype = None
+        L.oracle_mlp_loss_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp]
+        L.oracle_mlp_loss_grad.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -82,3 +86,23 @@ def mine(path, support, threads=8, replicate=0, out=None, max_nodes=0, timeout=6
             k, v = ln.split(":", 1)
             stats[k.strip()] = v.strip()
     return stats
+
+
+def mlp_loss_grad(params, X, Y, M, want_grad=True):
+    """Loss, logits[n][2] and gradients (float64) of the policy MLP."""
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    X, Y, M = (np.ascontiguousarray(a, dtype=np.float32) for a in (X, Y, M))
+    logits = np.empty((X.shape[0], 2), dtype=np.float64)
+    g = np.empty(P.shape[0], dtype=np.float64) if want_grad else None
+    loss = lib().oracle_mlp_loss_grad(_p(P), _p(X), _p(Y), _p(M), X.shape[0], _p(logits),
+                                      _p(g) if want_grad else None)
+    return loss, logits, g
+
+
+def proto_bitmaps():
+    """batch_proto.cu's synthetic bitmaps: [512][2^23/64] u64 plus patterns."""
+    B = np.empty((512, (1 << 23) // 64), dtype=np.uint64)
+    pat = np.zeros(80 * 6, dtype=np.int32)
+    plen = np.zeros(80, dtype=np.int32)
+    lib().oracle_proto_gen(_p(B), _p(pat), _p(plen))
+    return B, [pat[6 * i:6 * i + plen[i]].tolist() for i in range(80)]
