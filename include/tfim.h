@@ -103,6 +103,13 @@ int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item
                      int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
                      int32_t *out, int32_t path);
 
+/* Tuning knobs of the batched path (ablation and autotuning; the reference's
+ * equivalents are environment variables, kernel_bmma.cu:351-368):
+ *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel
+ *   "tile_cols"    columns per work tile (8..256, default 32)
+ * Unknown keys and out-of-range values return TFIM_E_INVALID. */
+int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
+
 /* Statistics of the batched path since tfim_open (launches, graph replays,
  * rows, summed kernel milliseconds measured with HIP events). */
 int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *graph_replays,

@@ -49,6 +49,7 @@ _SIGS = {
     "tfim_count_node": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "tfim_count_nodes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "tfim_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "tfim_wave_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64),
                                         _p(ctypes.c_int64), _p(ctypes.c_double)]),
     "tfim_buf_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _p(ctypes.c_void_p)]),
@@ -214,6 +215,9 @@ class Context:
                                                _ptr(col_ids) if len(col_ids) else _ptr(np.zeros(1, np.int32)),
                                                _ptr(col_off), _ptr(out), _PATHS[path]))
         return [out[col_off[i]:col_off[i + 1]] for i in range(n)]
+
+    def config(self, key, value):
+        self._check(self._lib.tfim_config(self._h, key.encode(), int(value)))
 
     def wave_stats(self):
         a, b, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()

@@ -70,6 +70,9 @@ class LearnedPolicy : public Policy {
         thr_ = std::max(1.0, db.thr_abs);
         freq_ = &db.freq_dense;
         plain_.decide(ds, db);                       // fallback order and gate
+        if (!trained_)
+            for (Decision *d : ds)   // warm-up: label both outcomes (bounded tails only)
+                if (d->tail.size() >= 2 && d->tail.size() <= 64) d->lookahead = true;
         // features for every decision and tail item (inference batch)
         feats_.clear();
         std::vector<int64_t> node_row(ds.size(), -1), item_row0(ds.size(), -1);
@@ -255,10 +258,10 @@ class LearnedPolicy : public Policy {
         trained_ = true;
     }
 
-    static constexpr int64_t kTrainEvery = 2048;
+    static constexpr int64_t kTrainEvery = 8192;
     static constexpr int64_t kBatch = 8192;
     static constexpr int64_t kRing = 1 << 20;
-    static constexpr int kEpochs = 8;
+    static constexpr int kEpochs = 4;
     static constexpr size_t kMaxRankTail = 128;   // longer tails keep the plain order
 
     tfim_ctx *ctx_;

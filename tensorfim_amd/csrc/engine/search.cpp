@@ -178,14 +178,23 @@ int run_plain(tfim_ctx *ctx, const Database &db, int workers, std::vector<Mfi> &
     return rc_all;
 }
 
+// Waves whose rows touch few distinct items fit the LDS-staged kernel when
+// their total output stays within its accumulator budget; such waves are
+// capped by outputs.  Otherwise a wave's column traffic is kept near 4 GiB of
+// bitmap words so it amortises the launch while the DFS frontier stays small.
+constexpr int64_t kLdsOutBudget = 8192;   // = the library's LDS accumulator limit
+constexpr int32_t kLdsItemBudget = 90;    // survivors (+ item 0) that fit its slots
+
+bool lds_shape(const Database &db) { return db.d_count <= kLdsItemBudget; }
+
 int32_t auto_wave_cap(const Database &db) {
-    // Keep one wave's column traffic near 4 GiB of bitmap words so a wave is
-    // long enough to amortise the launch and short enough to keep the DFS
-    // frontier small.
+    if (lds_shape(db)) return 2048;
     const double row_bytes = (double)db.n_bits() / 8.0 * std::max(1, db.n_surv);
     const double cap = 4.0 * (1ull << 30) / std::max(1.0, row_bytes);
     return (int32_t)std::min(2048.0, std::max(64.0, cap));
 }
+
+int64_t out_budget(const Database &db) { return lds_shape(db) ? kLdsOutBudget : INT64_MAX; }
 
 int run_wave(tfim_ctx *ctx, const Database &db, int32_t cap, std::vector<Mfi> &out, SearchStats &st,
              std::string &err) {
@@ -194,11 +203,14 @@ int run_wave(tfim_ctx *ctx, const Database &db, int32_t cap, std::vector<Mfi> &o
     std::iota(all.begin(), all.end(), 0);
     WaveBatch wb;
     std::vector<RefNode> wave;
+    const int64_t budget = out_budget(db);
     while (!stack.empty()) {
         wave.clear();
-        while ((int)wave.size() < cap && !stack.empty()) {
+        int64_t outs = 0;
+        while ((int)wave.size() < cap && !stack.empty() && (wave.empty() || outs + db.n_surv <= budget)) {
             wave.push_back(std::move(stack.back()));
             stack.pop_back();
+            outs += db.n_surv;
         }
         wb.clear();
         for (auto &nd : wave) wb.add(nd.items.data(), (int)nd.items.size(), all.data(), db.n_surv);
@@ -311,9 +323,14 @@ class Mafia {
             root.lookahead = false;   // the empty head has no HUT row
             push_children(root);
         }
+        const int64_t budget = out_budget(db_);
         while (!stack_.empty()) {
             wave_.clear();
+            int64_t outs = 0;
             while ((int)wave_.size() < cap_ && !stack_.empty()) {
+                const int64_t o = row_outputs(stack_.back());
+                if (!wave_.empty() && outs + o > budget) break;
+                outs += o;
                 wave_.push_back(std::move(stack_.back()));
                 stack_.pop_back();
             }
@@ -324,6 +341,15 @@ class Mafia {
     }
 
    private:
+    int64_t row_outputs(const MNode &n) const {
+        switch (n.kind) {
+            case MNode::EXPAND: return (int64_t)n.tail.size();
+            case MNode::HUT: return 1 + db_.n_surv - (int64_t)(n.head.size() + n.tail.size());
+            case MNode::LEAF: return db_.n_surv - (int64_t)n.head.size();
+            default: return db_.n_surv;
+        }
+    }
+
     void complement(const std::vector<int32_t> &x, std::vector<int32_t> &cols, bool lead) {
         mark_.assign(db_.n_surv, 0);
         for (int32_t i : x) mark_[i] = 1;

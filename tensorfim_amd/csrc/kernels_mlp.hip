@@ -82,33 +82,45 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float *__restrict__ P
     if (lane == 0) lossv[s] = loss * inv_count;
 }
 
-// One thread per parameter: G[p] = sum over samples, in sample order.
+// Gradient reduction in two deterministic passes: blockIdx.y = slice z of
+// kSlice samples; thread p sums its parameter's terms over the slice in
+// sample order into part[z][p]; k_mlp_grad_reduce adds the slices in order.
+constexpr int kSlice = 256;
 __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ X, const float *__restrict__ H1,
                                                   const float *__restrict__ H2, const float *__restrict__ D1,
                                                   const float *__restrict__ D2, const float *__restrict__ DY,
-                                                  int n, float *__restrict__ G) {
+                                                  int n, float *__restrict__ part) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= MLP_NPARAM) return;
+    const int s0 = blockIdx.y * kSlice, s1 = min(n, s0 + kSlice);
     float g = 0.0f;
     if (p < OFF_B1) {
         const int j = (p - OFF_W1) / MLP_IN, i = (p - OFF_W1) % MLP_IN;
-        for (int s = 0; s < n; s++) g += D1[(int64_t)s * MLP_H + j] * X[(int64_t)s * MLP_IN + i];
+        for (int s = s0; s < s1; s++) g += D1[(int64_t)s * MLP_H + j] * X[(int64_t)s * MLP_IN + i];
     } else if (p < OFF_W2) {
         const int j = p - OFF_B1;
-        for (int s = 0; s < n; s++) g += D1[(int64_t)s * MLP_H + j];
+        for (int s = s0; s < s1; s++) g += D1[(int64_t)s * MLP_H + j];
     } else if (p < OFF_B2) {
         const int j = (p - OFF_W2) / MLP_H, i = (p - OFF_W2) % MLP_H;
-        for (int s = 0; s < n; s++) g += D2[(int64_t)s * MLP_H + j] * H1[(int64_t)s * MLP_H + i];
+        for (int s = s0; s < s1; s++) g += D2[(int64_t)s * MLP_H + j] * H1[(int64_t)s * MLP_H + i];
     } else if (p < OFF_W3) {
         const int j = p - OFF_B2;
-        for (int s = 0; s < n; s++) g += D2[(int64_t)s * MLP_H + j];
+        for (int s = s0; s < s1; s++) g += D2[(int64_t)s * MLP_H + j];
     } else if (p < OFF_B3) {
         const int o = (p - OFF_W3) / MLP_H, j = (p - OFF_W3) % MLP_H;
-        for (int s = 0; s < n; s++) g += DY[(int64_t)s * MLP_OUT + o] * H2[(int64_t)s * MLP_H + j];
+        for (int s = s0; s < s1; s++) g += DY[(int64_t)s * MLP_OUT + o] * H2[(int64_t)s * MLP_H + j];
     } else {
         const int o = p - OFF_B3;
-        for (int s = 0; s < n; s++) g += DY[(int64_t)s * MLP_OUT + o];
+        for (int s = s0; s < s1; s++) g += DY[(int64_t)s * MLP_OUT + o];
     }
+    part[(int64_t)blockIdx.y * MLP_NPARAM + p] = g;
+}
+
+__global__ void k_mlp_grad_reduce(const float *__restrict__ part, int slices, float *__restrict__ G) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= MLP_NPARAM) return;
+    float g = 0.0f;
+    for (int z = 0; z < slices; z++) g += part[(int64_t)z * MLP_NPARAM + p];
     G[p] = g;
 }
 
@@ -179,9 +191,14 @@ void launch_mlp_fwd_bwd(const float *P, const float *X, const float *Y, const fl
                        D2, DY, lossv);
 }
 
+int mlp_grad_slices(int n) { return (n + kSlice - 1) / kSlice; }
+
 void launch_mlp_grad(const float *X, const float *H1, const float *H2, const float *D1, const float *D2,
-                     const float *DY, int n, float *G, hipStream_t s) {
-    hipLaunchKernelGGL(k_mlp_grad, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s, X, H1, H2, D1, D2, DY, n, G);
+                     const float *DY, int n, float *part, float *G, hipStream_t s) {
+    const int slices = mlp_grad_slices(n);
+    hipLaunchKernelGGL(k_mlp_grad, dim3((MLP_NPARAM + 255) / 256, slices), dim3(256), 0, s, X, H1, H2, D1, D2, DY,
+                       n, part);
+    hipLaunchKernelGGL(k_mlp_grad_reduce, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s, part, slices, G);
 }
 
 void launch_mlp_adam(float *P, const float *G, float *Mo, float *Vo, uint16_t *Pb, float lr, float b1, float b2,

@@ -210,6 +210,106 @@ void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged fused wave kernel.  Workgroup g handles word chunks g, g+G, ...;
+// per chunk it copies the chunk of every slot (item row) the wave uses into
+// LDS with 128-bit loads, then its 4 waves take the tiles round-robin: prefix
+// AND from LDS into registers, column counts from LDS, 8-column
+// reduce-scatter, LDS atomic accumulation.  Partial counts leave the
+// workgroup once, as one global atomic per non-zero output.  Global traffic
+// per wave is one pass over the slots' rows whatever the number of rows.
+template <int RW>
+__global__ __launch_bounds__(256) void k_count_wave_lds(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                        WaveLdsDesc d, int64_t n_chunks) {
+    extern __shared__ uint4 lds[];
+    constexpr int C4 = 64 * RW;   // uint4 per slot per chunk
+    int *acc = reinterpret_cast<int *>(lds + (size_t)d.n_slots * C4);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int i = tid; i < d.n_out; i += 256) acc[i] = 0;
+    for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        __syncthreads();
+        const int64_t base = c * C4;
+        for (int i = tid; i < d.n_slots * C4; i += 256) {
+            const int slot = i / C4, off = i % C4;
+            const int64_t g = base + off;
+            lds[i] = g < pitch4 ? bm[(int64_t)d.slot_rows[slot] * pitch4 + g] : make_uint4(0, 0, 0, 0);
+        }
+        __syncthreads();
+        for (int t = wave; t < d.n_tiles; t += 4) {
+            const WaveTile tile = d.tiles[t];
+            const int r = tile.row;
+            const int ib = d.item_off[r], ie = d.item_off[r + 1];
+            uint4 m[RW];
+            {
+                const uint4 *row = lds + d.item_slots[ib] * C4;
+#pragma unroll
+                for (int q = 0; q < RW; q++) m[q] = row[q * 64 + lane];
+            }
+            for (int i = ib + 1; i < ie; i++) {
+                const uint4 *row = lds + d.item_slots[i] * C4;
+#pragma unroll
+                for (int q = 0; q < RW; q++) m[q] = and4(m[q], row[q * 64 + lane]);
+            }
+            const int ob = d.col_off[r];
+            for (int j = tile.col_begin; j < tile.col_end; j += 8) {
+                int v8[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    uint32_t a = 0;
+                    if (j + q < tile.col_end) {
+                        const uint4 *row = lds + d.col_slots[ob + j + q] * C4;
+#pragma unroll
+                        for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
+                    }
+                    v8[q] = (int)a;
+                }
+                const int y = wave_reduce_scatter8(v8);
+                if ((lane & 7) == 0) {
+                    const int q = (lane >> 3) & 7;
+                    if (j + q < tile.col_end && y) atomicAdd(&acc[ob + j + q], y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < d.n_out; i += 256)
+        if (acc[i]) atomicAdd(&d.out[i], acc[i]);
+}
+
+bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
+                           hipStream_t s) {
+    if (d.n_slots < 1 || d.n_slots > kLdsMaxSlots || d.n_out > kLdsMaxOut) return false;
+    const int64_t pitch4 = pitch_words / 2;
+    const size_t acc_bytes = ((size_t)d.n_out * 4 + 15) / 16 * 16;
+    auto bytes = [&](int rw) { return (size_t)d.n_slots * 64 * rw * 16 + acc_bytes; };
+    // Prefer two workgroups per CU (<= 80 KB each); the widest chunk that fits.
+    constexpr size_t kHalf = 80 * 1024, kFull = 160 * 1024;
+    int rw = 4;
+    while (rw > 1 && bytes(rw) > kHalf) rw /= 2;
+    if (bytes(rw) > kHalf && bytes(1) > kFull) return false;
+    const size_t lds_bytes = bytes(rw);
+    const int64_t n_chunks = (pitch4 + 64 * rw - 1) / (64 * rw);
+    const int per_cu = lds_bytes <= kHalf ? 2 : 1;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_chunks, (int64_t)n_cu * per_cu));
+    const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
+    switch (rw) {
+        case 4:
+            hipFuncSetAttribute((const void *)k_count_wave_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
+            hipLaunchKernelGGL(k_count_wave_lds<4>, dim3((unsigned)grid), dim3(256), lds_bytes, s, b4, pitch4, d, n_chunks);
+            break;
+        case 2:
+            hipFuncSetAttribute((const void *)k_count_wave_lds<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
+            hipLaunchKernelGGL(k_count_wave_lds<2>, dim3((unsigned)grid), dim3(256), lds_bytes, s, b4, pitch4, d, n_chunks);
+            break;
+        default:
+            hipFuncSetAttribute((const void *)k_count_wave_lds<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
+            hipLaunchKernelGGL(k_count_wave_lds<1>, dim3((unsigned)grid), dim3(256), lds_bytes, s, b4, pitch4, d, n_chunks);
+            break;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 __global__ void k_zero_i32(int32_t *p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0;

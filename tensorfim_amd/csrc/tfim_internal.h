@@ -50,6 +50,26 @@ void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &
                        int grid_blocks, int chunk_uint4_per_lane, hipStream_t s);
 int wave_chunk_words(int chunk_uint4_per_lane);
 
+// LDS-staged wave: every item the wave touches gets an LDS slot; prefixes and
+// columns are given as slot indices.  Each workgroup stages one word chunk of
+// all slots at a time and runs every tile of the wave against it.
+struct WaveLdsDesc {
+    const int32_t *slot_rows;  // [n_slots] bitmap row of each slot
+    const int32_t *item_slots; // prefix slots, CSR by item_off
+    const int32_t *item_off;   // [n_rows+1]
+    const int32_t *col_slots;  // column slots, CSR by col_off
+    const int32_t *col_off;    // [n_rows+1] (also output offsets)
+    const WaveTile *tiles;
+    int32_t *out;              // [n_out], zeroed
+    int32_t n_slots, n_tiles, n_out;
+};
+// Largest slot count / output count the LDS kernel accepts.
+constexpr int kLdsMaxSlots = 96;
+constexpr int kLdsMaxOut = 8192;
+// Returns false if the shape does not fit LDS (caller uses launch_count_wave).
+bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
+                           hipStream_t s);
+
 // Zero `n` int32 with a fill kernel (graph-capture safe).
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
 
@@ -74,8 +94,9 @@ void launch_row_popcount(const uint64_t *bm, int64_t pitch_words, int32_t rows, 
 int mlp_nparam();
 void launch_mlp_fwd_bwd(const float *P, const float *X, const float *Y, const float *M, int n, float inv_count,
                         float *H1, float *H2, float *D1, float *D2, float *DY, float *lossv, hipStream_t s);
+int mlp_grad_slices(int n);
 void launch_mlp_grad(const float *X, const float *H1, const float *H2, const float *D1, const float *D2,
-                     const float *DY, int n, float *G, hipStream_t s);
+                     const float *DY, int n, float *part, float *G, hipStream_t s);
 void launch_mlp_adam(float *P, const float *G, float *Mo, float *Vo, uint16_t *Pb, float lr, float b1, float b2,
                      float eps, float c1, float c2, hipStream_t s);
 void launch_mlp_to_bf16(const float *P, uint16_t *Pb, hipStream_t s);

@@ -30,17 +30,19 @@ uint64_t sm64(uint64_t &s) {
 }
 
 struct Scratch {
-    float *X, *Y, *M, *H1, *H2, *D1, *D2, *DY, *L, *O;
+    float *X, *Y, *M, *H1, *H2, *D1, *D2, *DY, *L, *O, *part;
 };
 
 int ensure(tfim_mlp *m, int64_t n, Scratch &sc) {
     const int64_t per = IN + OUT + OUT + 4 * H + OUT + 1 + OUT;
+    const int64_t part_floats = (int64_t)mlp_grad_slices((int)std::max<int64_t>(n, 4096)) * mlp_nparam();
     if (n > m->cap) {
         hipFree(m->buf);
         m->buf = nullptr;
         m->cap = 0;
         const int64_t c = std::max<int64_t>(n, 4096);
-        if (hipMalloc(&m->buf, (size_t)(c * per) * sizeof(float)) != hipSuccess)
+        const int64_t pf = (int64_t)mlp_grad_slices((int)c) * mlp_nparam();
+        if (hipMalloc(&m->buf, (size_t)(c * per + pf) * sizeof(float)) != hipSuccess)
             return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp scratch allocation failed");
         m->cap = c;
     }
@@ -55,7 +57,9 @@ int ensure(tfim_mlp *m, int64_t n, Scratch &sc) {
     sc.D2 = p; p += c * H;
     sc.DY = p; p += c * OUT;
     sc.L = p; p += c;
-    sc.O = p;
+    sc.O = p; p += c * OUT;
+    sc.part = p;
+    (void)part_floats;
     return TFIM_OK;
 }
 
@@ -76,7 +80,7 @@ int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int3
     MCHK(m, hipMemcpyAsync(sc.Y, y, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
     MCHK(m, hipMemcpyAsync(sc.M, mask, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
     launch_mlp_fwd_bwd(m->P, sc.X, sc.Y, sc.M, n, inv, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, sc.L, s);
-    launch_mlp_grad(sc.X, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, n, m->G, s);
+    launch_mlp_grad(sc.X, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, n, sc.part, m->G, s);
     MCHK(m, hipGetLastError());
     if (loss) {
         std::vector<float> lv((size_t)n);

@@ -48,8 +48,11 @@ struct tfim_ctx {
     int64_t wave_out_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    int64_t wave_launches = 0, wave_graph_replays = 0, wave_rows = 0;
+    int64_t wave_launches = 0, wave_graph_replays = 0, wave_rows = 0, wave_lds_launches = 0;
     double wave_ms = 0;
+    int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel
+    int tile_cols = 32;
+    std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
 };
 
 namespace {
@@ -302,60 +305,120 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
         n_out = (int64_t)n_rows * c->n_cols;
     }
     if (n_out > INT32_MAX) return fail(c, TFIM_E_RANGE, "count_nodes: output too large");
+    if (n_out == 0) return TFIM_OK;
     if (int r = set_device(c)) return r;
 
-    // tiles of <= 32 columns per row
-    constexpr int kTileCols = 32;
+    // tiles of <= tile_cols columns per row
     std::vector<WaveTile> tiles;
     tiles.reserve((size_t)n_rows * 2);
     for (int32_t r = 0; r < n_rows; r++) {
         const int32_t nc = col_ids ? col_off[r + 1] - col_off[r] : c->n_cols;
-        for (int32_t b = 0; b < nc; b += kTileCols) tiles.push_back({r, b, std::min(nc, b + kTileCols), 0});
+        for (int32_t b = 0; b < nc; b += c->tile_cols) tiles.push_back({r, b, std::min(nc, b + c->tile_cols), 0});
     }
-    if (tiles.empty()) return TFIM_OK;
+    if (tiles.empty()) {
+        memset(out, 0, (size_t)n_out * 4);
+        return TFIM_OK;
+    }
 
-    // pack the descriptor block: item_ids | item_off | col_off | col_ids | tiles
+    // LDS slots: the distinct rows this wave touches
+    std::vector<int32_t> slot_rows;
+    bool lds = c->wave_kernel != 1 && n_out <= kLdsMaxOut;
+    if (lds) {
+        if ((int32_t)c->slot_of.size() < c->n_items) c->slot_of.assign(c->n_items, -1);
+        auto slot = [&](int32_t item) {
+            int32_t &sl = c->slot_of[item];
+            if (sl < 0) {
+                sl = (int32_t)slot_rows.size();
+                slot_rows.push_back(item);
+            }
+            return sl;
+        };
+        for (int64_t i = 0; i < n_item_ids && (int)slot_rows.size() <= kLdsMaxSlots; i++) slot(item_ids[i]);
+        if (col_ids)
+            for (int64_t i = 0; i < n_out && (int)slot_rows.size() <= kLdsMaxSlots; i++) slot(col_ids[i]);
+        else
+            for (int32_t i = 0; i < c->n_cols && (int)slot_rows.size() <= kLdsMaxSlots; i++) slot(i);
+        lds = (int)slot_rows.size() <= kLdsMaxSlots;
+        if (!lds)
+            for (int32_t it : slot_rows) c->slot_of[it] = -1;
+    }
+    if (c->wave_kernel == 2 && !lds)
+        return fail(c, TFIM_E_RANGE, "count_nodes: wave does not fit the LDS kernel (%zu items, %lld outputs)",
+                    slot_rows.size(), (long long)n_out);
+
+    // pack the descriptor block: [slot_rows] | item ids or slots | item_off | col_off | col ids or slots | tiles
     auto align16 = [](int64_t x) { return (x + 15) / 16 * 16; };
-    const int64_t o_items = 0;
+    const int64_t o_slots = 0;
+    const int64_t o_items = align16(o_slots + (int64_t)slot_rows.size() * 4);
     const int64_t o_ioff = align16(o_items + n_item_ids * 4);
     const int64_t o_coff = align16(o_ioff + (int64_t)(n_rows + 1) * 4);
     const int64_t o_cids = align16(o_coff + (int64_t)(n_rows + 1) * 4);
-    const int64_t o_tiles = align16(o_cids + (col_ids ? n_out * 4 : 0));
+    const bool need_cols = col_ids != nullptr || lds;
+    const int64_t o_tiles = align16(o_cids + (need_cols ? n_out * 4 : 0));
     const int64_t total = o_tiles + (int64_t)tiles.size() * (int64_t)sizeof(WaveTile);
     if (int r = ensure_pair(c, &c->d_desc, &c->h_desc, &c->desc_cap, total)) return r;
     if (int r = ensure_pair(c, &c->d_wave_out, &c->h_wave_out, &c->wave_out_cap, n_out)) return r;
     char *h = c->h_desc;
-    memcpy(h + o_items, item_ids, (size_t)n_item_ids * 4);
+    int32_t *hi = reinterpret_cast<int32_t *>(h + o_items);
+    int32_t *hc = reinterpret_cast<int32_t *>(h + o_cids);
+    if (lds) {
+        memcpy(h + o_slots, slot_rows.data(), slot_rows.size() * 4);
+        for (int64_t i = 0; i < n_item_ids; i++) hi[i] = c->slot_of[item_ids[i]];
+        if (col_ids)
+            for (int64_t i = 0; i < n_out; i++) hc[i] = c->slot_of[col_ids[i]];
+        else
+            for (int32_t r = 0; r < n_rows; r++)
+                for (int32_t j = 0; j < c->n_cols; j++) hc[(int64_t)r * c->n_cols + j] = c->slot_of[j];
+        for (int32_t it : slot_rows) c->slot_of[it] = -1;
+    } else {
+        memcpy(hi, item_ids, (size_t)n_item_ids * 4);
+        if (col_ids) memcpy(hc, col_ids, (size_t)n_out * 4);
+    }
     memcpy(h + o_ioff, item_off, (size_t)(n_rows + 1) * 4);
     int32_t *coff = reinterpret_cast<int32_t *>(h + o_coff);
     if (col_ids) {
         memcpy(coff, col_off, (size_t)(n_rows + 1) * 4);
-        memcpy(h + o_cids, col_ids, (size_t)n_out * 4);
     } else {
         for (int32_t r = 0; r <= n_rows; r++) coff[r] = r * c->n_cols;
     }
     memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
     HIPCHK(c, hipMemcpyAsync(c->d_desc, c->h_desc, (size_t)total, hipMemcpyHostToDevice, c->stream));
 
-    WaveDesc d;
-    d.item_ids = reinterpret_cast<const int32_t *>(c->d_desc + o_items);
-    d.item_off = reinterpret_cast<const int32_t *>(c->d_desc + o_ioff);
-    d.col_off = reinterpret_cast<const int32_t *>(c->d_desc + o_coff);
-    d.col_ids = col_ids ? reinterpret_cast<const int32_t *>(c->d_desc + o_cids) : nullptr;
-    d.tiles = reinterpret_cast<const WaveTile *>(c->d_desc + o_tiles);
-    d.out = c->d_wave_out;
-    d.n_tiles = (int32_t)tiles.size();
-    const int rw = c->pitch >= 2048 ? 4 : (c->pitch >= 512 ? 2 : 1);
-    const int64_t cw = wave_chunk_words(rw);
-    d.n_chunks = (int32_t)((c->pitch + cw - 1) / cw);
-    const int64_t units = (int64_t)d.n_chunks * d.n_tiles;
-    int64_t blocks = (units + 3) / 4;
-    blocks = std::min<int64_t>(blocks, (int64_t)c->n_cu * 8);
-    blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
-
     launch_zero_i32(c->d_wave_out, n_out, c->stream);
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    launch_count_wave(c->d_bm, c->pitch, d, (int)blocks, rw, c->stream);
+    if (lds) {
+        WaveLdsDesc d;
+        d.slot_rows = reinterpret_cast<const int32_t *>(c->d_desc + o_slots);
+        d.item_slots = reinterpret_cast<const int32_t *>(c->d_desc + o_items);
+        d.item_off = reinterpret_cast<const int32_t *>(c->d_desc + o_ioff);
+        d.col_slots = reinterpret_cast<const int32_t *>(c->d_desc + o_cids);
+        d.col_off = reinterpret_cast<const int32_t *>(c->d_desc + o_coff);
+        d.tiles = reinterpret_cast<const WaveTile *>(c->d_desc + o_tiles);
+        d.out = c->d_wave_out;
+        d.n_slots = (int32_t)slot_rows.size();
+        d.n_tiles = (int32_t)tiles.size();
+        d.n_out = (int32_t)n_out;
+        if (!launch_count_wave_lds(c->d_bm, c->pitch, d, c->n_cu, c->stream))
+            return fail(c, TFIM_E_RANGE, "count_nodes: LDS kernel rejected %d slots", d.n_slots);
+        c->wave_lds_launches++;
+    } else {
+        WaveDesc d;
+        d.item_ids = reinterpret_cast<const int32_t *>(c->d_desc + o_items);
+        d.item_off = reinterpret_cast<const int32_t *>(c->d_desc + o_ioff);
+        d.col_off = reinterpret_cast<const int32_t *>(c->d_desc + o_coff);
+        d.col_ids = col_ids ? reinterpret_cast<const int32_t *>(c->d_desc + o_cids) : nullptr;
+        d.tiles = reinterpret_cast<const WaveTile *>(c->d_desc + o_tiles);
+        d.out = c->d_wave_out;
+        d.n_tiles = (int32_t)tiles.size();
+        const int rw = c->pitch >= 2048 ? 4 : (c->pitch >= 512 ? 2 : 1);
+        const int64_t cw = wave_chunk_words(rw);
+        d.n_chunks = (int32_t)((c->pitch + cw - 1) / cw);
+        const int64_t units = (int64_t)d.n_chunks * d.n_tiles;
+        int64_t blocks = (units + 3) / 4;
+        blocks = std::min<int64_t>(blocks, (int64_t)c->n_cu * 8);
+        blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+        launch_count_wave(c->d_bm, c->pitch, d, (int)blocks, rw, c->stream);
+    }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->h_wave_out, c->d_wave_out, (size_t)n_out * 4, hipMemcpyDeviceToHost, c->stream));
@@ -367,6 +430,20 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
     c->wave_rows += n_rows;
     memcpy(out, c->h_wave_out, (size_t)n_out * 4);
     return TFIM_OK;
+}
+
+int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
+    if (!c || !key) return TFIM_E_INVALID;
+    const std::string k(key);
+    if (k == "wave_kernel" && value >= 0 && value <= 2) {
+        c->wave_kernel = (int)value;
+        return TFIM_OK;
+    }
+    if (k == "tile_cols" && value >= 8 && value <= 256) {
+        c->tile_cols = (int)value;
+        return TFIM_OK;
+    }
+    return fail(c, TFIM_E_INVALID, "config: unknown key or value %s=%lld", key, (long long)value);
 }
 
 int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *replays, int64_t *rows, double *ms) {

@@ -137,3 +137,29 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
     np.testing.assert_array_equal(got, oracle_lib.popcount_matrix(A, B))
     for x in (dA, dB, dC):
         x.free()
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("n_bits,n_items", [(1100, 30), (3200, 75), (400_003, 90), (2_000_000, 38)])
+def test_wave_kernels_forced(ctx, kernel, n_bits, n_items):
+    """Both batched kernels (1 = L2-resident, 2 = LDS-staged) on the same waves:
+    uneven prefixes, uneven and empty tails, full-column rows."""
+    rng = np.random.default_rng(kernel * 100 + n_items)
+    rows = rand_rows(rng, n_items, n_bits, 0.65)
+    n_cols = n_items - 1
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    ctx.config("wave_kernel", kernel)
+    try:
+        prefixes, tails = [], []
+        for i in range(120):
+            prefixes.append(rng.choice(n_items, size=int(rng.integers(1, 9)), replace=False))
+            t = int(rng.integers(0, 40)) if i % 7 else 0
+            tails.append(np.sort(rng.choice(n_cols, size=min(t, n_cols), replace=False)))
+        got = ctx.count_nodes(prefixes, tails)
+        for p, t, g in zip(prefixes, tails, got):
+            np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols)[t])
+        full = ctx.count_nodes(prefixes[:60])
+        for p, g in zip(prefixes[:60], full):
+            np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols))
+    finally:
+        ctx.config("wave_kernel", 0)
