@@ -106,13 +106,17 @@ int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item
 /* Tuning knobs of the batched path (ablation and autotuning; the reference's
  * equivalents are environment variables, kernel_bmma.cu:351-368):
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel
- *   "tile_cols"    columns per work tile (8..256, default 32)
+ *   "tile_cols"    columns per work tile (8..64, default 32)
+ *   "mfma_tile"    0 auto, 128 or 256: workgroup tile of the int8-MFMA
+ *                  matrix count (auto = 256 when both dimensions reach 256)
  * Unknown keys and out-of-range values return TFIM_E_INVALID. */
 int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
 
-/* Statistics of the batched path since tfim_open (launches, graph replays,
- * rows, summed kernel milliseconds measured with HIP events). */
-int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *graph_replays,
+/* Statistics of the batched path since tfim_open: launches, how many used
+ * the LDS-staged kernel, rows, summed kernel milliseconds (HIP events).  In
+ * TFIM_PATH_AUTO the library times both kernels on waves that fit either and
+ * keeps the cheaper per row-word of work (re-sampling every 64 waves). */
+int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,
                     int64_t *rows, double *kernel_ms);
 
 /* ---- device buffers and the isolated count (microbench) ---------------- */

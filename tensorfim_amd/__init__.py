@@ -223,7 +223,7 @@ class Context:
         a, b, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
         self._check(self._lib.tfim_wave_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                               ctypes.byref(d)))
-        return {"launches": a.value, "graph_replays": b.value, "rows": c.value, "kernel_ms": d.value}
+        return {"launches": a.value, "lds_launches": b.value, "rows": c.value, "kernel_ms": d.value}
 
     def buffer(self, nbytes):
         return DeviceBuffer(self, nbytes)

@@ -2,8 +2,8 @@
 //
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
-//             [--workers N] [--learned] [--gate indep|always] [--model PATH]
-//             [--trace PATH] [--verbose]
+//             [--workers N] [--learned] [--rank] [--gate indep|always]
+//             [--wave-kernel 0|1|2] [--model PATH] [--trace PATH] [--verbose]
 //
 // Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the
 // reference layout and prints run statistics as "key: value" lines.  Exit
@@ -30,6 +30,7 @@ static double now() {
 int main(int argc, char **argv) {
     SearchOptions opt;
     std::string out_path;
+    int wave_kernel = 0;
     int replicate = 0, device = 0;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
@@ -46,6 +47,8 @@ int main(int argc, char **argv) {
         else if (a == "--learned") opt.learned = true;
         else if (a == "--gate") opt.gate = next();
         else if (a == "--workers") opt.workers = atoi(next().c_str());
+        else if (a == "--rank") opt.rank = true;
+        else if (a == "--wave-kernel") wave_kernel = atoi(next().c_str());
         else if (a == "--model") opt.model_path = next();
         else if (a == "--trace") opt.trace_path = next();
         else if (a == "--verbose") opt.verbose = true;
@@ -62,6 +65,11 @@ int main(int argc, char **argv) {
     tfim_ctx *ctx = nullptr;
     if (tfim_open(device, &ctx) != TFIM_OK) {
         fprintf(stderr, "tfim_open(%d) failed: no usable GPU\n", device);
+        return 1;
+    }
+    if (wave_kernel && tfim_config(ctx, "wave_kernel", wave_kernel) != TFIM_OK) {
+        fprintf(stderr, "%s\n", tfim_last_error(ctx));
+        tfim_close(ctx);
         return 1;
     }
     Database db;
@@ -130,7 +138,8 @@ int main(int argc, char **argv) {
            db.preprocessing_seconds, t_up1 - t_up0);
     printf("computation_time(s): %g\nwall_time(s): %g\n", info.computation_time, wall);
     printf("count_time(s): %g\nkernel_ms: %g\n", st.count_seconds, st.kernel_ms);
-    printf("rows: %lld\nwaves: %lld\n", (long long)st.nodes, (long long)st.waves);
+    printf("rows: %lld\nwaves: %lld\nlds_waves: %lld\n", (long long)st.nodes, (long long)st.waves,
+           (long long)st.lds_waves);
     printf("expand_rows: %lld\nhut_rows: %lld\nleaf_rows: %lld\nref_rows: %lld\n", (long long)st.expand_rows,
            (long long)st.hut_rows, (long long)st.leaf_rows, (long long)st.ref_rows);
     printf("hut_success: %lld\nhut_skipped: %lld\npep_moves: %lld\nhutmfi_pruned: %lld\nleaf_subsumed: %lld\n",

@@ -72,7 +72,7 @@ class LearnedPolicy : public Policy {
         plain_.decide(ds, db);                       // fallback order and gate
         if (!trained_)
             for (Decision *d : ds)   // warm-up: label both outcomes (bounded tails only)
-                if (d->tail.size() >= 2 && d->tail.size() <= 64) d->lookahead = true;
+                if (d->tail.size() >= 2 && d->tail.size() <= 64 && rand01() < 0.25f) d->lookahead = true;
         // features for every decision and tail item (inference batch)
         feats_.clear();
         std::vector<int64_t> node_row(ds.size(), -1), item_row0(ds.size(), -1);
@@ -99,15 +99,15 @@ class LearnedPolicy : public Policy {
             Decision *d = ds[q];
             if (node_row[q] < 0) continue;
             if (trained_) {
+                // the plain rule's lookaheads stay; the model adds confident ones
                 const float pg = sigmoid(logits_[node_row[q] * 2 + 0]);
-                const bool explore = rand01() < 0.1;
-                d->lookahead = d->tail.size() >= 2 && (pg >= 0.25f || explore);
-                if (item_row0[q] >= 0) {
-                    std::vector<float> sc(d->tail.size());
-                    for (size_t i = 0; i < d->tail.size(); i++) sc[i] = logits_[(item_row0[q] + i) * 2 + 1];
-                    std::stable_sort(d->order.begin(), d->order.end(), [&](int32_t a, int32_t b) {
-                        if (sc[a] != sc[b]) return sc[a] < sc[b];
-                        return d->tail_sup[a] < d->tail_sup[b];
+                const bool explore = d->tail.size() <= 64 && rand01() < 0.03f;
+                d->lookahead = d->tail.size() >= 2 && (d->lookahead || pg >= 0.6f || explore);
+                if (item_row0[q] >= 0 && opt_.rank) {
+                    // items predicted to keep >= 90% of their child's tail go
+                    // last (they sit in every earlier child's head+tail)
+                    std::stable_partition(d->order.begin(), d->order.end(), [&](int32_t a) {
+                        return sigmoid(logits_[(item_row0[q] + a) * 2 + 1]) < 0.9f;
                     });
                 }
             }
@@ -186,8 +186,8 @@ class LearnedPolicy : public Policy {
         const NodeAgg a = agg(d);
         const double margin = std::log(a.s / thr_);
         const float f[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
-                             (float)(d.tail.size() / 16.0), (float)margin, (float)a.sum_log_r,
-                             (float)(margin + a.sum_log_r), (float)a.min_r, (float)a.mean_r, (float)a.max_r,
+                             (float)(d.tail.size() / 16.0), (float)(10 * margin), (float)a.sum_log_r,
+                             (float)(10 * (margin + a.sum_log_r)), (float)a.min_r, (float)a.mean_r, (float)a.max_r,
                              (float)a.frac_hi, (float)(d.depth / 16.0), (float)a.mean_freq, 0.0f};
         feats_.insert(feats_.end(), f, f + NF);
         cur_ = a;
@@ -198,12 +198,12 @@ class LearnedPolicy : public Policy {
         const double r = std::max(1, d.tail_sup[i]) / a.s;
         const int rank = rank_[i];
         const float f[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
-                             (float)(d.tail.size() / 16.0), (float)margin, (float)std::log(r),
-                             (float)std::log(std::max(1, d.tail_sup[i]) / thr_),
+                             (float)(d.tail.size() / 16.0), (float)(10 * margin), (float)(10 * std::log(r)),
+                             (float)(10 * std::log(std::max(1, d.tail_sup[i]) / thr_)),
                              (float)(rank / std::max<double>(1, d.tail.size() - 1)),
                              (float)((double)(*freq_)[d.tail[i]] / N_), (float)(r - a.mean_r),
-                             (float)(margin + a.sum_log_r), (float)(d.depth / 16.0),
-                             (float)(margin + a.sum_log_r - std::log(r)), 1.0f};
+                             (float)(10 * (margin + a.sum_log_r)), (float)(d.depth / 16.0),
+                             (float)(10 * (margin + a.sum_log_r - std::log(r))), 1.0f};
         feats_.insert(feats_.end(), f, f + NF);
     }
 
@@ -242,7 +242,7 @@ class LearnedPolicy : public Policy {
         bm_[b * 2] = head == 0 ? 1.0f : 0.0f;
         bm_[b * 2 + 1] = head == 1 ? 1.0f : 0.0f;
         n_labelled_++;
-        if (n_labelled_ - last_train_ >= kTrainEvery) train();
+        if (n_labelled_ - last_train_ >= (last_train_ == 0 ? kFirstTrain : kTrainEvery)) train();
     }
     void train() {
         if (!mlp_) return;
@@ -259,6 +259,7 @@ class LearnedPolicy : public Policy {
     }
 
     static constexpr int64_t kTrainEvery = 8192;
+    static constexpr int64_t kFirstTrain = 1024;
     static constexpr int64_t kBatch = 8192;
     static constexpr int64_t kRing = 1 << 20;
     static constexpr int kEpochs = 4;

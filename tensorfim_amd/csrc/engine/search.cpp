@@ -575,7 +575,7 @@ int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std:
     if (opt.engine == "wave") {
         const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
         const int rc = run_wave(ctx, db, cap, out, st, err);
-        tfim_wave_stats(ctx, nullptr, nullptr, nullptr, &st.kernel_ms);
+        tfim_wave_stats(ctx, nullptr, &st.lds_waves, nullptr, &st.kernel_ms);
         return rc;
     }
     if (opt.engine == "mafia") {
@@ -590,7 +590,7 @@ int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std:
         Mafia m(ctx, db, opt, *pol, st);
         const int rc = m.run(out, err);
         pol->finish(st);
-        tfim_wave_stats(ctx, nullptr, nullptr, nullptr, &st.kernel_ms);
+        tfim_wave_stats(ctx, nullptr, &st.lds_waves, nullptr, &st.kernel_ms);
         return rc;
     }
     err = "unknown engine " + opt.engine;

@@ -30,6 +30,7 @@ struct SearchOptions {
     int32_t workers = 1;          // plain engine: host workers (one stream each)
     bool learned = false;         // mafia: use the MLP ranker/gate
     std::string gate = "indep";   // mafia plain lookahead rule: indep | always
+    bool rank = false;            // mafia learned: reorder tails with the ranker head
     std::string model_path;       // mafia: load MLP weights (else trained online)
     std::string trace_path;       // mafia: write (features, outcome) records
     bool verbose = false;
@@ -48,6 +49,7 @@ struct SearchStats {
     double counted_bitops = 0;    // sum over rows of columns x transactions
     double count_seconds = 0;     // host time inside count calls
     double kernel_ms = 0;         // device time of count kernels (HIP events)
+    int64_t lds_waves = 0;        // waves counted by the LDS-staged kernel
     double learn_seconds = 0;     // host time in the learned part
     int64_t mlp_train_steps = 0;
 };

@@ -129,6 +129,142 @@ __global__ __launch_bounds__(256) void k_popmat_mfma(const uint64_t *__restrict_
             }
 }
 
+// ---------------------------------------------------------------------------
+// Large-tile variant: 256x256 per workgroup, each wave 128x128 (4x4 MFMA
+// tiles, 256 int32 accumulators), so every 1 KB fragment read from LDS feeds
+// 4 MFMAs instead of 2.  Thread t owns row t of the A tile and row t of the B
+// tile: it keeps one 1024-bit block of each in registers (eight 128-bit
+// loads), prefetches the next block while the current one is consumed, and
+// per 128-bit sub-stage unpacks 16 bytes of each row into a double-buffered
+// LDS stage (2 x 64 KB, 16-byte slots XOR-swizzled by row & 7).  One barrier
+// per sub-stage: a stage is rewritten only after every wave has passed the
+// barrier that follows its last read.
+constexpr int M2_T = 256;
+constexpr int M2_SLOTS = 8;   // 16-byte slots per row per 128-bit sub-stage
+
+__global__ __launch_bounds__(256) void k_popmat_mfma256(const uint64_t *__restrict__ A,
+                                                        const uint64_t *__restrict__ B,
+                                                        int32_t *__restrict__ C, int64_t b, int64_t d,
+                                                        int64_t pitch, int64_t k_per_split, int atomic,
+                                                        int tiles_n) {
+    extern __shared__ uint4 sm[];
+    uint4 *As = sm;                              // [2][256][8]
+    uint4 *Bs = sm + 2 * M2_T * M2_SLOTS;        // [2][256][8]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
+    const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
+    const int64_t row0 = (lin / tiles_n) * M2_T, col0 = (lin % tiles_n) * M2_T;
+    const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
+    const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
+    const bool a_ok = row0 + tid < b, b_ok = col0 + tid < d;
+    const uint4 *Arow = reinterpret_cast<const uint4 *>(A + (row0 + tid) * pitch);
+    const uint4 *Brow = reinterpret_cast<const uint4 *>(B + (col0 + tid) * pitch);
+    const uint4 z4 = make_uint4(0, 0, 0, 0);
+
+    const int wr = wave >> 1, wc = wave & 1;
+    v16i acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[i][j][q] = 0;
+
+    uint4 ra[8], rb[8], na[8], nb[8];
+    if (kbeg < kend) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            ra[u] = a_ok ? Arow[(kbeg >> 1) + u] : z4;
+            rb[u] = b_ok ? Brow[(kbeg >> 1) + u] : z4;
+        }
+    }
+    int buf = 0;
+    for (int64_t kw = kbeg; kw < kend; kw += 16) {
+        const bool more = kw + 16 < kend;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {   // prefetch the next 1024-bit block
+            na[u] = (more && a_ok) ? Arow[((kw + 16) >> 1) + u] : z4;
+            nb[u] = (more && b_ok) ? Brow[((kw + 16) >> 1) + u] : z4;
+        }
+#pragma unroll
+        for (int sub = 0; sub < 8; sub++) {
+            uint4 *as = As + buf * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
+            uint4 *bs = Bs + buf * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
+            const uint32_t aw[4] = {ra[sub].x, ra[sub].y, ra[sub].z, ra[sub].w};
+            const uint32_t bw[4] = {rb[sub].x, rb[sub].y, rb[sub].z, rb[sub].w};
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                as[j ^ (tid & 7)] = unpack16(aw[j >> 1] >> ((j & 1) * 16));
+                bs[j ^ (tid & 7)] = unpack16(bw[j >> 1] >> ((j & 1) * 16));
+            }
+            __syncthreads();
+            const uint4 *ab = As + buf * (M2_T * M2_SLOTS);
+            const uint4 *bb = Bs + buf * (M2_T * M2_SLOTS);
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const int slot = kk * 2 + (lane >> 5);
+                v4i af[4], bf[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int r = wr * 128 + i * 32 + (lane & 31);
+                    const uint4 x = ab[r * M2_SLOTS + (slot ^ (r & 7))];
+                    af[i] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+                }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int c = wc * 128 + j * 32 + (lane & 31);
+                    const uint4 x = bb[c * M2_SLOTS + (slot ^ (c & 7))];
+                    bf[j] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+            }
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            ra[u] = na[u];
+            rb[u] = nb[u];
+        }
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int64_t r = row0 + wr * 128 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                const int64_t c = col0 + wc * 128 + j * 32 + (lane & 31);
+                if (r < b && c < d) {
+                    if (atomic) atomicAdd(&C[r * d + c], acc[i][j][q]);
+                    else C[r * d + c] = acc[i][j][q];
+                }
+            }
+}
+
+void launch_popmat_mfma256(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                           int64_t pitch_words, int split_k, hipStream_t s) {
+    if (split_k < 1) split_k = 1;
+    int64_t kps = (pitch_words + split_k - 1) / split_k;
+    kps = (kps + 15) / 16 * 16;
+    const int64_t zs = (pitch_words + kps - 1) / kps;
+    const int64_t tm = (b + M2_T - 1) / M2_T, tn = (d + M2_T - 1) / M2_T;
+    const size_t lds = (size_t)4 * M2_T * M2_SLOTS * sizeof(uint4);   // 128 KB
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute((const void *)k_popmat_mfma256, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    dim3 grid((unsigned)(tm * tn), (unsigned)zs);
+    hipLaunchKernelGGL(k_popmat_mfma256, grid, dim3(256), lds, s, A, B, C, b, d, pitch_words, kps,
+                       zs > 1 ? 1 : 0, (int)tn);
+}
+
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                         int64_t pitch_words, int split_k, hipStream_t s) {
     if (split_k < 1) split_k = 1;

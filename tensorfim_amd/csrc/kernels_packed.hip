@@ -210,54 +210,64 @@ void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &
 }
 
 // ---------------------------------------------------------------------------
-// LDS-staged fused wave kernel.  Workgroup g handles word chunks g, g+G, ...;
-// per chunk it copies the chunk of every slot (item row) the wave uses into
-// LDS with 128-bit loads, then its 4 waves take the tiles round-robin: prefix
-// AND from LDS into registers, column counts from LDS, 8-column
-// reduce-scatter, LDS atomic accumulation.  Partial counts leave the
+// LDS-staged fused wave kernel.  Workgroup g (1024 threads = 16 waves) handles
+// word chunks g, g+G, ...; per chunk it copies the chunk of every slot (item
+// row) the wave uses into LDS with 128-bit loads, then its waves take the
+// tiles round-robin.  Per tile the prefix and column slot numbers are loaded
+// lane-parallel once and broadcast with readlane, so the column loop issues
+// only LDS reads and VALU work: prefix AND into registers, 8 columns at a
+// time, reduce-scatter, LDS atomic accumulation.  Partial counts leave the
 // workgroup once, as one global atomic per non-zero output.  Global traffic
 // per wave is one pass over the slots' rows whatever the number of rows.
+constexpr int kLdsThreads = 1024;
+constexpr int kLdsWaves = kLdsThreads / 64;
+
 template <int RW>
-__global__ __launch_bounds__(256) void k_count_wave_lds(const uint4 *__restrict__ bm, int64_t pitch4,
-                                                        WaveLdsDesc d, int64_t n_chunks) {
+__global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                                WaveLdsDesc d, int64_t n_chunks) {
     extern __shared__ uint4 lds[];
     constexpr int C4 = 64 * RW;   // uint4 per slot per chunk
     int *acc = reinterpret_cast<int *>(lds + (size_t)d.n_slots * C4);
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    for (int i = tid; i < d.n_out; i += 256) acc[i] = 0;
+    for (int i = tid; i < d.n_out; i += kLdsThreads) acc[i] = 0;
     for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
         __syncthreads();
         const int64_t base = c * C4;
-        for (int i = tid; i < d.n_slots * C4; i += 256) {
+        for (int i = tid; i < d.n_slots * C4; i += kLdsThreads) {
             const int slot = i / C4, off = i % C4;
             const int64_t g = base + off;
             lds[i] = g < pitch4 ? bm[(int64_t)d.slot_rows[slot] * pitch4 + g] : make_uint4(0, 0, 0, 0);
         }
         __syncthreads();
-        for (int t = wave; t < d.n_tiles; t += 4) {
+        for (int t = wave; t < d.n_tiles; t += kLdsWaves) {
             const WaveTile tile = d.tiles[t];
             const int r = tile.row;
-            const int ib = d.item_off[r], ie = d.item_off[r + 1];
+            const int ib = d.item_off[r], k = d.item_off[r + 1] - ib;
+            const int ob = d.col_off[r];
+            const int nc = tile.col_end - tile.col_begin;   // <= 64 (tile_cols)
+            // slot numbers, one per lane, broadcast below with readlane
+            const int pslot = lane < k ? d.item_slots[ib + lane] : 0;
+            const int cslot = lane < nc ? d.col_slots[ob + tile.col_begin + lane] : 0;
             uint4 m[RW];
             {
-                const uint4 *row = lds + d.item_slots[ib] * C4;
+                const uint4 *row = lds + __builtin_amdgcn_readlane(pslot, 0) * C4;
 #pragma unroll
                 for (int q = 0; q < RW; q++) m[q] = row[q * 64 + lane];
             }
-            for (int i = ib + 1; i < ie; i++) {
-                const uint4 *row = lds + d.item_slots[i] * C4;
+            for (int i = 1; i < k; i++) {
+                const int sl = i < 64 ? __builtin_amdgcn_readlane(pslot, i) : d.item_slots[ib + i];
+                const uint4 *row = lds + sl * C4;
 #pragma unroll
                 for (int q = 0; q < RW; q++) m[q] = and4(m[q], row[q * 64 + lane]);
             }
-            const int ob = d.col_off[r];
-            for (int j = tile.col_begin; j < tile.col_end; j += 8) {
+            for (int j = 0; j < nc; j += 8) {
                 int v8[8];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
                     uint32_t a = 0;
-                    if (j + q < tile.col_end) {
-                        const uint4 *row = lds + d.col_slots[ob + j + q] * C4;
+                    if (j + q < nc) {
+                        const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, j + q) * C4;
 #pragma unroll
                         for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
                     }
@@ -266,13 +276,13 @@ __global__ __launch_bounds__(256) void k_count_wave_lds(const uint4 *__restrict_
                 const int y = wave_reduce_scatter8(v8);
                 if ((lane & 7) == 0) {
                     const int q = (lane >> 3) & 7;
-                    if (j + q < tile.col_end && y) atomicAdd(&acc[ob + j + q], y);
+                    if (j + q < nc && y) atomicAdd(&acc[ob + tile.col_begin + j + q], y);
                 }
             }
         }
     }
     __syncthreads();
-    for (int i = tid; i < d.n_out; i += 256)
+    for (int i = tid; i < d.n_out; i += kLdsThreads)
         if (acc[i]) atomicAdd(&d.out[i], acc[i]);
 }
 
@@ -284,6 +294,13 @@ bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLd
     auto bytes = [&](int rw) { return (size_t)d.n_slots * 64 * rw * 16 + acc_bytes; };
     // Prefer two workgroups per CU (<= 80 KB each); the widest chunk that fits.
     constexpr size_t kHalf = 80 * 1024, kFull = 160 * 1024;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipFuncSetAttribute((const void *)k_count_wave_lds<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
+        hipFuncSetAttribute((const void *)k_count_wave_lds<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
+        hipFuncSetAttribute((const void *)k_count_wave_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
+        attr_set = true;
+    }
     int rw = 4;
     while (rw > 1 && bytes(rw) > kHalf) rw /= 2;
     if (bytes(rw) > kHalf && bytes(1) > kFull) return false;
@@ -294,16 +311,13 @@ bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLd
     const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
     switch (rw) {
         case 4:
-            hipFuncSetAttribute((const void *)k_count_wave_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
-            hipLaunchKernelGGL(k_count_wave_lds<4>, dim3((unsigned)grid), dim3(256), lds_bytes, s, b4, pitch4, d, n_chunks);
+            hipLaunchKernelGGL(k_count_wave_lds<4>, dim3((unsigned)grid), dim3(kLdsThreads), lds_bytes, s, b4, pitch4, d, n_chunks);
             break;
         case 2:
-            hipFuncSetAttribute((const void *)k_count_wave_lds<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
-            hipLaunchKernelGGL(k_count_wave_lds<2>, dim3((unsigned)grid), dim3(256), lds_bytes, s, b4, pitch4, d, n_chunks);
+            hipLaunchKernelGGL(k_count_wave_lds<2>, dim3((unsigned)grid), dim3(kLdsThreads), lds_bytes, s, b4, pitch4, d, n_chunks);
             break;
         default:
-            hipFuncSetAttribute((const void *)k_count_wave_lds<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
-            hipLaunchKernelGGL(k_count_wave_lds<1>, dim3((unsigned)grid), dim3(256), lds_bytes, s, b4, pitch4, d, n_chunks);
+            hipLaunchKernelGGL(k_count_wave_lds<1>, dim3((unsigned)grid), dim3(kLdsThreads), lds_bytes, s, b4, pitch4, d, n_chunks);
             break;
     }
     return true;

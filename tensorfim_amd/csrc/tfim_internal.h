@@ -76,6 +76,9 @@ void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
 // Matrix AND-popcount, packed (VALU) path.  C must be zeroed when split_k>1.
 void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                           int64_t pitch_words, int split_k, hipStream_t s);
+// Matrix AND-popcount, int8 MFMA path, 256x256 tiles (large shapes).
+void launch_popmat_mfma256(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                           int64_t pitch_words, int split_k, hipStream_t s);
 // Matrix AND-popcount, int8 MFMA path.  C must be zeroed when split_k>1.
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                         int64_t pitch_words, int split_k, hipStream_t s);

@@ -51,7 +51,13 @@ struct tfim_ctx {
     int64_t wave_launches = 0, wave_graph_replays = 0, wave_rows = 0, wave_lds_launches = 0;
     double wave_ms = 0;
     int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel
+    // online kernel choice for waves both kernels accept: EMA of kernel
+    // milliseconds per (row prefix + column) word, per kernel
+    double tune_cost[2] = {0, 0};
+    int64_t tune_samples[2] = {0, 0};
+    int64_t tune_calls = 0;
     int tile_cols = 32;
+    int mfma_tile = 0;      // 0 auto, 128 or 256
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
 };
 
@@ -320,6 +326,8 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
         return TFIM_OK;
     }
 
+    // work of this wave in row-words: sum over rows of (prefix + columns) x pitch
+    const double work = ((double)n_item_ids + (double)n_out) * (double)c->pitch;
     // LDS slots: the distinct rows this wave touches
     std::vector<int32_t> slot_rows;
     bool lds = c->wave_kernel != 1 && n_out <= kLdsMaxOut;
@@ -345,6 +353,23 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
     if (c->wave_kernel == 2 && !lds)
         return fail(c, TFIM_E_RANGE, "count_nodes: wave does not fit the LDS kernel (%zu items, %lld outputs)",
                     slot_rows.size(), (long long)n_out);
+    // autotune: both kernels accept the wave; sample each, then take the
+    // cheaper, re-sampling the other every 64 calls
+    int tuned = -1;
+    if (lds && c->wave_kernel == 0) {
+        c->tune_calls++;
+        if (c->tune_samples[0] < 3) tuned = 0;
+        else if (c->tune_samples[1] < 3) tuned = 1;
+        else {
+            const int best = c->tune_cost[1] < c->tune_cost[0] ? 1 : 0;
+            tuned = (c->tune_calls % 64 == 0) ? 1 - best : best;
+        }
+        if (tuned == 0) {
+            lds = false;
+            for (int32_t it : slot_rows) c->slot_of[it] = -1;
+            slot_rows.clear();
+        }
+    }
 
     // pack the descriptor block: [slot_rows] | item ids or slots | item_off | col_off | col ids or slots | tiles
     auto align16 = [](int64_t x) { return (x + 15) / 16 * 16; };
@@ -428,6 +453,12 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
     c->wave_ms += ms;
     c->wave_launches++;
     c->wave_rows += n_rows;
+    if (tuned >= 0 && work > 0) {
+        const double per = (double)ms / work;
+        double &cst = c->tune_cost[tuned];
+        cst = c->tune_samples[tuned] == 0 ? per : 0.8 * cst + 0.2 * per;
+        c->tune_samples[tuned]++;
+    }
     memcpy(out, c->h_wave_out, (size_t)n_out * 4);
     return TFIM_OK;
 }
@@ -439,17 +470,21 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
         c->wave_kernel = (int)value;
         return TFIM_OK;
     }
-    if (k == "tile_cols" && value >= 8 && value <= 256) {
+    if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256)) {
+        c->mfma_tile = (int)value;
+        return TFIM_OK;
+    }
+    if (k == "tile_cols" && value >= 8 && value <= 64) {
         c->tile_cols = (int)value;
         return TFIM_OK;
     }
     return fail(c, TFIM_E_INVALID, "config: unknown key or value %s=%lld", key, (long long)value);
 }
 
-int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *replays, int64_t *rows, double *ms) {
+int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *lds_launches, int64_t *rows, double *ms) {
     if (!c) return TFIM_E_INVALID;
+    if (lds_launches) *lds_launches = c->wave_lds_launches;
     if (launches) *launches = c->wave_launches;
-    if (replays) *replays = c->wave_graph_replays;
     if (rows) *rows = c->wave_rows;
     if (ms) *ms = c->wave_ms;
     return TFIM_OK;
@@ -523,10 +558,15 @@ static void popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfi
                           int64_t d, int64_t pitch, int32_t path, int32_t split_k) {
     if (split_k <= 0) split_k = 1;
     if (split_k > 1) launch_zero_i32((int32_t *)C->ptr, b * d, c->stream);
-    if (path == TFIM_PATH_PACKED)
-        launch_popmat_packed((const uint64_t *)A->ptr, (const uint64_t *)B->ptr, (int32_t *)C->ptr, b, d, pitch, split_k, c->stream);
-    else
-        launch_popmat_mfma((const uint64_t *)A->ptr, (const uint64_t *)B->ptr, (int32_t *)C->ptr, b, d, pitch, split_k, c->stream);
+    const uint64_t *a = (const uint64_t *)A->ptr, *bb = (const uint64_t *)B->ptr;
+    int32_t *cc = (int32_t *)C->ptr;
+    if (path == TFIM_PATH_PACKED) {
+        launch_popmat_packed(a, bb, cc, b, d, pitch, split_k, c->stream);
+    } else {
+        const int tile = c->mfma_tile ? c->mfma_tile : (b >= 256 && d >= 256 ? 256 : 128);
+        if (tile == 256) launch_popmat_mfma256(a, bb, cc, b, d, pitch, split_k, c->stream);
+        else launch_popmat_mfma(a, bb, cc, b, d, pitch, split_k, c->stream);
+    }
 }
 
 int tfim_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,

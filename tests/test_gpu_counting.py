@@ -117,10 +117,10 @@ def test_fill_bernoulli_matches_oracle(ctx):
     buf.free()
 
 
-@pytest.mark.parametrize("path", ["packed", "mfma"])
+@pytest.mark.parametrize("path", ["packed", "mfma", "mfma128", "mfma256"])
 @pytest.mark.parametrize("b,d,n_bits,split", [(130, 70, 4096 + 65, 1), (256, 256, 65_536, 1),
                                                (200, 513, 40_000, 3), (64, 33, 2048 * 9, 8),
-                                               (1, 1, 64, 1)])
+                                               (1, 1, 64, 1), (515, 300, 2048 * 3 + 1, 2)])
 def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
     rng = np.random.default_rng(b * 1000 + d)
     pitch = tf.pitch_words(n_bits)
@@ -132,7 +132,12 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
     dA, dB, dC = ctx.buffer(A.nbytes), ctx.buffer(B.nbytes), ctx.buffer(b * d * 4)
     dA.write(A)
     dB.write(B)
-    ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path=path, split_k=split)
+    tile = {"mfma128": 128, "mfma256": 256}.get(path, 0)
+    ctx.config("mfma_tile", tile)
+    try:
+        ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path="packed" if path == "packed" else "mfma", split_k=split)
+    finally:
+        ctx.config("mfma_tile", 0)
     got = dC.read(np.int32, b * d).reshape(b, d)
     np.testing.assert_array_equal(got, oracle_lib.popcount_matrix(A, B))
     for x in (dA, dB, dC):
