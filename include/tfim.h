@@ -103,12 +103,25 @@ int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item
                      int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
                      int32_t *out, int32_t path);
 
+/* The same count split in two, so the host can prepare or process one wave
+ * while the GPU counts another: submit packs and launches the wave
+ * asynchronously on the ctx stream and returns a ticket; wait blocks until
+ * that wave's counts are done and copies them into `out` (layout as in
+ * tfim_count_nodes).  At most two waves may be in flight per ctx; tickets
+ * are waited in submission order.  A ticket of -1 (nothing to count) waits
+ * trivially. */
+int tfim_count_nodes_submit(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item_off,
+                            int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
+                            int32_t path, int64_t *ticket);
+int tfim_count_nodes_wait(tfim_ctx *ctx, int64_t ticket, int32_t *out);
+
 /* Tuning knobs of the batched path (ablation and autotuning; the reference's
  * equivalents are environment variables, kernel_bmma.cu:351-368):
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel
  *   "tile_cols"    columns per work tile (8..64, default 32)
- *   "mfma_tile"    0 auto, 128 or 256: workgroup tile of the int8-MFMA
- *                  matrix count (auto = 256 when both dimensions reach 256)
+ *   "mfma_tile"    workgroup tile of the int8-MFMA matrix count: 0 auto
+ *                  (= 128), 128, 256, or 257 (256 with the unpack of the
+ *                  next K sub-stage software-pipelined into the MFMA loop)
  * Unknown keys and out-of-range values return TFIM_E_INVALID. */
 int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
 

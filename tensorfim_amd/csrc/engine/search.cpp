@@ -38,28 +38,43 @@ class WaveBatch {
     }
     int rows() const { return (int)item_off_.size() - 1; }
     int64_t cols_total() const { return (int64_t)col_ids_.size(); }
-    int run(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, std::string &err) {
+    // Launches the wave asynchronously (tfim_count_nodes_submit); wait()
+    // makes its counts available.  A wave with no columns submits nothing.
+    int submit(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, std::string &err) {
         out_.resize(std::max<size_t>(1, col_ids_.size()));
         st.nodes += rows();
+        ticket_ = -1;
         if (col_ids_.empty()) return TFIM_OK;   // nothing to count (e.g. a leaf holding every survivor)
         const double t0 = now();
-        const int rc = tfim_count_nodes(ctx, item_ids_.data(), item_off_.data(), rows(),
-                                        col_ids_.empty() ? nullptr : col_ids_.data(),
-                                        col_ids_.empty() ? nullptr : col_off_.data(), out_.data(), TFIM_PATH_AUTO);
+        const int rc = tfim_count_nodes_submit(ctx, item_ids_.data(), item_off_.data(), rows(), col_ids_.data(),
+                                               col_off_.data(), TFIM_PATH_AUTO, &ticket_);
         st.count_seconds += now() - t0;
         if (rc != TFIM_OK) {
-            err = std::string("tfim_count_nodes: ") + tfim_last_error(ctx);
+            err = std::string("tfim_count_nodes_submit: ") + tfim_last_error(ctx);
             return rc;
         }
         st.waves++;
         st.counted_bitops += (double)cols_total() * (double)n_bits;
         return TFIM_OK;
     }
+    int wait(tfim_ctx *ctx, SearchStats &st, std::string &err) {
+        const double t0 = now();
+        const int rc = tfim_count_nodes_wait(ctx, ticket_, out_.data());
+        st.count_seconds += now() - t0;
+        ticket_ = -1;
+        if (rc != TFIM_OK) err = std::string("tfim_count_nodes_wait: ") + tfim_last_error(ctx);
+        return rc;
+    }
+    int run(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, std::string &err) {
+        if (int rc = submit(ctx, st, n_bits, err)) return rc;
+        return wait(ctx, st, err);
+    }
     const int32_t *result(int r) const { return out_.data() + col_off_[r]; }
     int ncols(int r) const { return col_off_[r + 1] - col_off_[r]; }
 
    private:
     std::vector<int32_t> item_ids_, item_off_{0}, col_ids_, col_off_{0}, out_;
+    int64_t ticket_ = -1;
 };
 
 bool frequent(const Database &db, int32_t s) { return !((double)s < db.thr_abs); }
@@ -201,22 +216,39 @@ int run_wave(tfim_ctx *ctx, const Database &db, int32_t cap, std::vector<Mfi> &o
     std::vector<RefNode> stack = ref_roots(db, false);
     std::vector<int32_t> all(db.n_surv);
     std::iota(all.begin(), all.end(), 0);
-    WaveBatch wb;
-    std::vector<RefNode> wave;
     const int64_t budget = out_budget(db);
-    while (!stack.empty()) {
-        wave.clear();
+    // two waves in flight: the next one is counted while this one expands
+    struct Flight {
+        std::vector<RefNode> wave;
+        WaveBatch wb;
+        bool live = false;
+    } fl[2];
+    auto launch = [&](Flight &f) -> int {
+        f.wave.clear();
         int64_t outs = 0;
-        while ((int)wave.size() < cap && !stack.empty() && (wave.empty() || outs + db.n_surv <= budget)) {
-            wave.push_back(std::move(stack.back()));
+        while ((int)f.wave.size() < cap && !stack.empty() && (f.wave.empty() || outs + db.n_surv <= budget)) {
+            f.wave.push_back(std::move(stack.back()));
             stack.pop_back();
             outs += db.n_surv;
         }
-        wb.clear();
-        for (auto &nd : wave) wb.add(nd.items.data(), (int)nd.items.size(), all.data(), db.n_surv);
-        if (int rc = wb.run(ctx, st, db.n_bits(), err)) return rc;
-        st.ref_rows += (int64_t)wave.size();
-        for (size_t i = 0; i < wave.size(); i++) ref_expand(db, wave[i], wb.result((int)i), stack, out);
+        f.live = !f.wave.empty();
+        if (!f.live) return TFIM_OK;
+        f.wb.clear();
+        for (auto &nd : f.wave) f.wb.add(nd.items.data(), (int)nd.items.size(), all.data(), db.n_surv);
+        st.ref_rows += (int64_t)f.wave.size();
+        return f.wb.submit(ctx, st, db.n_bits(), err);
+    };
+    int cur = 0;
+    if (int rc = launch(fl[cur])) return rc;
+    while (fl[cur].live) {
+        if (int rc = launch(fl[1 - cur])) return rc;
+        Flight &f = fl[cur];
+        if (int rc = f.wb.wait(ctx, st, err)) return rc;
+        for (size_t i = 0; i < f.wave.size(); i++) ref_expand(db, f.wave[i], f.wb.result((int)i), stack, out);
+        f.live = false;
+        cur = 1 - cur;
+        if (!fl[cur].live)
+            if (int rc = launch(fl[cur])) return rc;
     }
     return TFIM_OK;
 }
@@ -323,24 +355,45 @@ class Mafia {
             root.lookahead = false;   // the empty head has no HUT row
             push_children(root);
         }
-        const int64_t budget = out_budget(db_);
-        while (!stack_.empty()) {
-            wave_.clear();
-            int64_t outs = 0;
-            while ((int)wave_.size() < cap_ && !stack_.empty()) {
-                const int64_t o = row_outputs(stack_.back());
-                if (!wave_.empty() && outs + o > budget) break;
-                outs += o;
-                wave_.push_back(std::move(stack_.back()));
-                stack_.pop_back();
-            }
-            if (int rc = count_wave(err)) return rc;
-            if (int rc = process_wave(out)) return rc;
+        budget_ = out_budget(db_);
+        // two waves in flight: the next one is counted while this one is processed
+        int cur = 0;
+        if (int rc = launch(fl_[cur], err)) return rc;
+        while (fl_[cur].live) {
+            if (int rc = launch(fl_[1 - cur], err)) return rc;
+            Flight &f = fl_[cur];
+            if (int rc = f.wb.wait(ctx_, st_, err)) return rc;
+            if (int rc = process_wave(f, out)) return rc;
+            f.live = false;
+            cur = 1 - cur;
+            if (!fl_[cur].live)
+                if (int rc = launch(fl_[cur], err)) return rc;
         }
         return TFIM_OK;
     }
 
    private:
+    struct Flight {
+        std::vector<MNode> wave;
+        WaveBatch wb;
+        bool live = false;
+    };
+
+    int launch(Flight &f, std::string &err) {
+        f.wave.clear();
+        int64_t outs = 0;
+        while ((int)f.wave.size() < cap_ && !stack_.empty()) {
+            const int64_t o = row_outputs(stack_.back());
+            if (!f.wave.empty() && outs + o > budget_) break;
+            outs += o;
+            f.wave.push_back(std::move(stack_.back()));
+            stack_.pop_back();
+        }
+        f.live = !f.wave.empty();
+        if (!f.live) return TFIM_OK;
+        return count_wave(f, err);
+    }
+
     int64_t row_outputs(const MNode &n) const {
         switch (n.kind) {
             case MNode::EXPAND: return (int64_t)n.tail.size();
@@ -359,10 +412,11 @@ class Mafia {
             if (!mark_[i]) cols.push_back(i);
     }
 
-    int count_wave(std::string &err) {
+    int count_wave(Flight &f, std::string &err) {
+        WaveBatch &wb_ = f.wb;
         wb_.clear();
         std::vector<int32_t> cols;
-        for (auto &n : wave_) {
+        for (auto &n : f.wave) {
             switch (n.kind) {
                 case MNode::EXPAND:
                     wb_.add(n.head.data(), (int)n.head.size(), n.tail.data(), (int)n.tail.size());
@@ -389,7 +443,7 @@ class Mafia {
                     break;
             }
         }
-        return wb_.run(ctx_, st_, db_.n_bits(), err);
+        return wb_.submit(ctx_, st_, db_.n_bits(), err);
     }
 
     void emit(std::vector<int32_t> items, int32_t sup, std::vector<Mfi> &out) {
@@ -427,7 +481,9 @@ class Mafia {
         }
     }
 
-    int process_wave(std::vector<Mfi> &out) {
+    int process_wave(Flight &f, std::vector<Mfi> &out) {
+        std::vector<MNode> &wave_ = f.wave;
+        WaveBatch &wb_ = f.wb;
         std::vector<Decision> decisions;
         decisions.reserve(wave_.size());
         for (int r = 0; r < (int)wave_.size(); r++) {
@@ -560,11 +616,12 @@ class Mafia {
     SearchStats &st_;
     MfiIndex index_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
-    std::vector<MNode> stack_, wave_, pending_;
+    std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_;
     std::vector<uint8_t> mark_;
-    WaveBatch wb_;
+    Flight fl_[2];
     int32_t cap_ = 256;
+    int64_t budget_ = INT64_MAX;
 };
 
 }  // namespace

@@ -247,8 +247,135 @@ __global__ __launch_bounds__(256) void k_popmat_mfma256(const uint64_t *__restri
             }
 }
 
+// Software-pipelined 256x256 variant: while a wave runs the 64 MFMAs of
+// sub-stage g from LDS buffer g%2, it unpacks sub-stage g+1 (2 of the 8 slots
+// of its A row and of its B row per K=32 step) into the other buffer, so the
+// VALU/DS work of the unpack sits in the MFMA shadow of the same wave (one
+// wave per SIMD at 256 accumulators).  One barrier per sub-stage orders the
+// buffer swap; the raw 1024-bit blocks are prefetched one block ahead.
+__device__ __forceinline__ void unpack_pair(uint4 *dst, int row, int j, const uint4 &raw) {
+    const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+    dst[j ^ (row & 7)] = unpack16(w[j >> 1] >> ((j & 1) * 16));
+}
+
+__global__ __launch_bounds__(256) void k_popmat_mfma256p(const uint64_t *__restrict__ A,
+                                                         const uint64_t *__restrict__ B,
+                                                         int32_t *__restrict__ C, int64_t b, int64_t d,
+                                                         int64_t pitch, int64_t k_per_split, int atomic,
+                                                         int tiles_n) {
+    extern __shared__ uint4 sm[];
+    uint4 *As = sm;                              // [2][256][8]
+    uint4 *Bs = sm + 2 * M2_T * M2_SLOTS;        // [2][256][8]
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
+    const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
+    const int64_t row0 = (lin / tiles_n) * M2_T, col0 = (lin % tiles_n) * M2_T;
+    const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
+    const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
+    const bool a_ok = row0 + tid < b, b_ok = col0 + tid < d;
+    const uint4 *Arow = reinterpret_cast<const uint4 *>(A + (row0 + tid) * pitch);
+    const uint4 *Brow = reinterpret_cast<const uint4 *>(B + (col0 + tid) * pitch);
+    const uint4 z4 = make_uint4(0, 0, 0, 0);
+    const int wr = wave >> 1, wc = wave & 1;
+    v16i acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[i][j][q] = 0;
+    const int64_t nsub = (kend - kbeg) / 2;   // 128-bit sub-stages (uint4 per row)
+    if (nsub > 0) {
+        uint4 ra[8], rb[8], na[8], nb[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            ra[u] = a_ok ? Arow[(kbeg >> 1) + u] : z4;
+            rb[u] = b_ok ? Brow[(kbeg >> 1) + u] : z4;
+            na[u] = z4;
+            nb[u] = z4;
+        }
+        {   // prologue: sub-stage 0 into buffer 0
+            uint4 *as = As + tid * M2_SLOTS, *bs = Bs + tid * M2_SLOTS;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                unpack_pair(as, tid, j, ra[0]);
+                unpack_pair(bs, tid, j, rb[0]);
+            }
+        }
+        __syncthreads();
+        for (int64_t g0 = 0; g0 < nsub; g0 += 8) {   // one 1024-bit block per iteration
+            const bool more = g0 + 8 < nsub;
+            const int64_t nxt = (kbeg >> 1) + g0 + 8;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                na[u] = (more && a_ok) ? Arow[nxt + u] : z4;
+                nb[u] = (more && b_ok) ? Brow[nxt + u] : z4;
+            }
+#pragma unroll
+            for (int sub = 0; sub < 8; sub++) {
+                const int buf = sub & 1;   // g0 is a multiple of 8: parity of g is parity of sub
+                const uint4 *ab = As + buf * (M2_T * M2_SLOTS);
+                const uint4 *bb = Bs + buf * (M2_T * M2_SLOTS);
+                uint4 *an = As + (buf ^ 1) * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
+                uint4 *bn = Bs + (buf ^ 1) * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
+                const bool has_next = sub < 7 || more;
+                const uint4 &rna = sub < 7 ? ra[sub + 1] : na[0];
+                const uint4 &rnb = sub < 7 ? rb[sub + 1] : nb[0];
+#pragma unroll
+                for (int kk = 0; kk < 4; kk++) {
+                    const int slot = kk * 2 + (lane >> 5);
+                    v4i af[4], bf[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int r = wr * 128 + i * 32 + (lane & 31);
+                        const uint4 x = ab[r * M2_SLOTS + (slot ^ (r & 7))];
+                        af[i] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int c = wc * 128 + j * 32 + (lane & 31);
+                        const uint4 x = bb[c * M2_SLOTS + (slot ^ (c & 7))];
+                        bf[j] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    if (has_next) {
+                        unpack_pair(an, tid, 2 * kk, rna);
+                        unpack_pair(an, tid, 2 * kk + 1, rna);
+                        unpack_pair(bn, tid, 2 * kk, rnb);
+                        unpack_pair(bn, tid, 2 * kk + 1, rnb);
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                ra[u] = na[u];
+                rb[u] = nb[u];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int64_t r = row0 + wr * 128 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
+                const int64_t c = col0 + wc * 128 + j * 32 + (lane & 31);
+                if (r < b && c < d) {
+                    if (atomic) atomicAdd(&C[r * d + c], acc[i][j][q]);
+                    else C[r * d + c] = acc[i][j][q];
+                }
+            }
+}
+
 void launch_popmat_mfma256(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                           int64_t pitch_words, int split_k, hipStream_t s) {
+                           int64_t pitch_words, int split_k, hipStream_t s, bool pipelined) {
     if (split_k < 1) split_k = 1;
     int64_t kps = (pitch_words + split_k - 1) / split_k;
     kps = (kps + 15) / 16 * 16;
@@ -258,11 +385,16 @@ void launch_popmat_mfma256(const uint64_t *A, const uint64_t *B, int32_t *C, int
     static bool attr = false;
     if (!attr) {
         hipFuncSetAttribute((const void *)k_popmat_mfma256, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipFuncSetAttribute((const void *)k_popmat_mfma256p, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
     dim3 grid((unsigned)(tm * tn), (unsigned)zs);
-    hipLaunchKernelGGL(k_popmat_mfma256, grid, dim3(256), lds, s, A, B, C, b, d, pitch_words, kps,
-                       zs > 1 ? 1 : 0, (int)tn);
+    if (pipelined)
+        hipLaunchKernelGGL(k_popmat_mfma256p, grid, dim3(256), lds, s, A, B, C, b, d, pitch_words, kps,
+                           zs > 1 ? 1 : 0, (int)tn);
+    else
+        hipLaunchKernelGGL(k_popmat_mfma256, grid, dim3(256), lds, s, A, B, C, b, d, pitch_words, kps,
+                           zs > 1 ? 1 : 0, (int)tn);
 }
 
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,

@@ -78,7 +78,7 @@ void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int6
                           int64_t pitch_words, int split_k, hipStream_t s);
 // Matrix AND-popcount, int8 MFMA path, 256x256 tiles (large shapes).
 void launch_popmat_mfma256(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                           int64_t pitch_words, int split_k, hipStream_t s);
+                           int64_t pitch_words, int split_k, hipStream_t s, bool pipelined);
 // Matrix AND-popcount, int8 MFMA path.  C must be zeroed when split_k>1.
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                         int64_t pitch_words, int split_k, hipStream_t s);

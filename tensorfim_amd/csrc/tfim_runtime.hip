@@ -41,11 +41,21 @@ struct tfim_ctx {
     int32_t *h_node_items = nullptr, *h_node_out = nullptr;   // pinned
     int64_t node_items_cap = 0, node_out_cap = 0;
 
-    // wave scratch: one descriptor block + one output block
-    char *d_desc = nullptr, *h_desc = nullptr;   // h_desc pinned
-    int64_t desc_cap = 0;
-    int32_t *d_wave_out = nullptr, *h_wave_out = nullptr;
-    int64_t wave_out_cap = 0;
+    // wave scratch: two in-flight slots, each a descriptor block and an
+    // output block (device + pinned host) and its events
+    struct Slot {
+        char *d_desc = nullptr, *h_desc = nullptr;
+        int64_t desc_cap = 0;
+        int32_t *d_out = nullptr, *h_out = nullptr;
+        int64_t out_cap = 0;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+        bool busy = false;
+        int64_t ticket = -1, n_out = 0, n_rows = 0;
+        int tuned = -1;
+        double work = 0;
+    };
+    Slot slots[2];
+    int64_t next_ticket = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     int64_t wave_launches = 0, wave_graph_replays = 0, wave_rows = 0, wave_lds_launches = 0;
@@ -131,6 +141,9 @@ int tfim_open(int device, tfim_ctx **out) {
     c->clock_mhz = p.clockRate / 1000;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TFIM_E_HIP; }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return TFIM_E_HIP; }
+    for (auto &sl : c->slots)
+        if (hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) { delete c; return TFIM_E_HIP; }
     *out = c;
     return TFIM_OK;
 }
@@ -142,8 +155,13 @@ void tfim_close(tfim_ctx *c) {
     if (c->owns_bm) hipFree(c->d_bm);
     hipFree(c->d_node_items); hipFree(c->d_node_out);
     hipHostFree(c->h_node_items); hipHostFree(c->h_node_out);
-    hipFree(c->d_desc); hipHostFree(c->h_desc);
-    hipFree(c->d_wave_out); hipHostFree(c->h_wave_out);
+    for (auto &sl : c->slots) {
+        hipFree(sl.d_desc); hipHostFree(sl.h_desc);
+        hipFree(sl.d_out); hipHostFree(sl.h_out);
+        if (sl.ev0) hipEventDestroy(sl.ev0);
+        if (sl.ev1) hipEventDestroy(sl.ev1);
+        if (sl.done) hipEventDestroy(sl.done);
+    }
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -282,15 +300,18 @@ int tfim_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t *out) 
     return TFIM_OK;
 }
 
-int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
-                     const int32_t *col_ids, const int32_t *col_off, int32_t *out, int32_t path) {
-    if (!c || !item_ids || !item_off || !out || n_rows < 0)
+int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
+                            const int32_t *col_ids, const int32_t *col_off, int32_t path, int64_t *ticket) {
+    if (!c || !item_ids || !item_off || !ticket || n_rows < 0)
         return fail(c, TFIM_E_INVALID, "count_nodes: null argument");
+    *ticket = -1;
     if (col_ids && !col_off) return fail(c, TFIM_E_INVALID, "count_nodes: col_ids without col_off");
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_nodes: no bitmaps loaded");
     if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED)
         return fail(c, TFIM_E_INVALID, "count_nodes: path %d not available for node waves", path);
     if (n_rows == 0) return TFIM_OK;
+    tfim_ctx::Slot &sl = c->slots[c->next_ticket & 1];
+    if (sl.busy) return fail(c, TFIM_E_STATE, "count_nodes_submit: both slots in flight; wait on a ticket first");
     if (item_off[0] != 0) return fail(c, TFIM_E_INVALID, "count_nodes: item_off[0] != 0");
     for (int32_t r = 0; r < n_rows; r++)
         if (item_off[r + 1] <= item_off[r]) return fail(c, TFIM_E_INVALID, "count_nodes: row %d has no prefix item", r);
@@ -321,10 +342,7 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
         const int32_t nc = col_ids ? col_off[r + 1] - col_off[r] : c->n_cols;
         for (int32_t b = 0; b < nc; b += c->tile_cols) tiles.push_back({r, b, std::min(nc, b + c->tile_cols), 0});
     }
-    if (tiles.empty()) {
-        memset(out, 0, (size_t)n_out * 4);
-        return TFIM_OK;
-    }
+    if (tiles.empty()) return TFIM_OK;   // only empty tails: n_out == 0, handled above
 
     // work of this wave in row-words: sum over rows of (prefix + columns) x pitch
     const double work = ((double)n_item_ids + (double)n_out) * (double)c->pitch;
@@ -381,9 +399,9 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
     const bool need_cols = col_ids != nullptr || lds;
     const int64_t o_tiles = align16(o_cids + (need_cols ? n_out * 4 : 0));
     const int64_t total = o_tiles + (int64_t)tiles.size() * (int64_t)sizeof(WaveTile);
-    if (int r = ensure_pair(c, &c->d_desc, &c->h_desc, &c->desc_cap, total)) return r;
-    if (int r = ensure_pair(c, &c->d_wave_out, &c->h_wave_out, &c->wave_out_cap, n_out)) return r;
-    char *h = c->h_desc;
+    if (int r = ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, total)) return r;
+    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, n_out)) return r;
+    char *h = sl.h_desc;
     int32_t *hi = reinterpret_cast<int32_t *>(h + o_items);
     int32_t *hc = reinterpret_cast<int32_t *>(h + o_cids);
     if (lds) {
@@ -407,19 +425,19 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
         for (int32_t r = 0; r <= n_rows; r++) coff[r] = r * c->n_cols;
     }
     memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
-    HIPCHK(c, hipMemcpyAsync(c->d_desc, c->h_desc, (size_t)total, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sl.d_desc, sl.h_desc, (size_t)total, hipMemcpyHostToDevice, c->stream));
 
-    launch_zero_i32(c->d_wave_out, n_out, c->stream);
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    launch_zero_i32(sl.d_out, n_out, c->stream);
+    HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
     if (lds) {
         WaveLdsDesc d;
-        d.slot_rows = reinterpret_cast<const int32_t *>(c->d_desc + o_slots);
-        d.item_slots = reinterpret_cast<const int32_t *>(c->d_desc + o_items);
-        d.item_off = reinterpret_cast<const int32_t *>(c->d_desc + o_ioff);
-        d.col_slots = reinterpret_cast<const int32_t *>(c->d_desc + o_cids);
-        d.col_off = reinterpret_cast<const int32_t *>(c->d_desc + o_coff);
-        d.tiles = reinterpret_cast<const WaveTile *>(c->d_desc + o_tiles);
-        d.out = c->d_wave_out;
+        d.slot_rows = reinterpret_cast<const int32_t *>(sl.d_desc + o_slots);
+        d.item_slots = reinterpret_cast<const int32_t *>(sl.d_desc + o_items);
+        d.item_off = reinterpret_cast<const int32_t *>(sl.d_desc + o_ioff);
+        d.col_slots = reinterpret_cast<const int32_t *>(sl.d_desc + o_cids);
+        d.col_off = reinterpret_cast<const int32_t *>(sl.d_desc + o_coff);
+        d.tiles = reinterpret_cast<const WaveTile *>(sl.d_desc + o_tiles);
+        d.out = sl.d_out;
         d.n_slots = (int32_t)slot_rows.size();
         d.n_tiles = (int32_t)tiles.size();
         d.n_out = (int32_t)n_out;
@@ -428,12 +446,12 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
         c->wave_lds_launches++;
     } else {
         WaveDesc d;
-        d.item_ids = reinterpret_cast<const int32_t *>(c->d_desc + o_items);
-        d.item_off = reinterpret_cast<const int32_t *>(c->d_desc + o_ioff);
-        d.col_off = reinterpret_cast<const int32_t *>(c->d_desc + o_coff);
-        d.col_ids = col_ids ? reinterpret_cast<const int32_t *>(c->d_desc + o_cids) : nullptr;
-        d.tiles = reinterpret_cast<const WaveTile *>(c->d_desc + o_tiles);
-        d.out = c->d_wave_out;
+        d.item_ids = reinterpret_cast<const int32_t *>(sl.d_desc + o_items);
+        d.item_off = reinterpret_cast<const int32_t *>(sl.d_desc + o_ioff);
+        d.col_off = reinterpret_cast<const int32_t *>(sl.d_desc + o_coff);
+        d.col_ids = col_ids ? reinterpret_cast<const int32_t *>(sl.d_desc + o_cids) : nullptr;
+        d.tiles = reinterpret_cast<const WaveTile *>(sl.d_desc + o_tiles);
+        d.out = sl.d_out;
         d.n_tiles = (int32_t)tiles.size();
         const int rw = c->pitch >= 2048 ? 4 : (c->pitch >= 512 ? 2 : 1);
         const int64_t cw = wave_chunk_words(rw);
@@ -445,22 +463,49 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
         launch_count_wave(c->d_bm, c->pitch, d, (int)blocks, rw, c->stream);
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->h_wave_out, c->d_wave_out, (size_t)n_out * 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sl.h_out, sl.d_out, (size_t)n_out * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(sl.done, c->stream));
+    sl.busy = true;
+    sl.ticket = c->next_ticket++;
+    sl.n_out = n_out;
+    sl.n_rows = n_rows;
+    sl.tuned = tuned;
+    sl.work = work;
+    *ticket = sl.ticket;
+    return TFIM_OK;
+}
+
+int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
+    if (!c) return TFIM_E_INVALID;
+    if (ticket < 0) return TFIM_OK;   // nothing was submitted (no rows or no columns)
+    tfim_ctx::Slot &sl = c->slots[ticket & 1];
+    if (!sl.busy || sl.ticket != ticket) return fail(c, TFIM_E_STATE, "count_nodes_wait: unknown ticket %lld", (long long)ticket);
+    if (!out) return fail(c, TFIM_E_INVALID, "count_nodes_wait: null output");
+    if (int r = set_device(c)) return r;
+    sl.busy = false;
+    HIPCHK(c, hipEventSynchronize(sl.done));
     float ms = 0;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&ms, sl.ev0, sl.ev1);
     c->wave_ms += ms;
     c->wave_launches++;
-    c->wave_rows += n_rows;
-    if (tuned >= 0 && work > 0) {
-        const double per = (double)ms / work;
-        double &cst = c->tune_cost[tuned];
-        cst = c->tune_samples[tuned] == 0 ? per : 0.8 * cst + 0.2 * per;
-        c->tune_samples[tuned]++;
+    c->wave_rows += sl.n_rows;
+    if (sl.tuned >= 0 && sl.work > 0) {
+        const double per = (double)ms / sl.work;
+        double &cst = c->tune_cost[sl.tuned];
+        cst = c->tune_samples[sl.tuned] == 0 ? per : 0.8 * cst + 0.2 * per;
+        c->tune_samples[sl.tuned]++;
     }
-    memcpy(out, c->h_wave_out, (size_t)n_out * 4);
+    memcpy(out, sl.h_out, (size_t)sl.n_out * 4);
     return TFIM_OK;
+}
+
+int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
+                     const int32_t *col_ids, const int32_t *col_off, int32_t *out, int32_t path) {
+    if (!out) return fail(c, TFIM_E_INVALID, "count_nodes: null output");
+    int64_t ticket = -1;
+    if (int r = tfim_count_nodes_submit(c, item_ids, item_off, n_rows, col_ids, col_off, path, &ticket)) return r;
+    return tfim_count_nodes_wait(c, ticket, out);
 }
 
 int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
@@ -470,7 +515,7 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
         c->wave_kernel = (int)value;
         return TFIM_OK;
     }
-    if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256)) {
+    if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256 || value == 257)) {
         c->mfma_tile = (int)value;
         return TFIM_OK;
     }
@@ -563,8 +608,8 @@ static void popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfi
     if (path == TFIM_PATH_PACKED) {
         launch_popmat_packed(a, bb, cc, b, d, pitch, split_k, c->stream);
     } else {
-        const int tile = c->mfma_tile ? c->mfma_tile : (b >= 256 && d >= 256 ? 256 : 128);
-        if (tile == 256) launch_popmat_mfma256(a, bb, cc, b, d, pitch, split_k, c->stream);
+        const int tile = c->mfma_tile ? c->mfma_tile : 128;
+        if (tile == 256 || tile == 257) launch_popmat_mfma256(a, bb, cc, b, d, pitch, split_k, c->stream, tile == 257);
         else launch_popmat_mfma(a, bb, cc, b, d, pitch, split_k, c->stream);
     }
 }

@@ -1,0 +1,39 @@
+// TEST INFRASTRUCTURE: a CPU stand-in for libtfim.so, linked only into the
+// test build of the engine (tests/test_engine_host_logic.py) so the search
+// logic can be checked where there is no GPU.  The product never loads it.
+#include "tfim.h"
+#include <cstring>
+#include <vector>
+#include <string>
+#include <cstdlib>
+struct tfim_ctx { std::vector<uint64_t> bm; int64_t W=0, n_bits=0; int32_t n_items=0, n_cols=0; std::string err; };
+extern "C" {
+int tfim_open(int, tfim_ctx **o) { *o = new tfim_ctx(); return 0; }
+void tfim_close(tfim_ctx *c) { delete c; }
+int tfim_open_shared(tfim_ctx *p, tfim_ctx **o) { *o = new tfim_ctx(*p); return 0; }
+const char *tfim_last_error(const tfim_ctx *c) { return c->err.c_str(); }
+int tfim_device_info(tfim_ctx *, char *n, int l, int32_t *cu, int64_t *h, int32_t *clk) { snprintf(n, l, "mock"); if(cu)*cu=0; if(h)*h=0; if(clk)*clk=0; return 0; }
+int tfim_config(tfim_ctx *, const char *, int64_t) { return 0; }
+int tfim_load_bitmaps(tfim_ctx *c, const uint64_t *rows, int32_t n, int64_t nb, int64_t w, int32_t nc) {
+  c->W = (nb+63)/64; c->bm.assign((size_t)n*c->W, 0); for (int i=0;i<n;i++) memcpy(&c->bm[(size_t)i*c->W], rows+(size_t)i*w, c->W*8);
+  c->n_bits=nb; c->n_items=n; c->n_cols=nc; return 0; }
+int tfim_load_bitmaps_replicated(tfim_ctx *c, const uint64_t *rows, int32_t n, int64_t nb, int64_t w, int32_t nc, int32_t) { return tfim_load_bitmaps(c, rows, n, nb, w, nc); }
+static void cnt(tfim_ctx *c, const int32_t *it, int k, const int32_t *cols, int nc, int32_t *out) {
+  std::vector<uint64_t> m(c->bm.begin()+(size_t)it[0]*c->W, c->bm.begin()+(size_t)(it[0]+1)*c->W);
+  for (int i=1;i<k;i++) for (int64_t w=0;w<c->W;w++) m[w]&=c->bm[(size_t)it[i]*c->W+w];
+  for (int j=0;j<nc;j++){ int col=cols?cols[j]:j; const uint64_t*r=&c->bm[(size_t)col*c->W]; int64_t s=0; for(int64_t w=0;w<c->W;w++) s+=__builtin_popcountll(m[w]&r[w]); out[j]=(int32_t)s; } }
+int tfim_count_node(tfim_ctx *c, const int32_t *it, int32_t k, int32_t *out) { cnt(c,it,k,nullptr,c->n_cols,out); return 0; }
+int tfim_count_nodes(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co, int32_t *out, int32_t) {
+  for (int r=0;r<n;r++){ if(cids) cnt(c,ids+io[r],io[r+1]-io[r],cids+co[r],co[r+1]-co[r],out+co[r]); else cnt(c,ids+io[r],io[r+1]-io[r],nullptr,c->n_cols,out+(size_t)r*c->n_cols);} return 0; }
+static std::vector<int32_t> g_pending[2]; static int64_t g_next=0;
+int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co, int32_t p, int64_t *t) {
+  int64_t tk=g_next++; auto &v=g_pending[tk&1]; v.assign(cids?co[n]:(size_t)n*c->n_cols,0); tfim_count_nodes(c,ids,io,n,cids,co,v.data(),p); *t=tk; return 0; }
+int tfim_count_nodes_wait(tfim_ctx *, int64_t t, int32_t *out) { if(t<0) return 0; auto &v=g_pending[t&1]; memcpy(out,v.data(),v.size()*4); return 0; }
+int tfim_wave_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *r, double *ms) { if(a)*a=0; if(b)*b=0; if(r)*r=0; if(ms)*ms=0; return 0; }
+int tfim_mlp_create(tfim_ctx *, uint64_t, tfim_mlp **) { return -2; }
+void tfim_mlp_destroy(tfim_mlp *) {}
+int tfim_mlp_get_params(tfim_mlp *, float *) { return -2; }
+int tfim_mlp_set_params(tfim_mlp *, const float *) { return -2; }
+int tfim_mlp_forward(tfim_mlp *, const float *, int32_t, float *, int32_t) { return -2; }
+int tfim_mlp_train_step(tfim_mlp *, const float *, const float *, const float *, int32_t, float, float *) { return -2; }
+}

@@ -1,0 +1,66 @@
+"""CPU: the engine's search logic (plain, wave and MAFIA engines, policies,
+quirks) against the oracle, with the counting library replaced by a CPU
+stand-in (tests/engine_mock/mock_tfim.cpp) so it runs without a GPU.  The
+GPU tests run the same engines on the real library."""
+import os
+import random
+import subprocess
+
+import pytest
+
+import oracle_lib
+import tensorfim_amd as tf
+from conftest import GOLD_PAIRS, gold_path
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ENG = os.path.join(ROOT, "tensorfim_amd", "csrc", "engine")
+
+
+@pytest.fixture(scope="module")
+def mock_engine(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("mock") / "tfim_mine_mock")
+    srcs = [os.path.join(ENG, f) for f in ("dataset.cpp", "results.cpp", "search.cpp", "policy.cpp", "main_mine.cpp")]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + ENG] + srcs +
+                   [os.path.join(ROOT, "tests", "engine_mock", "mock_tfim.cpp"), "-o", out, "-lpthread"], check=True)
+    return out
+
+
+ENGINES = [["--engine", "mafia"], ["--engine", "mafia", "--gate", "always"], ["--engine", "wave"],
+           ["--engine", "plain", "--workers", "3"]]
+
+
+@pytest.mark.parametrize("ds,thr", GOLD_PAIRS[:5])
+def test_engines_reproduce_golds(mock_engine, fimi_dir, tmp_path, ds, thr):
+    for eng in ENGINES[:3]:
+        out = tmp_path / "m.res"
+        subprocess.run([mock_engine, str(fimi_dir / ds), repr(thr), "--out", str(out)] + eng, check=True,
+                       capture_output=True)
+        assert tf.load_results(str(out)) == tf.load_results(gold_path(ds, thr)), eng
+
+
+def test_engines_match_oracle_on_random_databases(mock_engine, tmp_path):
+    """Random small databases with frequent item 0 (its own enumeration rule),
+    repeated ids, empty lines and exact-threshold items."""
+    rnd = random.Random(7)
+    for it in range(80):
+        n_items = rnd.randint(3, 11)
+        rows = []
+        for _ in range(rnd.randint(4, 50)):
+            r = rnd.sample(range(0, n_items), rnd.randint(0, n_items))
+            if rnd.random() < 0.05 and r:
+                r.append(r[0])
+            rows.append(r)
+        txt = "\n".join(" ".join(map(str, r)) for r in rows) + ("\n" if rnd.random() < 0.5 else "")
+        if rnd.random() < 0.2:
+            txt += "\n\n"
+        db = tmp_path / "db.txt"
+        db.write_text(txt)
+        sup = rnd.choice([0.1, 0.2, 0.25, 0.3, 0.4, 0.5])
+        ref = tmp_path / "o.res"
+        oracle_lib.mine(db, sup, threads=2, out=ref)
+        exp = tf.load_results(str(ref))
+        for eng in ENGINES:
+            out = tmp_path / "m.res"
+            subprocess.run([mock_engine, str(db), repr(sup), "--out", str(out)] + eng, check=True,
+                           capture_output=True)
+            assert tf.load_results(str(out)) == exp, (it, eng, sup)

@@ -117,7 +117,7 @@ def test_fill_bernoulli_matches_oracle(ctx):
     buf.free()
 
 
-@pytest.mark.parametrize("path", ["packed", "mfma", "mfma128", "mfma256"])
+@pytest.mark.parametrize("path", ["packed", "mfma", "mfma128", "mfma256", "mfma257"])
 @pytest.mark.parametrize("b,d,n_bits,split", [(130, 70, 4096 + 65, 1), (256, 256, 65_536, 1),
                                                (200, 513, 40_000, 3), (64, 33, 2048 * 9, 8),
                                                (1, 1, 64, 1), (515, 300, 2048 * 3 + 1, 2)])
@@ -132,7 +132,7 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
     dA, dB, dC = ctx.buffer(A.nbytes), ctx.buffer(B.nbytes), ctx.buffer(b * d * 4)
     dA.write(A)
     dB.write(B)
-    tile = {"mfma128": 128, "mfma256": 256}.get(path, 0)
+    tile = {"mfma128": 128, "mfma256": 256, "mfma257": 257}.get(path, 0)
     ctx.config("mfma_tile", tile)
     try:
         ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path="packed" if path == "packed" else "mfma", split_k=split)
