@@ -1,17 +1,19 @@
 """Benchmark: exact support counting on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--paths mfma,packed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--no-cpu] [--paths mfma,packed]
+    python bench.py --copy-calibration MIB     (a plain device copy, for counter scaling)
 
-A step is one pass of the hot path over one batch: the isolated count of
-configuration 2, C[i][j] = popcount(A_i AND B_j) for 65,536 prefix bitmaps x
+A step is one pass of the hot path over one batch: the isolated count
+C[i][j] = popcount(A_i AND B_j) for 65,536 prefix bitmaps x
 512 item bitmaps over 2^22 transactions (i.i.d. Bernoulli(0.3) bits, seed 1),
 resident in HBM before timing.  `value` is support-count Tbit-op/s of the
 int8-MFMA path (bit-ops = rows x items x transactions).  Also reported in the
 same run: the packed AND+v_bcnt path on the same data, the CPU baseline (the
 oracle's popcount on a 512-row sample, host threads stated), and unless
---no-e2e the end-to-end mining time of configuration 4 (49,046-row base x256
-= 12,555,776 transactions, support 0.85) for the plain per-node GPU engine and
-the batched MAFIA engine with the learned policy, with their MFI sets compared.
+--no-e2e the end-to-end mining time of the dense replicated workload (49,046-row base x256
+= 12,555,776 transactions, support 0.85) for the plain per-node GPU engine
+(one and eight host workers) and the batched MAFIA engine with and without the
+learned policy, with their MFI sets compared.
 """
 import argparse
 import json
@@ -33,6 +35,15 @@ HBM_PEAK_GBS = 8000.0
 I8_PEAK_TOPS = 5000.0      # dense int8 MFMA, MI355X
 VALU_PEAK_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz int32 ops
 
+# HBM bytes per launch of the default int8-MFMA count kernel on this workload,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# `bench.py --no-e2e --no-cpu --paths mfma`, each scaled by the same counter's
+# reading for a device copy of known size (`bench.py --copy-calibration 1024`);
+# profiles/popmat_mfma_rp_65536x512x4M_traffic.md holds the readings.  None until
+# measured for the kernel below.
+TRAFFIC_KERNEL = "k_popmat_mfma_rp"
+TRAFFIC_BYTES = None
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -52,7 +63,7 @@ def cpu_baseline(A_rows, B_rows, n_bits, threads):
 def bench_isolated(ctx, args):
     b, d, n_bits = 65536, 512, 1 << 22
     pitch = tf.pitch_words(n_bits)
-    log("config 2: A %d x %d bits (%.1f GiB), B %d x %d bits" % (b, n_bits, b * pitch * 8 / 2 ** 30, d, n_bits))
+    log("isolated count: A %d x %d bits (%.1f GiB), B %d x %d bits" % (b, n_bits, b * pitch * 8 / 2 ** 30, d, n_bits))
     A = ctx.buffer(b * pitch * 8)
     B = ctx.buffer(d * pitch * 8)
     C = ctx.buffer(b * d * 4)
@@ -102,25 +113,45 @@ def run_engine(data, engine, extra, out):
 
 def bench_e2e(args):
     tmp = tempfile.mkdtemp(prefix="tfim_bench_")
-    data = os.path.join(tmp, "c4_base.txt")
-    subprocess.run([tf.binary("tfim_gen"), "4", data], check=True)
+    data = os.path.join(tmp, "dense_replicated_base.txt")
+    subprocess.run([tf.binary("tfim_gen"), "dense-replicated", data], check=True)
     out = {}
     sets = {}
-    for name, engine, extra in [("batched_learned", "mafia", ["--learned"]), ("plain", "plain", [])]:
-        log("config 4 e2e: %s ..." % name)
+    runs = [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
+            ("plain_w8", "plain", ["--workers", "8"]), ("plain", "plain", [])]
+    for name, engine, extra in runs:
+        log("dense replicated e2e: %s ..." % name)
         path, st, wall = run_engine(data, engine, extra, os.path.join(tmp, name + ".res"))
         sets[name] = tf.load_results(path)
         out[name] = {"wall_s": wall, "computation_s": float(st["computation_time(s)"]),
                      "rows": int(st["rows"]), "waves": int(st["waves"]), "mfis": int(st["MFIsNumber"]),
                      "kernel_ms": float(st.get("kernel_ms", 0)), "counted_bitops": float(st["counted_bitops"])}
-        log("  %s: %.2f s wall, %.2f s computation, %d rows" % (name, wall, out[name]["computation_s"],
-                                                               out[name]["rows"]))
-    out["identical_mfi_sets"] = sets["plain"] == sets["batched_learned"]
-    out["speedup_computation"] = out["plain"]["computation_s"] / out["batched_learned"]["computation_s"]
-    out["speedup_wall"] = out["plain"]["wall_s"] / out["batched_learned"]["wall_s"]
+        for k in ("hut_rows", "hut_success", "model_lookaheads", "mlp_train_steps", "graph_launches"):
+            if k in st:
+                out[name][k] = int(st[k])
+        log("  %s: %.2f s wall, %.3f s computation, %d rows" % (name, wall, out[name]["computation_s"],
+                                                                out[name]["rows"]))
+    out["identical_mfi_sets"] = all(sets[n] == sets["plain"] for n, _, _ in runs)
+    best_plain = min(out["plain"]["computation_s"], out["plain_w8"]["computation_s"])
+    out["speedup_computation_vs_plain_1worker"] = out["plain"]["computation_s"] / out["batched_learned"]["computation_s"]
+    out["speedup_computation_vs_best_plain"] = best_plain / out["batched_learned"]["computation_s"]
+    out["speedup_wall_vs_plain_1worker"] = out["plain"]["wall_s"] / out["batched_learned"]["wall_s"]
     if not out["identical_mfi_sets"]:
-        raise SystemExit("batched+learned MFI set differs from the plain engine")
+        raise SystemExit("a batched MFI set differs from the plain engine")
     return out
+
+
+def copy_calibration(mib):
+    """A plain device copy of `mib` MiB (torch), three times: the reference
+    point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * 2 ** 20 // 4
+    src = torch.ones(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copied %d MiB x3" % mib)
 
 
 def main():
@@ -131,12 +162,17 @@ def main():
     ap.add_argument("--no-e2e", action="store_true")
     ap.add_argument("--paths", default="mfma,packed")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     args.paths = [p for p in args.paths.split(",") if p]
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (transaction or subtree "
               "sharding is a design note); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     ctx = tf.Context(0)
     info = ctx.device_info()
     log("device:", info)
@@ -146,12 +182,14 @@ def main():
     head_path = "mfma" if "mfma" in res else args.paths[0]
 
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    log("cpu baseline: oracle popcount, %d threads, 512 rows x 512 items x 2^22 ..." % threads)
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib
-    rng_rows = np.repeat(A_sample, 103, axis=0)[:512]   # 512 device-generated rows (5 distinct)
-    _, cpu_tbit, cpu_s = cpu_baseline(rng_rows, B_host, n_bits, threads)
-    log("cpu: %.3f Tbit-op/s (%.1f s)" % (cpu_tbit, cpu_s))
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline: oracle popcount, %d threads, 512 rows x 512 items x 2^22 ..." % threads)
+        rng_rows = np.repeat(A_sample, 103, axis=0)[:512]   # 512 device-generated rows (5 distinct)
+        _, cpu_tbit, cpu_s = cpu_baseline(rng_rows, B_host, n_bits, threads)
+        log("cpu: %.3f Tbit-op/s (%.1f s)" % (cpu_tbit, cpu_s))
+        cpu = {"value": cpu_tbit, "unit": "Tbit-op/s", "cores": threads, "kind": "port",
+               "sample": "oracle AND-popcount of 512 rows x 512 items x 2^22 transactions (%.1f s)" % cpu_s}
 
     e2e = None if args.no_e2e else bench_e2e(args)
 
@@ -159,7 +197,8 @@ def main():
     if head_path == "mfma":
         achieved = 2.0 * bitops / kernel_s / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": I8_PEAK_TOPS, "unit": "TFLOP/s",
-                "frac": achieved / I8_PEAK_TOPS, "traffic": None}
+                "frac": achieved / I8_PEAK_TOPS, "traffic": TRAFFIC_BYTES, "kernel": TRAFFIC_KERNEL,
+                "algorithmic_bytes": alg_bytes}
     else:
         achieved = alg_bytes / kernel_s / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -178,17 +217,16 @@ def main():
         "dtype": "i8" if head_path == "mfma" else "u32",
         "data": "synthetic (Bernoulli(0.3) bits generated on the device, seed 1)",
         "config": {"workload": "isolated support count: 65,536 prefix bitmaps x 512 item bitmaps x 2^22 "
-                               "transactions, int8-MFMA path, split-k 8",
+                               "transactions, int8-MFMA path (256x128 tiles, operand bytes expanded "
+                               "in registers from packed bits in LDS), split-k 8",
                    "rows": 65536, "items": 512, "transactions": 1 << 22},
         "roofline": roof,
-        "cpu_baseline": {"value": cpu_tbit, "unit": "Tbit-op/s", "cores": threads, "kind": "port",
-                         "sample": "oracle AND-popcount of 512 rows x 512 items x 2^22 transactions "
-                                   "(%.1f s)" % cpu_s},
+        "cpu_baseline": cpu,
         "paths": res,
         "device": info,
     }
     if e2e is not None:
-        line["e2e_config4"] = e2e
+        line["e2e_dense_replicated"] = e2e
     print(json.dumps(line), flush=True)
 
 

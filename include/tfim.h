@@ -120,8 +120,13 @@ int tfim_count_nodes_wait(tfim_ctx *ctx, int64_t ticket, int32_t *out);
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel
  *   "tile_cols"    columns per work tile (8..64, default 32)
  *   "mfma_tile"    workgroup tile of the int8-MFMA matrix count: 0 auto
- *                  (= 128), 128, 256, or 257 (256 with the unpack of the
- *                  next K sub-stage software-pipelined into the MFMA loop)
+ *                  (= 256), 256 (256 x 128 outputs, operand bytes expanded
+ *                  in registers from packed bits in LDS) or 128 (128 x 128,
+ *                  bytes expanded into LDS)
+ *   "graphs"       1 (default): each wave is replayed from an instantiated
+ *                  HIP graph (descriptor copy, output fill, count kernel,
+ *                  output copy) whose node parameters are updated per wave;
+ *                  0: the same operations issued one by one
  * Unknown keys and out-of-range values return TFIM_E_INVALID. */
 int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
 
@@ -131,6 +136,11 @@ int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
  * keeps the cheaper per row-word of work (re-sampling every 64 waves). */
 int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,
                     int64_t *rows, double *kernel_ms);
+
+/* Batched-path graphs instantiated and graph launches since tfim_open.  The
+ * reference launches each wave directly (runWaveEngine, kernel_bmma.cu:896-910);
+ * no counterpart. */
+int tfim_wave_graph_stats(tfim_ctx *ctx, int64_t *graphs, int64_t *launches);
 
 /* ---- device buffers and the isolated count (microbench) ---------------- */
 
@@ -185,7 +195,9 @@ int tfim_mlp_forward(tfim_mlp *mlp, const float *x, int32_t n, float *logits, in
 /* Gradient of the loss w.r.t. all parameters (no update). */
 int tfim_mlp_gradients(tfim_mlp *mlp, const float *x, const float *labels, const float *mask, int32_t n,
                        float *grads, float *loss);
-/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch. */
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch.  The MLP
+ * works on a stream of its own; with loss == NULL the step is queued and the
+ * call returns once the inputs are copied (later MLP calls order after it). */
 int tfim_mlp_train_step(tfim_mlp *mlp, const float *x, const float *labels, const float *mask, int32_t n,
                         float lr, float *loss);
 

@@ -52,6 +52,7 @@ _SIGS = {
     "tfim_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "tfim_wave_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64),
                                         _p(ctypes.c_int64), _p(ctypes.c_double)]),
+    "tfim_wave_graph_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64)]),
     "tfim_buf_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _p(ctypes.c_void_p)]),
     "tfim_buf_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "tfim_buf_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
@@ -223,7 +224,10 @@ class Context:
         a, b, c, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
         self._check(self._lib.tfim_wave_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
                                               ctypes.byref(d)))
-        return {"launches": a.value, "lds_launches": b.value, "rows": c.value, "kernel_ms": d.value}
+        g, r = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.tfim_wave_graph_stats(self._h, ctypes.byref(g), ctypes.byref(r)))
+        return {"launches": a.value, "lds_launches": b.value, "rows": c.value, "kernel_ms": d.value,
+                "graphs": g.value, "graph_launches": r.value}
 
     def buffer(self, nbytes):
         return DeviceBuffer(self, nbytes)

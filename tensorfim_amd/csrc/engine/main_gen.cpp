@@ -1,30 +1,31 @@
-// tfim_gen -- deterministic synthetic transaction databases for the five
-// benchmark configurations, written in the reference's input format (one
-// transaction per line, ascending item ids >= 1, space separated).
+// tfim_gen -- deterministic synthetic transaction databases for the benchmark
+// workloads, written in the reference's input format (one transaction per
+// line, ascending item ids >= 1, space separated).
 //
-//   tfim_gen <config 1|3|4|5> <out.txt> [--rows N]
+//   tfim_gen <dense-small|clickstream|dense-replicated|gene> <out.txt> [--rows N]
 //
-// Every draw comes from one SplitMix64 stream seeded by the configuration
-// seed, so a file is a pure function of (config, rows).  Config 2 (the
-// isolated count) has no transaction file: its bitmaps are generated on the
-// device (tfim_fill_bernoulli).  Config 4 writes the 49,046-row base table;
-// the engine replicates it x256 (--replicate 256 = data/make_replica.py).
+// Every draw comes from one SplitMix64 stream seeded by the workload's seed,
+// so a file is a pure function of (workload, rows).  The isolated count
+// (65,536 x 512 bitmaps over 2^22 transactions) has no transaction file: its
+// bitmaps are generated on the device (tfim_fill_bernoulli).
+// dense-replicated writes the 49,046-row base table; the engine replicates it
+// x256 (--replicate 256 = data/make_replica.py).
 //
-// Config 1  3,200 x items 1..75, exactly 37 distinct items per transaction:
+// dense-small       3,200 x items 1..75, exactly 37 distinct items per transaction:
 //           6 blocks of 8 items (a seeded permutation of 1..75), visited in a
 //           random order, each added with p=0.7 when it still fits; the rest
 //           drawn without replacement with weight 1/(i+1)^0.8.  seed 0.
-// Config 3  990,000 transactions over 41,270 items: 1+Poisson(7) distinct
+// clickstream       990,000 transactions over 41,270 items: 1+Poisson(7) distinct
 //           items by Zipf(1.05) popularity (id = rank), plus 200 planted
 //           patterns of 3..8 uniform items, each inserted into exactly 4,950
 //           (0.5%) distinct transactions.  seed 2.
-// Config 4  49,046 transactions over 2,113 items, exactly 74 per transaction
+// dense-replicated  49,046 transactions over 2,113 items, exactly 74 per transaction
 //           (pumsb-like: 74 attribute slots, slot k owns ids k+1, k+75, ...,
 //           values Zipf(0.7) in id order).  20 blocks of 12 slots drawn from
 //           the first 40 slots fire independently with p=0.8; a core slot
 //           takes its first value when at least 60% of its blocks fire or,
 //           independently, with p=0.45; otherwise its Zipf draw.  seed 3.
-// Config 5  1,100 transactions over 20,000 items: each item present with
+// gene              1,100 transactions over 20,000 items: each item present with
 //           p=0.05; 40 co-regulated modules of 50..300 items; transaction t
 //           has a latent factor z_t ~ N(0,1) and module m switches on with
 //           probability sigmoid(a_m z_t + b_m), a_m ~ U(1,3), b_m ~ U(-3.5,-1.5);
@@ -243,15 +244,20 @@ void gen5(FILE *f, int rows) {
 
 int main(int argc, char **argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: tfim_gen <1|3|4|5> <out.txt> [--rows N]\n");
+        fprintf(stderr, "usage: tfim_gen <dense-small|clickstream|dense-replicated|gene> <out.txt> [--rows N]\n");
         return 1;
     }
-    const int cfg = atoi(argv[1]);
+    const std::string w = argv[1];
+    const int cfg = w == "dense-small" ? 1 : w == "clickstream" ? 3 : w == "dense-replicated" ? 4 : w == "gene" ? 5 : 0;
     int rows = cfg == 1 ? 3200 : cfg == 3 ? 990000 : cfg == 4 ? 49046 : cfg == 5 ? 1100 : 0;
+    if (!cfg) {
+        fprintf(stderr, "unknown workload %s\n", argv[1]);
+        return 1;
+    }
     for (int i = 3; i + 1 < argc; i++)
         if (!strcmp(argv[i], "--rows")) rows = atoi(argv[i + 1]);
     if (rows <= 0) {
-        fprintf(stderr, "unknown config %d\n", cfg);
+        fprintf(stderr, "bad --rows\n");
         return 1;
     }
     FILE *f = fopen(argv[2], "w");

@@ -3,7 +3,8 @@
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
 //             [--workers N] [--learned] [--rank] [--gate indep|always]
-//             [--wave-kernel 0|1|2] [--model PATH] [--trace PATH] [--verbose]
+//             [--wave-kernel 0|1|2] [--graphs 0|1] [--model PATH] [--trace PATH]
+//             [--verbose]
 //
 // Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the
 // reference layout and prints run statistics as "key: value" lines.  Exit
@@ -31,6 +32,7 @@ int main(int argc, char **argv) {
     SearchOptions opt;
     std::string out_path;
     int wave_kernel = 0;
+    int graphs = -1;   // -1: library default
     int replicate = 0, device = 0;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
@@ -49,6 +51,7 @@ int main(int argc, char **argv) {
         else if (a == "--workers") opt.workers = atoi(next().c_str());
         else if (a == "--rank") opt.rank = true;
         else if (a == "--wave-kernel") wave_kernel = atoi(next().c_str());
+        else if (a == "--graphs") graphs = atoi(next().c_str());
         else if (a == "--model") opt.model_path = next();
         else if (a == "--trace") opt.trace_path = next();
         else if (a == "--verbose") opt.verbose = true;
@@ -65,6 +68,11 @@ int main(int argc, char **argv) {
     tfim_ctx *ctx = nullptr;
     if (tfim_open(device, &ctx) != TFIM_OK) {
         fprintf(stderr, "tfim_open(%d) failed: no usable GPU\n", device);
+        return 1;
+    }
+    if (graphs >= 0 && tfim_config(ctx, "graphs", graphs) != TFIM_OK) {
+        fprintf(stderr, "%s\n", tfim_last_error(ctx));
+        tfim_close(ctx);
         return 1;
     }
     if (wave_kernel && tfim_config(ctx, "wave_kernel", wave_kernel) != TFIM_OK) {
@@ -140,13 +148,16 @@ int main(int argc, char **argv) {
     printf("count_time(s): %g\nkernel_ms: %g\n", st.count_seconds, st.kernel_ms);
     printf("rows: %lld\nwaves: %lld\nlds_waves: %lld\n", (long long)st.nodes, (long long)st.waves,
            (long long)st.lds_waves);
+    int64_t n_graphs = 0, graph_launches = 0;
+    tfim_wave_graph_stats(ctx, &n_graphs, &graph_launches);
+    printf("graphs: %lld\ngraph_launches: %lld\n", (long long)n_graphs, (long long)graph_launches);
     printf("expand_rows: %lld\nhut_rows: %lld\nleaf_rows: %lld\nref_rows: %lld\n", (long long)st.expand_rows,
            (long long)st.hut_rows, (long long)st.leaf_rows, (long long)st.ref_rows);
     printf("hut_success: %lld\nhut_skipped: %lld\npep_moves: %lld\nhutmfi_pruned: %lld\nleaf_subsumed: %lld\n",
            (long long)st.hut_success, (long long)st.hut_skipped, (long long)st.pep_moves,
            (long long)st.hutmfi_pruned, (long long)st.leaf_subsumed);
-    printf("counted_bitops: %.6g\nlearn_time(s): %g\nmlp_train_steps: %lld\n", st.counted_bitops, st.learn_seconds,
-           (long long)st.mlp_train_steps);
+    printf("counted_bitops: %.6g\nlearn_time(s): %g\nmlp_train_steps: %lld\nmodel_lookaheads: %lld\n",
+           st.counted_bitops, st.learn_seconds, (long long)st.mlp_train_steps, (long long)st.model_lookaheads);
     tfim_close(ctx);
     return 0;
 }

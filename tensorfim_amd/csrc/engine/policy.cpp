@@ -1,6 +1,7 @@
 #include "policy.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <numeric>
@@ -47,7 +48,8 @@ constexpr int NF = TFIM_MLP_INPUTS;
 // any tail order gives the same set enumeration.
 class LearnedPolicy : public Policy {
    public:
-    LearnedPolicy(tfim_ctx *ctx, const SearchOptions &opt) : ctx_(ctx), opt_(opt), plain_(opt.gate == "always") {
+    LearnedPolicy(tfim_ctx *ctx, const SearchOptions &opt) : ctx_(ctx), opt_(opt), plain_(false) {
+        const auto t0 = std::chrono::steady_clock::now();
         if (tfim_mlp_create(ctx, 0x5EEDull, &mlp_) != TFIM_OK) mlp_ = nullptr;
         if (mlp_ && !opt.model_path.empty()) {
             std::vector<float> p(TFIM_MLP_PARAMS);
@@ -59,6 +61,7 @@ class LearnedPolicy : public Policy {
             if (f) fclose(f);
         }
         if (!opt.trace_path.empty()) trace_ = fopen(opt.trace_path.c_str(), "wb");
+        extra_seconds_ += secs_since(t0);
     }
     ~LearnedPolicy() override {
         if (mlp_) tfim_mlp_destroy(mlp_);
@@ -69,23 +72,22 @@ class LearnedPolicy : public Policy {
         N_ = (double)std::max(1, db.trans_num_reduced);
         thr_ = std::max(1.0, db.thr_abs);
         freq_ = &db.freq_dense;
-        plain_.decide(ds, db);                       // fallback order and gate
-        if (!trained_)
-            for (Decision *d : ds)   // warm-up: label both outcomes (bounded tails only)
-                if (d->tail.size() >= 2 && d->tail.size() <= 64 && rand01() < 0.25f) d->lookahead = true;
-        // features for every decision and tail item (inference batch)
+        plain_.decide(ds, db);   // plain order and the independence-estimate rule
+        // features: one gate row per node with a tail of two or more; item rows
+        // for the ranker only when it reorders tails
         feats_.clear();
-        std::vector<int64_t> node_row(ds.size(), -1), item_row0(ds.size(), -1);
+        node_row_.assign(ds.size(), -1);
+        item_row0_.assign(ds.size(), -1);
         for (size_t q = 0; q < ds.size(); q++) {
             Decision *d = ds[q];
-            if (d->head.empty() || d->tail.empty()) continue;
+            if (d->head.empty() || d->tail.size() < 2) continue;
             node_features(*d);
-            node_row[q] = (int64_t)(feats_.size() / NF) - 1;
-            if (d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail) {
+            node_row_[q] = (int64_t)(feats_.size() / NF) - 1;
+            if (opt_.rank && d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail) {
                 // rank of each item in the plain (increasing support) order
                 rank_.assign(d->tail.size(), 0);
                 for (size_t j = 0; j < d->order.size(); j++) rank_[d->order[j]] = (int32_t)j;
-                item_row0[q] = node_row[q] + 1;
+                item_row0_[q] = node_row_[q] + 1;
                 for (size_t i = 0; i < d->tail.size(); i++) item_features(*d, i);
             }
         }
@@ -97,26 +99,33 @@ class LearnedPolicy : public Policy {
         }
         for (size_t q = 0; q < ds.size(); q++) {
             Decision *d = ds[q];
-            if (node_row[q] < 0) continue;
-            if (trained_) {
-                // the plain rule's lookaheads stay; the model adds confident ones
-                const float pg = sigmoid(logits_[node_row[q] * 2 + 0]);
-                const bool explore = d->tail.size() <= 64 && rand01() < 0.03f;
-                d->lookahead = d->tail.size() >= 2 && (d->lookahead || pg >= 0.6f || explore);
-                if (item_row0[q] >= 0 && opt_.rank) {
+            if (node_row_[q] < 0) continue;
+            const bool rule = d->lookahead;
+            if (!trained_) {
+                // warm-up: classical lookahead on bounded tails so that both
+                // outcomes get labelled
+                d->lookahead = rule || d->tail.size() <= 64;
+            } else {
+                // the model adds the lookaheads it expects to succeed to the
+                // rule's; a small exploration share keeps negative labels coming
+                const float pg = sigmoid(logits_[node_row_[q] * 2 + 0]);
+                const bool explore = d->tail.size() <= 64 && rand01() < kExplore;
+                d->lookahead = rule || pg >= kGateP || explore;
+                if (d->lookahead && !rule) model_gated_++;
+                if (item_row0_[q] >= 0) {
                     // items predicted to keep >= 90% of their child's tail go
                     // last (they sit in every earlier child's head+tail)
                     std::stable_partition(d->order.begin(), d->order.end(), [&](int32_t a) {
-                        return sigmoid(logits_[(item_row0[q] + a) * 2 + 1]) < 0.9f;
+                        return sigmoid(logits_[(item_row0_[q] + a) * 2 + 1]) < 0.9f;
                     });
                 }
             }
-            // records: the gate record is labelled only if the lookahead runs
-            d->record = d->lookahead ? add_record(&feats_[node_row[q] * NF], 0) : -1;
-            if (item_row0[q] >= 0) {
+            // the gate record is labelled only if the lookahead runs
+            d->record = d->lookahead ? add_record(&feats_[node_row_[q] * NF], 0) : -1;
+            if (item_row0_[q] >= 0) {
                 d->item_records.resize(d->tail.size());
                 for (size_t i = 0; i < d->tail.size(); i++)
-                    d->item_records[i] = add_record(&feats_[(item_row0[q] + i) * NF], 1);
+                    d->item_records[i] = add_record(&feats_[(item_row0_[q] + i) * NF], 1);
             }
         }
     }
@@ -132,7 +141,8 @@ class LearnedPolicy : public Policy {
 
     void finish(SearchStats &st) override {
         st.mlp_train_steps = steps_;
-        st.learn_seconds += 0;
+        st.model_lookaheads = model_gated_;
+        st.learn_seconds += extra_seconds_;   // set-up and training (decide() is timed by the engine)
         if (mlp_ && !opt_.model_path.empty() && opt_.learned && trained_) {
             // keep what was learned for the next run on the same data
             std::vector<float> p(TFIM_MLP_PARAMS);
@@ -147,6 +157,9 @@ class LearnedPolicy : public Policy {
     }
 
    private:
+    static double secs_since(std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    }
     static float sigmoid(float x) { return 1.0f / (1.0f + std::exp(-x)); }
     float rand01() {
         uint64_t z = (rng_ += 0x9E3779B97F4A7C15ull);
@@ -229,8 +242,9 @@ class LearnedPolicy : public Policy {
             const float yh[2] = {(float)head, v};
             fwrite(yh, 4, 2, trace_);
         }
-        // the batch keeps the most recent labelled records
-        const size_t b = (size_t)(n_labelled_ % kBatch);
+        // each head keeps its most recent labelled records (half a batch each)
+        const int64_t nh = n_head_[head]++;
+        const size_t b = (size_t)(head * kHalf + nh % kHalf);
         if (bx_.empty()) {
             bx_.assign((size_t)kBatch * NF, 0.0f);
             by_.assign((size_t)kBatch * 2, 0.0f);
@@ -246,24 +260,46 @@ class LearnedPolicy : public Policy {
     }
     void train() {
         if (!mlp_) return;
+        const auto t0 = std::chrono::steady_clock::now();
+        struct Acc {
+            double &s;
+            std::chrono::steady_clock::time_point t;
+            ~Acc() { s += secs_since(t); }
+        } acc{extra_seconds_, t0};
         last_train_ = n_labelled_;
-        const int64_t take = std::min<int64_t>(n_labelled_, kBatch);
+        // batch = filled part of each head's half; snapshot so queued steps
+        // read a stable copy
+        tx_.clear();
+        ty_.clear();
+        tm_.clear();
+        for (int h = 0; h < 2; h++) {
+            const int64_t n = std::min<int64_t>(n_head_[h], kHalf);
+            tx_.insert(tx_.end(), bx_.begin() + h * kHalf * NF, bx_.begin() + (h * kHalf + n) * NF);
+            ty_.insert(ty_.end(), by_.begin() + h * kHalf * 2, by_.begin() + (h * kHalf + n) * 2);
+            tm_.insert(tm_.end(), bm_.begin() + h * kHalf * 2, bm_.begin() + (h * kHalf + n) * 2);
+        }
+        const int32_t take = (int32_t)(tx_.size() / NF);
+        if (take == 0) return;
         float loss = 0;
         for (int e = 0; e < kEpochs; e++) {
-            if (tfim_mlp_train_step(mlp_, bx_.data(), by_.data(), bm_.data(), (int32_t)take, 2e-3f, &loss) != TFIM_OK)
-                return;
+            // the last step reports the loss (and waits); the others stay queued
+            float *lp = (e == kEpochs - 1 && opt_.verbose) ? &loss : nullptr;
+            if (tfim_mlp_train_step(mlp_, tx_.data(), ty_.data(), tm_.data(), take, 2e-3f, lp) != TFIM_OK) return;
             steps_++;
         }
-        if (opt_.verbose) fprintf(stderr, "mlp train: %lld records, loss %.4f\n", (long long)take, loss);
+        if (opt_.verbose) fprintf(stderr, "mlp train: %d records, loss %.4f\n", take, loss);
         trained_ = true;
     }
 
     static constexpr int64_t kTrainEvery = 8192;
     static constexpr int64_t kFirstTrain = 1024;
     static constexpr int64_t kBatch = 8192;
-    static constexpr int64_t kRing = 1 << 20;
+    static constexpr int64_t kHalf = kBatch / 2;
+    static constexpr int64_t kRing = 1 << 16;   // labels arrive within a few waves
     static constexpr int kEpochs = 4;
     static constexpr size_t kMaxRankTail = 128;   // longer tails keep the plain order
+    static constexpr float kGateP = 0.5f;          // model-added lookahead: P(HUT frequent) >= this
+    static constexpr float kExplore = 0.02f;       // share of rule-declined lookaheads run anyway
 
     tfim_ctx *ctx_;
     const SearchOptions &opt_;
@@ -275,12 +311,16 @@ class LearnedPolicy : public Policy {
     const std::vector<int32_t> *freq_ = nullptr;
     NodeAgg cur_{};
     std::vector<int32_t> rank_;
+    std::vector<int64_t> node_row_, item_row0_;
+    int64_t model_gated_ = 0;
+    double extra_seconds_ = 0;
     std::vector<float> feats_, logits_;
     std::vector<float> rec_x_ = std::vector<float>((size_t)kRing * NF);
     std::vector<float> rec_y_ = std::vector<float>((size_t)kRing * 2), rec_m_ = std::vector<float>((size_t)kRing * 2);
     std::vector<int64_t> rec_id_ = std::vector<int64_t>((size_t)kRing, -1);
-    std::vector<float> bx_, by_, bm_;
+    std::vector<float> bx_, by_, bm_, tx_, ty_, tm_;
     int64_t next_id_ = 0, n_labelled_ = 0, last_train_ = 0, steps_ = 0;
+    int64_t n_head_[2] = {0, 0};
     uint64_t rng_ = 0xC0FFEEull;
 };
 

@@ -52,6 +52,7 @@ struct SearchStats {
     int64_t lds_waves = 0;        // waves counted by the LDS-staged kernel
     double learn_seconds = 0;     // host time in the learned part
     int64_t mlp_train_steps = 0;
+    int64_t model_lookaheads = 0;  // lookaheads run because the model (not the rule) asked
 };
 
 // Runs the engine; returns MFIs (dense ids, insertion order) with frequencies.

@@ -5,20 +5,19 @@
 // int8, so C[i][j] = sum_t a_it * b_jt is exactly popcount(A_i AND B_j), and
 // v_mfma_i32_32x32x32_i8 accumulates it in int32 without rounding.
 //
-// Tile: 128 rows x 128 columns per 256-thread workgroup; waves in a 2x2 grid
-// each own 64x64 = 2x2 MFMA tiles (64 accumulator registers).  K is staged
-// 256 bits per step: the packed words are loaded with 128-bit loads and
-// unpacked to bytes straight into LDS (32 KB for A, 32 KB for B), with a
-// 16-byte-slot XOR swizzle so the ds_read_b128 of a 32-row fragment hits
-// distinct banks.
+// Two kernels:
+//   k_popmat_mfma     128 x 128 outputs per workgroup, bits expanded to bytes
+//                     into LDS (every wave reads bytes back);
+//   k_popmat_mfma_rp  256 x 128 outputs per workgroup, packed bits in LDS and
+//                     each lane expanding its own operands in registers (the
+//                     default: LDS traffic per MFMA is 1/8 of the first).
 //
 // Operand layout.  For v_mfma_i32_32x32x32_i8, lane l supplies 16 bytes of
 // row (l % 32) of A and the same 16 k-positions of column (l % 32) of B; the
 // half l / 32 selects which 16 of the 32 k.  The instruction sums over all 64
 // (half, byte) pairs, so any consistent assignment of k to (half, byte) is
-// exact; here byte b of half h is k = 16h + b.  The accumulator layout is the
-// 32x32 one of the CDNA3/CDNA4 ISA: register q of lane l holds
-// C[8*(q/4) + 4*(l/32) + (q%4)][l % 32].
+// exact.  The accumulator layout is the 32x32 one of the CDNA3/CDNA4 ISA:
+// register q of lane l holds C[8*(q/4) + 4*(l/32) + (q%4)][l % 32].
 #include "tfim_internal.h"
 
 namespace tfim {
@@ -36,6 +35,11 @@ __device__ __forceinline__ uint4 unpack16(uint32_t x16) {
     return make_uint4(nib2bytes(x16), nib2bytes(x16 >> 4), nib2bytes(x16 >> 8), nib2bytes(x16 >> 12));
 }
 
+// 128 x 128 tile: waves in a 2x2 grid each own 64x64 = 2x2 MFMA tiles (64
+// accumulator registers).  K is staged 256 bits per step: the packed words are
+// loaded with 128-bit loads and unpacked to bytes straight into LDS (32 KB for
+// A, 32 KB for B), byte b of half h being k = 16h + b, with a 16-byte-slot XOR
+// swizzle so the ds_read_b128 of a 32-row fragment hits distinct banks.
 constexpr int MM_T = 128;     // rows and columns per workgroup tile
 constexpr int MM_SLOTS = 16;  // 16-byte slots per row per 256-bit K step
 
@@ -130,271 +134,198 @@ __global__ __launch_bounds__(256) void k_popmat_mfma(const uint64_t *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Large-tile variant: 256x256 per workgroup, each wave 128x128 (4x4 MFMA
-// tiles, 256 int32 accumulators), so every 1 KB fragment read from LDS feeds
-// 4 MFMAs instead of 2.  Thread t owns row t of the A tile and row t of the B
-// tile: it keeps one 1024-bit block of each in registers (eight 128-bit
-// loads), prefetches the next block while the current one is consumed, and
-// per 128-bit sub-stage unpacks 16 bytes of each row into a double-buffered
-// LDS stage (2 x 64 KB, 16-byte slots XOR-swizzled by row & 7).  One barrier
-// per sub-stage: a stage is rewritten only after every wave has passed the
-// barrier that follows its last read.
-constexpr int M2_T = 256;
-constexpr int M2_SLOTS = 8;   // 16-byte slots per row per 128-bit sub-stage
+// Register-expanded variant.  LDS holds the PACKED bits (512 per row per
+// stage, double-buffered, one barrier per stage); each lane expands its own
+// operand bytes in registers, so LDS carries 1/8 of the bytes of the 128x128
+// kernel.  A 32-bit word w gives four operand dwords per parity p: dword t
+// holds the bits 8i + q (i = 0..3, q = 4p + t), one bit per byte.  Operand A
+// keeps each bit in place (weight 2^q, one AND); operand B rotates it to the
+// complementary position (weight 2^(6-q), rotate + AND); bit 7 goes to
+// position 3 on both sides.  Every matching bit therefore contributes exactly
+// 64 and the accumulator is 64 x popcount (recovered with a shift; a split
+// holds at most 2^24 bits, so an accumulator stays below 2^30).  All byte
+// values are 0..64, inside the signed int8 range.
+//
+// K map.  Per stage a row has 16 words of 32 bits.  At MFMA step s (0..15)
+// lane half h supplies word 8h + s/2 with parity p = s & 1: every bit of the
+// stage is used exactly once, by the same (step, half, byte) in A and B.
+//
+// Tile: waves in a 2x2 grid, each owning TI x TJ fragments of 32x32, so a
+// workgroup covers (64 TI) x (64 TJ); TI = 4, TJ = 2 gives 256 x 128 with
+// 128 accumulators per lane and two waves per SIMD.  The A side has more
+// fragments, so it gets the cheaper expansion.
+constexpr int RP_STRIDE = 20;   // u32 per LDS row: 16 data + 4 pad (bank spread)
 
-__global__ __launch_bounds__(256) void k_popmat_mfma256(const uint64_t *__restrict__ A,
+// A side: bit 8i + q (q < 7) stays in place, byte value 2^q (one AND);
+// q = 7 goes to bit 8i + 3 (value 8).
+__device__ __forceinline__ v4i rp_expand_a(uint32_t w, int parity) {
+    v4i r;
+    if (parity == 0) {
+        r[0] = (int)(w & 0x01010101u);
+        r[1] = (int)(w & 0x02020202u);
+        r[2] = (int)(w & 0x04040404u);
+        r[3] = (int)(w & 0x08080808u);
+    } else {
+        r[0] = (int)(w & 0x10101010u);
+        r[1] = (int)(w & 0x20202020u);
+        r[2] = (int)(w & 0x40404040u);
+        r[3] = (int)(__builtin_amdgcn_alignbit(w, w, 4) & 0x08080808u);
+    }
+    return r;
+}
+
+// B side: bit 8i + q (q < 7) goes to bit 8i + 6 - q, byte value 2^(6-q), so
+// every matching pair contributes 2^q * 2^(6-q) = 64; q = 7 goes to bit
+// 8i + 3 (8 * 8 = 64).  alignbit(w, w, n) rotates right by n.
+__device__ __forceinline__ v4i rp_expand_b(uint32_t w, int parity) {
+    v4i r;
+    if (parity == 0) {
+        r[0] = (int)(__builtin_amdgcn_alignbit(w, w, 26) & 0x40404040u);
+        r[1] = (int)(__builtin_amdgcn_alignbit(w, w, 28) & 0x20202020u);
+        r[2] = (int)(__builtin_amdgcn_alignbit(w, w, 30) & 0x10101010u);
+        r[3] = (int)(w & 0x08080808u);
+    } else {
+        r[0] = (int)(__builtin_amdgcn_alignbit(w, w, 2) & 0x04040404u);
+        r[1] = (int)(__builtin_amdgcn_alignbit(w, w, 4) & 0x02020202u);
+        r[2] = (int)(__builtin_amdgcn_alignbit(w, w, 6) & 0x01010101u);
+        r[3] = (int)(__builtin_amdgcn_alignbit(w, w, 4) & 0x08080808u);
+    }
+    return r;
+}
+
+template <int TI, int TJ>
+__global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__restrict__ A,
                                                         const uint64_t *__restrict__ B,
                                                         int32_t *__restrict__ C, int64_t b, int64_t d,
                                                         int64_t pitch, int64_t k_per_split, int atomic,
                                                         int tiles_n) {
-    extern __shared__ uint4 sm[];
-    uint4 *As = sm;                              // [2][256][8]
-    uint4 *Bs = sm + 2 * M2_T * M2_SLOTS;        // [2][256][8]
+    constexpr int RA = 64 * TI, RB = 64 * TJ;          // rows of A / B per workgroup
+    constexpr int LA = RA * 4 / 256, LB = RB * 4 / 256;  // uint4 loads per thread per stage
+    __shared__ uint32_t As[2][RA * RP_STRIDE];
+    __shared__ uint32_t Bs[2][RB * RP_STRIDE];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int wr = wave >> 1, wc = wave & 1;
     const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
     const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
-    const int64_t row0 = (lin / tiles_n) * M2_T, col0 = (lin % tiles_n) * M2_T;
+    const int64_t row0 = (lin / tiles_n) * RA, col0 = (lin % tiles_n) * RB;
     const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
     const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
-    const bool a_ok = row0 + tid < b, b_ok = col0 + tid < d;
-    const uint4 *Arow = reinterpret_cast<const uint4 *>(A + (row0 + tid) * pitch);
-    const uint4 *Brow = reinterpret_cast<const uint4 *>(B + (col0 + tid) * pitch);
     const uint4 z4 = make_uint4(0, 0, 0, 0);
 
-    const int wr = wave >> 1, wc = wave & 1;
-    v16i acc[4][4];
+    // loader: load j of thread t is uint4 part (idx & 3) of row (idx >> 2), idx = 256 j + t
+    uint4 ra[LA], rb[LB];
+    auto gload = [&](int64_t kw) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < LA; j++) {
+            const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
+            ra[j] = row0 + r < b ? *reinterpret_cast<const uint4 *>(A + (row0 + r) * pitch + kw + 2 * part) : z4;
+        }
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < LB; j++) {
+            const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
+            rb[j] = col0 + r < d ? *reinterpret_cast<const uint4 *>(B + (col0 + r) * pitch + kw + 2 * part) : z4;
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < LA; j++) {
+            const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
+            *reinterpret_cast<uint4 *>(&As[buf][r * RP_STRIDE + 4 * part]) = ra[j];
+        }
+#pragma unroll
+        for (int j = 0; j < LB; j++) {
+            const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
+            *reinterpret_cast<uint4 *>(&Bs[buf][r * RP_STRIDE + 4 * part]) = rb[j];
+        }
+    };
+
+    v16i acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++)
 #pragma unroll
             for (int q = 0; q < 16; q++) acc[i][j][q] = 0;
 
-    uint4 ra[8], rb[8], na[8], nb[8];
     if (kbeg < kend) {
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            ra[u] = a_ok ? Arow[(kbeg >> 1) + u] : z4;
-            rb[u] = b_ok ? Brow[(kbeg >> 1) + u] : z4;
-        }
+        gload(kbeg);
+        lstore(0);
     }
+    __syncthreads();
     int buf = 0;
-    for (int64_t kw = kbeg; kw < kend; kw += 16) {
-        const bool more = kw + 16 < kend;
+    for (int64_t kw = kbeg; kw < kend; kw += 8) {
+        const bool more = kw + 8 < kend;
+        if (more) gload(kw + 8);
 #pragma unroll
-        for (int u = 0; u < 8; u++) {   // prefetch the next 1024-bit block
-            na[u] = (more && a_ok) ? Arow[((kw + 16) >> 1) + u] : z4;
-            nb[u] = (more && b_ok) ? Brow[((kw + 16) >> 1) + u] : z4;
-        }
+        for (int hk = 0; hk < 2; hk++) {
+            // this lane's 4 words (of its half) for steps 8 hk .. 8 hk + 7
+            uint4 pa[TI], pb[TJ];
 #pragma unroll
-        for (int sub = 0; sub < 8; sub++) {
-            uint4 *as = As + buf * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
-            uint4 *bs = Bs + buf * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
-            const uint32_t aw[4] = {ra[sub].x, ra[sub].y, ra[sub].z, ra[sub].w};
-            const uint32_t bw[4] = {rb[sub].x, rb[sub].y, rb[sub].z, rb[sub].w};
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                as[j ^ (tid & 7)] = unpack16(aw[j >> 1] >> ((j & 1) * 16));
-                bs[j ^ (tid & 7)] = unpack16(bw[j >> 1] >> ((j & 1) * 16));
+            for (int i = 0; i < TI; i++) {
+                const int r = wr * (32 * TI) + i * 32 + l32;
+                pa[i] = *reinterpret_cast<const uint4 *>(&As[buf][r * RP_STRIDE + 8 * half + 4 * hk]);
             }
-            __syncthreads();
-            const uint4 *ab = As + buf * (M2_T * M2_SLOTS);
-            const uint4 *bb = Bs + buf * (M2_T * M2_SLOTS);
 #pragma unroll
-            for (int kk = 0; kk < 4; kk++) {
-                const int slot = kk * 2 + (lane >> 5);
-                v4i af[4], bf[4];
+            for (int j = 0; j < TJ; j++) {
+                const int r = wc * (32 * TJ) + j * 32 + l32;
+                pb[j] = *reinterpret_cast<const uint4 *>(&Bs[buf][r * RP_STRIDE + 8 * half + 4 * hk]);
+            }
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int r = wr * 128 + i * 32 + (lane & 31);
-                    const uint4 x = ab[r * M2_SLOTS + (slot ^ (r & 7))];
-                    af[i] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+            for (int s2 = 0; s2 < 8; s2++) {
+                const int wi = s2 >> 1, par = s2 & 1;
+                v4i ea[TI], eb[TJ];
+#pragma unroll
+                for (int i = 0; i < TI; i++) {
+                    const uint32_t w = wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w;
+                    ea[i] = rp_expand_a(w, par);
                 }
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int c = wc * 128 + j * 32 + (lane & 31);
-                    const uint4 x = bb[c * M2_SLOTS + (slot ^ (c & 7))];
-                    bf[j] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+                for (int j = 0; j < TJ; j++) {
+                    const uint32_t w = wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w;
+                    eb[j] = rp_expand_b(w, par);
                 }
 #pragma unroll
-                for (int i = 0; i < 4; i++)
+                for (int i = 0; i < TI; i++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-            }
-            buf ^= 1;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            ra[u] = na[u];
-            rb[u] = nb[u];
-        }
-    }
-
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int64_t r = row0 + wr * 128 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                const int64_t c = col0 + wc * 128 + j * 32 + (lane & 31);
-                if (r < b && c < d) {
-                    if (atomic) atomicAdd(&C[r * d + c], acc[i][j][q]);
-                    else C[r * d + c] = acc[i][j][q];
-                }
-            }
-}
-
-// Software-pipelined 256x256 variant: while a wave runs the 64 MFMAs of
-// sub-stage g from LDS buffer g%2, it unpacks sub-stage g+1 (2 of the 8 slots
-// of its A row and of its B row per K=32 step) into the other buffer, so the
-// VALU/DS work of the unpack sits in the MFMA shadow of the same wave (one
-// wave per SIMD at 256 accumulators).  One barrier per sub-stage orders the
-// buffer swap; the raw 1024-bit blocks are prefetched one block ahead.
-__device__ __forceinline__ void unpack_pair(uint4 *dst, int row, int j, const uint4 &raw) {
-    const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
-    dst[j ^ (row & 7)] = unpack16(w[j >> 1] >> ((j & 1) * 16));
-}
-
-__global__ __launch_bounds__(256) void k_popmat_mfma256p(const uint64_t *__restrict__ A,
-                                                         const uint64_t *__restrict__ B,
-                                                         int32_t *__restrict__ C, int64_t b, int64_t d,
-                                                         int64_t pitch, int64_t k_per_split, int atomic,
-                                                         int tiles_n) {
-    extern __shared__ uint4 sm[];
-    uint4 *As = sm;                              // [2][256][8]
-    uint4 *Bs = sm + 2 * M2_T * M2_SLOTS;        // [2][256][8]
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
-    const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
-    const int64_t row0 = (lin / tiles_n) * M2_T, col0 = (lin % tiles_n) * M2_T;
-    const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
-    const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
-    const bool a_ok = row0 + tid < b, b_ok = col0 + tid < d;
-    const uint4 *Arow = reinterpret_cast<const uint4 *>(A + (row0 + tid) * pitch);
-    const uint4 *Brow = reinterpret_cast<const uint4 *>(B + (col0 + tid) * pitch);
-    const uint4 z4 = make_uint4(0, 0, 0, 0);
-    const int wr = wave >> 1, wc = wave & 1;
-    v16i acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 16; q++) acc[i][j][q] = 0;
-    const int64_t nsub = (kend - kbeg) / 2;   // 128-bit sub-stages (uint4 per row)
-    if (nsub > 0) {
-        uint4 ra[8], rb[8], na[8], nb[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            ra[u] = a_ok ? Arow[(kbeg >> 1) + u] : z4;
-            rb[u] = b_ok ? Brow[(kbeg >> 1) + u] : z4;
-            na[u] = z4;
-            nb[u] = z4;
-        }
-        {   // prologue: sub-stage 0 into buffer 0
-            uint4 *as = As + tid * M2_SLOTS, *bs = Bs + tid * M2_SLOTS;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                unpack_pair(as, tid, j, ra[0]);
-                unpack_pair(bs, tid, j, rb[0]);
+                    for (int j = 0; j < TJ; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[i], eb[j], acc[i][j], 0, 0, 0);
             }
         }
+        if (more) lstore(buf ^ 1);
         __syncthreads();
-        for (int64_t g0 = 0; g0 < nsub; g0 += 8) {   // one 1024-bit block per iteration
-            const bool more = g0 + 8 < nsub;
-            const int64_t nxt = (kbeg >> 1) + g0 + 8;
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                na[u] = (more && a_ok) ? Arow[nxt + u] : z4;
-                nb[u] = (more && b_ok) ? Brow[nxt + u] : z4;
-            }
-#pragma unroll
-            for (int sub = 0; sub < 8; sub++) {
-                const int buf = sub & 1;   // g0 is a multiple of 8: parity of g is parity of sub
-                const uint4 *ab = As + buf * (M2_T * M2_SLOTS);
-                const uint4 *bb = Bs + buf * (M2_T * M2_SLOTS);
-                uint4 *an = As + (buf ^ 1) * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
-                uint4 *bn = Bs + (buf ^ 1) * (M2_T * M2_SLOTS) + tid * M2_SLOTS;
-                const bool has_next = sub < 7 || more;
-                const uint4 &rna = sub < 7 ? ra[sub + 1] : na[0];
-                const uint4 &rnb = sub < 7 ? rb[sub + 1] : nb[0];
-#pragma unroll
-                for (int kk = 0; kk < 4; kk++) {
-                    const int slot = kk * 2 + (lane >> 5);
-                    v4i af[4], bf[4];
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int r = wr * 128 + i * 32 + (lane & 31);
-                        const uint4 x = ab[r * M2_SLOTS + (slot ^ (r & 7))];
-                        af[i] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
-                    }
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int c = wc * 128 + j * 32 + (lane & 31);
-                        const uint4 x = bb[c * M2_SLOTS + (slot ^ (c & 7))];
-                        bf[j] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-#pragma unroll
-                        for (int j = 0; j < 4; j++)
-                            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-                    if (has_next) {
-                        unpack_pair(an, tid, 2 * kk, rna);
-                        unpack_pair(an, tid, 2 * kk + 1, rna);
-                        unpack_pair(bn, tid, 2 * kk, rnb);
-                        unpack_pair(bn, tid, 2 * kk + 1, rnb);
-                    }
-                }
-                __syncthreads();
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                ra[u] = na[u];
-                rb[u] = nb[u];
-            }
-        }
+        buf ^= 1;
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < TI; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < TJ; j++)
 #pragma unroll
             for (int q = 0; q < 16; q++) {
-                const int64_t r = row0 + wr * 128 + i * 32 + 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                const int64_t c = col0 + wc * 128 + j * 32 + (lane & 31);
+                const int64_t r = row0 + wr * (32 * TI) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
+                const int64_t c = col0 + wc * (32 * TJ) + j * 32 + l32;
                 if (r < b && c < d) {
-                    if (atomic) atomicAdd(&C[r * d + c], acc[i][j][q]);
-                    else C[r * d + c] = acc[i][j][q];
+                    const int32_t v = acc[i][j][q] >> 6;
+                    if (atomic) atomicAdd(&C[r * d + c], v);
+                    else C[r * d + c] = v;
                 }
             }
 }
 
-void launch_popmat_mfma256(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                           int64_t pitch_words, int split_k, hipStream_t s, bool pipelined) {
+void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                           int64_t pitch_words, int split_k, hipStream_t s) {
     if (split_k < 1) split_k = 1;
     int64_t kps = (pitch_words + split_k - 1) / split_k;
-    kps = (kps + 15) / 16 * 16;
+    kps = (kps + 7) / 8 * 8;                                     // whole 512-bit stages
+    kps = std::min<int64_t>(kps, (int64_t)1 << 18);              // <= 2^24 bits per split
     const int64_t zs = (pitch_words + kps - 1) / kps;
-    const int64_t tm = (b + M2_T - 1) / M2_T, tn = (d + M2_T - 1) / M2_T;
-    const size_t lds = (size_t)4 * M2_T * M2_SLOTS * sizeof(uint4);   // 128 KB
-    static bool attr = false;
-    if (!attr) {
-        hipFuncSetAttribute((const void *)k_popmat_mfma256, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipFuncSetAttribute((const void *)k_popmat_mfma256p, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        attr = true;
-    }
+    if (zs > 1 && split_k == 1) launch_zero_i32(C, b * d, s);    // the caller zeroes C only for split_k > 1
+    constexpr int RA = 256, RB = 128;
+    const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
     dim3 grid((unsigned)(tm * tn), (unsigned)zs);
-    if (pipelined)
-        hipLaunchKernelGGL(k_popmat_mfma256p, grid, dim3(256), lds, s, A, B, C, b, d, pitch_words, kps,
-                           zs > 1 ? 1 : 0, (int)tn);
-    else
-        hipLaunchKernelGGL(k_popmat_mfma256, grid, dim3(256), lds, s, A, B, C, b, d, pitch_words, kps,
-                           zs > 1 ? 1 : 0, (int)tn);
+    hipLaunchKernelGGL((k_popmat_mfma_rp<4, 2>), grid, dim3(256), 0, s, A, B, C, b, d, pitch_words, kps,
+                       zs > 1 ? 1 : 0, (int)tn);
 }
 
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,

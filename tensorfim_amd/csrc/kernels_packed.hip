@@ -197,16 +197,28 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint4 *__restrict__ bm
 
 int wave_chunk_words(int rw) { return 64 * rw * 2; }
 
+void plan_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d, int grid_blocks, int rw,
+                     KernelLaunch &L) {
+    switch (rw) {
+        case 1: L.func = (const void *)k_count_wave<1>; break;
+        case 2: L.func = (const void *)k_count_wave<2>; break;
+        case 8: L.func = (const void *)k_count_wave<8>; break;
+        default: L.func = (const void *)k_count_wave<4>; break;
+    }
+    L.grid = dim3((unsigned)grid_blocks);
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.n_args = L.used = 0;
+    L.arg(reinterpret_cast<const uint4 *>(bm));
+    L.arg((int64_t)(pitch_words / 2));
+    L.arg(d);
+}
+
 void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d, int grid_blocks,
                        int rw, hipStream_t s) {
-    const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
-    const int64_t pitch4 = pitch_words / 2;
-    switch (rw) {
-        case 1: hipLaunchKernelGGL(k_count_wave<1>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
-        case 2: hipLaunchKernelGGL(k_count_wave<2>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
-        case 8: hipLaunchKernelGGL(k_count_wave<8>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
-        default: hipLaunchKernelGGL(k_count_wave<4>, dim3(grid_blocks), dim3(256), 0, s, b4, pitch4, d); break;
-    }
+    KernelLaunch L;
+    plan_count_wave(bm, pitch_words, d, grid_blocks, rw, L);
+    L.launch(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -286,8 +298,8 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
         if (acc[i]) atomicAdd(&d.out[i], acc[i]);
 }
 
-bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
-                           hipStream_t s) {
+bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
+                         KernelLaunch &L) {
     if (d.n_slots < 1 || d.n_slots > kLdsMaxSlots || d.n_out > kLdsMaxOut) return false;
     const int64_t pitch4 = pitch_words / 2;
     const size_t acc_bytes = ((size_t)d.n_out * 4 + 15) / 16 * 16;
@@ -308,18 +320,24 @@ bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLd
     const int64_t n_chunks = (pitch4 + 64 * rw - 1) / (64 * rw);
     const int per_cu = lds_bytes <= kHalf ? 2 : 1;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_chunks, (int64_t)n_cu * per_cu));
-    const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
-    switch (rw) {
-        case 4:
-            hipLaunchKernelGGL(k_count_wave_lds<4>, dim3((unsigned)grid), dim3(kLdsThreads), lds_bytes, s, b4, pitch4, d, n_chunks);
-            break;
-        case 2:
-            hipLaunchKernelGGL(k_count_wave_lds<2>, dim3((unsigned)grid), dim3(kLdsThreads), lds_bytes, s, b4, pitch4, d, n_chunks);
-            break;
-        default:
-            hipLaunchKernelGGL(k_count_wave_lds<1>, dim3((unsigned)grid), dim3(kLdsThreads), lds_bytes, s, b4, pitch4, d, n_chunks);
-            break;
-    }
+    L.func = rw == 4 ? (const void *)k_count_wave_lds<4>
+                     : (rw == 2 ? (const void *)k_count_wave_lds<2> : (const void *)k_count_wave_lds<1>);
+    L.grid = dim3((unsigned)grid);
+    L.block = dim3(kLdsThreads);
+    L.shmem = (uint32_t)lds_bytes;
+    L.n_args = L.used = 0;
+    L.arg(reinterpret_cast<const uint4 *>(bm));
+    L.arg(pitch4);
+    L.arg(d);
+    L.arg(n_chunks);
+    return true;
+}
+
+bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
+                           hipStream_t s) {
+    KernelLaunch L;
+    if (!plan_count_wave_lds(bm, pitch_words, d, n_cu, L)) return false;
+    L.launch(s);
     return true;
 }
 
@@ -329,10 +347,22 @@ __global__ void k_zero_i32(int32_t *p, int64_t n) {
         p[i] = 0;
 }
 
+void plan_zero_i32(int32_t *p, int64_t n, KernelLaunch &L) {
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+    L.func = (const void *)k_zero_i32;
+    L.grid = dim3((unsigned)blocks);
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.n_args = L.used = 0;
+    L.arg(p);
+    L.arg(n);
+}
+
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) {
     if (n <= 0) return;
-    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    KernelLaunch L;
+    plan_zero_i32(p, n, L);
+    L.launch(s);
 }
 
 // ---------------------------------------------------------------------------

@@ -17,6 +17,36 @@ inline int64_t pitch_for_bits(int64_t n_bits) {
     return ((n_bits + 2047) / 2048) * kPitchWords;
 }
 
+// One kernel launch, described so that it can be issued directly or set as
+// the parameters of a graph kernel node.  Arguments live in `buf`; params()
+// returns the pointer array HIP expects (valid while this object lives).
+struct KernelLaunch {
+    const void *func = nullptr;
+    dim3 grid{1, 1, 1}, block{1, 1, 1};
+    uint32_t shmem = 0;
+    alignas(16) unsigned char buf[192];
+    int off[6] = {0, 0, 0, 0, 0, 0};
+    int n_args = 0, used = 0;
+    void *ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+    template <class T>
+    void arg(const T &v) {
+        const int a = (int)alignof(T) < 8 ? 8 : (int)alignof(T);
+        used = (used + a - 1) / a * a;
+        static_assert(sizeof(T) <= sizeof(buf), "argument too large");
+        __builtin_memcpy(buf + used, &v, sizeof(T));
+        off[n_args++] = used;
+        used += (int)sizeof(T);
+    }
+    void **params() {
+        for (int i = 0; i < n_args; i++) ptrs[i] = buf + off[i];
+        return ptrs;
+    }
+    hipError_t launch(hipStream_t s) {
+        return hipLaunchKernel(func, grid, block, params(), shmem, s);
+    }
+};
+
 // ---- launchers (defined in kernels_*.hip) --------------------------------
 
 // Plain per-node count: out[c] += popcount(AND rows[items] & rows[c]) for
@@ -48,6 +78,8 @@ struct WaveDesc {
 // (atomics).  `out` must be zeroed.  grid_blocks must be a multiple of 8.
 void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d,
                        int grid_blocks, int chunk_uint4_per_lane, hipStream_t s);
+void plan_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d, int grid_blocks,
+                     int chunk_uint4_per_lane, KernelLaunch &L);
 int wave_chunk_words(int chunk_uint4_per_lane);
 
 // LDS-staged wave: every item the wave touches gets an LDS slot; prefixes and
@@ -69,17 +101,22 @@ constexpr int kLdsMaxOut = 8192;
 // Returns false if the shape does not fit LDS (caller uses launch_count_wave).
 bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
                            hipStream_t s);
+bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
+                         KernelLaunch &L);
 
 // Zero `n` int32 with a fill kernel (graph-capture safe).
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
+void plan_zero_i32(int32_t *p, int64_t n, KernelLaunch &L);
 
 // Matrix AND-popcount, packed (VALU) path.  C must be zeroed when split_k>1.
 void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                           int64_t pitch_words, int split_k, hipStream_t s);
-// Matrix AND-popcount, int8 MFMA path, 256x256 tiles (large shapes).
-void launch_popmat_mfma256(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                           int64_t pitch_words, int split_k, hipStream_t s, bool pipelined);
-// Matrix AND-popcount, int8 MFMA path.  C must be zeroed when split_k>1.
+// Matrix AND-popcount, int8 MFMA path, 256x128 tiles with the operand bytes
+// expanded in registers from packed LDS.  C must be zeroed when split_k>1.
+void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                           int64_t pitch_words, int split_k, hipStream_t s);
+// Matrix AND-popcount, int8 MFMA path, 128x128 tiles with the operand bytes
+// expanded into LDS.  C must be zeroed when split_k>1.
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                         int64_t pitch_words, int split_k, hipStream_t s);
 

@@ -10,6 +10,7 @@ using namespace tfim;
 
 struct tfim_mlp {
     tfim_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;   // own stream: inference never queues behind counting
     float *P = nullptr, *G = nullptr, *Mo = nullptr, *Vo = nullptr;
     uint16_t *Pb = nullptr;
     int64_t step = 0;
@@ -72,7 +73,7 @@ int ensure(tfim_mlp *m, int64_t n, Scratch &sc) {
 // forward+backward into scratch; returns the loss and leaves G filled
 int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float *loss, Scratch &sc) {
     if (int r = ensure(m, n, sc)) return r;
-    hipStream_t s = ctx_stream(m->ctx);
+    hipStream_t s = m->stream;
     double cnt = 0;
     for (int64_t i = 0; i < (int64_t)n * OUT; i++) cnt += mask[i];
     const float inv = (float)(1.0 / std::max(1.0, cnt));
@@ -102,7 +103,8 @@ int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out) {
     tfim_mlp *m = new tfim_mlp();
     m->ctx = ctx;
     const int np = mlp_nparam();
-    if (hipSetDevice(ctx_device(ctx)) != hipSuccess || hipMalloc(&m->P, np * 4) != hipSuccess ||
+    if (hipSetDevice(ctx_device(ctx)) != hipSuccess ||
+        hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess || hipMalloc(&m->P, np * 4) != hipSuccess ||
         hipMalloc(&m->G, np * 4) != hipSuccess || hipMalloc(&m->Mo, np * 4) != hipSuccess ||
         hipMalloc(&m->Vo, np * 4) != hipSuccess || hipMalloc(&m->Pb, np * 2) != hipSuccess) {
         tfim_mlp_destroy(m);
@@ -124,13 +126,15 @@ int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out) {
 
 void tfim_mlp_destroy(tfim_mlp *m) {
     if (!m) return;
+    if (m->stream) hipStreamSynchronize(m->stream);
     hipFree(m->P); hipFree(m->G); hipFree(m->Mo); hipFree(m->Vo); hipFree(m->Pb); hipFree(m->buf);
+    if (m->stream) hipStreamDestroy(m->stream);
     delete m;
 }
 
 int tfim_mlp_get_params(tfim_mlp *m, float *params) {
     if (!m || !params) return TFIM_E_INVALID;
-    hipStream_t s = ctx_stream(m->ctx);
+    hipStream_t s = m->stream;
     MCHK(m, hipMemcpyAsync(params, m->P, (size_t)mlp_nparam() * 4, hipMemcpyDeviceToHost, s));
     MCHK(m, hipStreamSynchronize(s));
     return TFIM_OK;
@@ -138,7 +142,7 @@ int tfim_mlp_get_params(tfim_mlp *m, float *params) {
 
 int tfim_mlp_set_params(tfim_mlp *m, const float *params) {
     if (!m || !params) return TFIM_E_INVALID;
-    hipStream_t s = ctx_stream(m->ctx);
+    hipStream_t s = m->stream;
     const size_t b = (size_t)mlp_nparam() * 4;
     MCHK(m, hipMemcpyAsync(m->P, params, b, hipMemcpyHostToDevice, s));
     MCHK(m, hipMemsetAsync(m->Mo, 0, b, s));
@@ -155,7 +159,7 @@ int tfim_mlp_forward(tfim_mlp *m, const float *x, int32_t n, float *logits, int3
     if (n == 0) return TFIM_OK;
     Scratch sc;
     if (int r = ensure(m, n, sc)) return r;
-    hipStream_t s = ctx_stream(m->ctx);
+    hipStream_t s = m->stream;
     MCHK(m, hipMemcpyAsync(sc.X, x, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
     launch_mlp_infer(m->P, bf16 ? m->Pb : nullptr, sc.X, n, sc.O, s);
     MCHK(m, hipGetLastError());
@@ -169,7 +173,7 @@ int tfim_mlp_gradients(tfim_mlp *m, const float *x, const float *y, const float 
     if (!m || !x || !y || !mask || !grads || n < 1) return TFIM_E_INVALID;
     Scratch sc;
     if (int r = fwd_bwd(m, x, y, mask, n, loss, sc)) return r;
-    hipStream_t s = ctx_stream(m->ctx);
+    hipStream_t s = m->stream;
     MCHK(m, hipMemcpyAsync(grads, m->G, (size_t)mlp_nparam() * 4, hipMemcpyDeviceToHost, s));
     MCHK(m, hipStreamSynchronize(s));
     return TFIM_OK;
@@ -184,10 +188,10 @@ int tfim_mlp_train_step(tfim_mlp *m, const float *x, const float *y, const float
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = (float)(1.0 - std::pow(0.9, (double)m->step));
     const float c2 = (float)(1.0 - std::pow(0.999, (double)m->step));
-    hipStream_t s = ctx_stream(m->ctx);
+    hipStream_t s = m->stream;
     launch_mlp_adam(m->P, m->G, m->Mo, m->Vo, m->Pb, lr, b1, b2, 1e-8f, c1, c2, s);
     MCHK(m, hipGetLastError());
-    MCHK(m, hipStreamSynchronize(s));
+    if (loss) MCHK(m, hipStreamSynchronize(s));   // without a loss request the step stays queued
     return TFIM_OK;
 }
 

@@ -53,12 +53,21 @@ struct tfim_ctx {
         int64_t ticket = -1, n_out = 0, n_rows = 0;
         int tuned = -1;
         double work = 0;
+        // one executable graph per count-kernel function: H2D descriptor
+        // copy -> output fill -> ev0 -> count kernel -> ev1 -> D2H copy
+        struct Graph {
+            const void *func = nullptr;
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            hipGraphNode_t h2d = nullptr, fill = nullptr, e0 = nullptr, kern = nullptr, e1 = nullptr, d2h = nullptr;
+        };
+        std::vector<Graph> graphs;
     };
     Slot slots[2];
     int64_t next_ticket = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
-    int64_t wave_launches = 0, wave_graph_replays = 0, wave_rows = 0, wave_lds_launches = 0;
+    int64_t wave_launches = 0, wave_graph_replays = 0, wave_graphs = 0, wave_rows = 0, wave_lds_launches = 0;
     double wave_ms = 0;
     int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel
     // online kernel choice for waves both kernels accept: EMA of kernel
@@ -67,7 +76,8 @@ struct tfim_ctx {
     int64_t tune_samples[2] = {0, 0};
     int64_t tune_calls = 0;
     int tile_cols = 32;
-    int mfma_tile = 0;      // 0 auto, 128 or 256
+    int mfma_tile = 0;      // 0 auto (= 256), 128 or 256
+    int use_graphs = 1;     // waves replayed as instantiated graphs
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
 };
 
@@ -161,6 +171,10 @@ void tfim_close(tfim_ctx *c) {
         if (sl.ev0) hipEventDestroy(sl.ev0);
         if (sl.ev1) hipEventDestroy(sl.ev1);
         if (sl.done) hipEventDestroy(sl.done);
+        for (auto &g : sl.graphs) {
+            if (g.x) hipGraphExecDestroy(g.x);
+            if (g.g) hipGraphDestroy(g.g);
+        }
     }
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -300,6 +314,71 @@ int tfim_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t *out) 
     return TFIM_OK;
 }
 
+namespace {
+
+hipKernelNodeParams node_params(KernelLaunch &L) {
+    hipKernelNodeParams p{};
+    p.func = const_cast<void *>(L.func);
+    p.gridDim = L.grid;
+    p.blockDim = L.block;
+    p.sharedMemBytes = L.shmem;
+    p.kernelParams = L.params();
+    p.extra = nullptr;
+    return p;
+}
+
+// Issues one wave on a slot: descriptor copy, output fill, the count kernel
+// between the slot's timing events, output copy.  With graphs on, the six
+// operations form one graph per (slot, kernel function), instantiated on
+// first use and replayed with this wave's node parameters; the slot is idle
+// (its last wave waited on) whenever it is issued again.
+int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_out, KernelLaunch &K) {
+    KernelLaunch Z;
+    plan_zero_i32(sl.d_out, n_out, Z);
+    const size_t out_bytes = (size_t)n_out * 4;
+    if (!c->use_graphs) {
+        HIPCHK(c, hipMemcpyAsync(sl.d_desc, sl.h_desc, (size_t)desc_bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, Z.launch(c->stream));
+        HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
+        HIPCHK(c, K.launch(c->stream));
+        HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
+        HIPCHK(c, hipMemcpyAsync(sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost, c->stream));
+        return TFIM_OK;
+    }
+    hipKernelNodeParams zp = node_params(Z), kp = node_params(K);
+    tfim_ctx::Slot::Graph *G = nullptr;
+    for (auto &g : sl.graphs)
+        if (g.func == K.func) G = &g;
+    if (!G) {
+        sl.graphs.emplace_back();
+        G = &sl.graphs.back();
+        G->func = K.func;
+        HIPCHK(c, hipGraphCreate(&G->g, 0));
+        HIPCHK(c, hipGraphAddMemcpyNode1D(&G->h2d, G->g, nullptr, 0, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
+                                          hipMemcpyHostToDevice));
+        HIPCHK(c, hipGraphAddKernelNode(&G->fill, G->g, &G->h2d, 1, &zp));
+        HIPCHK(c, hipGraphAddEventRecordNode(&G->e0, G->g, &G->fill, 1, sl.ev0));
+        HIPCHK(c, hipGraphAddKernelNode(&G->kern, G->g, &G->e0, 1, &kp));
+        HIPCHK(c, hipGraphAddEventRecordNode(&G->e1, G->g, &G->kern, 1, sl.ev1));
+        HIPCHK(c, hipGraphAddMemcpyNode1D(&G->d2h, G->g, &G->e1, 1, sl.h_out, sl.d_out, out_bytes,
+                                          hipMemcpyDeviceToHost));
+        HIPCHK(c, hipGraphInstantiate(&G->x, G->g, nullptr, nullptr, 0));
+        c->wave_graphs++;
+    } else {
+        HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->h2d, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
+                                                    hipMemcpyHostToDevice));
+        HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->fill, &zp));
+        HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->kern, &kp));
+        HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->d2h, sl.h_out, sl.d_out, out_bytes,
+                                                    hipMemcpyDeviceToHost));
+    }
+    HIPCHK(c, hipGraphLaunch(G->x, c->stream));
+    c->wave_graph_replays++;
+    return TFIM_OK;
+}
+
+}  // namespace
+
 int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
                             const int32_t *col_ids, const int32_t *col_off, int32_t path, int64_t *ticket) {
     if (!c || !item_ids || !item_off || !ticket || n_rows < 0)
@@ -425,10 +504,7 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         for (int32_t r = 0; r <= n_rows; r++) coff[r] = r * c->n_cols;
     }
     memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
-    HIPCHK(c, hipMemcpyAsync(sl.d_desc, sl.h_desc, (size_t)total, hipMemcpyHostToDevice, c->stream));
-
-    launch_zero_i32(sl.d_out, n_out, c->stream);
-    HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
+    KernelLaunch K;
     if (lds) {
         WaveLdsDesc d;
         d.slot_rows = reinterpret_cast<const int32_t *>(sl.d_desc + o_slots);
@@ -441,7 +517,7 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         d.n_slots = (int32_t)slot_rows.size();
         d.n_tiles = (int32_t)tiles.size();
         d.n_out = (int32_t)n_out;
-        if (!launch_count_wave_lds(c->d_bm, c->pitch, d, c->n_cu, c->stream))
+        if (!plan_count_wave_lds(c->d_bm, c->pitch, d, c->n_cu, K))
             return fail(c, TFIM_E_RANGE, "count_nodes: LDS kernel rejected %d slots", d.n_slots);
         c->wave_lds_launches++;
     } else {
@@ -460,11 +536,9 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         int64_t blocks = (units + 3) / 4;
         blocks = std::min<int64_t>(blocks, (int64_t)c->n_cu * 8);
         blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
-        launch_count_wave(c->d_bm, c->pitch, d, (int)blocks, rw, c->stream);
+        plan_count_wave(c->d_bm, c->pitch, d, (int)blocks, rw, K);
     }
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
-    HIPCHK(c, hipMemcpyAsync(sl.h_out, sl.d_out, (size_t)n_out * 4, hipMemcpyDeviceToHost, c->stream));
+    if (int r = issue_wave(c, sl, total, n_out, K)) return r;
     HIPCHK(c, hipEventRecord(sl.done, c->stream));
     sl.busy = true;
     sl.ticket = c->next_ticket++;
@@ -515,8 +589,12 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
         c->wave_kernel = (int)value;
         return TFIM_OK;
     }
-    if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256 || value == 257)) {
+    if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256)) {
         c->mfma_tile = (int)value;
+        return TFIM_OK;
+    }
+    if (k == "graphs" && (value == 0 || value == 1)) {
+        c->use_graphs = (int)value;
         return TFIM_OK;
     }
     if (k == "tile_cols" && value >= 8 && value <= 64) {
@@ -532,6 +610,13 @@ int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *lds_launches, int64
     if (launches) *launches = c->wave_launches;
     if (rows) *rows = c->wave_rows;
     if (ms) *ms = c->wave_ms;
+    return TFIM_OK;
+}
+
+int tfim_wave_graph_stats(tfim_ctx *c, int64_t *graphs, int64_t *replays) {
+    if (!c) return TFIM_E_INVALID;
+    if (graphs) *graphs = c->wave_graphs;
+    if (replays) *replays = c->wave_graph_replays;
     return TFIM_OK;
 }
 
@@ -608,8 +693,8 @@ static void popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfi
     if (path == TFIM_PATH_PACKED) {
         launch_popmat_packed(a, bb, cc, b, d, pitch, split_k, c->stream);
     } else {
-        const int tile = c->mfma_tile ? c->mfma_tile : 128;
-        if (tile == 256 || tile == 257) launch_popmat_mfma256(a, bb, cc, b, d, pitch, split_k, c->stream, tile == 257);
+        const int tile = c->mfma_tile ? c->mfma_tile : 256;
+        if (tile == 256) launch_popmat_mfma_rp(a, bb, cc, b, d, pitch, split_k, c->stream);
         else launch_popmat_mfma(a, bb, cc, b, d, pitch, split_k, c->stream);
     }
 }

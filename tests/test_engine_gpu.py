@@ -50,16 +50,17 @@ def _gen(cfg, path, rows=None):
     subprocess.run(cmd, check=True)
 
 
-@pytest.mark.parametrize("cfg,thr,rows", [(1, 0.6, None), (5, 0.12, None), (4, 0.9, 6000), (3, 0.02, 100000)])
+@pytest.mark.parametrize("cfg,thr,rows", [("dense-small", 0.6, None), ("gene", 0.12, None),
+                                          ("dense-replicated", 0.9, 6000), ("clickstream", 0.02, 100000)])
 def test_synthetic_configs_match_oracle(tmp_path, cfg, thr, rows):
-    data = tmp_path / ("c%d.txt" % cfg)
+    data = tmp_path / ("%s.txt" % cfg)
     _gen(cfg, data, rows)
     ref = tmp_path / "oracle.res"
     oracle_lib.mine(data, thr, threads=16, out=ref)
     exp = tf.load_results(str(ref))
     assert exp, "oracle found no MFI"
     for name, extra in ENGINES:
-        if name == "plain" and cfg == 5:
+        if name == "plain" and cfg == "gene":
             continue   # 19k launches; the wave engine runs the same enumeration
         out = tmp_path / ("%s%s.res" % (name, "-l" if extra else ""))
         path, stats = tf.mine(str(data), thr, engine=name, out=str(out), extra=extra, timeout=600)
@@ -67,11 +68,11 @@ def test_synthetic_configs_match_oracle(tmp_path, cfg, thr, rows):
         assert tf.load_results(path) == exp, (name, extra)
 
 
-def test_replicated_config4_scales_exactly(tmp_path):
+def test_replicated_dense_scales_exactly(tmp_path):
     """x k replication scales every support by k; thresholds 0.9*N are
     non-integers for both sizes, so the MFI set is unchanged."""
     data = tmp_path / "c4.txt"
-    _gen(4, data, 3001)
+    _gen("dense-replicated", data, 3001)
     base = tmp_path / "base.res"
     oracle_lib.mine(data, 0.9, threads=16, out=base)
     exp = {s: 16 * f for s, f in tf.load_results(str(base)).items()}

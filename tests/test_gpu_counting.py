@@ -117,10 +117,11 @@ def test_fill_bernoulli_matches_oracle(ctx):
     buf.free()
 
 
-@pytest.mark.parametrize("path", ["packed", "mfma", "mfma128", "mfma256", "mfma257"])
+@pytest.mark.parametrize("path", ["packed", "mfma", "mfma128", "mfma256"])
 @pytest.mark.parametrize("b,d,n_bits,split", [(130, 70, 4096 + 65, 1), (256, 256, 65_536, 1),
                                                (200, 513, 40_000, 3), (64, 33, 2048 * 9, 8),
-                                               (1, 1, 64, 1), (515, 300, 2048 * 3 + 1, 2)])
+                                               (1, 1, 64, 1), (515, 300, 2048 * 3 + 1, 2),
+                                               (40, 24, (1 << 24) + 2048 * 3, 1)])
 def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
     rng = np.random.default_rng(b * 1000 + d)
     pitch = tf.pitch_words(n_bits)
@@ -132,7 +133,7 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
     dA, dB, dC = ctx.buffer(A.nbytes), ctx.buffer(B.nbytes), ctx.buffer(b * d * 4)
     dA.write(A)
     dB.write(B)
-    tile = {"mfma128": 128, "mfma256": 256, "mfma257": 257}.get(path, 0)
+    tile = {"mfma128": 128, "mfma256": 256}.get(path, 0)
     ctx.config("mfma_tile", tile)
     try:
         ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path="packed" if path == "packed" else "mfma", split_k=split)
@@ -144,16 +145,19 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
         x.free()
 
 
+@pytest.mark.parametrize("graphs", [1, 0])
 @pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("n_bits,n_items", [(1100, 30), (3200, 75), (400_003, 90), (2_000_000, 38)])
-def test_wave_kernels_forced(ctx, kernel, n_bits, n_items):
-    """Both batched kernels (1 = L2-resident, 2 = LDS-staged) on the same waves:
-    uneven prefixes, uneven and empty tails, full-column rows."""
+def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
+    """Both batched kernels (1 = L2-resident, 2 = LDS-staged) on the same waves,
+    replayed from graphs or issued directly: uneven prefixes, uneven and empty
+    tails, full-column rows."""
     rng = np.random.default_rng(kernel * 100 + n_items)
     rows = rand_rows(rng, n_items, n_bits, 0.65)
     n_cols = n_items - 1
     ctx.load_bitmaps(rows, n_bits, n_cols)
     ctx.config("wave_kernel", kernel)
+    ctx.config("graphs", graphs)
     try:
         prefixes, tails = [], []
         for i in range(120):
@@ -168,3 +172,25 @@ def test_wave_kernels_forced(ctx, kernel, n_bits, n_items):
             np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols))
     finally:
         ctx.config("wave_kernel", 0)
+        ctx.config("graphs", 1)
+
+
+def test_wave_graph_replays(ctx):
+    """Graph replays with changing wave sizes, interleaved with eager work and
+    host synchronisation, each checked against the oracle."""
+    rng = np.random.default_rng(77)
+    n_bits, n_items = 50_001, 40
+    rows = rand_rows(rng, n_items, n_bits, 0.6)
+    ctx.load_bitmaps(rows, n_bits, n_items)
+    before = ctx.wave_stats()
+    for rep, n in enumerate([50, 3, 200, 1, 120, 7, 60]):
+        prefixes = [rng.choice(n_items, size=int(rng.integers(1, 6)), replace=False) for _ in range(n)]
+        tails = [np.sort(rng.choice(n_items, size=int(rng.integers(0, 25)), replace=False)) for _ in range(n)]
+        got = ctx.count_nodes(prefixes, tails)
+        for p, t, g in zip(prefixes, tails, got):
+            np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_items)[t], err_msg="replay %d" % rep)
+        # eager work between replays
+        np.testing.assert_array_equal(ctx.count_node(prefixes[0]), oracle_lib.node_counts(rows, prefixes[0], n_items))
+    after = ctx.wave_stats()
+    assert after["graph_launches"] - before["graph_launches"] >= 7
+    assert after["graphs"] - before["graphs"] <= 8   # instantiated once per (slot, kernel)
