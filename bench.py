@@ -1,6 +1,6 @@
 """Benchmark: exact support counting on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--no-cpu] [--paths mfma,packed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--no-cpu] [--tuned] [--paths mfma,packed]
     python bench.py --copy-calibration MIB     (a plain device copy, for counter scaling)
 
 A step is one pass of the hot path over one batch: the isolated count
@@ -38,26 +38,32 @@ VALU_PEAK_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz int32 ops
 # HBM bytes per launch of the default int8-MFMA count kernel on this workload,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
 # `bench.py --no-e2e --no-cpu --paths mfma`, each scaled by the same counter's
-# reading for a device copy of known size (`bench.py --copy-calibration 1024`);
-# profiles/popmat_mfma_rp_65536x512x4M_traffic.md holds the readings.  None until
-# measured for the kernel below.
+# reading for device traffic of known size (`bench.py --copy-calibration 1024`:
+# WRITE_SIZE exact; FETCH_SIZE exact for 64-byte reads, the pattern of this
+# kernel); profiles/popmat_mfma_rp_65536x512x4M_traffic.md holds the readings.
+# Read 37,393,180 KiB + written 23,137,734 KiB (mean over dispatches).
 TRAFFIC_KERNEL = "k_popmat_mfma_rp"
-TRAFFIC_BYTES = None
+TRAFFIC_BYTES = (37393180 + 23137734) * 1024.0
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(A_rows, B_rows, n_bits, threads):
-    """The oracle's popcount (bmma_bench.cu cpu_ref semantics) on a sample."""
+def cpu_baseline(A_rows, B_rows, n_bits, threads, min_seconds=10.0):
+    """The oracle's popcount (bmma_bench.cu cpu_ref semantics) on a sample,
+    repeated until at least `min_seconds` of CPU work has been timed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    t0 = time.perf_counter()
-    C = oracle_lib.popcount_matrix(A_rows, B_rows, threads=threads)
-    dt = time.perf_counter() - t0
-    bitops = A_rows.shape[0] * B_rows.shape[0] * n_bits
-    return C, bitops / dt / 1e12, dt
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        C = oracle_lib.popcount_matrix(A_rows, B_rows, threads=threads)
+        reps += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            break
+    bitops = reps * A_rows.shape[0] * B_rows.shape[0] * n_bits
+    return C, bitops / dt / 1e12, dt, reps
 
 
 def bench_isolated(ctx, args):
@@ -99,6 +105,14 @@ def bench_isolated(ctx, args):
         log("%-6s %.3f ms/launch  %.1f Tbit-op/s  exact=%s" % (path, ms, tbit, ok))
         if not ok:
             raise SystemExit("support counts differ from the oracle on sampled rows (%s path)" % path)
+        if args.tuned:
+            # the library's tuner (split_k = 0): its choice for this shape and its time
+            tile, tsplit = ctx.popcount_matrix_plan(A, B, C, b, d, pitch, path=path)
+            tms = ctx.time_popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=0, warmup=1, reps=args.steps)
+            got = np.stack([C.read(np.int32, d, offset=int(r) * d * 4) for r in sample_rows])
+            res[path]["tuned"] = {"tile": tile, "split_k": tsplit, "kernel_ms": tms,
+                                  "tbitops": bitops / (tms * 1e-3) / 1e12, "exact_on_sample": bool((got == C_exp).all())}
+            log("%-6s tuned: tile %d split %d  %.3f ms" % (path, tile, tsplit, tms))
     for x in (A, B, C):
         x.free()
     return res, bitops, alg_bytes, B_host, A_sample, n_bits
@@ -142,16 +156,19 @@ def bench_e2e(args):
 
 
 def copy_calibration(mib):
-    """A plain device copy of `mib` MiB (torch), three times: the reference
-    point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    """Plain device traffic of known size (torch): a copy of `mib` MiB and a
+    read-only sum over the same `mib` MiB, three times each -- the reference
+    points that scale the FETCH_SIZE / WRITE_SIZE counters to bytes."""
     import torch
     n = mib * 2 ** 20 // 4
     src = torch.ones(n, dtype=torch.int32, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
+    for _ in range(3):
+        s = src.sum()
     torch.cuda.synchronize()
-    log("copied %d MiB x3" % mib)
+    log("copied %d MiB x3, summed %d MiB x3 (sum %d)" % (mib, mib, int(s.item())))
 
 
 def main():
@@ -163,6 +180,7 @@ def main():
     ap.add_argument("--paths", default="mfma,packed")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuned", action="store_true", help="also time the tuner's tile/split choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     args.paths = [p for p in args.paths.split(",") if p]
@@ -186,10 +204,11 @@ def main():
     if not args.no_cpu:
         log("cpu baseline: oracle popcount, %d threads, 512 rows x 512 items x 2^22 ..." % threads)
         rng_rows = np.repeat(A_sample, 103, axis=0)[:512]   # 512 device-generated rows (5 distinct)
-        _, cpu_tbit, cpu_s = cpu_baseline(rng_rows, B_host, n_bits, threads)
-        log("cpu: %.3f Tbit-op/s (%.1f s)" % (cpu_tbit, cpu_s))
+        _, cpu_tbit, cpu_s, reps = cpu_baseline(rng_rows, B_host, n_bits, threads)
+        log("cpu: %.3f Tbit-op/s (%d passes, %.1f s)" % (cpu_tbit, reps, cpu_s))
         cpu = {"value": cpu_tbit, "unit": "Tbit-op/s", "cores": threads, "kind": "port",
-               "sample": "oracle AND-popcount of 512 rows x 512 items x 2^22 transactions (%.1f s)" % cpu_s}
+               "sample": "oracle AND-popcount of 512 rows x 512 items x 2^22 transactions, %d passes "
+                         "(%.1f s)" % (reps, cpu_s)}
 
     e2e = None if args.no_e2e else bench_e2e(args)
 

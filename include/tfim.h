@@ -118,7 +118,8 @@ int tfim_count_nodes_wait(tfim_ctx *ctx, int64_t ticket, int32_t *out);
 /* Tuning knobs of the batched path (ablation and autotuning; the reference's
  * equivalents are environment variables, kernel_bmma.cu:351-368):
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel
- *   "tile_cols"    columns per work tile (8..64, default 32)
+ *   "tile_cols"    columns per work tile of the batched path: 8..64, or 0
+ *                  (default) to let the wave tuner choose
  *   "mfma_tile"    workgroup tile of the int8-MFMA matrix count: 0 auto
  *                  (= 256), 256 (256 x 128 outputs, operand bytes expanded
  *                  in registers from packed bits in LDS) or 128 (128 x 128,
@@ -132,8 +133,10 @@ int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
 
 /* Statistics of the batched path since tfim_open: launches, how many used
  * the LDS-staged kernel, rows, summed kernel milliseconds (HIP events).  In
- * TFIM_PATH_AUTO the library times both kernels on waves that fit either and
- * keeps the cheaper per row-word of work (re-sampling every 64 waves). */
+ * TFIM_PATH_AUTO the library's wave tuner picks, for waves that fit both
+ * kernels, the kernel and the columns per work tile (L2 kernel with 32; LDS
+ * kernel with 16, 32 or 64): each arm is timed three times, then the cheapest
+ * per row-word of work is kept and another arm re-timed every 16 waves. */
 int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,
                     int64_t *rows, double *kernel_ms);
 
@@ -163,10 +166,20 @@ int tfim_fill_bernoulli(tfim_ctx *ctx, tfim_buf *dst, int64_t rows, int64_t n_bi
 /* C[i][j] = popcount(A_i AND B_j) over pitch_words words, int32, row-major
  * [b][d] in C.  Replaces bmma_kernel / bmma_smem_kernel / bitmap_kernel of
  * code/microbench/bmma_bench.cu:50-174.  split_k >= 1 splits the word axis;
- * 0 lets the library choose.  Asynchronous on the ctx stream. */
+ * 0 lets the library's tuner choose the split and (int8-MFMA path, unless
+ * "mfma_tile" is configured) the tile: the first call for a (path, b, d,
+ * pitch) shape times every candidate once on these buffers and blocks; the
+ * choice is kept for the shape.  Otherwise asynchronous on the ctx stream. */
 int tfim_popcount_matrix(tfim_ctx *ctx, const tfim_buf *A, const tfim_buf *B, tfim_buf *C,
                          int64_t b, int64_t d, int64_t pitch_words, int32_t path,
                          int32_t split_k);
+
+/* The tile (64 for the packed path) and split_k that split_k == 0 selects for
+ * this shape, tuning first if needed.  No counterpart in the reference (its
+ * split factor is fixed, kernel_bmma.cu:715-758). */
+int tfim_popcount_matrix_plan(tfim_ctx *ctx, const tfim_buf *A, const tfim_buf *B, tfim_buf *C,
+                              int64_t b, int64_t d, int64_t pitch_words, int32_t path,
+                              int32_t *tile, int32_t *split_k);
 
 /* Times `reps` back-to-back tfim_popcount_matrix launches after `warmup`
  * untimed ones with HIP events on the ctx stream; avg_ms = mean per launch. */
@@ -192,6 +205,12 @@ int tfim_mlp_get_params(tfim_mlp *mlp, float *params);
 int tfim_mlp_set_params(tfim_mlp *mlp, const float *params);
 /* logits[n][2] for x[n][14]; bf16 != 0 uses the bf16 weight copy. */
 int tfim_mlp_forward(tfim_mlp *mlp, const float *x, int32_t n, float *logits, int32_t bf16);
+/* The same forward split in two: submit copies x (n >= 1) and queues the
+ * inference on the model's stream, returning at once; wait blocks until it
+ * has finished and writes logits[n][2].  One request may be outstanding;
+ * training steps queued before or after it run in submission order. */
+int tfim_mlp_forward_submit(tfim_mlp *mlp, const float *x, int32_t n, int32_t bf16);
+int tfim_mlp_forward_wait(tfim_mlp *mlp, float *logits);
 /* Gradient of the loss w.r.t. all parameters (no update). */
 int tfim_mlp_gradients(tfim_mlp *mlp, const float *x, const float *labels, const float *mask, int32_t n,
                        float *grads, float *loss);

@@ -65,6 +65,9 @@ _SIGS = {
     "tfim_popcount_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32]),
+    "tfim_popcount_matrix_plan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                                  _p(ctypes.c_int32), _p(ctypes.c_int32)]),
     "tfim_time_popcount_matrix": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -76,6 +79,8 @@ _SIGS = {
     "tfim_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "tfim_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_int32]),
+    "tfim_mlp_forward_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "tfim_mlp_forward_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "tfim_mlp_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_int32, ctypes.c_void_p, _p(ctypes.c_float)]),
     "tfim_mlp_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -241,6 +246,13 @@ class Context:
         self._check(self._lib.tfim_popcount_matrix(self._h, A._h, B._h, C._h, int(b), int(d), int(pitch),
                                                    _PATHS[path], int(split_k)))
 
+    def popcount_matrix_plan(self, A, B, C, b, d, pitch, path="auto"):
+        """(tile, split_k) the tuner picks for this shape (split_k=0 calls)."""
+        t, k = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self._lib.tfim_popcount_matrix_plan(self._h, A._h, B._h, C._h, int(b), int(d), int(pitch),
+                                                        _PATHS[path], ctypes.byref(t), ctypes.byref(k)))
+        return t.value, k.value
+
     def time_popcount_matrix(self, A, B, C, b, d, pitch, path="auto", split_k=1, warmup=1, reps=3):
         ms = ctypes.c_double()
         self._check(self._lib.tfim_time_popcount_matrix(self._h, A._h, B._h, C._h, int(b), int(d), int(pitch),
@@ -282,6 +294,16 @@ class Mlp:
         x = np.ascontiguousarray(x, dtype=np.float32)
         out = np.empty((x.shape[0], MLP_OUTPUTS), dtype=np.float32)
         self.ctx._check(self.ctx._lib.tfim_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(out), int(bf16)))
+        return out
+
+    def forward_submit(self, x, bf16=False):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        self.ctx._check(self.ctx._lib.tfim_mlp_forward_submit(self._h, _ptr(x), x.shape[0], int(bf16)))
+        self._pending_rows = x.shape[0]
+
+    def forward_wait(self):
+        out = np.empty((self._pending_rows, MLP_OUTPUTS), dtype=np.float32)
+        self.ctx._check(self.ctx._lib.tfim_mlp_forward_wait(self._h, _ptr(out)))
         return out
 
     def gradients(self, x, y, m):

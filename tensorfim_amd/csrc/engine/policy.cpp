@@ -73,29 +73,61 @@ class LearnedPolicy : public Policy {
         thr_ = std::max(1.0, db.thr_abs);
         freq_ = &db.freq_dense;
         plain_.decide(ds, db);   // plain order and the independence-estimate rule
-        // features: one gate row per node with a tail of two or more; item rows
-        // for the ranker only when it reorders tails
+        // features: one gate row per node with a tail of two or more (kept as
+        // a training record when its lookahead runs); the model is asked only
+        // about nodes the rule declines with tails of at most kMaxGateTail,
+        // plus the ranker rows of tails it reorders
         feats_.clear();
+        infer_.clear();
         node_row_.assign(ds.size(), -1);
         item_row0_.assign(ds.size(), -1);
+        infer_row_.assign(ds.size(), -1);
+        item_infer0_.assign(ds.size(), -1);
+        const bool use_model = trained_ && mlp_;
+        // gate-only: the model's verdicts are settled one wave later (resolve)
+        const bool deferred = use_model && !opt_.rank;
         for (size_t q = 0; q < ds.size(); q++) {
             Decision *d = ds[q];
             if (d->head.empty() || d->tail.size() < 2) continue;
             node_features(*d);
             node_row_[q] = (int64_t)(feats_.size() / NF) - 1;
+            if (use_model && !d->lookahead && d->tail.size() <= kMaxGateTail) {
+                infer_row_[q] = (int64_t)(infer_.size() / NF);
+                infer_.insert(infer_.end(), feats_.end() - NF, feats_.end());
+            }
             if (opt_.rank && d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail) {
                 // rank of each item in the plain (increasing support) order
                 rank_.assign(d->tail.size(), 0);
                 for (size_t j = 0; j < d->order.size(); j++) rank_[d->order[j]] = (int32_t)j;
-                item_row0_[q] = node_row_[q] + 1;
+                item_row0_[q] = (int64_t)(feats_.size() / NF);
                 for (size_t i = 0; i < d->tail.size(); i++) item_features(*d, i);
+                if (use_model) {
+                    const int64_t first = (int64_t)(infer_.size() / NF);
+                    infer_.insert(infer_.end(), feats_.end() - (ptrdiff_t)(d->tail.size() * NF), feats_.end());
+                    item_infer0_[q] = first;
+                }
             }
         }
-        const int64_t n = (int64_t)(feats_.size() / NF);
-        if (n == 0) return;
-        if (trained_ && mlp_) {
+        if (feats_.empty()) return;
+        const int64_t n = (int64_t)(infer_.size() / NF);
+        if (deferred) {
+            if (n > 0) {
+                if (tfim_mlp_forward_submit(mlp_, infer_.data(), (int32_t)n, 1) != TFIM_OK) {
+                    trained_ = false;   // no model: the rule decides from here on
+                } else {
+                    pend_feats_ = infer_;
+                    pend_n_ = n;
+                    forward_calls_++;
+                    forward_rows_ += n;
+                }
+            }
+        } else if (use_model && n > 0) {
+            const auto tf0 = std::chrono::steady_clock::now();
             logits_.resize((size_t)n * 2);
-            if (tfim_mlp_forward(mlp_, feats_.data(), (int32_t)n, logits_.data(), 1) != TFIM_OK) trained_ = false;
+            if (tfim_mlp_forward(mlp_, infer_.data(), (int32_t)n, logits_.data(), 1) != TFIM_OK) trained_ = false;
+            forward_seconds_ += secs_since(tf0);
+            forward_calls_++;
+            forward_rows_ += n;
         }
         for (size_t q = 0; q < ds.size(); q++) {
             Decision *d = ds[q];
@@ -104,21 +136,25 @@ class LearnedPolicy : public Policy {
             if (!trained_) {
                 // warm-up: classical lookahead on bounded tails so that both
                 // outcomes get labelled
-                d->lookahead = rule || d->tail.size() <= 64;
-            } else {
+                d->lookahead = rule || d->tail.size() <= kMaxGateTail;
+            } else if (!rule && d->tail.size() <= kMaxGateTail) {
+                if (deferred) {
+                    d->pending = true;   // settled by resolve() from the queued inference
+                    continue;
+                }
                 // the model adds the lookaheads it expects to succeed to the
                 // rule's; a small exploration share keeps negative labels coming
-                const float pg = sigmoid(logits_[node_row_[q] * 2 + 0]);
-                const bool explore = d->tail.size() <= 64 && rand01() < kExplore;
-                d->lookahead = rule || pg >= kGateP || explore;
-                if (d->lookahead && !rule) model_gated_++;
-                if (item_row0_[q] >= 0) {
-                    // items predicted to keep >= 90% of their child's tail go
-                    // last (they sit in every earlier child's head+tail)
-                    std::stable_partition(d->order.begin(), d->order.end(), [&](int32_t a) {
-                        return sigmoid(logits_[(item_row0_[q] + a) * 2 + 1]) < 0.9f;
-                    });
-                }
+                const bool model = infer_row_[q] >= 0 && sigmoid(logits_[infer_row_[q] * 2 + 0]) >= kGateP;
+                d->lookahead = model || rand01() < kExplore;
+                if (d->lookahead) model_gated_++;
+            }
+            if (trained_ && item_row0_[q] >= 0 && n > 0 && !deferred) {
+                // items predicted to keep >= 90% of their child's tail go
+                // last (they sit in every earlier child's head+tail)
+                const int64_t base = item_infer0_[q];
+                std::stable_partition(d->order.begin(), d->order.end(), [&](int32_t a) {
+                    return sigmoid(logits_[(base + a) * 2 + 1]) < 0.9f;
+                });
             }
             // the gate record is labelled only if the lookahead runs
             d->record = d->lookahead ? add_record(&feats_[node_row_[q] * NF], 0) : -1;
@@ -128,6 +164,28 @@ class LearnedPolicy : public Policy {
                     d->item_records[i] = add_record(&feats_[(item_row0_[q] + i) * NF], 1);
             }
         }
+    }
+
+    void resolve(std::vector<Decision *> &ds) override {
+        // ds: the decisions decide() left pending, in the order it saw them;
+        // their feature rows are the rows of the queued inference
+        if (ds.empty()) return;
+        bool ok = pend_n_ == (int64_t)ds.size();
+        if (pend_n_ > 0) {
+            const auto t0 = std::chrono::steady_clock::now();
+            logits_.resize((size_t)pend_n_ * 2);
+            ok = tfim_mlp_forward_wait(mlp_, logits_.data()) == TFIM_OK && ok;
+            forward_seconds_ += secs_since(t0);
+        }
+        for (size_t q = 0; q < ds.size(); q++) {
+            Decision *d = ds[q];
+            const bool model = ok && sigmoid(logits_[q * 2 + 0]) >= kGateP;
+            d->lookahead = model || rand01() < kExplore;
+            d->pending = false;
+            if (d->lookahead) model_gated_++;
+            d->record = d->lookahead && ok ? add_record(&pend_feats_[q * NF], 0) : -1;
+        }
+        pend_n_ = 0;
     }
 
     void observe(int32_t record, int outcome) override {
@@ -143,6 +201,10 @@ class LearnedPolicy : public Policy {
         st.mlp_train_steps = steps_;
         st.model_lookaheads = model_gated_;
         st.learn_seconds += extra_seconds_;   // set-up and training (decide() is timed by the engine)
+        if (opt_.verbose)
+            fprintf(stderr, "learned policy: set-up+training %.4f s (%lld steps), forward %.4f s in %lld calls (%lld rows)\n",
+                    extra_seconds_, (long long)steps_, forward_seconds_, (long long)forward_calls_,
+                    (long long)forward_rows_);
         if (mlp_ && !opt_.model_path.empty() && opt_.learned && trained_) {
             // keep what was learned for the next run on the same data
             std::vector<float> p(TFIM_MLP_PARAMS);
@@ -298,6 +360,7 @@ class LearnedPolicy : public Policy {
     static constexpr int64_t kRing = 1 << 16;   // labels arrive within a few waves
     static constexpr int kEpochs = 4;
     static constexpr size_t kMaxRankTail = 128;   // longer tails keep the plain order
+    static constexpr size_t kMaxGateTail = 64;     // the model gates tails of 2..64 items
     static constexpr float kGateP = 0.5f;          // model-added lookahead: P(HUT frequent) >= this
     static constexpr float kExplore = 0.02f;       // share of rule-declined lookaheads run anyway
 
@@ -311,9 +374,12 @@ class LearnedPolicy : public Policy {
     const std::vector<int32_t> *freq_ = nullptr;
     NodeAgg cur_{};
     std::vector<int32_t> rank_;
-    std::vector<int64_t> node_row_, item_row0_;
+    std::vector<int64_t> node_row_, item_row0_, infer_row_, item_infer0_;
+    std::vector<float> infer_, pend_feats_;
+    int64_t pend_n_ = 0;   // rows of the inference queued for pending decisions
     int64_t model_gated_ = 0;
-    double extra_seconds_ = 0;
+    double extra_seconds_ = 0, forward_seconds_ = 0;
+    int64_t forward_calls_ = 0, forward_rows_ = 0;
     std::vector<float> feats_, logits_;
     std::vector<float> rec_x_ = std::vector<float>((size_t)kRing * NF);
     std::vector<float> rec_y_ = std::vector<float>((size_t)kRing * 2), rec_m_ = std::vector<float>((size_t)kRing * 2);

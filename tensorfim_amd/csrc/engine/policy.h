@@ -26,6 +26,7 @@ struct Decision {
     int32_t depth = 0;
     std::vector<int32_t> order;     // out: permutation of tail indices
     bool lookahead = false;         // out: count head+tail before expanding
+    bool pending = false;           // out: lookahead undecided until resolve()
     int32_t record = -1;            // out: gate record id for observe()
     std::vector<int32_t> item_records;  // out: ranker record id per tail index
 };
@@ -34,6 +35,10 @@ class Policy {
    public:
     virtual ~Policy() {}
     virtual void decide(std::vector<Decision *> &ds, const Database &db) = 0;
+    // Settles the decisions decide() left pending (lookahead, record).  The
+    // engine calls it one wave later, so work queued on the device for them
+    // (model inference) overlaps the next wave's count.
+    virtual void resolve(std::vector<Decision *> &ds) { (void)ds; }
     // outcome: 1 = HUT frequent, 0 = not frequent, -1 = lookahead not taken
     virtual void observe(int32_t record, int outcome) { (void)record; (void)outcome; }
     // ranker label of a child: fraction of its tail that stayed frequent

@@ -380,6 +380,7 @@ class Mafia {
     };
 
     int launch(Flight &f, std::string &err) {
+        if (stack_.empty()) settle_undecided();   // nothing else left to count
         f.wave.clear();
         int64_t outs = 0;
         while ((int)f.wave.size() < cap_ && !stack_.empty()) {
@@ -482,6 +483,7 @@ class Mafia {
     }
 
     int process_wave(Flight &f, std::vector<Mfi> &out) {
+        settle_undecided();
         std::vector<MNode> &wave_ = f.wave;
         WaveBatch &wb_ = f.wb;
         std::vector<Decision> decisions;
@@ -588,25 +590,44 @@ class Mafia {
         hut_children_.clear();
         for (size_t q = decisions.size(); q-- > 0;) {
             Decision &d = decisions[q];
-            if (d.lookahead && d.tail.size() >= 2) {
-                MNode h;
-                h.kind = MNode::HUT;
-                h.head = d.head;
-                h.sup = d.sup;
-                h.depth = d.depth;
-                for (int32_t o : d.order) {
-                    h.tail.push_back(d.tail[o]);
-                    h.tail_sup.push_back(d.tail_sup[o]);
-                }
-                h.decision = d.record;
-                stack_.push_back(std::move(h));
-            } else {
-                if (d.tail.size() >= 2) st_.hut_skipped++;
-                pol_.observe(d.record, -1);
-                push_children(d);
-            }
+            if (d.pending) undecided_.push_back(std::move(d));   // settled after the next wave
+            else apply(d);
         }
         return TFIM_OK;
+    }
+
+    // Stacks a decided node's work: its head-union-tail row, or its children.
+    void apply(Decision &d) {
+        if (d.lookahead && d.tail.size() >= 2) {
+            MNode h;
+            h.kind = MNode::HUT;
+            h.head = d.head;
+            h.sup = d.sup;
+            h.depth = d.depth;
+            for (int32_t o : d.order) {
+                h.tail.push_back(d.tail[o]);
+                h.tail_sup.push_back(d.tail_sup[o]);
+            }
+            h.decision = d.record;
+            stack_.push_back(std::move(h));
+        } else {
+            if (d.tail.size() >= 2) st_.hut_skipped++;
+            pol_.observe(d.record, -1);
+            push_children(d);
+        }
+    }
+
+    // Settles the decisions the policy left pending one wave earlier (kept in
+    // reverse stacking order) and stacks their work.
+    void settle_undecided() {
+        if (undecided_.empty()) return;
+        std::vector<Decision *> ptrs;
+        for (size_t q = undecided_.size(); q-- > 0;) ptrs.push_back(&undecided_[q]);
+        const double t0 = now();
+        pol_.resolve(ptrs);
+        st_.learn_seconds += now() - t0;
+        for (auto &d : undecided_) apply(d);
+        undecided_.clear();
     }
 
     tfim_ctx *ctx_;
@@ -617,7 +638,7 @@ class Mafia {
     MfiIndex index_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
     std::vector<MNode> stack_, pending_;
-    std::vector<Decision> hut_children_;
+    std::vector<Decision> hut_children_, undecided_;
     std::vector<uint8_t> mark_;
     Flight fl_[2];
     int32_t cap_ = 256;

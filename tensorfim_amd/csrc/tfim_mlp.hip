@@ -17,6 +17,12 @@ struct tfim_mlp {
     // batch scratch (grow-only): X, Y, M, H1, H2, D1, D2, DY, loss, out
     float *buf = nullptr;
     int64_t cap = 0;
+    // pinned staging (grow-only): inference input/output, training batch;
+    // `staged` marks when the last training batch has left its staging
+    float *h_x = nullptr, *h_o = nullptr, *h_t = nullptr;
+    int64_t h_cap = 0, t_cap = 0;
+    hipEvent_t fwd_done = nullptr, staged = nullptr;
+    int32_t fwd_n = -1;   // rows of the outstanding forward_submit, -1 = none
 };
 
 namespace {
@@ -38,6 +44,7 @@ int ensure(tfim_mlp *m, int64_t n, Scratch &sc) {
     const int64_t per = IN + OUT + OUT + 4 * H + OUT + 1 + OUT;
     const int64_t part_floats = (int64_t)mlp_grad_slices((int)std::max<int64_t>(n, 4096)) * mlp_nparam();
     if (n > m->cap) {
+        hipStreamSynchronize(m->stream);   // queued steps may still use the old block
         hipFree(m->buf);
         m->buf = nullptr;
         m->cap = 0;
@@ -77,9 +84,25 @@ int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int3
     double cnt = 0;
     for (int64_t i = 0; i < (int64_t)n * OUT; i++) cnt += mask[i];
     const float inv = (float)(1.0 / std::max(1.0, cnt));
-    MCHK(m, hipMemcpyAsync(sc.X, x, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
-    MCHK(m, hipMemcpyAsync(sc.Y, y, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
-    MCHK(m, hipMemcpyAsync(sc.M, mask, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
+    // stage the batch in pinned memory so that a queued step never reads the
+    // caller's buffers; the previous batch must have left the staging first
+    const int64_t per = IN + 2 * OUT;
+    if (m->t_cap > 0) MCHK(m, hipEventSynchronize(m->staged));
+    if (n > m->t_cap) {
+        hipHostFree(m->h_t);
+        m->h_t = nullptr;
+        m->t_cap = 0;
+        if (hipHostMalloc(&m->h_t, (size_t)n * per * 4) != hipSuccess)
+            return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp pinned staging allocation failed");
+        m->t_cap = n;
+    }
+    std::memcpy(m->h_t, x, (size_t)n * IN * 4);
+    std::memcpy(m->h_t + (size_t)n * IN, y, (size_t)n * OUT * 4);
+    std::memcpy(m->h_t + (size_t)n * (IN + OUT), mask, (size_t)n * OUT * 4);
+    MCHK(m, hipMemcpyAsync(sc.X, m->h_t, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
+    MCHK(m, hipMemcpyAsync(sc.Y, m->h_t + (size_t)n * IN, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
+    MCHK(m, hipMemcpyAsync(sc.M, m->h_t + (size_t)n * (IN + OUT), (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
+    MCHK(m, hipEventRecord(m->staged, s));
     launch_mlp_fwd_bwd(m->P, sc.X, sc.Y, sc.M, n, inv, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, sc.L, s);
     launch_mlp_grad(sc.X, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, n, sc.part, m->G, s);
     MCHK(m, hipGetLastError());
@@ -104,7 +127,9 @@ int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out) {
     m->ctx = ctx;
     const int np = mlp_nparam();
     if (hipSetDevice(ctx_device(ctx)) != hipSuccess ||
-        hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess || hipMalloc(&m->P, np * 4) != hipSuccess ||
+        hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&m->fwd_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->staged, hipEventDisableTiming) != hipSuccess || hipMalloc(&m->P, np * 4) != hipSuccess ||
         hipMalloc(&m->G, np * 4) != hipSuccess || hipMalloc(&m->Mo, np * 4) != hipSuccess ||
         hipMalloc(&m->Vo, np * 4) != hipSuccess || hipMalloc(&m->Pb, np * 2) != hipSuccess) {
         tfim_mlp_destroy(m);
@@ -128,6 +153,9 @@ void tfim_mlp_destroy(tfim_mlp *m) {
     if (!m) return;
     if (m->stream) hipStreamSynchronize(m->stream);
     hipFree(m->P); hipFree(m->G); hipFree(m->Mo); hipFree(m->Vo); hipFree(m->Pb); hipFree(m->buf);
+    hipHostFree(m->h_x); hipHostFree(m->h_o); hipHostFree(m->h_t);
+    if (m->fwd_done) hipEventDestroy(m->fwd_done);
+    if (m->staged) hipEventDestroy(m->staged);
     if (m->stream) hipStreamDestroy(m->stream);
     delete m;
 }
@@ -154,18 +182,47 @@ int tfim_mlp_set_params(tfim_mlp *m, const float *params) {
     return TFIM_OK;
 }
 
+int tfim_mlp_forward_submit(tfim_mlp *m, const float *x, int32_t n, int32_t bf16) {
+    if (!m || !x || n < 1) return TFIM_E_INVALID;
+    if (m->fwd_n >= 0) return ctx_fail(m->ctx, TFIM_E_STATE, "mlp forward: a request is outstanding; wait first");
+    Scratch sc;
+    if (int r = ensure(m, n, sc)) return r;
+    if (n > m->h_cap) {
+        hipHostFree(m->h_x);
+        hipHostFree(m->h_o);
+        m->h_x = m->h_o = nullptr;
+        m->h_cap = 0;
+        if (hipHostMalloc(&m->h_x, (size_t)n * IN * 4) != hipSuccess ||
+            hipHostMalloc(&m->h_o, (size_t)n * OUT * 4) != hipSuccess)
+            return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp pinned staging allocation failed");
+        m->h_cap = n;
+    }
+    std::memcpy(m->h_x, x, (size_t)n * IN * 4);
+    hipStream_t s = m->stream;
+    MCHK(m, hipMemcpyAsync(sc.X, m->h_x, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
+    launch_mlp_infer(m->P, bf16 ? m->Pb : nullptr, sc.X, n, sc.O, s);
+    MCHK(m, hipGetLastError());
+    MCHK(m, hipMemcpyAsync(m->h_o, sc.O, (size_t)n * OUT * 4, hipMemcpyDeviceToHost, s));
+    MCHK(m, hipEventRecord(m->fwd_done, s));
+    m->fwd_n = n;
+    return TFIM_OK;
+}
+
+int tfim_mlp_forward_wait(tfim_mlp *m, float *logits) {
+    if (!m || !logits) return TFIM_E_INVALID;
+    if (m->fwd_n < 0) return ctx_fail(m->ctx, TFIM_E_STATE, "mlp forward_wait: nothing submitted");
+    const int32_t n = m->fwd_n;
+    m->fwd_n = -1;
+    MCHK(m, hipEventSynchronize(m->fwd_done));
+    std::memcpy(logits, m->h_o, (size_t)n * OUT * 4);
+    return TFIM_OK;
+}
+
 int tfim_mlp_forward(tfim_mlp *m, const float *x, int32_t n, float *logits, int32_t bf16) {
     if (!m || !x || !logits || n < 0) return TFIM_E_INVALID;
     if (n == 0) return TFIM_OK;
-    Scratch sc;
-    if (int r = ensure(m, n, sc)) return r;
-    hipStream_t s = m->stream;
-    MCHK(m, hipMemcpyAsync(sc.X, x, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
-    launch_mlp_infer(m->P, bf16 ? m->Pb : nullptr, sc.X, n, sc.O, s);
-    MCHK(m, hipGetLastError());
-    MCHK(m, hipMemcpyAsync(logits, sc.O, (size_t)n * OUT * 4, hipMemcpyDeviceToHost, s));
-    MCHK(m, hipStreamSynchronize(s));
-    return TFIM_OK;
+    if (int r = tfim_mlp_forward_submit(m, x, n, bf16)) return r;
+    return tfim_mlp_forward_wait(m, logits);
 }
 
 int tfim_mlp_gradients(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float *grads,

@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 #include "tfim_internal.h"
@@ -19,6 +20,14 @@ struct tfim_buf {
     void *ptr = nullptr;
     int64_t bytes = 0;
 };
+
+namespace {
+// Online tuner arms for waves that both batched kernels accept:
+// (kernel 1 = L2-resident, 2 = LDS-staged; columns per work tile).
+constexpr int kTuneArms = 4;
+constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2};
+constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64};
+}  // namespace
 
 struct tfim_ctx {
     int device = 0;
@@ -70,15 +79,16 @@ struct tfim_ctx {
     int64_t wave_launches = 0, wave_graph_replays = 0, wave_graphs = 0, wave_rows = 0, wave_lds_launches = 0;
     double wave_ms = 0;
     int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel
-    // online kernel choice for waves both kernels accept: EMA of kernel
-    // milliseconds per (row prefix + column) word, per kernel
-    double tune_cost[2] = {0, 0};
-    int64_t tune_samples[2] = {0, 0};
+    // wave tuner: EMA of kernel milliseconds per (row prefix + column) word,
+    // per arm (kernel, columns per tile)
+    double tune_cost[kTuneArms] = {0, 0, 0, 0};
+    int64_t tune_samples[kTuneArms] = {0, 0, 0, 0};
     int64_t tune_calls = 0;
-    int tile_cols = 32;
+    int tile_cols = 0;      // 0: chosen by the tuner (32 when it does not run)
     int mfma_tile = 0;      // 0 auto (= 256), 128 or 256
     int use_graphs = 1;     // waves replayed as instantiated graphs
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
+    std::map<std::string, std::pair<int, int32_t>> popmat_tuned;   // shape -> (tile, split_k)
 };
 
 namespace {
@@ -146,7 +156,8 @@ int tfim_open(int device, tfim_ctx **out) {
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, device) != hipSuccess) { delete c; return TFIM_E_HIP; }
     c->n_cu = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
-    snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
+    if (p.name[0]) snprintf(c->name, sizeof c->name, "%s (%s)", p.name, p.gcnArchName);
+    else snprintf(c->name, sizeof c->name, "%s, %d CUs", p.gcnArchName, c->n_cu);   // the runtime gave no marketing name
     c->hbm = (int64_t)p.totalGlobalMem;
     c->clock_mhz = p.clockRate / 1000;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TFIM_E_HIP; }
@@ -414,15 +425,6 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     if (n_out == 0) return TFIM_OK;
     if (int r = set_device(c)) return r;
 
-    // tiles of <= tile_cols columns per row
-    std::vector<WaveTile> tiles;
-    tiles.reserve((size_t)n_rows * 2);
-    for (int32_t r = 0; r < n_rows; r++) {
-        const int32_t nc = col_ids ? col_off[r + 1] - col_off[r] : c->n_cols;
-        for (int32_t b = 0; b < nc; b += c->tile_cols) tiles.push_back({r, b, std::min(nc, b + c->tile_cols), 0});
-    }
-    if (tiles.empty()) return TFIM_OK;   // only empty tails: n_out == 0, handled above
-
     // work of this wave in row-words: sum over rows of (prefix + columns) x pitch
     const double work = ((double)n_item_ids + (double)n_out) * (double)c->pitch;
     // LDS slots: the distinct rows this wave touches
@@ -450,22 +452,39 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     if (c->wave_kernel == 2 && !lds)
         return fail(c, TFIM_E_RANGE, "count_nodes: wave does not fit the LDS kernel (%zu items, %lld outputs)",
                     slot_rows.size(), (long long)n_out);
-    // autotune: both kernels accept the wave; sample each, then take the
-    // cheaper, re-sampling the other every 64 calls
+    // autotune over (kernel, columns per tile) for waves both kernels accept:
+    // each arm is sampled three times, then the cheapest per row-word is
+    // taken, one other arm being re-sampled every 16 calls (round robin)
     int tuned = -1;
     if (lds && c->wave_kernel == 0) {
         c->tune_calls++;
-        if (c->tune_samples[0] < 3) tuned = 0;
-        else if (c->tune_samples[1] < 3) tuned = 1;
-        else {
-            const int best = c->tune_cost[1] < c->tune_cost[0] ? 1 : 0;
-            tuned = (c->tune_calls % 64 == 0) ? 1 - best : best;
+        for (int a = 0; a < kTuneArms && tuned < 0; a++)
+            if (c->tune_samples[a] < 3) tuned = a;
+        if (tuned < 0) {
+            int best = 0;
+            for (int a = 1; a < kTuneArms; a++)
+                if (c->tune_cost[a] < c->tune_cost[best]) best = a;
+            tuned = best;
+            if (c->tune_calls % 16 == 0) tuned = (best + 1 + (int)((c->tune_calls / 16) % (kTuneArms - 1))) % kTuneArms;
         }
-        if (tuned == 0) {
+        if (kArmKernel[tuned] == 1) {
             lds = false;
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
             slot_rows.clear();
         }
+    }
+    const int tile_cols = c->tile_cols ? c->tile_cols : (tuned >= 0 ? kArmTile[tuned] : 32);
+
+    // tiles of <= tile_cols columns per row
+    std::vector<WaveTile> tiles;
+    tiles.reserve((size_t)n_rows * 2);
+    for (int32_t r = 0; r < n_rows; r++) {
+        const int32_t nc = col_ids ? col_off[r + 1] - col_off[r] : c->n_cols;
+        for (int32_t b = 0; b < nc; b += tile_cols) tiles.push_back({r, b, std::min(nc, b + tile_cols), 0});
+    }
+    if (tiles.empty()) {   // only empty tails: n_out == 0, handled above
+        for (int32_t it : slot_rows) c->slot_of[it] = -1;
+        return TFIM_OK;
     }
 
     // pack the descriptor block: [slot_rows] | item ids or slots | item_off | col_off | col ids or slots | tiles
@@ -597,7 +616,7 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
         c->use_graphs = (int)value;
         return TFIM_OK;
     }
-    if (k == "tile_cols" && value >= 8 && value <= 64) {
+    if (k == "tile_cols" && (value == 0 || (value >= 8 && value <= 64))) {
         c->tile_cols = (int)value;
         return TFIM_OK;
     }
@@ -685,7 +704,7 @@ static int popmat_check(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_
 }
 
 static void popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
-                          int64_t d, int64_t pitch, int32_t path, int32_t split_k) {
+                          int64_t d, int64_t pitch, int32_t path, int32_t split_k, int tile) {
     if (split_k <= 0) split_k = 1;
     if (split_k > 1) launch_zero_i32((int32_t *)C->ptr, b * d, c->stream);
     const uint64_t *a = (const uint64_t *)A->ptr, *bb = (const uint64_t *)B->ptr;
@@ -693,18 +712,74 @@ static void popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfi
     if (path == TFIM_PATH_PACKED) {
         launch_popmat_packed(a, bb, cc, b, d, pitch, split_k, c->stream);
     } else {
-        const int tile = c->mfma_tile ? c->mfma_tile : 256;
         if (tile == 256) launch_popmat_mfma_rp(a, bb, cc, b, d, pitch, split_k, c->stream);
         else launch_popmat_mfma(a, bb, cc, b, d, pitch, split_k, c->stream);
     }
+}
+
+// Tile and split-k of a matrix count.  split_k >= 1 and a configured
+// mfma_tile are taken as given; split_k == 0 asks the tuner, which times every
+// candidate once on the caller's buffers the first time a (path, b, d, pitch)
+// shape is seen (blocking) and keeps the fastest for that shape.
+static int popmat_plan(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b, int64_t d,
+                       int64_t pitch, int32_t path, int32_t split_k, int *tile, int32_t *split) {
+    *tile = c->mfma_tile ? c->mfma_tile : 256;
+    *split = split_k;
+    if (split_k >= 1) return TFIM_OK;
+    const std::string key = std::to_string(path) + ":" + std::to_string(b) + ":" + std::to_string(d) + ":" +
+                            std::to_string(pitch) + ":" + std::to_string(c->mfma_tile);
+    auto it = c->popmat_tuned.find(key);
+    if (it != c->popmat_tuned.end()) {
+        *tile = it->second.first;
+        *split = it->second.second;
+        return TFIM_OK;
+    }
+    std::vector<int> tiles;
+    if (path == TFIM_PATH_PACKED) tiles = {0};
+    else if (c->mfma_tile) tiles = {c->mfma_tile};
+    else tiles = {256, 128};
+    double best = 1e300;
+    for (int t : tiles)
+        for (int32_t sk : {1, 2, 4, 8, 16}) {
+            HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            popmat_launch(c, A, B, C, b, d, pitch, path, sk, t);
+            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipEventSynchronize(c->ev1));
+            float ms = 0;
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            if (ms < best) {
+                best = ms;
+                *tile = t;
+                *split = sk;
+            }
+        }
+    c->popmat_tuned[key] = {*tile, *split};
+    return TFIM_OK;
 }
 
 int tfim_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
                          int64_t d, int64_t pitch, int32_t path, int32_t split_k) {
     if (int r = popmat_check(c, A, B, C, b, d, pitch, path)) return r;
     if (int r = set_device(c)) return r;
-    popmat_launch(c, A, B, C, b, d, pitch, path, split_k);
+    int tile;
+    int32_t split;
+    if (int r = popmat_plan(c, A, B, C, b, d, pitch, path, split_k, &tile, &split)) return r;
+    popmat_launch(c, A, B, C, b, d, pitch, path, split, tile);
     HIPCHK(c, hipGetLastError());
+    return TFIM_OK;
+}
+
+int tfim_popcount_matrix_plan(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
+                              int64_t d, int64_t pitch, int32_t path, int32_t *tile, int32_t *split_k) {
+    if (!tile || !split_k) return fail(c, TFIM_E_INVALID, "popcount_matrix_plan: null output");
+    if (int r = popmat_check(c, A, B, C, b, d, pitch, path)) return r;
+    if (int r = set_device(c)) return r;
+    int t;
+    int32_t sk;
+    if (int r = popmat_plan(c, A, B, C, b, d, pitch, path, 0, &t, &sk)) return r;
+    *tile = path == TFIM_PATH_PACKED ? 64 : t;
+    *split_k = sk;
     return TFIM_OK;
 }
 
@@ -714,10 +789,13 @@ int tfim_time_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B,
     if (int r = popmat_check(c, A, B, C, b, d, pitch, path)) return r;
     if (reps < 1 || !avg_ms) return fail(c, TFIM_E_INVALID, "time_popcount_matrix: reps < 1");
     if (int r = set_device(c)) return r;
-    for (int i = 0; i < warmup; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split_k);
+    int tile;
+    int32_t split;
+    if (int r = popmat_plan(c, A, B, C, b, d, pitch, path, split_k, &tile, &split)) return r;
+    for (int i = 0; i < warmup; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split, tile);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int i = 0; i < reps; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split_k);
+    for (int i = 0; i < reps; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split, tile);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));

@@ -194,3 +194,29 @@ def test_wave_graph_replays(ctx):
     after = ctx.wave_stats()
     assert after["graph_launches"] - before["graph_launches"] >= 7
     assert after["graphs"] - before["graphs"] <= 8   # instantiated once per (slot, kernel)
+
+
+@pytest.mark.parametrize("path", ["mfma", "packed"])
+def test_popcount_matrix_tuned_split(ctx, path):
+    """split_k = 0: the tuner times the candidates on first use of the shape,
+    keeps its choice, and the result stays exact."""
+    rng = np.random.default_rng(99)
+    b, d, n_bits = 300, 140, 2048 * 40 + 5
+    pitch = tf.pitch_words(n_bits)
+    w = (n_bits + 63) // 64
+    A = np.zeros((b, pitch), np.uint64)
+    B = np.zeros((d, pitch), np.uint64)
+    A[:, :w] = rand_rows(rng, b, n_bits, 0.4)
+    B[:, :w] = rand_rows(rng, d, n_bits, 0.4)
+    dA, dB, dC = ctx.buffer(A.nbytes), ctx.buffer(B.nbytes), ctx.buffer(b * d * 4)
+    dA.write(A)
+    dB.write(B)
+    exp = oracle_lib.popcount_matrix(A, B)
+    for _ in range(2):   # first call tunes, second reuses the choice
+        ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path=path, split_k=0)
+        np.testing.assert_array_equal(dC.read(np.int32, b * d).reshape(b, d), exp)
+    tile, split = ctx.popcount_matrix_plan(dA, dB, dC, b, d, pitch, path=path)
+    assert split in (1, 2, 4, 8, 16)
+    assert tile in ((64,) if path == "packed" else (128, 256))
+    for x in (dA, dB, dC):
+        x.free()

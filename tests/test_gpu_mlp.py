@@ -71,3 +71,25 @@ def test_training_learns_a_separable_rule(ctx):
     acc = ((mlp.forward(X, bf16=True) > 0) == (Y > 0.5)).mean()
     assert acc > 0.95
     mlp.close()
+
+
+def test_forward_submit_wait_orders_with_training(ctx):
+    """An inference queued before a training step sees the weights before the
+    step; one queued after sees the updated ones; both match the synchronous
+    forward at those weights."""
+    rng = np.random.default_rng(3)
+    mlp = tf.Mlp(ctx, seed=11)
+    X = rng.normal(size=(300, 14)).astype(np.float32)
+    before = mlp.forward(X)
+    mlp.forward_submit(X)
+    Xt, Yt, Mt = _batch(rng, 2048)
+    mlp.train_step(Xt, Yt, Mt, lr=1e-2)
+    got_before = mlp.forward_wait()
+    np.testing.assert_array_equal(got_before, before)
+    mlp.forward_submit(X[:17])
+    got_after = mlp.forward_wait()
+    np.testing.assert_array_equal(got_after, mlp.forward(X[:17]))
+    assert np.abs(got_after - before[:17]).max() > 0   # the step changed the weights
+    with pytest.raises(tf.TfimError):
+        mlp.forward_wait()                            # nothing outstanding
+    mlp.close()
