@@ -146,6 +146,8 @@ def bench_e2e(args):
         log("  %s: %.2f s wall, %.3f s computation, %d rows" % (name, wall, out[name]["computation_s"],
                                                                 out[name]["rows"]))
     out["identical_mfi_sets"] = all(sets[n] == sets["plain"] for n, _, _ in runs)
+    if not args.no_cpu:
+        out["cpu_engine"] = cpu_engine_baseline(data, out["plain"]["rows"], args)
     best_plain = min(out["plain"]["computation_s"], out["plain_w8"]["computation_s"])
     out["speedup_computation_vs_plain_1worker"] = out["plain"]["computation_s"] / out["batched_learned"]["computation_s"]
     out["speedup_computation_vs_best_plain"] = best_plain / out["batched_learned"]["computation_s"]
@@ -153,6 +155,28 @@ def bench_e2e(args):
     if not out["identical_mfi_sets"]:
         raise SystemExit("a batched MFI set differs from the plain engine")
     return out
+
+
+def cpu_engine_baseline(data, total_nodes, args):
+    """The reference bitmap engine's logic on the host cores: the oracle's
+    worker-pool DFS (kernel_bitmap.cu:853-894 semantics, 64-bit popcount
+    counter) on the same replicated table, timed over its first M nodes at 1
+    thread and at all usable threads, extrapolated to the full node count of
+    the plain GPU run (same enumeration)."""
+    exe = os.path.join(ROOT, "oracle", "_build", "tfim_oracle")
+    res = []
+    nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    for threads, nodes in ((1, 400), (nthreads, 400 * nthreads)):
+        log("cpu engine baseline: %d threads, first %d DFS nodes ..." % (threads, nodes))
+        r = subprocess.run([exe, data, "0.85", "--replicate", "256", "--threads", str(threads), "--max-nodes",
+                            str(nodes), "--out", data + ".cpu.res"], capture_output=True, text=True, check=True)
+        st = dict(ln.split(":", 1) for ln in r.stdout.splitlines() if ":" in ln)
+        secs, done = float(st["computation_time(s)"]), int(st["nodes"])
+        res.append({"threads": threads, "nodes_timed": done, "seconds": secs,
+                    "extrapolated_computation_s": secs * total_nodes / max(1, done), "nodes_total": total_nodes,
+                    "kind": "port", "note": "extrapolated from the first nodes of the reference DFS"})
+        log("  %.2f s for %d nodes -> %.0f s extrapolated" % (secs, done, res[-1]["extrapolated_computation_s"]))
+    return res
 
 
 def copy_calibration(mib):

@@ -53,6 +53,33 @@ __device__ __forceinline__ int wave_reduce_scatter8(const int v[8]) {
     return y;
 }
 
+// Sums 8 per-lane counts (each < 2^12) over every 16-lane row of the wave with
+// DPP row ops only (no LDS traffic).  Two columns share a 32-bit register (16
+// bits each; a row total is < 2^16) through the reduction.  Afterwards the
+// lanes with (lane & 3) == 0 hold, for group g = (lane >> 2) & 3, the row
+// totals of columns 2g (low half) and 2g + 1 (high half).
+//   step 1: partner 15 - l (row mirror); lanes 0-7 keep columns 0-3, 8-15 keep 4-7
+//   step 2: partner 7 - (l & 7) (half mirror); (l & 4) keeps the upper pair
+//   step 3: sum over the quad (xor 1, then xor 2)
+__device__ __forceinline__ uint32_t row_reduce8_packed(const uint32_t v[8]) {
+    const int lane = threadIdx.x & 63;
+    const bool hi = lane & 8, h2 = lane & 4;
+    uint32_t p[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) p[i] = v[2 * i] | (v[2 * i + 1] << 16);
+    uint32_t q[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const uint32_t keep = hi ? p[2 + i] : p[i], send = hi ? p[i] : p[2 + i];
+        q[i] = keep + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x140, 0xF, 0xF, false);
+    }
+    const uint32_t keep = h2 ? q[1] : q[0], send = h2 ? q[0] : q[1];
+    uint32_t r = keep + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x141, 0xF, 0xF, false);
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Plain per-node count.  grid.x = word chunks of 256*R uint4, grid.y = column
 // blocks of <= 64 columns.  Every block ANDs the k prefix rows over its chunk
@@ -226,11 +253,13 @@ void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &
 // word chunks g, g+G, ...; per chunk it copies the chunk of every slot (item
 // row) the wave uses into LDS with 128-bit loads, then its waves take the
 // tiles round-robin.  Per tile the prefix and column slot numbers are loaded
-// lane-parallel once and broadcast with readlane, so the column loop issues
-// only LDS reads and VALU work: prefix AND into registers, 8 columns at a
-// time, reduce-scatter, LDS atomic accumulation.  Partial counts leave the
-// workgroup once, as one global atomic per non-zero output.  Global traffic
-// per wave is one pass over the slots' rows whatever the number of rows.
+// lane-parallel once (prefetched during the previous tile) and broadcast with
+// readlane, so the column loop issues only LDS reads and VALU work: prefix
+// AND into registers, 8 columns at a time without branches, a DPP reduction
+// over each 16-lane row, and one LDS atomic per (row, column).  Partial
+// counts leave the workgroup once, as one global atomic per non-zero output.
+// Global traffic per wave is one pass over the slots' rows whatever the
+// number of rows.
 constexpr int kLdsThreads = 1024;
 constexpr int kLdsWaves = kLdsThreads / 64;
 
@@ -252,15 +281,26 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
             lds[i] = g < pitch4 ? bm[(int64_t)d.slot_rows[slot] * pitch4 + g] : make_uint4(0, 0, 0, 0);
         }
         __syncthreads();
-        for (int t = wave; t < d.n_tiles; t += kLdsWaves) {
-            const WaveTile tile = d.tiles[t];
-            const int r = tile.row;
-            const int ib = d.item_off[r], k = d.item_off[r + 1] - ib;
-            const int ob = d.col_off[r];
-            const int nc = tile.col_end - tile.col_begin;   // <= 64 (tile_cols)
-            // slot numbers, one per lane, broadcast below with readlane
-            const int pslot = lane < k ? d.item_slots[ib + lane] : 0;
-            const int cslot = lane < nc ? d.col_slots[ob + tile.col_begin + lane] : 0;
+        // tiles round-robin over the workgroup's waves; the slot numbers of
+        // the next tile are loaded while this one is counted
+        int t = wave;
+        LdsTile tl = {0, 0, 0, 0};
+        int pslot = 0, cslot = 0;
+        if (t < d.n_tiles) {
+            tl = d.tiles[t];
+            pslot = lane < tl.k ? d.item_slots[tl.ib + lane] : 0;
+            cslot = lane < tl.nc ? d.col_slots[tl.cb + lane] : 0;
+        }
+        for (; t < d.n_tiles; t += kLdsWaves) {
+            const int tn = t + kLdsWaves;
+            LdsTile nx = {0, 0, 0, 0};
+            int npslot = 0, ncslot = 0;
+            if (tn < d.n_tiles) {
+                nx = d.tiles[tn];
+                npslot = lane < nx.k ? d.item_slots[nx.ib + lane] : 0;
+                ncslot = lane < nx.nc ? d.col_slots[nx.cb + lane] : 0;
+            }
+            const int k = tl.k, nc = tl.nc;   // nc <= 64 (tile_cols)
             uint4 m[RW];
             {
                 const uint4 *row = lds + __builtin_amdgcn_readlane(pslot, 0) * C4;
@@ -268,29 +308,35 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                 for (int q = 0; q < RW; q++) m[q] = row[q * 64 + lane];
             }
             for (int i = 1; i < k; i++) {
-                const int sl = i < 64 ? __builtin_amdgcn_readlane(pslot, i) : d.item_slots[ib + i];
+                const int sl = i < 64 ? __builtin_amdgcn_readlane(pslot, i) : d.item_slots[tl.ib + i];
                 const uint4 *row = lds + sl * C4;
 #pragma unroll
                 for (int q = 0; q < RW; q++) m[q] = and4(m[q], row[q * 64 + lane]);
             }
             for (int j = 0; j < nc; j += 8) {
-                int v8[8];
+                // eight columns, branch-free: past the tile's end the last
+                // column is read again and its count dropped
+                uint32_t v[8];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
+                    const int col = j + q < nc ? j + q : nc - 1;
+                    const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, col) * C4;
                     uint32_t a = 0;
-                    if (j + q < nc) {
-                        const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, j + q) * C4;
 #pragma unroll
-                        for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
-                    }
-                    v8[q] = (int)a;
+                    for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
+                    v[q] = j + q < nc ? a : 0u;
                 }
-                const int y = wave_reduce_scatter8(v8);
-                if ((lane & 7) == 0) {
-                    const int q = (lane >> 3) & 7;
-                    if (j + q < nc && y) atomicAdd(&acc[ob + tile.col_begin + j + q], y);
+                const uint32_t r = row_reduce8_packed(v);
+                if ((lane & 3) == 0) {
+                    const int c0 = j + 2 * ((lane >> 2) & 3);
+                    const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
+                    if (c0 < nc && lo) atomicAdd(&acc[tl.cb + c0], lo);
+                    if (c0 + 1 < nc && hi) atomicAdd(&acc[tl.cb + c0 + 1], hi);
                 }
             }
+            tl = nx;
+            pslot = npslot;
+            cslot = ncslot;
         }
     }
     __syncthreads();

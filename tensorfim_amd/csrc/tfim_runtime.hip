@@ -522,7 +522,20 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     } else {
         for (int32_t r = 0; r <= n_rows; r++) coff[r] = r * c->n_cols;
     }
-    memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
+    if (lds) {
+        // the LDS kernel takes tiles resolved to prefix and output ranges
+        static_assert(sizeof(LdsTile) == sizeof(WaveTile), "tile records share the descriptor layout");
+        LdsTile *lt = reinterpret_cast<LdsTile *>(h + o_tiles);
+        for (size_t i = 0; i < tiles.size(); i++) {
+            const WaveTile &w = tiles[i];
+            lt[i].ib = item_off[w.row];
+            lt[i].k = item_off[w.row + 1] - item_off[w.row];
+            lt[i].cb = coff[w.row] + w.col_begin;
+            lt[i].nc = w.col_end - w.col_begin;
+        }
+    } else {
+        memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
+    }
     KernelLaunch K;
     if (lds) {
         WaveLdsDesc d;
@@ -531,7 +544,7 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         d.item_off = reinterpret_cast<const int32_t *>(sl.d_desc + o_ioff);
         d.col_slots = reinterpret_cast<const int32_t *>(sl.d_desc + o_cids);
         d.col_off = reinterpret_cast<const int32_t *>(sl.d_desc + o_coff);
-        d.tiles = reinterpret_cast<const WaveTile *>(sl.d_desc + o_tiles);
+        d.tiles = reinterpret_cast<const LdsTile *>(sl.d_desc + o_tiles);
         d.out = sl.d_out;
         d.n_slots = (int32_t)slot_rows.size();
         d.n_tiles = (int32_t)tiles.size();
