@@ -15,11 +15,17 @@
 //     :1224-1233, quirk A.2).
 #include "dataset.h"
 
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cstdio>
 #include <numeric>
+#include <thread>
 
 namespace tfim {
 
@@ -29,14 +35,17 @@ double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Parses a byte range into CSR rows.
-void parse(const char *p, const char *end, std::vector<int32_t> &ids, std::vector<int64_t> &off) {
-    off.push_back(0);
+// Parses bytes [p, end) with the reader's rules; appends ids and the length
+// of every row ended by '\n'.  The bytes after the last '\n' form a final row
+// when `final_row` is 1 (always) or 2 (only if it holds an id); 0 ignores
+// them (a piece that ends right after a '\n' has none).
+void parse_piece(const char *p, const char *end, int final_row, std::vector<int32_t> &ids,
+                 std::vector<int32_t> &lens, int32_t &idmax) {
     size_t row_start = ids.size();
     while (p < end) {
         const char c = *p;
         if (c == '\n') {
-            off.push_back((int64_t)ids.size());
+            lens.push_back((int32_t)(ids.size() - row_start));
             row_start = ids.size();
             ++p;
             continue;
@@ -45,6 +54,7 @@ void parse(const char *p, const char *end, std::vector<int32_t> &ids, std::vecto
             long long v = 0;
             while (p < end && *p >= '0' && *p <= '9') v = v * 10 + (*p++ - '0');
             ids.push_back((int32_t)v);
+            idmax = std::max(idmax, (int32_t)v);
             continue;
         }
         if (c == '-') {
@@ -54,36 +64,98 @@ void parse(const char *p, const char *end, std::vector<int32_t> &ids, std::vecto
         }
         ++p;
     }
-    if (ids.size() > row_start) off.push_back((int64_t)ids.size());
+    if (final_row == 1 || (final_row == 2 && ids.size() > row_start)) lens.push_back((int32_t)(ids.size() - row_start));
+}
+
+// Parses a whole file image into CSR rows with up to `threads` threads: the
+// bytes are cut right after '\n' bytes into one piece per thread, each piece
+// is parsed on its own and the pieces are joined in order, so the rows are
+// those of one sequential pass.  `newline_at_end` treats the image as if a
+// '\n' followed it (the replicated database appends one when missing).
+void parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, std::vector<int32_t> &ids,
+                    std::vector<int64_t> &off, int32_t &idmax) {
+    threads = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, threads), n / 2 + 1));
+    std::vector<size_t> cut(threads + 1, n);
+    cut[0] = 0;
+    for (int i = 1; i < threads; i++) {
+        size_t b = std::max(cut[i - 1], n / threads * i);
+        while (b < n && p[b - 1] != '\n') b++;
+        cut[i] = b;
+    }
+    std::vector<std::vector<int32_t>> pid(threads), plen(threads);
+    std::vector<int32_t> pmax(threads, 0);
+    auto work = [&](int i) {
+        pid[i].reserve((cut[i + 1] - cut[i]) / 3 + 16);
+        // the bytes after the file's last '\n' belong to the piece that ends
+        // the image (trailing pieces past it are empty)
+        const bool ends = cut[i + 1] == n && (cut[i] < n || i == 0);
+        const int final_row = ends ? (newline_at_end ? 1 : 2) : 0;
+        parse_piece(p + cut[i], p + cut[i + 1], final_row, pid[i], plen[i], pmax[i]);
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < threads; i++) pool.emplace_back(work, i);
+    work(0);
+    for (auto &t : pool) t.join();
+    size_t n_ids = 0, n_rows = 0;
+    for (int i = 0; i < threads; i++) {
+        n_ids += pid[i].size();
+        n_rows += plen[i].size();
+        idmax = std::max(idmax, pmax[i]);
+    }
+    ids.resize(n_ids);
+    off.resize(n_rows + 1);
+    off[0] = 0;
+    size_t io = 0, ro = 0;
+    for (int i = 0; i < threads; i++) {
+        std::copy(pid[i].begin(), pid[i].end(), ids.begin() + (ptrdiff_t)io);
+        for (int32_t l : plen[i]) {
+            off[ro + 1] = off[ro] + l;
+            ro++;
+        }
+        io += pid[i].size();
+        std::vector<int32_t>().swap(pid[i]);
+    }
 }
 
 }  // namespace
 
 bool load_database(const std::string &path, double support, int replicate, Database &db,
-                   std::string &err) {
+                   std::string &err, int parse_threads) {
     const double t0 = now();
-    FILE *fp = fopen(path.c_str(), "rb");
-    if (!fp) {
+    const int fd = open(path.c_str(), O_RDONLY);
+    if (fd < 0) {
         err = path + " open failed!";
         return false;
     }
-    fseeko(fp, 0, SEEK_END);
-    const long long sz = ftello(fp);
-    fseeko(fp, 0, SEEK_SET);
-    std::vector<char> buf((size_t)sz + 1);
-    if (sz > 0 && fread(buf.data(), 1, (size_t)sz, fp) != (size_t)sz) {
-        fclose(fp);
-        err = path + " read failed!";
+    struct stat stt;
+    if (fstat(fd, &stt) != 0) {
+        close(fd);
+        err = path + " open failed!";
         return false;
     }
-    fclose(fp);
-    size_t n = (size_t)sz;
-    if (replicate >= 1 && (n == 0 || buf[n - 1] != '\n')) buf[n++] = '\n';
+    const size_t n = (size_t)stt.st_size;
+    const char *img = nullptr;
+    void *map = MAP_FAILED;
+    if (n > 0) {
+        map = mmap(nullptr, n, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (map == MAP_FAILED) {
+            close(fd);
+            err = path + " read failed!";
+            return false;
+        }
+        img = static_cast<const char *>(map);
+    }
+    // the replicated database is the file plus '\n' when it lacks one
+    const bool newline_at_end = replicate >= 1 && (n == 0 || img[n - 1] != '\n');
     std::vector<int32_t> ids;
     std::vector<int64_t> off;
-    ids.reserve(n / 3 + 16);
-    parse(buf.data(), buf.data() + n, ids, off);
-    std::vector<char>().swap(buf);
+    int32_t parsed_max = 0;
+    const int threads = parse_threads > 0 ? parse_threads
+                        : n < ((size_t)4 << 20) ? 1
+                                                : (int)std::min(32u, std::max(1u, std::thread::hardware_concurrency()));
+    parse_parallel(img, n, newline_at_end, threads, ids, off, parsed_max);
+    if (map != MAP_FAILED) munmap(map, n);
+    close(fd);
     const int64_t base_rows = (int64_t)off.size() - 1;
     const int64_t rep = replicate >= 1 ? replicate : 1;
     db.replicate = (int32_t)rep;

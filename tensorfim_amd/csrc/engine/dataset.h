@@ -51,8 +51,10 @@ struct Database {
 // Reads `path` (whitespace-separated ids, one transaction per line) and
 // prepares it at `support`.  replicate >= 1 gives the database
 // data/make_replica.py writes (file + '\n' if missing, concatenated
-// `replicate` times); 0 reads the file as is.  Returns false if unreadable.
+// `replicate` times); 0 reads the file as is.  parse_threads = 0 uses the
+// hardware threads (files under 4 MB are parsed by one).  Returns false if
+// unreadable.
 bool load_database(const std::string &path, double support, int replicate, Database &db,
-                   std::string &err);
+                   std::string &err, int parse_threads = 0);
 
 }  // namespace tfim

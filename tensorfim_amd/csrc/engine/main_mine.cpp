@@ -3,8 +3,8 @@
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
 //             [--workers N] [--learned] [--rank] [--gate indep|always]
-//             [--wave-kernel 0|1|2] [--graphs 0|1] [--model PATH] [--trace PATH]
-//             [--verbose]
+//             [--wave-kernel 0|1|2] [--graphs 0|1] [--parse-threads N]
+//             [--model PATH] [--trace PATH] [--verbose]
 //
 // Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the
 // reference layout and prints run statistics as "key: value" lines.  Exit
@@ -33,6 +33,7 @@ int main(int argc, char **argv) {
     std::string out_path;
     int wave_kernel = 0;
     int graphs = -1;   // -1: library default
+    int parse_threads = 0;   // 0: hardware threads for files of 4 MB and more
     int replicate = 0, device = 0;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
@@ -52,6 +53,7 @@ int main(int argc, char **argv) {
         else if (a == "--rank") opt.rank = true;
         else if (a == "--wave-kernel") wave_kernel = atoi(next().c_str());
         else if (a == "--graphs") graphs = atoi(next().c_str());
+        else if (a == "--parse-threads") parse_threads = atoi(next().c_str());
         else if (a == "--model") opt.model_path = next();
         else if (a == "--trace") opt.trace_path = next();
         else if (a == "--verbose") opt.verbose = true;
@@ -82,7 +84,7 @@ int main(int argc, char **argv) {
     }
     Database db;
     std::string err;
-    if (!load_database(file, support, replicate, db, err)) {
+    if (!load_database(file, support, replicate, db, err, parse_threads)) {
         fprintf(stderr, "%s\n", err.c_str());
         tfim_close(ctx);
         return 1;

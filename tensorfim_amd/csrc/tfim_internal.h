@@ -85,40 +85,23 @@ int wave_chunk_words(int chunk_uint4_per_lane);
 // LDS-staged wave: every item the wave touches gets an LDS slot; prefixes and
 // columns are given as slot indices.  Each workgroup stages one word chunk of
 // all slots at a time and runs every tile of the wave against it.
-//
-// Rows are grouped, on the host, into blocks of up to kLdsBlockRows
-// consecutive rows that share most of their columns (siblings of one search
-// node do): the block counts every row against the union U of its rows'
-// columns, so each column word read from LDS serves all of the block's rows,
-// and the prefix items common to the block are ANDed once.  Counts land in a
-// dense accumulator (entry acc_b + r * nu + u for row r, union column u);
-// `map` sends each entry to its output, or nowhere (-1) when row r did not
-// ask for column U[u].
-constexpr int kLdsBlockRows = 4;
-struct LdsBlock {
-    int32_t pre_b, pre_k;              // common prefix: item_slots[pre_b .. pre_b + pre_k)
-    int32_t nr, nu;                    // rows (1..kLdsBlockRows), union columns
-    int32_t ext_b[kLdsBlockRows];      // row r's other prefix items: item_slots[ext_b[r] ..
-    int32_t ext_k[kLdsBlockRows];      //                                 .. ext_b[r] + ext_k[r])
-    int32_t ucol_b;                    // union column slots: col_slots[ucol_b .. ucol_b + nu)
-    int32_t acc_b;                     // first dense accumulator entry
-    int32_t pad[2];
-};
-// One unit of LDS-kernel work: union columns [u_begin, u_end) of a block.
+// A tile of the LDS kernel, resolved on the host: prefix slots
+// item_slots[ib .. ib+k), column slots col_slots[cb .. cb+nc), outputs
+// out[cb .. cb+nc).
 struct LdsTile {
-    int32_t block, u_begin, u_end, pad;
+    int32_t ib, k, cb, nc;
 };
 struct WaveLdsDesc {
     const int32_t *slot_rows;  // [n_slots] bitmap row of each slot
-    const int32_t *item_slots; // prefix slots (common and per-row parts)
-    const int32_t *col_slots;  // union column slots of every block
-    const LdsBlock *blocks;
+    const int32_t *item_slots; // prefix slots, CSR by item_off
+    const int32_t *item_off;   // [n_rows+1]
+    const int32_t *col_slots;  // column slots, CSR by col_off
+    const int32_t *col_off;    // [n_rows+1] (also output offsets)
     const LdsTile *tiles;
-    const int32_t *map;        // [n_acc] output index of each accumulator entry, or -1
     int32_t *out;              // [n_out], zeroed
-    int32_t n_slots, n_tiles, n_acc, n_out;
+    int32_t n_slots, n_tiles, n_out;
 };
-// Largest slot count / accumulator entries the LDS kernel accepts.
+// Largest slot count / output count the LDS kernel accepts.
 constexpr int kLdsMaxSlots = 96;
 constexpr int kLdsMaxOut = 8192;
 // Returns false if the shape does not fit LDS (caller uses launch_count_wave).

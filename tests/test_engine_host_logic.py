@@ -64,3 +64,32 @@ def test_engines_match_oracle_on_random_databases(mock_engine, tmp_path):
             subprocess.run([mock_engine, str(db), repr(sup), "--out", str(out)] + eng, check=True,
                            capture_output=True)
             assert tf.load_results(str(out)) == exp, (it, eng, sup)
+
+
+def test_parallel_parser_matches_one_pass(mock_engine, tmp_path):
+    """The reader cut into pieces at '\\n' bytes gives the rows of one
+    sequential pass: empty lines, '-' numbers, CR bytes, separators of any
+    kind, a last line with or without '\\n', replicated or not."""
+    rnd = random.Random(11)
+    keys = ["transNum", "dimension", "itemIDmax", "dimensionReduced", "transNumReduced", "MFIsNumber",
+            "transLengthMean", "transLengthReducedMean"]
+    for it in range(40):
+        lines = []
+        for _ in range(rnd.randint(0, 60)):
+            toks = [str(rnd.randint(0, 9)) for _ in range(rnd.randint(0, 8))]
+            if rnd.random() < 0.2:
+                toks.insert(rnd.randint(0, len(toks)), "-" + str(rnd.randint(1, 9)))
+            sep = rnd.choice([" ", "  ", "\t", ",", " \r"])
+            lines.append(sep.join(toks) + (" \r" if rnd.random() < 0.2 else ""))
+        txt = "\n".join(lines) + rnd.choice(["", "\n", "\n\n", "  "])
+        db = tmp_path / "db.txt"
+        db.write_text(txt)
+        rep = rnd.choice([[], ["--replicate", "3"]])
+        outs = []
+        for threads in (1, 2, 5, 17):
+            out = tmp_path / ("p%d.res" % threads)
+            r = subprocess.run([mock_engine, str(db), "0.3", "--out", str(out), "--engine", "wave",
+                                "--parse-threads", str(threads)] + rep, capture_output=True, text=True)
+            outs.append((r.returncode, tf.load_results(str(out)) if r.returncode == 0 else None,
+                         {k: tf.load_header(str(out)).get(k) for k in keys} if r.returncode == 0 else None))
+        assert all(o == outs[0] for o in outs), (it, txt[:200], rep, outs)
