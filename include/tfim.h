@@ -187,6 +187,28 @@ int tfim_time_popcount_matrix(tfim_ctx *ctx, const tfim_buf *A, const tfim_buf *
                               int64_t b, int64_t d, int64_t pitch_words, int32_t path,
                               int32_t split_k, int32_t warmup, int32_t reps, double *avg_ms);
 
+/* ---- device preprocessing ---------------------------------------------- */
+/* The reference's frequency count, reduction and bitmap build
+ * (kernel_bmma.cu:272-276, :1075-1086, buildBitmap :388-426) on the device,
+ * for a database parsed into CSR rows (ids[row_off[t] .. row_off[t+1]) of
+ * transaction t, ids >= 0).
+ *   tfim_prep_begin  uploads the rows and returns freq[0..id_max], the
+ *                    occurrence count of every id; an id outside [0, id_max]
+ *                    is an error.
+ *   tfim_prep_finish keeps item x as dense row dense_of[x] (-1 = dropped),
+ *                    drops the rows left empty and makes the result the
+ *                    context's resident bitmaps, exactly as tfim_load_bitmaps
+ *                    of the host-built rows would (n_items rows, the first
+ *                    n_cols of them columns).  Returns the kept row count and
+ *                    the min, max and sum of kept row lengths, and
+ *                    *duplicates = 1 if some transaction repeats a kept id.
+ * The uploaded rows are released by tfim_prep_finish. */
+int tfim_prep_begin(tfim_ctx *ctx, const int32_t *ids, int64_t n_ids, const int64_t *row_off, int64_t n_rows,
+                    int32_t id_max, int32_t *freq);
+int tfim_prep_finish(tfim_ctx *ctx, const int32_t *dense_of, int32_t n_items, int32_t n_cols,
+                     int64_t *rows_reduced, int64_t *len_min, int64_t *len_max, int64_t *len_sum,
+                     int32_t *duplicates);
+
 /* ---- learned search policy (MLP 14 -> 64 -> 64 -> 2) -------------------- */
 /* No counterpart in the reference (its search learns nothing).  The MAFIA
  * engine's tail ranker (head 1) and head-union-tail lookahead gate (head 0).

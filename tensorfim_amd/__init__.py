@@ -53,6 +53,11 @@ _SIGS = {
     "tfim_wave_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64),
                                         _p(ctypes.c_int64), _p(ctypes.c_double)]),
     "tfim_wave_graph_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64)]),
+    "tfim_prep_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                        ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
+    "tfim_prep_finish": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                         _p(ctypes.c_int64), _p(ctypes.c_int64), _p(ctypes.c_int64),
+                                         _p(ctypes.c_int64), _p(ctypes.c_int32)]),
     "tfim_buf_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _p(ctypes.c_void_p)]),
     "tfim_buf_free": (None, [ctypes.c_void_p, ctypes.c_void_p]),
     "tfim_buf_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,

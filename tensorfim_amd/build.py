@@ -30,7 +30,8 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 INCLUDES = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
-LIB_SOURCES = ["tfim_runtime.hip", "tfim_mlp.hip", "kernels_packed.hip", "kernels_mfma.hip", "kernels_mlp.hip"]
+LIB_SOURCES = ["tfim_runtime.hip", "tfim_mlp.hip", "kernels_packed.hip", "kernels_mfma.hip", "kernels_mlp.hip",
+               "kernels_prep.hip"]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-O3", "-std=c++17", "-march=x86-64-v2", "-Wall", "-Wno-unused-result"]
 

@@ -15,6 +15,8 @@
 //     :1224-1233, quirk A.2).
 #include "dataset.h"
 
+#include "tfim.h"
+
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -35,26 +37,30 @@ double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// Parses bytes [p, end) with the reader's rules; appends ids and the length
-// of every row ended by '\n'.  The bytes after the last '\n' form a final row
-// when `final_row` is 1 (always) or 2 (only if it holds an id); 0 ignores
-// them (a piece that ends right after a '\n' has none).
-void parse_piece(const char *p, const char *end, int final_row, std::vector<int32_t> &ids,
-                 std::vector<int32_t> &lens, int32_t &idmax) {
-    size_t row_start = ids.size();
+// The reader's rules over bytes [p, end): a digit run is an id, '-' and its
+// digits are dropped, any other byte separates, '\n' ends a row.  The bytes
+// after the last '\n' form a final row when `final_row` is 1 (always) or 2
+// (only if it holds an id); 0 ignores them (a piece that ends right after a
+// '\n' has none).  With ids == nullptr only counts ids and rows; otherwise
+// writes ids and row lengths from the given positions.
+void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int32_t *lens, int64_t &n_ids,
+                int64_t &n_rows, int32_t &idmax) {
+    int64_t ni = 0, nr = 0, row_start = 0;
     while (p < end) {
         const char c = *p;
         if (c == '\n') {
-            lens.push_back((int32_t)(ids.size() - row_start));
-            row_start = ids.size();
+            if (lens) lens[nr] = (int32_t)(ni - row_start);
+            nr++;
+            row_start = ni;
             ++p;
             continue;
         }
         if (c >= '0' && c <= '9') {
             long long v = 0;
             while (p < end && *p >= '0' && *p <= '9') v = v * 10 + (*p++ - '0');
-            ids.push_back((int32_t)v);
+            if (ids) ids[ni] = (int32_t)v;
             idmax = std::max(idmax, (int32_t)v);
+            ni++;
             continue;
         }
         if (c == '-') {
@@ -64,14 +70,20 @@ void parse_piece(const char *p, const char *end, int final_row, std::vector<int3
         }
         ++p;
     }
-    if (final_row == 1 || (final_row == 2 && ids.size() > row_start)) lens.push_back((int32_t)(ids.size() - row_start));
+    if (final_row == 1 || (final_row == 2 && ni > row_start)) {
+        if (lens) lens[nr] = (int32_t)(ni - row_start);
+        nr++;
+    }
+    n_ids = ni;
+    n_rows = nr;
 }
 
 // Parses a whole file image into CSR rows with up to `threads` threads: the
-// bytes are cut right after '\n' bytes into one piece per thread, each piece
-// is parsed on its own and the pieces are joined in order, so the rows are
-// those of one sequential pass.  `newline_at_end` treats the image as if a
-// '\n' followed it (the replicated database appends one when missing).
+// bytes are cut right after '\n' bytes into one piece per thread; a first
+// pass counts each piece's ids and rows, a second writes them straight into
+// the joined arrays, so the rows are those of one sequential pass and no
+// thread allocates memory of its own.  `newline_at_end` treats the image as if
+// a '\n' followed it (the replicated database appends one when missing).
 void parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, std::vector<int32_t> &ids,
                     std::vector<int64_t> &off, int32_t &idmax) {
     threads = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, threads), n / 2 + 1));
@@ -82,45 +94,95 @@ void parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, s
         while (b < n && p[b - 1] != '\n') b++;
         cut[i] = b;
     }
-    std::vector<std::vector<int32_t>> pid(threads), plen(threads);
-    std::vector<int32_t> pmax(threads, 0);
-    auto work = [&](int i) {
-        pid[i].reserve((cut[i + 1] - cut[i]) / 3 + 16);
-        // the bytes after the file's last '\n' belong to the piece that ends
-        // the image (trailing pieces past it are empty)
+    // the bytes after the file's last '\n' belong to the piece that ends the
+    // image (trailing pieces past it are empty)
+    auto final_row = [&](int i) {
         const bool ends = cut[i + 1] == n && (cut[i] < n || i == 0);
-        const int final_row = ends ? (newline_at_end ? 1 : 2) : 0;
-        parse_piece(p + cut[i], p + cut[i + 1], final_row, pid[i], plen[i], pmax[i]);
+        return ends ? (newline_at_end ? 1 : 2) : 0;
     };
-    std::vector<std::thread> pool;
-    for (int i = 1; i < threads; i++) pool.emplace_back(work, i);
-    work(0);
-    for (auto &t : pool) t.join();
-    size_t n_ids = 0, n_rows = 0;
+    std::vector<int64_t> pids(threads + 1, 0), prows(threads + 1, 0);
+    std::vector<int32_t> pmax(threads, 0);
+    auto run = [&](auto &&fn) {
+        std::vector<std::thread> pool;
+        for (int i = 1; i < threads; i++) pool.emplace_back(fn, i);
+        fn(0);
+        for (auto &t : pool) t.join();
+    };
+    run([&](int i) {
+        scan_piece(p + cut[i], p + cut[i + 1], final_row(i), nullptr, nullptr, pids[i + 1], prows[i + 1], pmax[i]);
+    });
     for (int i = 0; i < threads; i++) {
-        n_ids += pid[i].size();
-        n_rows += plen[i].size();
+        pids[i + 1] += pids[i];
+        prows[i + 1] += prows[i];
         idmax = std::max(idmax, pmax[i]);
     }
-    ids.resize(n_ids);
-    off.resize(n_rows + 1);
+    // large arrays: ask for transparent huge pages before the first touch
+    // (page faults of 4 KB pages dominate filling gigabytes)
+    auto huge = [](void *ptr, size_t bytes) {
+        const uintptr_t a = ((uintptr_t)ptr + (2u << 20) - 1) & ~(uintptr_t)((2u << 20) - 1);
+        const uintptr_t e = ((uintptr_t)ptr + bytes) & ~(uintptr_t)((2u << 20) - 1);
+        if (e > a) madvise((void *)a, e - a, MADV_HUGEPAGE);
+    };
+    ids.reserve((size_t)pids[threads]);
+    huge(ids.data(), (size_t)pids[threads] * 4);
+    ids.resize((size_t)pids[threads]);
+    std::vector<int32_t> lens((size_t)prows[threads]);
+    run([&](int i) {
+        int64_t a, b;
+        int32_t mx = 0;
+        scan_piece(p + cut[i], p + cut[i + 1], final_row(i), ids.data() + pids[i], lens.data() + prows[i], a, b, mx);
+    });
+    off.resize(lens.size() + 1);
     off[0] = 0;
-    size_t io = 0, ro = 0;
-    for (int i = 0; i < threads; i++) {
-        std::copy(pid[i].begin(), pid[i].end(), ids.begin() + (ptrdiff_t)io);
-        for (int32_t l : plen[i]) {
-            off[ro + 1] = off[ro] + l;
-            ro++;
+    for (size_t r = 0; r < lens.size(); r++) off[r + 1] = off[r] + lens[r];
+}
+
+// Dense namespace from occurrence counts: survivors (id >= 1, ascending),
+// a frequent item 0 last, the frequency order and the roots.  Returns the
+// map original id -> dense id (-1 = dropped).
+std::vector<int32_t> dense_namespace(Database &db, const std::vector<int32_t> &freq, int32_t idmax, double support) {
+    const double thr = db.thr_abs;
+    auto frequent = [&](int32_t x) { return !((double)freq[x] < thr); };
+    int32_t dred = 0;
+    for (int32_t x = 0; x <= idmax; x++)
+        if ((double)freq[x] / (double)db.trans_num > support) dred++;
+    db.dimension_reduced = dred;
+
+    // dense compaction
+    std::vector<int32_t> o2d((size_t)idmax + 1, -1);
+    db.dense_to_orig.clear();
+    db.freq_dense.clear();
+    for (int32_t i = 1; i <= idmax; i++)
+        if (frequent(i)) {
+            o2d[i] = (int32_t)db.dense_to_orig.size();
+            db.dense_to_orig.push_back(i);
+            db.freq_dense.push_back(freq[i]);
         }
-        io += pid[i].size();
-        std::vector<int32_t>().swap(pid[i]);
+    db.n_surv = (int32_t)db.dense_to_orig.size();
+    if (frequent(0)) {
+        o2d[0] = db.n_surv;
+        db.dense_to_orig.push_back(0);
+        db.freq_dense.push_back(freq[0]);
     }
+    db.d_count = (int32_t)db.dense_to_orig.size();
+    db.sorted_items.resize(db.d_count);
+    std::iota(db.sorted_items.begin(), db.sorted_items.end(), 0);
+    std::stable_sort(db.sorted_items.begin(), db.sorted_items.end(),
+                     [&](int32_t a, int32_t b) { return db.freq_dense[a] > db.freq_dense[b]; });
+    // The reference pushes items_inFreqSort[0..dimensionReduced); its sorted
+    // list carries one zero-frequency slot past d_count without a bitmap row,
+    // so the root count is clamped to real items.
+    db.n_roots = std::min(dred, db.d_count);
+    db.is_root.assign(db.d_count, 0);
+    for (int32_t i = 0; i < db.n_roots; i++) db.is_root[db.sorted_items[i]] = 1;
+
+    return o2d;
 }
 
 }  // namespace
 
 bool load_database(const std::string &path, double support, int replicate, Database &db,
-                   std::string &err, int parse_threads) {
+                   std::string &err, int parse_threads, tfim_ctx *ctx) {
     const double t0 = now();
     const int fd = open(path.c_str(), O_RDONLY);
     if (fd < 0) {
@@ -162,6 +224,58 @@ bool load_database(const std::string &path, double support, int replicate, Datab
     db.support = support;
     db.parse_seconds = now() - t0;
     const double t1 = now();
+
+    if (ctx && replicate < 1) {
+        // occurrence counts, reduction and bitmaps on the device; lengths
+        // from the row offsets here
+        int64_t lmin = INT_MAX, lmax = 0;
+        double lsum = 0;
+        for (int64_t t = 0; t < base_rows; t++) {
+            const int64_t len = off[t + 1] - off[t];
+            lmin = std::min(lmin, len);
+            lmax = std::max(lmax, len);
+            lsum += (double)len;
+        }
+        std::vector<int32_t> freq((size_t)parsed_max + 1, 0);
+        if (tfim_prep_begin(ctx, ids.data(), (int64_t)ids.size(), off.data(), base_rows, parsed_max, freq.data()) !=
+            TFIM_OK) {
+            err = std::string("device preprocessing: ") + tfim_last_error(ctx);
+            return false;
+        }
+        std::vector<int32_t>().swap(ids);
+        std::vector<int64_t>().swap(off);
+        int32_t idmin = INT_MAX, dim = 0;
+        for (int32_t x = 0; x <= parsed_max; x++)
+            if (freq[x] > 0) {
+                idmin = std::min(idmin, x);
+                dim++;
+            }
+        db.trans_num = (int32_t)base_rows;
+        db.item_id_min = idmin;
+        db.item_id_max_raw = parsed_max;
+        db.len_min = lmin;
+        db.len_max = lmax;
+        db.len_mean = base_rows > 0 ? lsum / (double)base_rows : 0.0;
+        db.dimension = dim;
+        db.thr_abs = (double)db.trans_num * support;
+        std::vector<int32_t> o2d = dense_namespace(db, freq, parsed_max, support);
+        int64_t n_red = 0, rmin = 0, rmax = 0, rsum = 0;
+        int32_t dupf = 0;
+        if (tfim_prep_finish(ctx, o2d.data(), db.d_count, db.n_surv, &n_red, &rmin, &rmax, &rsum, &dupf) != TFIM_OK) {
+            err = std::string("device preprocessing: ") + tfim_last_error(ctx);
+            return false;
+        }
+        db.trans_num_reduced = (int32_t)n_red;
+        db.red_len_min = n_red > 0 ? rmin : INT_MAX;
+        db.red_len_max = rmax;
+        db.red_len_mean = (double)rsum / (double)n_red;   // NaN when nothing survives
+        db.duplicates = dupf != 0;
+        db.base_bits = n_red;
+        db.words = (n_red + 63) / 64;
+        db.on_device = true;
+        db.preprocessing_seconds = now() - t1;
+        return true;
+    }
 
     // statistics over the (replicated) database: every copy repeats the base
     int32_t idmin = INT_MAX, idmax = 0;
@@ -231,38 +345,7 @@ bool load_database(const std::string &path, double support, int replicate, Datab
     db.red_len_max = rmax;
     db.red_len_mean = (rsum * (double)rep) / (double)(base_red * rep);   // NaN when nothing survives
 
-    int32_t dred = 0;
-    for (int32_t x = 0; x <= idmax; x++)
-        if ((double)freq[x] / (double)db.trans_num > support) dred++;
-    db.dimension_reduced = dred;
-
-    // dense compaction
-    std::vector<int32_t> o2d((size_t)idmax + 1, -1);
-    db.dense_to_orig.clear();
-    db.freq_dense.clear();
-    for (int32_t i = 1; i <= idmax; i++)
-        if (frequent(i)) {
-            o2d[i] = (int32_t)db.dense_to_orig.size();
-            db.dense_to_orig.push_back(i);
-            db.freq_dense.push_back(freq[i]);
-        }
-    db.n_surv = (int32_t)db.dense_to_orig.size();
-    if (frequent(0)) {
-        o2d[0] = db.n_surv;
-        db.dense_to_orig.push_back(0);
-        db.freq_dense.push_back(freq[0]);
-    }
-    db.d_count = (int32_t)db.dense_to_orig.size();
-    db.sorted_items.resize(db.d_count);
-    std::iota(db.sorted_items.begin(), db.sorted_items.end(), 0);
-    std::stable_sort(db.sorted_items.begin(), db.sorted_items.end(),
-                     [&](int32_t a, int32_t b) { return db.freq_dense[a] > db.freq_dense[b]; });
-    // The reference pushes items_inFreqSort[0..dimensionReduced); its sorted
-    // list carries one zero-frequency slot past d_count without a bitmap row,
-    // so the root count is clamped to real items.
-    db.n_roots = std::min(dred, db.d_count);
-    db.is_root.assign(db.d_count, 0);
-    for (int32_t i = 0; i < db.n_roots; i++) db.is_root[db.sorted_items[i]] = 1;
+    std::vector<int32_t> o2d = dense_namespace(db, freq, idmax, support);
 
     // vertical bitmaps of one copy, LSB-first
     db.base_bits = base_red;

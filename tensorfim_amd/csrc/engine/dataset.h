@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+struct tfim_ctx;
+
 namespace tfim {
 
 struct Database {
@@ -41,6 +43,7 @@ struct Database {
     int64_t words = 0;            // u64 words per base row
     std::vector<uint64_t> bitmap; // [d_count][words]
     bool duplicates = false;      // some row repeats an item id
+    bool on_device = false;       // bitmaps built on the device (bitmap stays empty)
 
     double parse_seconds = 0, preprocessing_seconds = 0;
 
@@ -52,9 +55,11 @@ struct Database {
 // prepares it at `support`.  replicate >= 1 gives the database
 // data/make_replica.py writes (file + '\n' if missing, concatenated
 // `replicate` times); 0 reads the file as is.  parse_threads = 0 uses the
-// hardware threads (files under 4 MB are parsed by one).  Returns false if
-// unreadable.
+// hardware threads (files under 4 MB are parsed by one).  With a context and
+// replicate == 0 the counts, reduction and bitmaps are computed on the device
+// and left resident in `ctx` (db.on_device); otherwise on the host into
+// db.bitmap.  Returns false if unreadable (or on a device error).
 bool load_database(const std::string &path, double support, int replicate, Database &db,
-                   std::string &err, int parse_threads = 0);
+                   std::string &err, int parse_threads = 0, tfim_ctx *ctx = nullptr);
 
 }  // namespace tfim

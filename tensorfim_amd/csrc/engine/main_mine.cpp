@@ -4,7 +4,7 @@
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
 //             [--workers N] [--learned] [--rank] [--gate indep|always]
 //             [--wave-kernel 0|1|2] [--graphs 0|1] [--parse-threads N]
-//             [--model PATH] [--trace PATH] [--verbose]
+//             [--prep gpu|host] [--model PATH] [--trace PATH] [--verbose]
 //
 // Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the
 // reference layout and prints run statistics as "key: value" lines.  Exit
@@ -34,6 +34,7 @@ int main(int argc, char **argv) {
     int wave_kernel = 0;
     int graphs = -1;   // -1: library default
     int parse_threads = 0;   // 0: hardware threads for files of 4 MB and more
+    std::string prep = "gpu";   // counts, reduction and bitmaps: gpu | host
     int replicate = 0, device = 0;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
@@ -54,6 +55,7 @@ int main(int argc, char **argv) {
         else if (a == "--wave-kernel") wave_kernel = atoi(next().c_str());
         else if (a == "--graphs") graphs = atoi(next().c_str());
         else if (a == "--parse-threads") parse_threads = atoi(next().c_str());
+        else if (a == "--prep") prep = next();
         else if (a == "--model") opt.model_path = next();
         else if (a == "--trace") opt.trace_path = next();
         else if (a == "--verbose") opt.verbose = true;
@@ -82,16 +84,21 @@ int main(int argc, char **argv) {
         tfim_close(ctx);
         return 1;
     }
+    if (prep != "gpu" && prep != "host") {
+        fprintf(stderr, "--prep must be gpu or host\n");
+        tfim_close(ctx);
+        return 1;
+    }
     Database db;
     std::string err;
-    if (!load_database(file, support, replicate, db, err, parse_threads)) {
+    if (!load_database(file, support, replicate, db, err, parse_threads, prep == "gpu" ? ctx : nullptr)) {
         fprintf(stderr, "%s\n", err.c_str());
         tfim_close(ctx);
         return 1;
     }
     const double t_up0 = now();
     int rc = TFIM_OK;
-    if (db.d_count > 0 && db.base_bits > 0) {
+    if (db.d_count > 0 && db.base_bits > 0 && !db.on_device) {
         if (db.replicate > 1)
             rc = tfim_load_bitmaps_replicated(ctx, db.bitmap.data(), db.d_count, db.base_bits, db.words, db.n_surv,
                                               db.replicate);

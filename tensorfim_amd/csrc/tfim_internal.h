@@ -110,6 +110,17 @@ bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLd
 bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
                          KernelLaunch &L);
 
+// Device preprocessing (kernels_prep.hip).
+void launch_prep_hist(const int32_t *ids, int64_t n, int32_t nbins, int32_t *freq, int32_t *bad, int n_cu,
+                      hipStream_t s);
+int64_t prep_blocks(int64_t n_rows);
+void launch_prep_rows(const int32_t *ids, const int64_t *off, int64_t n_rows, const int32_t *dense_of, int32_t *cnt,
+                      int32_t *blk_keep, unsigned long long *stats, hipStream_t s);
+void launch_prep_scan(int32_t *blk, int64_t n_blk, int64_t *total, hipStream_t s);
+void launch_prep_index(const int32_t *cnt, int64_t n_rows, const int32_t *blk_off, int32_t *kept, hipStream_t s);
+void launch_prep_build(const int32_t *ids, const int64_t *off, const int32_t *kept, int64_t n_red,
+                       const int32_t *dense_of, uint64_t *bm, int64_t pitch, int32_t *dup, hipStream_t s);
+
 // Zero `n` int32 with a fill kernel (graph-capture safe).
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
 void plan_zero_i32(int32_t *p, int64_t n, KernelLaunch &L);

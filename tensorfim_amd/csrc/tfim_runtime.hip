@@ -89,6 +89,12 @@ struct tfim_ctx {
     int use_graphs = 1;     // waves replayed as instantiated graphs
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
     std::map<std::string, std::pair<int, int32_t>> popmat_tuned;   // shape -> (tile, split_k)
+
+    // device preprocessing state between tfim_prep_begin and tfim_prep_finish
+    int32_t *p_ids = nullptr;
+    int64_t *p_off = nullptr;
+    int64_t p_n_ids = 0, p_n_rows = -1;
+    int32_t p_id_max = -1;
 };
 
 namespace {
@@ -189,6 +195,7 @@ void tfim_close(tfim_ctx *c) {
     }
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    hipFree(c->p_ids); hipFree(c->p_off);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -815,6 +822,126 @@ int tfim_time_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B,
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *avg_ms = (double)ms / reps;
+    return TFIM_OK;
+}
+
+namespace {
+void prep_release(tfim_ctx *c) {
+    hipFree(c->p_ids);
+    hipFree(c->p_off);
+    c->p_ids = nullptr;
+    c->p_off = nullptr;
+    c->p_n_ids = 0;
+    c->p_n_rows = -1;
+    c->p_id_max = -1;
+}
+}  // namespace
+
+int tfim_prep_begin(tfim_ctx *c, const int32_t *ids, int64_t n_ids, const int64_t *row_off, int64_t n_rows,
+                    int32_t id_max, int32_t *freq) {
+    if (!c || !row_off || !freq || n_ids < 0 || n_rows < 0 || id_max < 0 || (n_ids > 0 && !ids))
+        return fail(c, TFIM_E_INVALID, "prep_begin: bad argument");
+    if (row_off[0] != 0 || row_off[n_rows] != n_ids)
+        return fail(c, TFIM_E_INVALID, "prep_begin: row offsets must run from 0 to n_ids");
+    if (n_rows > (int64_t)INT32_MAX) return fail(c, TFIM_E_RANGE, "prep_begin: %lld rows exceed int32", (long long)n_rows);
+    if (!c->owns_bm) return fail(c, TFIM_E_STATE, "prep_begin: context shares its parent's bitmaps");
+    if (int r = set_device(c)) return r;
+    prep_release(c);
+    int32_t *d_freq = nullptr, *d_bad = nullptr;
+    if (hipMalloc(&c->p_ids, (size_t)std::max<int64_t>(1, n_ids) * 4) != hipSuccess ||
+        hipMalloc(&c->p_off, (size_t)(n_rows + 1) * 8) != hipSuccess ||
+        hipMalloc(&d_freq, (size_t)(id_max + 1) * 4 + 4) != hipSuccess) {
+        hipFree(d_freq);
+        prep_release(c);
+        return fail(c, TFIM_E_NOMEM, "prep_begin: device allocation for %lld ids failed", (long long)n_ids);
+    }
+    d_bad = d_freq + (id_max + 1);
+    int rc = TFIM_OK;
+    int32_t bad = 0;
+    do {
+        if (n_ids > 0 && hipMemcpyAsync(c->p_ids, ids, (size_t)n_ids * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+        if (hipMemcpyAsync(c->p_off, row_off, (size_t)(n_rows + 1) * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+        if (hipMemsetAsync(d_freq, 0, (size_t)(id_max + 2) * 4, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+        launch_prep_hist(c->p_ids, n_ids, id_max + 1, d_freq, d_bad, c->n_cu, c->stream);
+        if (hipGetLastError() != hipSuccess) { rc = TFIM_E_HIP; break; }
+        if (hipMemcpyAsync(freq, d_freq, (size_t)(id_max + 1) * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(&bad, d_bad, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+    } while (0);
+    hipFree(d_freq);
+    if (rc) {
+        prep_release(c);
+        return fail(c, rc, "prep_begin: HIP error");
+    }
+    if (bad) {
+        prep_release(c);
+        return fail(c, TFIM_E_INVALID, "prep_begin: an id lies outside [0, %d]", id_max);
+    }
+    c->p_n_ids = n_ids;
+    c->p_n_rows = n_rows;
+    c->p_id_max = id_max;
+    return TFIM_OK;
+}
+
+int tfim_prep_finish(tfim_ctx *c, const int32_t *dense_of, int32_t n_items, int32_t n_cols, int64_t *rows_reduced,
+                     int64_t *len_min, int64_t *len_max, int64_t *len_sum, int32_t *duplicates) {
+    if (!c || !dense_of || !rows_reduced || n_items < 0 || n_cols < 0 || n_cols > n_items)
+        return fail(c, TFIM_E_INVALID, "prep_finish: bad argument");
+    if (c->p_n_rows < 0) return fail(c, TFIM_E_STATE, "prep_finish: no tfim_prep_begin before");
+    for (int32_t i = 0; i <= c->p_id_max; i++)
+        if (dense_of[i] < -1 || dense_of[i] >= n_items) {
+            prep_release(c);
+            return fail(c, TFIM_E_INVALID, "prep_finish: dense_of[%d] = %d outside [-1, %d)", i, dense_of[i], n_items);
+        }
+    if (int r = set_device(c)) return r;
+    const int64_t n_rows = c->p_n_rows, n_blk = prep_blocks(n_rows);
+    int32_t *d_map = nullptr, *d_cnt = nullptr, *d_blk = nullptr, *d_kept = nullptr, *d_dup = nullptr;
+    unsigned long long *d_stats = nullptr;
+    int64_t *d_total = nullptr;
+    int rc = TFIM_OK;
+    int64_t n_red = 0;
+    unsigned long long st[3] = {0, 0, 0};
+    int32_t dup = 0;
+    const unsigned long long init[3] = {0ull, ~0ull, 0ull};
+    do {
+        if (hipMalloc(&d_map, (size_t)(c->p_id_max + 1) * 4) != hipSuccess ||
+            hipMalloc(&d_cnt, (size_t)std::max<int64_t>(1, n_rows) * 4) != hipSuccess ||
+            hipMalloc(&d_blk, (size_t)std::max<int64_t>(1, n_blk) * 4) != hipSuccess ||
+            hipMalloc(&d_kept, (size_t)std::max<int64_t>(1, n_rows) * 4) != hipSuccess ||
+            hipMalloc(&d_stats, sizeof init + 8 + 4) != hipSuccess) { rc = TFIM_E_NOMEM; break; }
+        d_total = reinterpret_cast<int64_t *>(d_stats + 3);
+        d_dup = reinterpret_cast<int32_t *>(d_total + 1);
+        if (hipMemcpyAsync(d_map, dense_of, (size_t)(c->p_id_max + 1) * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemcpyAsync(d_stats, init, sizeof init, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemsetAsync(d_total, 0, 12, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+        if (n_rows > 0) {
+            launch_prep_rows(c->p_ids, c->p_off, n_rows, d_map, d_cnt, d_blk, d_stats, c->stream);
+            launch_prep_scan(d_blk, n_blk, d_total, c->stream);
+            if (hipGetLastError() != hipSuccess) { rc = TFIM_E_HIP; break; }
+        }
+        if (hipMemcpyAsync(&n_red, d_total, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+        if (n_red > 0 && n_items > 0) {
+            if (int r = load_common(c, n_items, n_red, n_cols)) { rc = r; break; }
+            if (hipMemsetAsync(c->d_bm, 0, (size_t)n_items * c->pitch * 8, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+            launch_prep_index(d_cnt, n_rows, d_blk, d_kept, c->stream);
+            launch_prep_build(c->p_ids, c->p_off, d_kept, n_red, d_map, c->d_bm, c->pitch, d_dup, c->stream);
+            if (hipGetLastError() != hipSuccess) { rc = TFIM_E_HIP; break; }
+        }
+        if (hipMemcpyAsync(st, d_stats, sizeof st, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(&dup, d_dup, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+    } while (0);
+    hipFree(d_map); hipFree(d_cnt); hipFree(d_blk); hipFree(d_kept); hipFree(d_stats);
+    prep_release(c);
+    if (rc == TFIM_E_NOMEM) return fail(c, rc, "prep_finish: device allocation failed");
+    if (rc == TFIM_E_HIP) return fail(c, rc, "prep_finish: HIP error");
+    if (rc) return rc;   // message set by load_common
+    *rows_reduced = n_red;
+    if (len_sum) *len_sum = (int64_t)st[0];
+    if (len_min) *len_min = n_red > 0 ? (int64_t)st[1] : 0;
+    if (len_max) *len_max = (int64_t)st[2];
+    if (duplicates) *duplicates = dup;
     return TFIM_OK;
 }
 

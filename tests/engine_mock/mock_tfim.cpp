@@ -2,11 +2,13 @@
 // test build of the engine (tests/test_engine_host_logic.py) so the search
 // logic can be checked where there is no GPU.  The product never loads it.
 #include "tfim.h"
+#include <algorithm>
 #include <cstring>
 #include <vector>
 #include <string>
 #include <cstdlib>
-struct tfim_ctx { std::vector<uint64_t> bm; int64_t W=0, n_bits=0; int32_t n_items=0, n_cols=0; std::string err; };
+struct tfim_ctx { std::vector<uint64_t> bm; int64_t W=0, n_bits=0; int32_t n_items=0, n_cols=0; std::string err;
+                  std::vector<int32_t> p_ids; std::vector<int64_t> p_off; int32_t p_max=-1; };
 extern "C" {
 int tfim_open(int, tfim_ctx **o) { *o = new tfim_ctx(); return 0; }
 void tfim_close(tfim_ctx *c) { delete c; }
@@ -17,6 +19,25 @@ int tfim_config(tfim_ctx *, const char *, int64_t) { return 0; }
 int tfim_load_bitmaps(tfim_ctx *c, const uint64_t *rows, int32_t n, int64_t nb, int64_t w, int32_t nc) {
   c->W = (nb+63)/64; c->bm.assign((size_t)n*c->W, 0); for (int i=0;i<n;i++) memcpy(&c->bm[(size_t)i*c->W], rows+(size_t)i*w, c->W*8);
   c->n_bits=nb; c->n_items=n; c->n_cols=nc; return 0; }
+int tfim_prep_begin(tfim_ctx *c, const int32_t *ids, int64_t n, const int64_t *off, int64_t rows, int32_t mx, int32_t *freq) {
+  c->p_ids.assign(ids, ids + n); c->p_off.assign(off, off + rows + 1); c->p_max = mx;
+  memset(freq, 0, (size_t)(mx + 1) * 4);
+  for (int64_t i = 0; i < n; i++) { if (ids[i] < 0 || ids[i] > mx) return -2; freq[ids[i]]++; }
+  return 0; }
+int tfim_prep_finish(tfim_ctx *c, const int32_t *dof, int32_t n_items, int32_t n_cols, int64_t *red, int64_t *lmin,
+                     int64_t *lmax, int64_t *lsum, int32_t *dup) {
+  std::vector<int64_t> kept; int64_t mn = 0, mx = 0, sm = 0; int d = 0;
+  for (int64_t t = 0; t + 1 < (int64_t)c->p_off.size(); t++) {
+    int64_t k = 0; for (int64_t j = c->p_off[t]; j < c->p_off[t+1]; j++) k += dof[c->p_ids[j]] >= 0;
+    if (k) { mn = kept.empty() ? k : std::min(mn, k); mx = std::max(mx, k); sm += k; kept.push_back(t); } }
+  *red = (int64_t)kept.size(); if (lmin) *lmin = mn; if (lmax) *lmax = mx; if (lsum) *lsum = sm;
+  if (!kept.empty() && n_items > 0) {
+    c->W = ((int64_t)kept.size() + 63) / 64; c->bm.assign((size_t)n_items * c->W, 0);
+    for (size_t r = 0; r < kept.size(); r++)
+      for (int64_t j = c->p_off[kept[r]]; j < c->p_off[kept[r]+1]; j++) { int x = dof[c->p_ids[j]]; if (x < 0) continue;
+        uint64_t &w = c->bm[(size_t)x * c->W + (r >> 6)]; if (w >> (r & 63) & 1) d = 1; w |= 1ull << (r & 63); }
+    c->n_bits = (int64_t)kept.size(); c->n_items = n_items; c->n_cols = n_cols; }
+  if (dup) *dup = d; c->p_ids.clear(); c->p_off.clear(); return 0; }
 int tfim_load_bitmaps_replicated(tfim_ctx *c, const uint64_t *rows, int32_t n, int64_t nb, int64_t w, int32_t nc, int32_t) { return tfim_load_bitmaps(c, rows, n, nb, w, nc); }
 static void cnt(tfim_ctx *c, const int32_t *it, int k, const int32_t *cols, int nc, int32_t *out) {
   std::vector<uint64_t> m(c->bm.begin()+(size_t)it[0]*c->W, c->bm.begin()+(size_t)(it[0]+1)*c->W);

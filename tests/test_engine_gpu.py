@@ -80,3 +80,40 @@ def test_replicated_dense_scales_exactly(tmp_path):
     got = tf.load_results(path)
     # single-item MFIs report occurrence counts, which also scale by k
     assert got == exp
+
+
+HEADER_ALL = HEADER_KEYS + ["itemIDmin", "transLengthMin", "transLengthMax", "transLengthReducedMin",
+                            "transLengthReducedMax"]
+
+
+@pytest.mark.parametrize("cfg,thr,rows", [("dense-small", 0.6, None), ("gene", 0.12, None),
+                                          ("clickstream", 0.02, 100000), ("dense-replicated", 0.9, 6000)])
+def test_device_preprocessing_matches_host(tmp_path, cfg, thr, rows):
+    """Counts, reduction and bitmaps built on the device (LDS or global
+    histogram by id range) give the host pass's header and MFI set."""
+    data = tmp_path / ("%s.txt" % cfg)
+    _gen(cfg, data, rows)
+    got = {}
+    for prep in ("gpu", "host"):
+        path, st = tf.mine(str(data), thr, engine="mafia", out=str(tmp_path / (prep + ".res")), extra=["--prep", prep],
+                           timeout=300)
+        got[prep] = (tf.load_results(path), {k: tf.load_header(path)[k] for k in HEADER_ALL})
+    assert got["gpu"] == got["host"]
+
+
+def test_device_preprocessing_edge_rows(tmp_path):
+    """Empty lines, a last line without newline, repeated ids (the engine then
+    falls back to the reference enumeration), item 0, rows that lose every
+    item: device and host preprocessing agree and match the oracle."""
+    lines = ["1 2 3", "", "2 3 3", "0 1 2", "7", "", "0 2 3 1", "3 2", "9 8", "1 2 0"]
+    data = tmp_path / "edge.txt"
+    data.write_text("\n".join(lines))
+    ref = tmp_path / "oracle.res"
+    oracle_lib.mine(data, 0.3, threads=2, out=ref)
+    for prep in ("gpu", "host"):
+        path, st = tf.mine(str(data), 0.3, engine="mafia", out=str(tmp_path / (prep + ".res")), extra=["--prep", prep],
+                           timeout=120)
+        assert tf.load_results(path) == tf.load_results(str(ref)), prep
+        hg, hr = tf.load_header(path), tf.load_header(str(ref))
+        for k in HEADER_KEYS:
+            assert hg[k] == hr[k], (prep, k)

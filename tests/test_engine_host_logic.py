@@ -93,3 +93,33 @@ def test_parallel_parser_matches_one_pass(mock_engine, tmp_path):
             outs.append((r.returncode, tf.load_results(str(out)) if r.returncode == 0 else None,
                          {k: tf.load_header(str(out)).get(k) for k in keys} if r.returncode == 0 else None))
         assert all(o == outs[0] for o in outs), (it, txt[:200], rep, outs)
+
+
+def test_device_preprocessing_flow_matches_host(mock_engine, tmp_path):
+    """--prep gpu (counts, reduction and bitmaps through tfim_prep_begin /
+    tfim_prep_finish, here the CPU stand-in) gives the host pass's header and
+    output on databases with empty lines, repeated ids and item 0."""
+    rnd = random.Random(5)
+    keys = ["transNum", "dimension", "itemIDmin", "itemIDmax", "dimensionReduced", "transNumReduced", "MFIsNumber",
+            "transLengthMin", "transLengthMax", "transLengthMean", "transLengthReducedMin", "transLengthReducedMax",
+            "transLengthReducedMean"]
+    for it in range(40):
+        n_items = rnd.randint(2, 12)
+        rows = []
+        for _ in range(rnd.randint(0, 40)):
+            r = rnd.sample(range(0, n_items), rnd.randint(0, n_items))
+            if rnd.random() < 0.1 and r:
+                r.append(r[-1])
+            rows.append(r)
+        txt = "\n".join(" ".join(map(str, r)) for r in rows) + rnd.choice(["", "\n", "\n\n"])
+        db = tmp_path / "db.txt"
+        db.write_text(txt)
+        sup = rnd.choice([0.05, 0.2, 0.3, 0.5, 0.9])
+        res = []
+        for prep in ("host", "gpu"):
+            out = tmp_path / ("%s.res" % prep)
+            r = subprocess.run([mock_engine, str(db), repr(sup), "--out", str(out), "--prep", prep],
+                               capture_output=True, text=True)
+            res.append((r.returncode, tf.load_results(str(out)) if r.returncode == 0 else None,
+                        {k: tf.load_header(str(out)).get(k) for k in keys} if r.returncode == 0 else None))
+        assert res[0] == res[1], (it, txt[:200], sup, res)
