@@ -117,7 +117,9 @@ int tfim_count_nodes_wait(tfim_ctx *ctx, int64_t ticket, int32_t *out);
 
 /* Tuning knobs of the batched path (ablation and autotuning; the reference's
  * equivalents are environment variables, kernel_bmma.cu:351-368):
- *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel
+ *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel (two
+ *                  workgroups per CU), 3 LDS-staged with wide staging (one
+ *                  workgroup per CU, up to 512 bits per lane per column)
  *   "tile_cols"    columns per work tile of the batched path: 8..64, or 0
  *                  (default) to let the wave tuner choose
  *   "mfma_tile"    workgroup tile of the int8-MFMA matrix count: 0 auto
@@ -135,7 +137,8 @@ int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
  * the LDS-staged kernel, rows, summed kernel milliseconds (HIP events).  In
  * TFIM_PATH_AUTO the library's wave tuner picks, for waves that fit both
  * kernels, the kernel and the columns per work tile (L2 kernel with 32; LDS
- * kernel with 16, 32 or 64): each arm is timed three times, then the cheapest
+ * kernel with 16, 32 or 64, or wide staging with 32): each arm is timed
+ * three times, then the cheapest
  * per row-word of work is kept and another arm re-timed every 16 waves. */
 int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,
                     int64_t *rows, double *kernel_ms);

@@ -3,7 +3,7 @@
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
 //             [--workers N] [--learned] [--rank] [--gate indep|always]
-//             [--wave-kernel 0|1|2] [--graphs 0|1] [--parse-threads N]
+//             [--wave-kernel 0|1|2|3] [--graphs 0|1] [--parse-threads N]
 //             [--prep gpu|host] [--model PATH] [--trace PATH] [--verbose]
 //
 // Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the

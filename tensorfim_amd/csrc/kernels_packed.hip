@@ -344,30 +344,43 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
         if (acc[i]) atomicAdd(&d.out[i], acc[i]);
 }
 
-bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
+bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu, bool wide,
                          KernelLaunch &L) {
     if (d.n_slots < 1 || d.n_slots > kLdsMaxSlots || d.n_out > kLdsMaxOut) return false;
     const int64_t pitch4 = pitch_words / 2;
     const size_t acc_bytes = ((size_t)d.n_out * 4 + 15) / 16 * 16;
     auto bytes = [&](int rw) { return (size_t)d.n_slots * 64 * rw * 16 + acc_bytes; };
-    // Prefer two workgroups per CU (<= 80 KB each); the widest chunk that fits.
+    // Narrow: two workgroups per CU (<= 80 KB each), the widest chunk that
+    // fits.  Wide: the widest chunk (up to 4 x 128 bits per lane) that fits
+    // one workgroup per CU, so each lane counts more bits per column between
+    // reductions, at half the occupancy.
     constexpr size_t kHalf = 80 * 1024, kFull = 160 * 1024;
     static bool attr_set = false;
     if (!attr_set) {
         hipFuncSetAttribute((const void *)k_count_wave_lds<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
         hipFuncSetAttribute((const void *)k_count_wave_lds<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
+        hipFuncSetAttribute((const void *)k_count_wave_lds<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
         hipFuncSetAttribute((const void *)k_count_wave_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
         attr_set = true;
     }
-    int rw = 4;
-    while (rw > 1 && bytes(rw) > kHalf) rw /= 2;
-    if (bytes(rw) > kHalf && bytes(1) > kFull) return false;
+    int rw;
+    if (wide) {
+        rw = 4;
+        while (rw > 1 && bytes(rw) > kFull) rw--;
+        if (bytes(rw) > kFull) return false;
+    } else {
+        rw = 4;
+        while (rw > 1 && bytes(rw) > kHalf) rw /= 2;
+        if (bytes(rw) > kHalf && bytes(1) > kFull) return false;
+    }
     const size_t lds_bytes = bytes(rw);
     const int64_t n_chunks = (pitch4 + 64 * rw - 1) / (64 * rw);
     const int per_cu = lds_bytes <= kHalf ? 2 : 1;
     const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_chunks, (int64_t)n_cu * per_cu));
-    L.func = rw == 4 ? (const void *)k_count_wave_lds<4>
-                     : (rw == 2 ? (const void *)k_count_wave_lds<2> : (const void *)k_count_wave_lds<1>);
+    L.func = rw == 4   ? (const void *)k_count_wave_lds<4>
+             : rw == 3 ? (const void *)k_count_wave_lds<3>
+             : rw == 2 ? (const void *)k_count_wave_lds<2>
+                       : (const void *)k_count_wave_lds<1>;
     L.grid = dim3((unsigned)grid);
     L.block = dim3(kLdsThreads);
     L.shmem = (uint32_t)lds_bytes;
@@ -382,7 +395,7 @@ bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsD
 bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
                            hipStream_t s) {
     KernelLaunch L;
-    if (!plan_count_wave_lds(bm, pitch_words, d, n_cu, L)) return false;
+    if (!plan_count_wave_lds(bm, pitch_words, d, n_cu, false, L)) return false;
     L.launch(s);
     return true;
 }

@@ -107,7 +107,7 @@ constexpr int kLdsMaxOut = 8192;
 // Returns false if the shape does not fit LDS (caller uses launch_count_wave).
 bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
                            hipStream_t s);
-bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
+bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu, bool wide,
                          KernelLaunch &L);
 
 // Device preprocessing (kernels_prep.hip).

@@ -23,10 +23,12 @@ struct tfim_buf {
 
 namespace {
 // Online tuner arms for waves that both batched kernels accept:
-// (kernel 1 = L2-resident, 2 = LDS-staged; columns per work tile).
-constexpr int kTuneArms = 4;
-constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2};
-constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64};
+// (kernel 1 = L2-resident, 2 = LDS-staged; columns per work tile; LDS
+// staging wide = one workgroup per CU with up to 4 x 128 bits per lane).
+constexpr int kTuneArms = 5;
+constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2, 2};
+constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64, 32};
+constexpr bool kArmWide[kTuneArms] = {false, false, false, false, true};
 }  // namespace
 
 struct tfim_ctx {
@@ -78,11 +80,11 @@ struct tfim_ctx {
 
     int64_t wave_launches = 0, wave_graph_replays = 0, wave_graphs = 0, wave_rows = 0, wave_lds_launches = 0;
     double wave_ms = 0;
-    int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel
+    int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel, 3 LDS kernel with wide staging
     // wave tuner: EMA of kernel milliseconds per (row prefix + column) word,
     // per arm (kernel, columns per tile)
-    double tune_cost[kTuneArms] = {0, 0, 0, 0};
-    int64_t tune_samples[kTuneArms] = {0, 0, 0, 0};
+    double tune_cost[kTuneArms] = {};
+    int64_t tune_samples[kTuneArms] = {};
     int64_t tune_calls = 0;
     int tile_cols = 0;      // 0: chosen by the tuner (32 when it does not run)
     int mfma_tile = 0;      // 0 auto (= 256), 128 or 256
@@ -456,7 +458,7 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         if (!lds)
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
     }
-    if (c->wave_kernel == 2 && !lds)
+    if (c->wave_kernel >= 2 && !lds)
         return fail(c, TFIM_E_RANGE, "count_nodes: wave does not fit the LDS kernel (%zu items, %lld outputs)",
                     slot_rows.size(), (long long)n_out);
     // autotune over (kernel, columns per tile) for waves both kernels accept:
@@ -556,7 +558,8 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         d.n_slots = (int32_t)slot_rows.size();
         d.n_tiles = (int32_t)tiles.size();
         d.n_out = (int32_t)n_out;
-        if (!plan_count_wave_lds(c->d_bm, c->pitch, d, c->n_cu, K))
+        const bool wide = c->wave_kernel == 3 || (tuned >= 0 && kArmWide[tuned]);
+        if (!plan_count_wave_lds(c->d_bm, c->pitch, d, c->n_cu, wide, K))
             return fail(c, TFIM_E_RANGE, "count_nodes: LDS kernel rejected %d slots", d.n_slots);
         c->wave_lds_launches++;
     } else {
@@ -624,7 +627,7 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
 int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
     if (!c || !key) return TFIM_E_INVALID;
     const std::string k(key);
-    if (k == "wave_kernel" && value >= 0 && value <= 2) {
+    if (k == "wave_kernel" && value >= 0 && value <= 3) {
         c->wave_kernel = (int)value;
         return TFIM_OK;
     }

@@ -146,10 +146,11 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
 
 
 @pytest.mark.parametrize("graphs", [1, 0])
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
 @pytest.mark.parametrize("n_bits,n_items", [(1100, 30), (3200, 75), (400_003, 90), (2_000_000, 38)])
 def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
-    """Both batched kernels (1 = L2-resident, 2 = LDS-staged) on the same waves,
+    """Both batched kernels (1 = L2-resident, 2 = LDS-staged, 3 = LDS-staged
+    with wide staging) on the same waves,
     replayed from graphs or issued directly: uneven prefixes, uneven and empty
     tails, full-column rows."""
     rng = np.random.default_rng(kernel * 100 + n_items)
