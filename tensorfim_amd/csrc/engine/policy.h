@@ -29,6 +29,7 @@ struct Decision {
     bool pending = false;           // out: lookahead undecided until resolve()
     int32_t record = -1;            // out: gate record id for observe()
     std::vector<int32_t> item_records;  // out: ranker record id per tail index
+    std::vector<uint64_t> known;    // survivors e with sup(head + e) < thr (bitset), engine-owned
 };
 
 class Policy {

@@ -277,6 +277,10 @@ struct MNode {
     int32_t depth = 0;
     int32_t decision = -1;           // HUT: policy record index for its outcome
     int32_t item_record = -1;        // EXPAND: ranker record of the item that made it
+    // survivors e already known to have sup(head + e) < thr (bitset): they
+    // stay infrequent for every superset of head, so HUT and LEAF rows skip
+    // them (a pruning of columns only; decisions are unchanged)
+    std::vector<uint64_t> known;
 };
 
 // Found MFIs, sliced by item: bits_[i] has bit m set when MFI m contains
@@ -404,9 +408,14 @@ class Mafia {
         }
     }
 
-    void complement(const std::vector<int32_t> &x, std::vector<int32_t> &cols, bool lead) {
+    // Survivors outside x that are not known infrequent with x's head; with
+    // `lead`, x[0] first (the HUT row's own support).
+    void complement(const std::vector<int32_t> &x, const std::vector<uint64_t> &known, std::vector<int32_t> &cols,
+                    bool lead) {
         mark_.assign(db_.n_surv, 0);
         for (int32_t i : x) mark_[i] = 1;
+        for (size_t w = 0; w < known.size(); w++)
+            for (uint64_t b = known[w]; b; b &= b - 1) mark_[(int32_t)(w * 64 + __builtin_ctzll(b))] = 1;
         cols.clear();
         if (lead) cols.push_back(x[0]);
         for (int32_t i = 0; i < db_.n_surv; i++)
@@ -426,13 +435,13 @@ class Mafia {
                 case MNode::HUT: {
                     std::vector<int32_t> hut = n.head;
                     hut.insert(hut.end(), n.tail.begin(), n.tail.end());
-                    complement(hut, cols, true);
+                    complement(hut, n.known, cols, true);
                     wb_.add(hut.data(), (int)hut.size(), cols.data(), (int)cols.size());
                     st_.hut_rows++;
                     break;
                 }
                 case MNode::LEAF:
-                    complement(n.head, cols, false);
+                    complement(n.head, n.known, cols, false);
                     wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
                     st_.leaf_rows++;
                     break;
@@ -471,6 +480,7 @@ class Mafia {
             c.head.push_back(d.tail[d.order[q]]);
             c.sup = d.tail_sup[d.order[q]];
             c.depth = d.depth + 1;
+            c.known = d.known;
             if (!d.item_records.empty()) c.item_record = d.item_records[d.order[q]];
             for (size_t r = q + 1; r < k; r++) c.tail.push_back(d.tail[d.order[r]]);
             if (c.tail.empty()) {
@@ -497,12 +507,17 @@ class Mafia {
                 d.head = n.head;
                 d.sup = n.sup;
                 d.depth = n.depth;
+                d.known = std::move(n.known);
+                if (d.known.empty()) d.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
                 int kept = 0;
                 for (int j = 0; j < nc; j++) kept += frequent(db_, res[j]);
                 if (n.item_record >= 0) pol_.observe_value(n.item_record, nc > 0 ? (float)kept / nc : 0.0f);
                 for (int j = 0; j < nc; j++) {
                     const int32_t t = n.tail[j], s = res[j];
-                    if (!frequent(db_, s)) continue;
+                    if (!frequent(db_, s)) {
+                        d.known[(size_t)t >> 6] |= 1ull << (t & 63);
+                        continue;
+                    }
                     if (s == n.sup) {   // PEP: every MFI above head contains t
                         d.head.push_back(t);
                         st_.pep_moves++;
@@ -519,6 +534,7 @@ class Mafia {
                         leaf.kind = MNode::LEAF;
                         leaf.head = d.head;
                         leaf.sup = d.sup;
+                        leaf.known = std::move(d.known);
                         pending_.push_back(std::move(leaf));
                     }
                     continue;
@@ -547,6 +563,7 @@ class Mafia {
                     d.depth = n.depth;
                     d.tail = n.tail;
                     d.tail_sup = n.tail_sup;
+                    d.known = std::move(n.known);
                     d.order.resize(d.tail.size());
                     std::iota(d.order.begin(), d.order.end(), 0);   // tail kept in decided order
                     hut_children_.push_back(std::move(d));
@@ -609,6 +626,7 @@ class Mafia {
                 h.tail_sup.push_back(d.tail_sup[o]);
             }
             h.decision = d.record;
+            h.known = d.known;
             stack_.push_back(std::move(h));
         } else {
             if (d.tail.size() >= 2) st_.hut_skipped++;
