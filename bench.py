@@ -166,7 +166,7 @@ def cpu_engine_baseline(data, total_nodes, args):
     exe = os.path.join(ROOT, "oracle", "_build", "tfim_oracle")
     res = []
     nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    for threads, nodes in ((1, 400), (nthreads, 400 * nthreads)):
+    for threads, nodes in ((1, 4000), (nthreads, 4000 * nthreads)):
         log("cpu engine baseline: %d threads, first %d DFS nodes ..." % (threads, nodes))
         r = subprocess.run([exe, data, "0.85", "--replicate", "256", "--threads", str(threads), "--max-nodes",
                             str(nodes), "--out", data + ".cpu.res"], capture_output=True, text=True, check=True)
