@@ -34,6 +34,11 @@ METRIC = "support-count Tbit-op/s; end-to-end MFI seconds, batched+learned vs pl
 HBM_PEAK_GBS = 8000.0
 I8_PEAK_TOPS = 5000.0      # dense int8 MFMA, MI355X
 VALU_PEAK_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz int32 ops
+# AND + popcount of 32 bit-pairs per lane takes two VALU ops: 16 bit-ops per op
+VALU_BITOP_PEAK_T = VALU_PEAK_TOPS * 16
+# memory-side bytes per launch of the packed kernel on this workload (FETCH_SIZE,
+# same runs as TRAFFIC_BYTES; its reads are 64-byte, factor 1.00)
+PACKED_TRAFFIC_BYTES = 141876016.75 * 1024.0
 
 # HBM bytes per launch of the default int8-MFMA count kernel on this workload,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
@@ -102,6 +107,16 @@ def bench_isolated(ctx, args):
         tbit = bitops / (ms * 1e-3) / 1e12
         res[path] = {"kernel_ms": ms, "wall_ms_per_step": wall_ms, "tbitops": tbit, "exact_on_sample": ok,
                      "split_k": split}
+        if path == "mfma":
+            res[path]["roofline"] = {"bound": "mfma", "achieved_tops": 2 * tbit, "peak_tops": I8_PEAK_TOPS,
+                                     "frac": 2 * tbit / I8_PEAK_TOPS,
+                                     "memory_gbs": TRAFFIC_BYTES / (ms * 1e-3) / 1e9,
+                                     "memory_frac": TRAFFIC_BYTES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        else:
+            res[path]["roofline"] = {"bound": "valu", "achieved_tbitops": tbit, "peak_tbitops": VALU_BITOP_PEAK_T,
+                                     "frac": tbit / VALU_BITOP_PEAK_T,
+                                     "memory_gbs": PACKED_TRAFFIC_BYTES / (ms * 1e-3) / 1e9,
+                                     "memory_frac": PACKED_TRAFFIC_BYTES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log("%-6s %.3f ms/launch  %.1f Tbit-op/s  exact=%s" % (path, ms, tbit, ok))
         if not ok:
             raise SystemExit("support counts differ from the oracle on sampled rows (%s path)" % path)

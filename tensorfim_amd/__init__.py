@@ -280,6 +280,7 @@ class Mlp:
         h = ctypes.c_void_p()
         ctx._check(ctx._lib.tfim_mlp_create(ctx._h, ctypes.c_uint64(seed), ctypes.byref(h)))
         self._h = h
+        self._pending_rows = 0
 
     def close(self):
         if self._h:

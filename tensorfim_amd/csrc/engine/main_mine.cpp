@@ -155,6 +155,8 @@ int main(int argc, char **argv) {
            db.preprocessing_seconds, t_up1 - t_up0);
     printf("computation_time(s): %g\nwall_time(s): %g\n", info.computation_time, wall);
     printf("count_time(s): %g\nkernel_ms: %g\n", st.count_seconds, st.kernel_ms);
+    printf("submit_time(s): %g\nbuild_time(s): %g\nprocess_time(s): %g\n", st.submit_seconds, st.build_seconds,
+           st.process_seconds);
     printf("rows: %lld\nwaves: %lld\nlds_waves: %lld\n", (long long)st.nodes, (long long)st.waves,
            (long long)st.lds_waves);
     int64_t n_graphs = 0, graph_launches = 0;

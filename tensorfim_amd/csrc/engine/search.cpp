@@ -49,6 +49,7 @@ class WaveBatch {
         const int rc = tfim_count_nodes_submit(ctx, item_ids_.data(), item_off_.data(), rows(), col_ids_.data(),
                                                col_off_.data(), TFIM_PATH_AUTO, &ticket_);
         st.count_seconds += now() - t0;
+        st.submit_seconds += now() - t0;
         if (rc != TFIM_OK) {
             err = std::string("tfim_count_nodes_submit: ") + tfim_last_error(ctx);
             return rc;
@@ -367,7 +368,9 @@ class Mafia {
             if (int rc = launch(fl_[1 - cur], err)) return rc;
             Flight &f = fl_[cur];
             if (int rc = f.wb.wait(ctx_, st_, err)) return rc;
+            const double tp = now();
             if (int rc = process_wave(f, out)) return rc;
+            st_.process_seconds += now() - tp;
             f.live = false;
             cur = 1 - cur;
             if (!fl_[cur].live)
@@ -385,6 +388,7 @@ class Mafia {
 
     int launch(Flight &f, std::string &err) {
         if (stack_.empty()) settle_undecided();   // nothing else left to count
+        const double t0 = now();
         f.wave.clear();
         int64_t outs = 0;
         while ((int)f.wave.size() < cap_ && !stack_.empty()) {
@@ -396,7 +400,9 @@ class Mafia {
         }
         f.live = !f.wave.empty();
         if (!f.live) return TFIM_OK;
-        return count_wave(f, err);
+        const int rc = count_wave(f, err);
+        st_.build_seconds += now() - t0;
+        return rc;
     }
 
     int64_t row_outputs(const MNode &n) const {

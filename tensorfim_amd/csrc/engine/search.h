@@ -47,7 +47,10 @@ struct SearchStats {
     int64_t leaf_subsumed = 0;    // leaves discarded by a found superset
     int64_t mfi_total = 0;        // MFIs of the survivor universe found
     double counted_bitops = 0;    // sum over rows of columns x transactions
-    double count_seconds = 0;     // host time inside count calls
+    double count_seconds = 0;     // host time inside count calls (submit + wait)
+    double submit_seconds = 0;    // of which: packing and issuing waves
+    double process_seconds = 0;   // host time turning wave results into decisions and new nodes
+    double build_seconds = 0;     // host time popping nodes into waves and issuing them (includes submit)
     double kernel_ms = 0;         // device time of count kernels (HIP events)
     int64_t lds_waves = 0;        // waves counted by the LDS-staged kernel
     double learn_seconds = 0;     // host time in the learned part

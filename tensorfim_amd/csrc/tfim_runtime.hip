@@ -370,19 +370,26 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_ou
     for (auto &g : sl.graphs)
         if (g.func == K.func) G = &g;
     if (!G) {
-        sl.graphs.emplace_back();
+        // build and instantiate; only a complete graph is kept
+        tfim_ctx::Slot::Graph g;
+        g.func = K.func;
+        hipError_t e = hipGraphCreate(&g.g, 0);
+        if (e == hipSuccess)
+            e = hipGraphAddMemcpyNode1D(&g.h2d, g.g, nullptr, 0, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
+                                        hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipGraphAddKernelNode(&g.fill, g.g, &g.h2d, 1, &zp);
+        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e0, g.g, &g.fill, 1, sl.ev0);
+        if (e == hipSuccess) e = hipGraphAddKernelNode(&g.kern, g.g, &g.e0, 1, &kp);
+        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e1, g.g, &g.kern, 1, sl.ev1);
+        if (e == hipSuccess)
+            e = hipGraphAddMemcpyNode1D(&g.d2h, g.g, &g.e1, 1, sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipGraphInstantiate(&g.x, g.g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (g.g) hipGraphDestroy(g.g);
+            return fail(c, TFIM_E_HIP, "wave graph construction failed: %s", hipGetErrorString(e));
+        }
+        sl.graphs.push_back(g);
         G = &sl.graphs.back();
-        G->func = K.func;
-        HIPCHK(c, hipGraphCreate(&G->g, 0));
-        HIPCHK(c, hipGraphAddMemcpyNode1D(&G->h2d, G->g, nullptr, 0, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
-                                          hipMemcpyHostToDevice));
-        HIPCHK(c, hipGraphAddKernelNode(&G->fill, G->g, &G->h2d, 1, &zp));
-        HIPCHK(c, hipGraphAddEventRecordNode(&G->e0, G->g, &G->fill, 1, sl.ev0));
-        HIPCHK(c, hipGraphAddKernelNode(&G->kern, G->g, &G->e0, 1, &kp));
-        HIPCHK(c, hipGraphAddEventRecordNode(&G->e1, G->g, &G->kern, 1, sl.ev1));
-        HIPCHK(c, hipGraphAddMemcpyNode1D(&G->d2h, G->g, &G->e1, 1, sl.h_out, sl.d_out, out_bytes,
-                                          hipMemcpyDeviceToHost));
-        HIPCHK(c, hipGraphInstantiate(&G->x, G->g, nullptr, nullptr, 0));
         c->wave_graphs++;
     } else {
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->h2d, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
@@ -436,8 +443,16 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
 
     // work of this wave in row-words: sum over rows of (prefix + columns) x pitch
     const double work = ((double)n_item_ids + (double)n_out) * (double)c->pitch;
-    // LDS slots: the distinct rows this wave touches
+    // LDS slots: the distinct rows this wave touches; slot_of is cleared
+    // again on every way out of this call
     std::vector<int32_t> slot_rows;
+    struct SlotReset {
+        tfim_ctx *c;
+        const std::vector<int32_t> &rows;
+        ~SlotReset() {
+            for (int32_t it : rows) c->slot_of[it] = -1;
+        }
+    } slot_reset{c, slot_rows};
     bool lds = c->wave_kernel != 1 && n_out <= kLdsMaxOut;
     if (lds) {
         if ((int32_t)c->slot_of.size() < c->n_items) c->slot_of.assign(c->n_items, -1);
