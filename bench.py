@@ -158,6 +158,11 @@ def bench_e2e(args):
         for k in ("hut_rows", "hut_success", "model_lookaheads", "mlp_train_steps", "graph_launches"):
             if k in st:
                 out[name][k] = int(st[k])
+        if out[name]["kernel_ms"] > 0:
+            # the batched count kernels against the AND + popcount VALU roof
+            tb = out[name]["counted_bitops"] / (out[name]["kernel_ms"] * 1e-3) / 1e12
+            out[name]["count_kernels_tbitops"] = tb
+            out[name]["count_kernels_valu_frac"] = tb / VALU_BITOP_PEAK_T
         log("  %s: %.2f s wall, %.3f s computation, %d rows" % (name, wall, out[name]["computation_s"],
                                                                 out[name]["rows"]))
     out["identical_mfi_sets"] = all(sets[n] == sets["plain"] for n, _, _ in runs)
