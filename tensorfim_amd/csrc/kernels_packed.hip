@@ -80,6 +80,24 @@ __device__ __forceinline__ uint32_t row_reduce8_packed(const uint32_t v[8]) {
     return r;
 }
 
+// The same for 4 columns: afterwards lanes with (lane & 7) == 0 hold the row
+// totals of columns 0 and 1 (lane & 8 clear) or 2 and 3 (lane & 8 set), low
+// half first.
+//   step 1: partner 15 - l (row mirror); lanes 0-7 keep columns 0-1, 8-15 keep 2-3
+//   step 2: partner 7 - (l & 7) (half mirror), both keep the same pair
+//   step 3: sum over the quad (xor 1, then xor 2)
+__device__ __forceinline__ uint32_t row_reduce4_packed(const uint32_t v[4]) {
+    const int lane = threadIdx.x & 63;
+    const bool hi = lane & 8;
+    const uint32_t p0 = v[0] | (v[1] << 16), p1 = v[2] | (v[3] << 16);
+    const uint32_t keep = hi ? p1 : p0, send = hi ? p0 : p1;
+    uint32_t r = keep + (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, 0x140, 0xF, 0xF, false);
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0x141, 0xF, 0xF, false);
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    r += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Plain per-node count.  grid.x = word chunks of 256*R uint4, grid.y = column
 // blocks of <= 64 columns.  Every block ANDs the k prefix rows over its chunk
@@ -255,7 +273,7 @@ void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &
 // tiles round-robin.  Per tile the prefix and column slot numbers are loaded
 // lane-parallel once (prefetched during the previous tile) and broadcast with
 // readlane, so the column loop issues only LDS reads and VALU work: prefix
-// AND into registers, 8 columns at a time without branches, a DPP reduction
+// AND into registers, 8 (then 4) columns at a time without branches, a DPP reduction
 // over each 16-lane row, and one LDS atomic per (row, column).  Partial
 // counts leave the workgroup once, as one global atomic per non-zero output.
 // Global traffic per wave is one pass over the slots' rows whatever the
@@ -313,12 +331,33 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
 #pragma unroll
                 for (int q = 0; q < RW; q++) m[q] = and4(m[q], row[q * 64 + lane]);
             }
-            for (int j = 0; j < nc; j += 8) {
-                // eight columns, branch-free: past the tile's end the last
-                // column is read again and its count dropped
+            // full groups of eight columns, then groups of four for the rest
+            // (branch-free: past the tile's end the last column is read
+            // again and its count dropped), so at most three columns of a
+            // tile are counted in vain
+            int j = 0;
+            for (; j + 8 <= nc; j += 8) {
                 uint32_t v[8];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
+                    const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, j + q) * C4;
+                    uint32_t a = 0;
+#pragma unroll
+                    for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
+                    v[q] = a;
+                }
+                const uint32_t r = row_reduce8_packed(v);
+                if ((lane & 3) == 0) {
+                    const int c0 = j + 2 * ((lane >> 2) & 3);
+                    const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
+                    if (lo) atomicAdd(&acc[tl.cb + c0], lo);
+                    if (hi) atomicAdd(&acc[tl.cb + c0 + 1], hi);
+                }
+            }
+            for (; j < nc; j += 4) {
+                uint32_t v[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
                     const int col = j + q < nc ? j + q : nc - 1;
                     const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, col) * C4;
                     uint32_t a = 0;
@@ -326,9 +365,9 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                     for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
                     v[q] = j + q < nc ? a : 0u;
                 }
-                const uint32_t r = row_reduce8_packed(v);
-                if ((lane & 3) == 0) {
-                    const int c0 = j + 2 * ((lane >> 2) & 3);
+                const uint32_t r = row_reduce4_packed(v);
+                if ((lane & 7) == 0) {
+                    const int c0 = j + ((lane & 8) ? 2 : 0);
                     const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
                     if (c0 < nc && lo) atomicAdd(&acc[tl.cb + c0], lo);
                     if (c0 + 1 < nc && hi) atomicAdd(&acc[tl.cb + c0 + 1], hi);
