@@ -137,7 +137,7 @@ int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
  * the LDS-staged kernel, rows, summed kernel milliseconds (HIP events).  In
  * TFIM_PATH_AUTO the library's wave tuner picks, for waves that fit both
  * kernels, the kernel and the columns per work tile (L2 kernel with 32; LDS
- * kernel with 16, 32 or 64, or wide staging with 32): each arm is timed
+ * kernel with 16, 32 or 64, narrow or wide staging): each arm is timed
  * three times, then the cheapest
  * per row-word of work is kept and another arm re-timed every 16 waves. */
 int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,

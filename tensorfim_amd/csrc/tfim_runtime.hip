@@ -25,10 +25,10 @@ namespace {
 // Online tuner arms for waves that both batched kernels accept:
 // (kernel 1 = L2-resident, 2 = LDS-staged; columns per work tile; LDS
 // staging wide = one workgroup per CU with up to 4 x 128 bits per lane).
-constexpr int kTuneArms = 5;
-constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2, 2};
-constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64, 32};
-constexpr bool kArmWide[kTuneArms] = {false, false, false, false, true};
+constexpr int kTuneArms = 7;
+constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2, 2, 2, 2};
+constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64, 16, 32, 64};
+constexpr bool kArmWide[kTuneArms] = {false, false, false, false, true, true, true};
 }  // namespace
 
 struct tfim_ctx {
