@@ -89,6 +89,13 @@ class LearnedPolicy : public Policy {
         for (size_t q = 0; q < ds.size(); q++) {
             Decision *d = ds[q];
             if (d->head.empty() || d->tail.size() < 2) continue;
+            // features only where they are used: every node during warm-up;
+            // afterwards the model's candidates, a sample of the rule's own
+            // lookaheads (their outcomes keep the labels balanced) and the
+            // ranker's nodes
+            bool need = !trained_ || (opt_.rank && d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail);
+            if (!need) need = d->lookahead ? rand01() < kRecordAccepted : d->tail.size() <= kMaxGateTail;
+            if (!need) continue;
             node_features(*d);
             node_row_[q] = (int64_t)(feats_.size() / NF) - 1;
             if (use_model && !d->lookahead && d->tail.size() <= kMaxGateTail) {
@@ -363,6 +370,7 @@ class LearnedPolicy : public Policy {
     static constexpr size_t kMaxGateTail = 64;     // the model gates tails of 2..64 items
     static constexpr float kGateP = 0.5f;          // model-added lookahead: P(HUT frequent) >= this
     static constexpr float kExplore = 0.02f;       // share of rule-declined lookaheads run anyway
+    static constexpr float kRecordAccepted = 0.25f;  // share of the rule's lookaheads kept as training records
 
     tfim_ctx *ctx_;
     const SearchOptions &opt_;
