@@ -141,9 +141,9 @@ class LearnedPolicy : public Policy {
             if (node_row_[q] < 0) continue;
             const bool rule = d->lookahead;
             if (!trained_) {
-                // warm-up: classical lookahead on bounded tails so that both
-                // outcomes get labelled
-                d->lookahead = rule || d->tail.size() <= kMaxGateTail;
+                // warm-up: the rule's lookaheads plus a sample of the
+                // declined bounded tails, so that both outcomes get labelled
+                d->lookahead = rule || (d->tail.size() <= kMaxGateTail && rand01() < kWarmSample);
             } else if (!rule && d->tail.size() <= kMaxGateTail) {
                 if (deferred) {
                     d->pending = true;   // settled by resolve() from the queued inference
@@ -151,9 +151,11 @@ class LearnedPolicy : public Policy {
                 }
                 // the model adds the lookaheads it expects to succeed to the
                 // rule's; a small exploration share keeps negative labels coming
-                const bool model = infer_row_[q] >= 0 && sigmoid(logits_[infer_row_[q] * 2 + 0]) >= kGateP;
-                d->lookahead = model || rand01() < kExplore;
+                const bool model = infer_row_[q] >= 0 && sigmoid(logits_[infer_row_[q] * 2 + 0]) >= gate_p_;
+                const bool explore = !model && rand01() < explore_p_;
+                d->lookahead = model || explore;
                 if (d->lookahead) model_gated_++;
+                gate_src_ = model ? kSrcModel : explore ? kSrcExplore : kSrcRule;
             }
             if (trained_ && item_row0_[q] >= 0 && n > 0 && !deferred) {
                 // items predicted to keep >= 90% of their child's tail go
@@ -164,7 +166,8 @@ class LearnedPolicy : public Policy {
                 });
             }
             // the gate record is labelled only if the lookahead runs
-            d->record = d->lookahead ? add_record(&feats_[node_row_[q] * NF], 0) : -1;
+            d->record = d->lookahead ? add_record(&feats_[node_row_[q] * NF], 0, gate_src_) : -1;
+            gate_src_ = kSrcRule;
             if (item_row0_[q] >= 0) {
                 d->item_records.resize(d->tail.size());
                 for (size_t i = 0; i < d->tail.size(); i++)
@@ -186,11 +189,12 @@ class LearnedPolicy : public Policy {
         }
         for (size_t q = 0; q < ds.size(); q++) {
             Decision *d = ds[q];
-            const bool model = ok && sigmoid(logits_[q * 2 + 0]) >= kGateP;
-            d->lookahead = model || rand01() < kExplore;
+            const bool model = ok && sigmoid(logits_[q * 2 + 0]) >= gate_p_;
+            const bool explore = !model && rand01() < explore_p_;
+            d->lookahead = model || explore;
             d->pending = false;
             if (d->lookahead) model_gated_++;
-            d->record = d->lookahead && ok ? add_record(&pend_feats_[q * NF], 0) : -1;
+            d->record = d->lookahead && ok ? add_record(&pend_feats_[q * NF], 0, model ? kSrcModel : kSrcExplore) : -1;
         }
         pend_n_ = 0;
     }
@@ -207,6 +211,9 @@ class LearnedPolicy : public Policy {
     void finish(SearchStats &st) override {
         st.mlp_train_steps = steps_;
         st.model_lookaheads = model_gated_;
+        st.model_lookahead_hits = model_success_;
+        st.gate_threshold = gate_p_;
+        st.explore_share = explore_p_;
         st.learn_seconds += extra_seconds_;   // set-up and training (decide() is timed by the engine)
         if (opt_.verbose)
             fprintf(stderr, "learned policy: set-up+training %.4f s (%lld steps), forward %.4f s in %lld calls (%lld rows)\n",
@@ -291,12 +298,15 @@ class LearnedPolicy : public Policy {
 
     // Records live in a ring of kRing slots; a label that arrives after its
     // slot was reused is dropped.
-    int32_t add_record(const float *f, int head) {
+    // src: who asked for the lookahead (kSrcModel / kSrcExplore outcomes
+    // calibrate the gate and the exploration share).
+    int32_t add_record(const float *f, int head, uint8_t src = kSrcRule) {
         (void)head;
         const int64_t id = next_id_++;
         const size_t slot = (size_t)(id % kRing);
         std::copy_n(f, NF, &rec_x_[slot * NF]);
         rec_id_[slot] = id;
+        rec_src_[slot] = src;
         rec_y_[slot * 2] = rec_y_[slot * 2 + 1] = 0.0f;
         rec_m_[slot * 2] = rec_m_[slot * 2 + 1] = 0.0f;
         return (int32_t)(id & 0x7FFFFFFF);
@@ -306,6 +316,7 @@ class LearnedPolicy : public Policy {
         if ((rec_id_[slot] & 0x7FFFFFFF) != id32) return;
         rec_y_[slot * 2 + head] = v;
         rec_m_[slot * 2 + head] = 1.0f;
+        if (head == 0 && rec_src_[slot] != kSrcRule) calibrate(rec_src_[slot], v > 0.5f);
         if (trace_) {
             fwrite(&rec_x_[slot * NF], 4, NF, trace_);
             const float yh[2] = {(float)head, v};
@@ -327,6 +338,33 @@ class LearnedPolicy : public Policy {
         n_labelled_++;
         if (n_labelled_ - last_train_ >= (last_train_ == 0 ? kFirstTrain : kTrainEvery)) train();
     }
+    // Self-calibration from the outcomes of the lookaheads the rule declined.
+    // Model-added: over each window of kGateWindow, a success share below
+    // kGateLow raises the threshold (those failures cost a row each and save
+    // nothing), one above kGateHigh lowers it again, never below kGateP.
+    // Exploration: where hardly any explored lookahead succeeds (fewer than
+    // kExploreMinHits in a window) the share halves, down to kExploreMin;
+    // where many do it doubles, up to kExplore.
+    void calibrate(uint8_t src, bool success) {
+        if (src == kSrcModel) {
+            gate_taken_++;
+            gate_success_ += success;
+            model_success_ += success;
+            if (gate_taken_ < kGateWindow) return;
+            const double prec = (double)gate_success_ / (double)gate_taken_;
+            if (prec < kGateLow) gate_p_ = std::min(kGateMax, gate_p_ + 0.1f);
+            else if (prec > kGateHigh) gate_p_ = std::max(kGateP, gate_p_ - 0.05f);
+            gate_taken_ = gate_success_ = 0;
+            return;
+        }
+        expl_taken_++;
+        expl_success_ += success;
+        if (expl_taken_ < kGateWindow) return;
+        if (expl_success_ < kExploreMinHits) explore_p_ = std::max(kExploreMin, explore_p_ * 0.5f);
+        else if (expl_success_ > kGateWindow / 4) explore_p_ = std::min(kExplore, explore_p_ * 2.0f);
+        expl_taken_ = expl_success_ = 0;
+    }
+
     void train() {
         if (!mlp_) return;
         const auto t0 = std::chrono::steady_clock::now();
@@ -368,8 +406,15 @@ class LearnedPolicy : public Policy {
     static constexpr int kEpochs = 4;
     static constexpr size_t kMaxRankTail = 128;   // longer tails keep the plain order
     static constexpr size_t kMaxGateTail = 64;     // the model gates tails of 2..64 items
-    static constexpr float kGateP = 0.5f;          // model-added lookahead: P(HUT frequent) >= this
-    static constexpr float kExplore = 0.02f;       // share of rule-declined lookaheads run anyway
+    static constexpr float kGateP = 0.5f;          // model-added lookahead: P(HUT frequent) >= this (start, floor)
+    static constexpr float kGateMax = 0.95f;       // ceiling of the self-calibrated threshold
+    static constexpr int64_t kGateWindow = 64;     // model-added outcomes per calibration step
+    static constexpr double kGateLow = 0.25, kGateHigh = 0.6;
+    static constexpr float kExplore = 0.02f;       // share of rule-declined lookaheads run anyway (start, cap)
+    static constexpr float kWarmSample = 0.25f;    // declined tails looked ahead before the first training
+    static constexpr float kExploreMin = 0.0025f;  // floor of the self-calibrated exploration share
+    static constexpr int64_t kExploreMinHits = 2;  // per window, below which exploration halves
+    static constexpr uint8_t kSrcRule = 0, kSrcModel = 1, kSrcExplore = 2;
     static constexpr float kRecordAccepted = 0.25f;  // share of the rule's lookaheads kept as training records
 
     tfim_ctx *ctx_;
@@ -385,13 +430,19 @@ class LearnedPolicy : public Policy {
     std::vector<int64_t> node_row_, item_row0_, infer_row_, item_infer0_;
     std::vector<float> infer_, pend_feats_;
     int64_t pend_n_ = 0;   // rows of the inference queued for pending decisions
-    int64_t model_gated_ = 0;
+    int64_t model_gated_ = 0, model_success_ = 0;
+    float gate_p_ = kGateP;
+    int64_t gate_taken_ = 0, gate_success_ = 0;
+    float explore_p_ = kExplore;
+    int64_t expl_taken_ = 0, expl_success_ = 0;
+    uint8_t gate_src_ = kSrcRule;
     double extra_seconds_ = 0, forward_seconds_ = 0;
     int64_t forward_calls_ = 0, forward_rows_ = 0;
     std::vector<float> feats_, logits_;
     std::vector<float> rec_x_ = std::vector<float>((size_t)kRing * NF);
     std::vector<float> rec_y_ = std::vector<float>((size_t)kRing * 2), rec_m_ = std::vector<float>((size_t)kRing * 2);
     std::vector<int64_t> rec_id_ = std::vector<int64_t>((size_t)kRing, -1);
+    std::vector<uint8_t> rec_src_ = std::vector<uint8_t>((size_t)kRing, 0);
     std::vector<float> bx_, by_, bm_, tx_, ty_, tm_;
     int64_t next_id_ = 0, n_labelled_ = 0, last_train_ = 0, steps_ = 0;
     int64_t n_head_[2] = {0, 0};

@@ -56,6 +56,9 @@ struct SearchStats {
     double learn_seconds = 0;     // host time in the learned part
     int64_t mlp_train_steps = 0;
     int64_t model_lookaheads = 0;  // lookaheads run because the model (not the rule) asked
+    int64_t model_lookahead_hits = 0;   // of those (exploration excluded), the ones whose HUT was frequent
+    double gate_threshold = 0;          // learned gate's final self-calibrated threshold
+    double explore_share = 0;           // and its final exploration share
 };
 
 // Runs the engine; returns MFIs (dense ids, insertion order) with frequencies.

@@ -4,8 +4,9 @@ The MAFIA-style engine without a model runs the classical head-union-tail
 lookahead at every node with a tail of two or more (``--gate always``).  With
 ``--learned`` the MLP is trained online on the GPU from the engine's own
 lookahead outcomes and decides which lookaheads to run (an analytic
-independence estimate proposes, the model adds, a 2% exploration share keeps
-labels coming).  Every decision is confirmed by exact counts, so the MFI set
+independence estimate proposes, the model adds, a small exploration share
+keeps labels coming; the gate threshold and the exploration share calibrate
+themselves from the outcomes of the lookaheads they add).  Every decision is confirmed by exact counts, so the MFI set
 must not change; what changes is the number of counted rows, mostly the
 lookaheads that fail.
 
