@@ -1,6 +1,7 @@
 """Benchmark: exact support counting on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--no-cpu] [--tuned] [--paths mfma,packed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--no-cpu] [--no-per-node] [--tuned]
+                    [--paths mfma,packed]
     python bench.py --copy-calibration MIB     (a plain device copy, for counter scaling)
 
 A step is one pass of the hot path over one batch: the isolated count
@@ -9,7 +10,8 @@ C[i][j] = popcount(A_i AND B_j) for 65,536 prefix bitmaps x
 resident in HBM before timing.  `value` is support-count Tbit-op/s of the
 int8-MFMA path (bit-ops = rows x items x transactions).  Also reported in the
 same run: the packed AND+v_bcnt path on the same data, the CPU baseline (the
-oracle's popcount on a 512-row sample, host threads stated), and unless
+oracle's popcount on a 512-row sample, host threads stated), the plain
+per-node count kernel's HBM fraction on two shapes (unless --no-per-node), and unless
 --no-e2e the end-to-end mining time of the dense replicated workload (49,046-row base x256
 = 12,555,776 transactions, support 0.85) for the plain per-node GPU engine
 (one and eight host workers) and the batched MAFIA engine with and without the
@@ -133,6 +135,39 @@ def bench_isolated(ctx, args):
     return res, bitops, alg_bytes, B_host, A_sample, n_bits
 
 
+def bench_per_node(ctx, args):
+    """The plain per-node path's kernel (tfim_count_node: prefix AND + count
+    against every survivor) on one node of 3 prefix items, timed with HIP
+    events over back-to-back launches.  Two shapes: the dense replicated
+    workload's (39 survivor bitmaps x 12,555,776 transactions, 61 MB, which
+    the 256 MB Infinity Cache can hold) and 512 bitmaps x 2^23 transactions
+    (512 MB, which it cannot).  Algorithmic bytes per node = (k + survivors)
+    bitmaps x N/8 + 4 bytes per survivor; the roof is HBM."""
+    out = {}
+    rng = np.random.default_rng(7)
+    for name, d, n_bits in (("dense_replicated_39x12555776", 39, 12555776), ("large_512x8388608", 512, 1 << 23)):
+        words = tf.pitch_words(n_bits)
+        rows = rng.integers(0, 2 ** 63, size=(d, words), dtype=np.int64).astype(np.uint64)
+        rows[:, n_bits // 64:] = 0          # zero pad past the last transaction (n_bits % 64 == 0 here)
+        ctx.load_bitmaps(rows, n_bits)
+        items = [0, 1, 2]
+        got = ctx.count_node(items)
+        m = rows[0] & rows[1] & rows[2]
+        cols = range(d) if d <= 64 else list(range(8)) + [d - 1]
+        ok = all(int(got[e]) == int(np.bitwise_count(m & rows[e]).sum()) for e in cols)
+        ms = ctx.time_count_node(items, warmup=5, reps=max(50, 20 * args.steps))
+        alg = (len(items) + d) * n_bits / 8 + 4 * d
+        gbs = alg / (ms * 1e-3) / 1e9
+        out[name] = {"kernel_ms": ms, "algorithmic_bytes": alg, "achieved_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS,
+                     "tbitops": d * n_bits / (ms * 1e-3) / 1e12, "exact_on_sample": ok}
+        log("per-node %s: %.4f ms  %.0f GB/s (%.0f%% of HBM)  exact=%s" % (name, ms, gbs, 100 * gbs / HBM_PEAK_GBS,
+                                                                          ok))
+        del rows
+        if not ok:
+            raise SystemExit("per-node counts differ from numpy's popcount (%s)" % name)
+    return out
+
+
 def run_engine(data, engine, extra, out):
     t0 = time.perf_counter()
     path, st = tf.mine(data, 0.85, engine=engine, out=out, extra=["--replicate", "256"] + extra, timeout=1800)
@@ -224,6 +259,7 @@ def main():
     ap.add_argument("--paths", default="mfma,packed")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-node", action="store_true")
     ap.add_argument("--tuned", action="store_true", help="also time the tuner's tile/split choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
@@ -239,6 +275,7 @@ def main():
     info = ctx.device_info()
     log("device:", info)
     res, bitops, alg_bytes, B_host, A_sample, n_bits = bench_isolated(ctx, args)
+    per_node = None if args.no_per_node else bench_per_node(ctx, args)
     ctx.close()
     head = res.get("mfma") or res[args.paths[0]]
     head_path = "mfma" if "mfma" in res else args.paths[0]
@@ -288,6 +325,8 @@ def main():
         "paths": res,
         "device": info,
     }
+    if per_node is not None:
+        line["per_node_kernel"] = per_node
     if e2e is not None:
         line["e2e_dense_replicated"] = e2e
     print(json.dumps(line), flush=True)

@@ -91,6 +91,14 @@ int tfim_item_supports(tfim_ctx *ctx, int32_t *out);
  * for every survivor e < n_cols; synchronous, `out` is host memory. */
 int tfim_count_node(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t *out);
 
+/* Times `reps` back-to-back launches of tfim_count_node's kernel for one node
+ * (after `warmup` untimed ones) with HIP events on the ctx stream; avg_ms =
+ * mean per launch.  Measurement only: no counts are returned.  Stands beside
+ * the reference's per-node timing of getResFrequencyCuda (kernel_bitmap.cu:
+ * 451-536), which it times without the copies and the synchronisation. */
+int tfim_time_count_node(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t warmup, int32_t reps,
+                         double *avg_ms);
+
 /* Batched path; replaces batchCountBMMA (kernel_bmma.cu:772-872), including
  * its mask build (:619-634), count (:636-758) and split-k reduce (:761-768),
  * as one fused launch.  Row r has prefix items item_ids[item_off[r] ..
@@ -115,8 +123,12 @@ int tfim_count_nodes_submit(tfim_ctx *ctx, const int32_t *item_ids, const int32_
                             int32_t path, int64_t *ticket);
 int tfim_count_nodes_wait(tfim_ctx *ctx, int64_t ticket, int32_t *out);
 
-/* Tuning knobs of the batched path (ablation and autotuning; the reference's
- * equivalents are environment variables, kernel_bmma.cu:351-368):
+/* Tuning knobs (ablation and autotuning; the reference's equivalents are
+ * environment variables, kernel_bmma.cu:351-368):
+ *   "node_shape"   launch shape of the per-node kernel (tfim_count_node):
+ *                  0 auto, else R * 100 + cpb with R in {1, 2, 4} 128-bit
+ *                  words per lane and cpb in {8, 16, 32, 64} columns per
+ *                  block
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel (two
  *                  workgroups per CU), 3 LDS-staged with wide staging (one
  *                  workgroup per CU, up to 512 bits per lane per column)

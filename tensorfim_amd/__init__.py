@@ -77,6 +77,8 @@ _SIGS = {
                                                   ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                   _p(ctypes.c_double)]),
+    "tfim_time_count_node": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, _p(ctypes.c_double)]),
     "tfim_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "tfim_mlp_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, _p(ctypes.c_void_p)]),
     "tfim_mlp_destroy": (None, [ctypes.c_void_p]),
@@ -203,6 +205,14 @@ class Context:
         out = np.empty(self.n_cols, dtype=np.int32)
         self._check(self._lib.tfim_count_node(self._h, _ptr(it), len(it), _ptr(out)))
         return out
+
+    def time_count_node(self, items, warmup=3, reps=100):
+        """Mean kernel ms of the plain per-node count for one node (HIP events)."""
+        it = np.ascontiguousarray(items, dtype=np.int32)
+        ms = ctypes.c_double()
+        self._check(self._lib.tfim_time_count_node(self._h, _ptr(it), len(it), int(warmup), int(reps),
+                                                   ctypes.byref(ms)))
+        return ms.value
 
     def count_nodes(self, prefixes, tails=None, path="auto"):
         """Batched path.  prefixes: list of item lists; tails: None (all

@@ -100,17 +100,18 @@ __device__ __forceinline__ uint32_t row_reduce4_packed(const uint32_t v[4]) {
 
 // ---------------------------------------------------------------------------
 // Plain per-node count.  grid.x = word chunks of 256*R uint4, grid.y = column
-// blocks of <= 64 columns.  Every block ANDs the k prefix rows over its chunk
-// once (registers), then streams its columns 8 at a time.
-template <int R>
+// blocks of cpb <= 64 columns.  Every block ANDs the k prefix rows over its
+// chunk once (registers), then streams its columns 8*G at a time: the R*8*G
+// loads of a batch are independent, so they are all in flight together.
+template <int R, int G>
 __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm, int64_t pitch4,
-                                                    const int32_t *__restrict__ items, int k,
-                                                    int n_cols, int32_t *__restrict__ out) {
+                                                    const int32_t *__restrict__ items, int k, int n_cols,
+                                                    int cpb, int32_t *__restrict__ out) {
     __shared__ int red[4][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t base = (int64_t)blockIdx.x * (256 * R) + tid;
-    const int c0 = blockIdx.y * 64;
-    const int c1 = min(n_cols, c0 + 64);
+    const int c0 = blockIdx.y * cpb;
+    const int c1 = min(n_cols, c0 + cpb);
     uint4 m[R];
     {
         const uint4 *row = bm + (int64_t)items[0] * pitch4;
@@ -128,10 +129,10 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
             if (idx < pitch4) m[r] = and4(m[r], row[idx]);
         }
     }
-    for (int c = c0; c < c1; c += 8) {
-        int v[8];
+    for (int c = c0; c < c1; c += 8 * G) {
+        int v[8 * G];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
+        for (int j = 0; j < 8 * G; j++) {
             uint32_t s = 0;
             if (c + j < c1) {
                 const uint4 *row = bm + (int64_t)(c + j) * pitch4;
@@ -143,10 +144,13 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
             }
             v[j] = (int)s;
         }
-        const int y = wave_reduce_scatter8(v);
-        if ((lane & 7) == 0) {
-            const int cj = (lane >> 3) & 7;
-            if (c + cj < c1) red[wave][c - c0 + cj] = y;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int y = wave_reduce_scatter8(&v[8 * g]);
+            if ((lane & 7) == 0) {
+                const int col = c + 8 * g + ((lane >> 3) & 7);
+                if (col < c1) red[wave][col - c0] = y;
+            }
         }
     }
     __syncthreads();
@@ -156,19 +160,38 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
     }
 }
 
+// shape = R * 100 + cpb (R in {1, 2, 4}, cpb in {8, 16, 32, 64}).  0 picks
+// from measurements on MI355X: (2, 32) once that gives >= 8 blocks per CU,
+// else (1, 16); short, narrow nodes are latency-bound, and more blocks with
+// fewer columns each keep more loads in flight.
 void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
-                       int32_t n_cols, int32_t *out, int n_cu, hipStream_t s) {
+                       int32_t n_cols, int32_t *out, int n_cu, int shape, hipStream_t s) {
     const int64_t pitch4 = pitch_words / 2;
-    const int ycols = (n_cols + 63) / 64;
-    auto chunks = [&](int R) { return (pitch4 + 256 * R - 1) / (256 * R); };
-    const int64_t want = 4LL * n_cu;
+    auto blocks = [&](int R, int cpb) { return ((pitch4 + 256 * R - 1) / (256 * R)) * ((n_cols + cpb - 1) / cpb); };
+    int R = shape / 100, cpb = shape % 100;
+    if (shape == 0) {
+        R = 1;
+        cpb = 16;
+        if (blocks(2, 32) >= 8LL * n_cu) {
+            R = 2;
+            cpb = 32;
+        }
+    }
+    const dim3 grid((unsigned)((pitch4 + 256 * R - 1) / (256 * R)), (unsigned)((n_cols + cpb - 1) / cpb));
     const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
-    if (chunks(4) * ycols >= want) {
-        hipLaunchKernelGGL(k_count_node<4>, dim3((unsigned)chunks(4), ycols), dim3(256), 0, s, b4, pitch4, items, k, n_cols, out);
-    } else if (chunks(2) * ycols >= want) {
-        hipLaunchKernelGGL(k_count_node<2>, dim3((unsigned)chunks(2), ycols), dim3(256), 0, s, b4, pitch4, items, k, n_cols, out);
+    if (cpb == 8) {
+        if (R == 4)
+            hipLaunchKernelGGL((k_count_node<4, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+        else if (R == 2)
+            hipLaunchKernelGGL((k_count_node<2, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+        else
+            hipLaunchKernelGGL((k_count_node<1, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+    } else if (R == 4) {
+        hipLaunchKernelGGL((k_count_node<4, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+    } else if (R == 2) {
+        hipLaunchKernelGGL((k_count_node<2, 2>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
     } else {
-        hipLaunchKernelGGL(k_count_node<1>, dim3((unsigned)chunks(1), ycols), dim3(256), 0, s, b4, pitch4, items, k, n_cols, out);
+        hipLaunchKernelGGL((k_count_node<1, 2>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
     }
 }
 

@@ -52,7 +52,7 @@ struct KernelLaunch {
 // Plain per-node count: out[c] += popcount(AND rows[items] & rows[c]) for
 // c < n_cols; `out` must be zeroed.  items are device ids.
 void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
-                       int32_t n_cols, int32_t *out, int n_cu, hipStream_t s);
+                       int32_t n_cols, int32_t *out, int n_cu, int shape, hipStream_t s);
 
 // One tile of a wave: a row's prefix, and a run of its columns.
 struct WaveTile {

@@ -88,6 +88,7 @@ struct tfim_ctx {
     int64_t tune_calls = 0;
     int tile_cols = 0;      // 0: chosen by the tuner (32 when it does not run)
     int mfma_tile = 0;      // 0 auto (= 256), 128 or 256
+    int node_shape = 0;     // per-node kernel: 0 auto, else R * 100 + columns per block
     int use_graphs = 1;     // waves replayed as instantiated graphs
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
     std::map<std::string, std::pair<int, int32_t>> popmat_tuned;   // shape -> (tile, split_k)
@@ -313,24 +314,57 @@ int tfim_item_supports(tfim_ctx *c, int32_t *out) {
     return TFIM_OK;
 }
 
+namespace {
+// Checks a node's items and stages them on the device for k_count_node.
+int node_stage(tfim_ctx *c, const char *what, const int32_t *items, int32_t k) {
+    if (!c->d_bm) return fail(c, TFIM_E_STATE, "%s: no bitmaps loaded", what);
+    for (int32_t i = 0; i < k; i++)
+        if (items[i] < 0 || items[i] >= c->n_items)
+            return fail(c, TFIM_E_INVALID, "%s: item %d out of range [0,%d)", what, items[i], c->n_items);
+    if (int r = set_device(c)) return r;
+    if (int r = ensure_pair(c, &c->d_node_items, &c->h_node_items, &c->node_items_cap, k)) return r;
+    if (int r = ensure_pair(c, &c->d_node_out, &c->h_node_out, &c->node_out_cap, std::max(1, c->n_cols))) return r;
+    memcpy(c->h_node_items, items, (size_t)k * 4);
+    HIPCHK(c, hipMemcpyAsync(c->d_node_items, c->h_node_items, (size_t)k * 4, hipMemcpyHostToDevice, c->stream));
+    return TFIM_OK;
+}
+}  // namespace
+
 int tfim_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t *out) {
     if (!c || !items || !out || k < 1) return fail(c, TFIM_E_INVALID, "count_node: null argument or k < 1");
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_node: no bitmaps loaded");
-    for (int32_t i = 0; i < k; i++)
-        if (items[i] < 0 || items[i] >= c->n_items)
-            return fail(c, TFIM_E_INVALID, "count_node: item %d out of range [0,%d)", items[i], c->n_items);
     if (c->n_cols == 0) return TFIM_OK;
-    if (int r = set_device(c)) return r;
-    if (int r = ensure_pair(c, &c->d_node_items, &c->h_node_items, &c->node_items_cap, k)) return r;
-    if (int r = ensure_pair(c, &c->d_node_out, &c->h_node_out, &c->node_out_cap, c->n_cols)) return r;
-    memcpy(c->h_node_items, items, (size_t)k * 4);
-    HIPCHK(c, hipMemcpyAsync(c->d_node_items, c->h_node_items, (size_t)k * 4, hipMemcpyHostToDevice, c->stream));
+    if (int r = node_stage(c, "count_node", items, k)) return r;
     launch_zero_i32(c->d_node_out, c->n_cols, c->stream);
-    launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->stream);
+    launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->node_shape, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->h_node_out, c->d_node_out, (size_t)c->n_cols * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     memcpy(out, c->h_node_out, (size_t)c->n_cols * 4);
+    return TFIM_OK;
+}
+
+int tfim_time_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t warmup, int32_t reps,
+                         double *avg_ms) {
+    if (!c || !items || k < 1 || reps < 1 || warmup < 0 || !avg_ms)
+        return fail(c, TFIM_E_INVALID, "time_count_node: null argument, k < 1 or reps < 1");
+    if (!c->d_bm) return fail(c, TFIM_E_STATE, "time_count_node: no bitmaps loaded");
+    if (c->n_cols == 0) return fail(c, TFIM_E_STATE, "time_count_node: no survivors to count");
+    if (int r = node_stage(c, "time_count_node", items, k)) return r;
+    // the launches accumulate into the library's node buffer, whose contents
+    // are not returned
+    for (int i = 0; i < warmup; i++)
+        launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->node_shape, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < reps; i++)
+        launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->node_shape, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *avg_ms = (double)ms / reps;
     return TFIM_OK;
 }
 
@@ -644,6 +678,12 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
     const std::string k(key);
     if (k == "wave_kernel" && value >= 0 && value <= 3) {
         c->wave_kernel = (int)value;
+        return TFIM_OK;
+    }
+    if (k == "node_shape" && (value == 0 || ((value / 100 == 1 || value / 100 == 2 || value / 100 == 4) &&
+                                             (value % 100 == 8 || value % 100 == 16 || value % 100 == 32 ||
+                                              value % 100 == 64)))) {
+        c->node_shape = (int)value;
         return TFIM_OK;
     }
     if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256)) {

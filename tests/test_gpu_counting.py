@@ -43,6 +43,25 @@ def test_count_node_matches_oracle(ctx, n_bits, n_items):
         np.testing.assert_array_equal(got, exp, err_msg="k=%d prefix=%s" % (k, prefix))
 
 
+@pytest.mark.parametrize("shape", [108, 116, 132, 164, 208, 216, 232, 264, 408, 416, 432, 464])
+def test_count_node_launch_shapes(ctx, shape):
+    """Every launch shape of the per-node kernel (words per lane x columns per
+    block) gives the oracle's counts, on ragged sizes."""
+    rng = np.random.default_rng(shape)
+    n_bits, n_items = 300_017, 139
+    rows = rand_rows(rng, n_items, n_bits, 0.5)
+    ctx.load_bitmaps(rows, n_bits, n_items - 2)
+    ctx.config("node_shape", shape)
+    try:
+        for k in (1, 4):
+            prefix = rng.choice(n_items, size=k, replace=False)
+            got = ctx.count_node(prefix)
+            np.testing.assert_array_equal(got, oracle_lib.node_counts(rows, prefix, n_items - 2),
+                                          err_msg="shape=%d k=%d" % (shape, k))
+    finally:
+        ctx.config("node_shape", 0)
+
+
 def test_item_supports(ctx):
     rng = np.random.default_rng(5)
     rows = rand_rows(rng, 37, 100_001, 0.2)
