@@ -17,8 +17,17 @@
 
 namespace tfim {
 
-__device__ __forceinline__ uint32_t popc_and4(const uint4 a, const uint4 b) {
-    return __popc(a.x & b.x) + __popc(a.y & b.y) + __popc(a.z & b.z) + __popc(a.w & b.w);
+// acc + popcount(a AND b) as a chain of v_bcnt_u32_b32 with accumulate (one
+// AND and one BCNT per dword): the empty asm statements keep the compiler
+// from re-associating the sum into separate adds.
+__device__ __forceinline__ uint32_t popc_and4_acc(const uint4 a, const uint4 b, uint32_t acc) {
+    acc = __popc(a.x & b.x) + acc;
+    asm("" : "+v"(acc));
+    acc = __popc(a.y & b.y) + acc;
+    asm("" : "+v"(acc));
+    acc = __popc(a.z & b.z) + acc;
+    asm("" : "+v"(acc));
+    return __popc(a.w & b.w) + acc;
 }
 
 __device__ __forceinline__ uint4 and4(const uint4 a, const uint4 b) {
@@ -139,7 +148,7 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const int64_t idx = base + r * 256;
-                    if (idx < pitch4) s += popc_and4(m[r], row[idx]);
+                    if (idx < pitch4) s = popc_and4_acc(m[r], row[idx], s);
                 }
             }
             v[j] = (int)s;
@@ -249,7 +258,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint4 *__restrict__ bm
 #pragma unroll
                     for (int qq = 0; qq < RW; qq++) {
                         const int64_t idx = base + qq * 64;
-                        if (idx < pitch4) acc += popc_and4(m[qq], row[idx]);
+                        if (idx < pitch4) acc = popc_and4_acc(m[qq], row[idx], acc);
                     }
                 }
                 v8[q] = (int)acc;
@@ -366,7 +375,7 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                     const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, j + q) * C4;
                     uint32_t a = 0;
 #pragma unroll
-                    for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
+                    for (int qq = 0; qq < RW; qq++) a = popc_and4_acc(m[qq], row[qq * 64 + lane], a);
                     v[q] = a;
                 }
                 const uint32_t r = row_reduce8_packed(v);
@@ -385,7 +394,7 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                     const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, col) * C4;
                     uint32_t a = 0;
 #pragma unroll
-                    for (int qq = 0; qq < RW; qq++) a += popc_and4(m[qq], row[qq * 64 + lane]);
+                    for (int qq = 0; qq < RW; qq++) a = popc_and4_acc(m[qq], row[qq * 64 + lane], a);
                     v[q] = j + q < nc ? a : 0u;
                 }
                 const uint32_t r = row_reduce4_packed(v);
