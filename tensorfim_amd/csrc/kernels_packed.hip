@@ -27,7 +27,18 @@ __device__ __forceinline__ uint32_t popc_and4_acc(const uint4 a, const uint4 b, 
     asm("" : "+v"(acc));
     acc = __popc(a.z & b.z) + acc;
     asm("" : "+v"(acc));
-    return __popc(a.w & b.w) + acc;
+    acc = __popc(a.w & b.w) + acc;
+    asm("" : "+v"(acc));
+    return acc;
+}
+
+// The same for one 64-bit word pair: two ANDs, two chained BCNTs.
+__device__ __forceinline__ uint32_t popc_and64_acc(const uint64_t a, const uint64_t b, uint32_t acc) {
+    acc = __popc((uint32_t)a & (uint32_t)b) + acc;
+    asm("" : "+v"(acc));
+    acc = __popc((uint32_t)(a >> 32) & (uint32_t)(b >> 32)) + acc;
+    asm("" : "+v"(acc));
+    return acc;
 }
 
 __device__ __forceinline__ uint4 and4(const uint4 a, const uint4 b) {
@@ -537,7 +548,7 @@ __global__ __launch_bounds__(256) void k_popmat_packed(const uint64_t *__restric
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] += __popcll(a[i] & bb[j]);
+                for (int j = 0; j < 4; j++) acc[i][j] = popc_and64_acc(a[i], bb[j], acc[i][j]);
         }
         __syncthreads();
     }
