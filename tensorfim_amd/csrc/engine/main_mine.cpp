@@ -2,7 +2,7 @@
 //
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
-//             [--workers N] [--learned] [--rank] [--gate indep|always]
+//             [--workers N] [--host-threads N] [--learned] [--rank] [--gate indep|always]
 //             [--wave-kernel 0|1|2|3] [--graphs 0|1] [--parse-threads N]
 //             [--prep gpu|host] [--model PATH] [--trace PATH] [--verbose]
 //
@@ -51,6 +51,7 @@ int main(int argc, char **argv) {
         else if (a == "--learned") opt.learned = true;
         else if (a == "--gate") opt.gate = next();
         else if (a == "--workers") opt.workers = atoi(next().c_str());
+        else if (a == "--host-threads") opt.host_threads = atoi(next().c_str());
         else if (a == "--rank") opt.rank = true;
         else if (a == "--wave-kernel") wave_kernel = atoi(next().c_str());
         else if (a == "--graphs") graphs = atoi(next().c_str());

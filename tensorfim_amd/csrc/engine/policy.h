@@ -30,6 +30,8 @@ struct Decision {
     int32_t record = -1;            // out: gate record id for observe()
     std::vector<int32_t> item_records;  // out: ranker record id per tail index
     std::vector<uint64_t> known;    // survivors e with sup(head + e) < thr (bitset), engine-owned
+    int8_t leaf_sub = -1;           // engine: last child's leaf subsumed (1) or not (0) by the
+    int64_t leaf_n = 0;             //   MFIs found before leaf_n; -1 = not yet checked
 };
 
 class Policy {

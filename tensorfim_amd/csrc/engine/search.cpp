@@ -1,7 +1,9 @@
 #include "search.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <functional>
 #include <condition_variable>
 #include <mutex>
 #include <cstdio>
@@ -289,15 +291,23 @@ struct MNode {
 // non-zero, scanned word by word with an early exit.
 class MfiIndex {
    public:
-    explicit MfiIndex(int32_t n) : bits_((size_t)std::max(1, n)) {}
-    bool subsumed(const std::vector<int32_t> &x) const {
-        if (x.empty()) return n_ > 0;
-        for (size_t w = 0; w < words_; w++) {
-            uint64_t acc = ~0ull;
-            for (int32_t i : x) {
-                acc &= bits_[i][w];
-                if (!acc) break;
-            }
+    explicit MfiIndex(int32_t n) : bits_((size_t)std::max(1, n)), cnt_((size_t)std::max(1, n), 0) {}
+    bool subsumed(const std::vector<int32_t> &x) const { return subsumed_from(x, 0); }
+    // Only the MFIs numbered >= n0 (and those sharing their first word).
+    bool subsumed_from(const std::vector<int32_t> &x, int64_t n0) const {
+        if (x.empty()) return n_ > n0;
+        // rarest items first, so the AND of a word empties sooner
+        int32_t ord[64] = {};
+        const size_t k = std::min<size_t>(x.size(), 64);
+        std::copy_n(x.begin(), k, ord);
+        std::sort(ord, ord + k, [&](int32_t a, int32_t b) { return cnt_[a] < cnt_[b]; });
+        if (cnt_[ord[0]] == 0) return false;
+        const uint64_t *p0 = bits_[ord[0]].data();
+        const uint64_t *p1 = bits_[ord[k > 1 ? 1 : 0]].data();
+        for (size_t w = (size_t)(n0 >> 6); w < words_; w++) {
+            uint64_t acc = p0[w] & p1[w];
+            for (size_t j = 2; j < k && acc; j++) acc &= bits_[ord[j]][w];
+            for (size_t j = k; j < x.size() && acc; j++) acc &= bits_[x[j]][w];
             if (acc) return true;
         }
         return false;
@@ -307,14 +317,90 @@ class MfiIndex {
             words_++;
             for (auto &b : bits_) b.push_back(0ull);
         }
-        for (int32_t i : x) bits_[i][n_ >> 6] |= 1ull << (n_ & 63);
+        for (int32_t i : x) {
+            bits_[i][n_ >> 6] |= 1ull << (n_ & 63);
+            cnt_[i]++;
+        }
         n_++;
     }
+    int64_t size() const { return n_; }
 
    private:
     std::vector<std::vector<uint64_t>> bits_;
+    std::vector<int64_t> cnt_;   // MFIs containing each item
     size_t words_ = 0;
     int64_t n_ = 0;
+};
+
+// Fork-join helper for the engine's independent per-row work: run(n, f)
+// calls f(0..n-1) on the calling thread and the pool's threads and returns
+// when all calls are done.  Between runs the threads spin on the job counter
+// for a short while (a run per wave arrives every few hundred microseconds)
+// before they sleep.
+class HostPool {
+   public:
+    explicit HostPool(int threads) {
+        for (int i = 1; i < threads; i++) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_.store(true);
+            gen_.fetch_add(1);
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    template <class F>
+    void run(int64_t n, F &&f) {
+        if (th_.empty() || n < 2 * kGrain) {
+            for (int64_t i = 0; i < n; i++) f(i);
+            return;
+        }
+        fn_ = std::function<void(int64_t)>(std::forward<F>(f));
+        n_ = n;
+        next_.store(0);
+        pending_.store((int)th_.size());
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            gen_.fetch_add(1);
+        }
+        cv_.notify_all();
+        work();
+        while (pending_.load() != 0) std::this_thread::yield();
+    }
+
+   private:
+    static constexpr int64_t kGrain = 8;
+    void work() {
+        for (int64_t i; (i = next_.fetch_add(kGrain)) < n_;)
+            for (int64_t j = i, e = std::min(n_, i + kGrain); j < e; j++) fn_(j);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        while (true) {
+            const auto t0 = std::chrono::steady_clock::now();
+            while (gen_.load() == seen && std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(500))
+                std::this_thread::yield();
+            if (gen_.load() == seen) {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return gen_.load() != seen; });
+            }
+            seen = gen_.load();
+            if (stop_.load()) return;
+            work();
+            pending_.fetch_sub(1);
+        }
+    }
+    std::vector<std::thread> th_;
+    std::function<void(int64_t)> fn_;
+    int64_t n_ = 0;
+    std::atomic<int64_t> next_{0};
+    std::atomic<int> pending_{0};
+    std::atomic<uint64_t> gen_{0};
+    std::atomic<bool> stop_{false};
+    std::mutex mu_;
+    std::condition_variable cv_;
 };
 
 struct VecHash {
@@ -328,7 +414,9 @@ struct VecHash {
 class Mafia {
    public:
     Mafia(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, Policy &pol, SearchStats &st)
-        : ctx_(ctx), db_(db), opt_(opt), pol_(pol), st_(st), index_(db.n_surv) {}
+        : ctx_(ctx), db_(db), opt_(opt), pol_(pol), st_(st), index_(db.n_surv),
+          pool_(opt.host_threads > 0 ? opt.host_threads
+                                     : (int)std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency()))) {}
 
     int run(std::vector<Mfi> &out, std::string &err) {
         const int32_t F = db_.n_surv;
@@ -475,6 +563,22 @@ class Mafia {
         }
     }
 
+    // Subsumption of the leaf child (the last one, with no tail) of each
+    // decision that will stack children, in parallel; push_children uses the
+    // answers (and checks any MFI found after them).
+    void check_leaves(std::vector<Decision *> &ds) {
+        const int64_t n_now = index_.size();
+        pool_.run((int64_t)ds.size(), [&](int64_t i) {
+            Decision &d = *ds[i];
+            d.leaf_sub = -1;
+            if (d.order.empty() || (d.lookahead && d.tail.size() >= 2)) return;   // a HUT row goes first
+            std::vector<int32_t> leaf = d.head;
+            leaf.push_back(d.tail[d.order.back()]);
+            d.leaf_sub = index_.subsumed(leaf) ? 1 : 0;
+            d.leaf_n = n_now;
+        });
+    }
+
     void push_children(const Decision &d) {
         // child i: head + t_i with tail t_{i+1..}; pushed last-first so the
         // first child (largest tail) is counted first
@@ -491,7 +595,9 @@ class Mafia {
             for (size_t r = q + 1; r < k; r++) c.tail.push_back(d.tail[d.order[r]]);
             if (c.tail.empty()) {
                 // no candidates left: the child is a leaf
-                if (index_.subsumed(c.head)) { st_.leaf_subsumed++; continue; }
+                const bool sub = d.leaf_sub >= 0 ? d.leaf_sub == 1 || index_.subsumed_from(c.head, d.leaf_n)
+                                                 : index_.subsumed(c.head);
+                if (sub) { st_.leaf_subsumed++; continue; }
                 c.kind = MNode::LEAF;
             }
             stack_.push_back(std::move(c));
@@ -504,6 +610,26 @@ class Mafia {
         WaveBatch &wb_ = f.wb;
         std::vector<Decision> decisions;
         decisions.reserve(wave_.size());
+        // Subsumption checks of the EXPAND rows, in parallel against the MFIs
+        // found so far.  An EXPAND row checks head + every frequent tail item
+        // (the leaf head after PEP when no other item stays frequent, else
+        // its HUT); the loop below adds a check against the MFIs that rows
+        // before it emit, so decisions equal a row-by-row pass.
+        const int64_t n_start = index_.size();
+        pre_sub_.assign(wave_.size(), 0);
+        pool_.run((int64_t)wave_.size(), [&](int64_t r) {
+            const MNode &n = wave_[r];
+            if (n.kind != MNode::EXPAND) return;
+            const int32_t *res = wb_.result((int)r);
+            const int nc = wb_.ncols((int)r);
+            std::vector<int32_t> q = n.head;
+            for (int j = 0; j < nc; j++)
+                if (frequent(db_, res[j])) q.push_back(n.tail[j]);
+            pre_sub_[r] = index_.subsumed(q);
+        });
+        auto subsumed_row = [&](int r, const std::vector<int32_t> &x) {
+            return pre_sub_[r] || index_.subsumed_from(x, n_start);
+        };
         for (int r = 0; r < (int)wave_.size(); r++) {
             MNode &n = wave_[r];
             const int32_t *res = wb_.result(r);
@@ -533,7 +659,7 @@ class Mafia {
                     }
                 }
                 if (d.tail.empty()) {
-                    if (index_.subsumed(d.head)) {
+                    if (subsumed_row(r, d.head)) {
                         st_.leaf_subsumed++;
                     } else {
                         MNode leaf;
@@ -547,7 +673,7 @@ class Mafia {
                 }
                 std::vector<int32_t> hut = d.head;
                 hut.insert(hut.end(), d.tail.begin(), d.tail.end());
-                if (index_.subsumed(hut)) {
+                if (subsumed_row(r, hut)) {
                     st_.hutmfi_pruned++;
                     continue;
                 }
@@ -609,6 +735,13 @@ class Mafia {
         // push in reverse so the first decision's work is on top
         for (auto &n : pending_) stack_.push_back(std::move(n));
         pending_.clear();
+        {
+            std::vector<Decision *> leaves;
+            for (auto &d : hut_children_) leaves.push_back(&d);
+            for (auto &d : decisions)
+                if (!d.pending) leaves.push_back(&d);
+            check_leaves(leaves);
+        }
         for (auto &d : hut_children_) push_children(d);
         hut_children_.clear();
         for (size_t q = decisions.size(); q-- > 0;) {
@@ -650,6 +783,7 @@ class Mafia {
         const double t0 = now();
         pol_.resolve(ptrs);
         st_.learn_seconds += now() - t0;
+        check_leaves(ptrs);
         for (auto &d : undecided_) apply(d);
         undecided_.clear();
     }
@@ -660,6 +794,8 @@ class Mafia {
     Policy &pol_;
     SearchStats &st_;
     MfiIndex index_;
+    HostPool pool_;
+    std::vector<uint8_t> pre_sub_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;

@@ -28,6 +28,7 @@ struct SearchOptions {
     std::string engine = "mafia";
     int32_t wave_cap = 0;         // rows per wave; 0 = automatic
     int32_t workers = 1;          // plain engine: host workers (one stream each)
+    int32_t host_threads = 0;     // mafia: threads for the subsumption checks (0 = up to 8)
     bool learned = false;         // mafia: use the MLP ranker/gate
     std::string gate = "indep";   // mafia plain lookahead rule: indep | always
     bool rank = false;            // mafia learned: reorder tails with the ranker head
