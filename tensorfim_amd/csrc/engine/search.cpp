@@ -552,7 +552,13 @@ class Mafia {
 
     void emit(std::vector<int32_t> items, int32_t sup, std::vector<Mfi> &out) {
         std::sort(items.begin(), items.end(), std::greater<int32_t>());   // insertion order: root first
-        if (!seen_.insert(items).second) return;
+        if (db_.n_surv <= 64) {   // the set itself as a 64-bit key
+            uint64_t key = 0;
+            for (int32_t i : items) key |= 1ull << i;
+            if (!seen64_.insert(key).second) return;
+        } else if (!seen_.insert(items).second) {
+            return;
+        }
         index_.add(items);
         st_.mfi_total++;
         if (db_.is_root[items[0]]) {
@@ -797,6 +803,7 @@ class Mafia {
     HostPool pool_;
     std::vector<uint8_t> pre_sub_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
+    std::unordered_set<uint64_t> seen64_;
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;
     std::vector<uint8_t> mark_;
