@@ -403,6 +403,59 @@ class HostPool {
     std::condition_variable cv_;
 };
 
+// One persistent helper thread running one job at a time: post(f) starts f,
+// join() returns when it is done.  It spins briefly between jobs (one per
+// wave) before it sleeps.
+class AsyncWorker {
+   public:
+    AsyncWorker() : th_([this] { loop(); }) {}
+    ~AsyncWorker() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_.store(true);
+            gen_.fetch_add(1);
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    void post(std::function<void()> f) {
+        job_ = std::move(f);
+        busy_.store(true);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            gen_.fetch_add(1);
+        }
+        cv_.notify_one();
+    }
+    void join() {
+        while (busy_.load()) std::this_thread::yield();
+    }
+
+   private:
+    void loop() {
+        uint64_t seen = 0;
+        while (true) {
+            const auto t0 = std::chrono::steady_clock::now();
+            while (gen_.load() == seen && std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(2000))
+                std::this_thread::yield();
+            if (gen_.load() == seen) {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return gen_.load() != seen; });
+            }
+            seen = gen_.load();
+            if (stop_.load()) return;
+            job_();
+            busy_.store(false);
+        }
+    }
+    std::function<void()> job_;
+    std::atomic<bool> busy_{false}, stop_{false};
+    std::atomic<uint64_t> gen_{0};
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::thread th_;
+};
+
 struct VecHash {
     size_t operator()(const std::vector<int32_t> &v) const {
         uint64_t h = 1469598103934665603ull;
@@ -449,20 +502,44 @@ class Mafia {
             push_children(root);
         }
         budget_ = out_budget(db_);
-        // two waves in flight: the next one is counted while this one is processed
+        // Three waves in flight: while this thread processes wave k, the
+        // device counts k+1 and then k+2, which the helper thread packs and
+        // issues meanwhile (the library allows one submit beside one wait).
+        // Wave k+2 is taken off the stack before k's children are on it;
+        // the answers do not depend on the order of the search.
         int cur = 0;
-        if (int rc = launch(fl_[cur], err)) return rc;
+        if (int rc = launch(fl_[cur], false, err)) return rc;
         while (fl_[cur].live) {
-            if (int rc = launch(fl_[1 - cur], err)) return rc;
+            Flight &n1 = fl_[(cur + 1) % kFlights];
+            if (!n1.live)
+                if (int rc = launch(n1, false, err)) return rc;
+            Flight &n2 = fl_[(cur + 2) % kFlights];
+            if (int rc = launch(n2, n1.live, err)) return rc;
+            struct Join {
+                AsyncWorker &w;
+                bool on;
+                ~Join() {
+                    if (on) w.join();
+                }
+            } join{helper_, n2.live && n1.live};
             Flight &f = fl_[cur];
             if (int rc = f.wb.wait(ctx_, st_, err)) return rc;
             const double tp = now();
             if (int rc = process_wave(f, out)) return rc;
             st_.process_seconds += now() - tp;
+            if (join.on) {
+                helper_.join();
+                join.on = false;
+                merge_build(n2);
+                if (n2.rc) {
+                    err = n2.err;
+                    return n2.rc;
+                }
+            }
             f.live = false;
-            cur = 1 - cur;
+            cur = (cur + 1) % kFlights;
             if (!fl_[cur].live)
-                if (int rc = launch(fl_[cur], err)) return rc;
+                if (int rc = launch(fl_[cur], false, err)) return rc;
         }
         return TFIM_OK;
     }
@@ -472,9 +549,18 @@ class Mafia {
         std::vector<MNode> wave;
         WaveBatch wb;
         bool live = false;
+        // packing state of the helper thread: its own scratch, statistics
+        // (added to the run's after it finishes) and status
+        std::vector<uint8_t> mark;
+        std::vector<int32_t> cols;
+        SearchStats st;
+        std::string err;
+        int rc = 0;
     };
 
-    int launch(Flight &f, std::string &err) {
+    // Pops the next wave off the stack (this thread) and packs and issues it,
+    // on the helper thread when `async`.
+    int launch(Flight &f, bool async, std::string &err) {
         if (stack_.empty()) settle_undecided();   // nothing else left to count
         const double t0 = now();
         f.wave.clear();
@@ -487,10 +573,39 @@ class Mafia {
             stack_.pop_back();
         }
         f.live = !f.wave.empty();
-        if (!f.live) return TFIM_OK;
-        const int rc = count_wave(f, err);
         st_.build_seconds += now() - t0;
-        return rc;
+        if (!f.live) return TFIM_OK;
+        f.rc = 0;
+        f.st = SearchStats();
+        if (async) {
+            helper_.post([this, &f] {
+                const double t1 = now();
+                f.rc = count_wave(f, f.err);
+                f.st.build_seconds += now() - t1;
+            });
+            return TFIM_OK;
+        }
+        const double t1 = now();
+        f.rc = count_wave(f, f.err);
+        f.st.build_seconds += now() - t1;
+        merge_build(f);
+        if (f.rc) err = f.err;
+        return f.rc;
+    }
+
+    // Adds the statistics of a wave's packing to the run's.
+    void merge_build(Flight &f) {
+        st_.nodes += f.st.nodes;
+        st_.waves += f.st.waves;
+        st_.expand_rows += f.st.expand_rows;
+        st_.hut_rows += f.st.hut_rows;
+        st_.leaf_rows += f.st.leaf_rows;
+        st_.ref_rows += f.st.ref_rows;
+        st_.counted_bitops += f.st.counted_bitops;
+        st_.count_seconds += f.st.count_seconds;
+        st_.submit_seconds += f.st.submit_seconds;
+        st_.build_seconds += f.st.build_seconds;
+        f.st = SearchStats();
     }
 
     int64_t row_outputs(const MNode &n) const {
@@ -505,49 +620,51 @@ class Mafia {
     // Survivors outside x that are not known infrequent with x's head; with
     // `lead`, x[0] first (the HUT row's own support).
     void complement(const std::vector<int32_t> &x, const std::vector<uint64_t> &known, std::vector<int32_t> &cols,
-                    bool lead) {
-        mark_.assign(db_.n_surv, 0);
-        for (int32_t i : x) mark_[i] = 1;
+                    bool lead, std::vector<uint8_t> &mark) const {
+        mark.assign(db_.n_surv, 0);
+        for (int32_t i : x) mark[i] = 1;
         for (size_t w = 0; w < known.size(); w++)
-            for (uint64_t b = known[w]; b; b &= b - 1) mark_[(int32_t)(w * 64 + __builtin_ctzll(b))] = 1;
+            for (uint64_t b = known[w]; b; b &= b - 1) mark[(int32_t)(w * 64 + __builtin_ctzll(b))] = 1;
         cols.clear();
         if (lead) cols.push_back(x[0]);
         for (int32_t i = 0; i < db_.n_surv; i++)
-            if (!mark_[i]) cols.push_back(i);
+            if (!mark[i]) cols.push_back(i);
     }
 
+    // Packs a wave and issues it; touches only the flight, the database and
+    // the library (so it may run beside process_wave).
     int count_wave(Flight &f, std::string &err) {
         WaveBatch &wb_ = f.wb;
         wb_.clear();
-        std::vector<int32_t> cols;
+        std::vector<int32_t> &cols = f.cols;
         for (auto &n : f.wave) {
             switch (n.kind) {
                 case MNode::EXPAND:
                     wb_.add(n.head.data(), (int)n.head.size(), n.tail.data(), (int)n.tail.size());
-                    st_.expand_rows++;
+                    f.st.expand_rows++;
                     break;
                 case MNode::HUT: {
                     std::vector<int32_t> hut = n.head;
                     hut.insert(hut.end(), n.tail.begin(), n.tail.end());
-                    complement(hut, n.known, cols, true);
+                    complement(hut, n.known, cols, true, f.mark);
                     wb_.add(hut.data(), (int)hut.size(), cols.data(), (int)cols.size());
-                    st_.hut_rows++;
+                    f.st.hut_rows++;
                     break;
                 }
                 case MNode::LEAF:
-                    complement(n.head, n.known, cols, false);
+                    complement(n.head, n.known, cols, false, f.mark);
                     wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
-                    st_.leaf_rows++;
+                    f.st.leaf_rows++;
                     break;
                 case MNode::REF:
                     cols.resize(db_.n_surv);
                     std::iota(cols.begin(), cols.end(), 0);
                     wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
-                    st_.ref_rows++;
+                    f.st.ref_rows++;
                     break;
             }
         }
-        return wb_.submit(ctx_, st_, db_.n_bits(), err);
+        return wb_.submit(ctx_, f.st, db_.n_bits(), err);
     }
 
     void emit(std::vector<int32_t> items, int32_t sup, std::vector<Mfi> &out) {
@@ -806,8 +923,9 @@ class Mafia {
     std::unordered_set<uint64_t> seen64_;
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;
-    std::vector<uint8_t> mark_;
-    Flight fl_[2];
+    static constexpr int kFlights = 3;
+    Flight fl_[kFlights];
+    AsyncWorker helper_;
     int32_t cap_ = 256;
     int64_t budget_ = INT64_MAX;
 };

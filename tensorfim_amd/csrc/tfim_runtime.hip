@@ -11,6 +11,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 
 #include "tfim_internal.h"
 
@@ -39,6 +40,10 @@ struct tfim_ctx {
     int64_t hbm = 0;
     int clock_mhz = 0;
     std::string err;
+    std::mutex err_mu;
+    // wave state (slots, tuner, counters): one thread may submit a wave while
+    // another waits on the earlier one
+    std::mutex wave_mu;
 
     // resident bitmaps
     uint64_t *d_bm = nullptr;
@@ -52,7 +57,7 @@ struct tfim_ctx {
     int32_t *h_node_items = nullptr, *h_node_out = nullptr;   // pinned
     int64_t node_items_cap = 0, node_out_cap = 0;
 
-    // wave scratch: two in-flight slots, each a descriptor block and an
+    // wave scratch: kWaveSlots in-flight slots, each a descriptor block and an
     // output block (device + pinned host) and its events
     struct Slot {
         char *d_desc = nullptr, *h_desc = nullptr;
@@ -74,7 +79,8 @@ struct tfim_ctx {
         };
         std::vector<Graph> graphs;
     };
-    Slot slots[2];
+    static constexpr int kWaveSlots = 3;
+    Slot slots[kWaveSlots];
     int64_t next_ticket = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -108,7 +114,10 @@ int fail(tfim_ctx *c, int code, const char *fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    if (c) c->err = buf;
+    if (c) {
+        std::lock_guard<std::mutex> g(c->err_mu);
+        c->err = buf;
+    }
     return code;
 }
 
@@ -444,14 +453,17 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
                             const int32_t *col_ids, const int32_t *col_off, int32_t path, int64_t *ticket) {
     if (!c || !item_ids || !item_off || !ticket || n_rows < 0)
         return fail(c, TFIM_E_INVALID, "count_nodes: null argument");
+    std::lock_guard<std::mutex> wave_lock(c->wave_mu);
     *ticket = -1;
     if (col_ids && !col_off) return fail(c, TFIM_E_INVALID, "count_nodes: col_ids without col_off");
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_nodes: no bitmaps loaded");
     if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED)
         return fail(c, TFIM_E_INVALID, "count_nodes: path %d not available for node waves", path);
     if (n_rows == 0) return TFIM_OK;
-    tfim_ctx::Slot &sl = c->slots[c->next_ticket & 1];
-    if (sl.busy) return fail(c, TFIM_E_STATE, "count_nodes_submit: both slots in flight; wait on a ticket first");
+    tfim_ctx::Slot &sl = c->slots[c->next_ticket % tfim_ctx::kWaveSlots];
+    if (sl.busy)
+        return fail(c, TFIM_E_STATE, "count_nodes_submit: %d waves in flight; wait on a ticket first",
+                    tfim_ctx::kWaveSlots);
     if (item_off[0] != 0) return fail(c, TFIM_E_INVALID, "count_nodes: item_off[0] != 0");
     for (int32_t r = 0; r < n_rows; r++)
         if (item_off[r + 1] <= item_off[r]) return fail(c, TFIM_E_INVALID, "count_nodes: row %d has no prefix item", r);
@@ -644,12 +656,31 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
 int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     if (!c) return TFIM_E_INVALID;
     if (ticket < 0) return TFIM_OK;   // nothing was submitted (no rows or no columns)
-    tfim_ctx::Slot &sl = c->slots[ticket & 1];
-    if (!sl.busy || sl.ticket != ticket) return fail(c, TFIM_E_STATE, "count_nodes_wait: unknown ticket %lld", (long long)ticket);
+    tfim_ctx::Slot &sl = c->slots[ticket % tfim_ctx::kWaveSlots];
+    {
+        std::lock_guard<std::mutex> g(c->wave_mu);
+        if (!sl.busy || sl.ticket != ticket)
+            return fail(c, TFIM_E_STATE, "count_nodes_wait: unknown ticket %lld", (long long)ticket);
+    }
     if (!out) return fail(c, TFIM_E_INVALID, "count_nodes_wait: null output");
     if (int r = set_device(c)) return r;
-    sl.busy = false;
-    HIPCHK(c, hipEventSynchronize(sl.done));
+    // the slot stays busy (a submit cannot take it) until its counts are copied
+    struct Release {
+        tfim_ctx *c;
+        tfim_ctx::Slot &sl;
+        ~Release() {
+            std::lock_guard<std::mutex> g(c->wave_mu);
+            sl.busy = false;
+        }
+    } release{c, sl};
+    // poll rather than block in the runtime, so that a submit from another
+    // thread is not held up behind this wait
+    for (hipError_t q; (q = hipEventQuery(sl.done)) != hipSuccess;) {
+        if (q != hipErrorNotReady) HIPCHK(c, q);
+        std::this_thread::yield();
+    }
+    memcpy(out, sl.h_out, (size_t)sl.n_out * 4);
+    std::lock_guard<std::mutex> g(c->wave_mu);
     float ms = 0;
     hipEventElapsedTime(&ms, sl.ev0, sl.ev1);
     c->wave_ms += ms;
@@ -661,7 +692,6 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
         cst = c->tune_samples[sl.tuned] == 0 ? per : 0.8 * cst + 0.2 * per;
         c->tune_samples[sl.tuned]++;
     }
-    memcpy(out, sl.h_out, (size_t)sl.n_out * 4);
     return TFIM_OK;
 }
 
@@ -703,6 +733,7 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
 
 int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *lds_launches, int64_t *rows, double *ms) {
     if (!c) return TFIM_E_INVALID;
+    std::lock_guard<std::mutex> g(c->wave_mu);
     if (lds_launches) *lds_launches = c->wave_lds_launches;
     if (launches) *launches = c->wave_launches;
     if (rows) *rows = c->wave_rows;
@@ -712,6 +743,7 @@ int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *lds_launches, int64
 
 int tfim_wave_graph_stats(tfim_ctx *c, int64_t *graphs, int64_t *replays) {
     if (!c) return TFIM_E_INVALID;
+    std::lock_guard<std::mutex> g(c->wave_mu);
     if (graphs) *graphs = c->wave_graphs;
     if (replays) *replays = c->wave_graph_replays;
     return TFIM_OK;

@@ -46,10 +46,10 @@ static void cnt(tfim_ctx *c, const int32_t *it, int k, const int32_t *cols, int 
 int tfim_count_node(tfim_ctx *c, const int32_t *it, int32_t k, int32_t *out) { cnt(c,it,k,nullptr,c->n_cols,out); return 0; }
 int tfim_count_nodes(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co, int32_t *out, int32_t) {
   for (int r=0;r<n;r++){ if(cids) cnt(c,ids+io[r],io[r+1]-io[r],cids+co[r],co[r+1]-co[r],out+co[r]); else cnt(c,ids+io[r],io[r+1]-io[r],nullptr,c->n_cols,out+(size_t)r*c->n_cols);} return 0; }
-static std::vector<int32_t> g_pending[2]; static int64_t g_next=0;
+static std::vector<int32_t> g_pending[3]; static int64_t g_next=0;
 int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co, int32_t p, int64_t *t) {
-  int64_t tk=g_next++; auto &v=g_pending[tk&1]; v.assign(cids?co[n]:(size_t)n*c->n_cols,0); tfim_count_nodes(c,ids,io,n,cids,co,v.data(),p); *t=tk; return 0; }
-int tfim_count_nodes_wait(tfim_ctx *, int64_t t, int32_t *out) { if(t<0) return 0; auto &v=g_pending[t&1]; memcpy(out,v.data(),v.size()*4); return 0; }
+  int64_t tk=g_next++; auto &v=g_pending[tk%3]; v.assign(cids?co[n]:(size_t)n*c->n_cols,0); tfim_count_nodes(c,ids,io,n,cids,co,v.data(),p); *t=tk; return 0; }
+int tfim_count_nodes_wait(tfim_ctx *, int64_t t, int32_t *out) { if(t<0) return 0; auto &v=g_pending[t%3]; memcpy(out,v.data(),v.size()*4); return 0; }
 int tfim_wave_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *r, double *ms) { if(a)*a=0; if(b)*b=0; if(r)*r=0; if(ms)*ms=0; return 0; }
 int tfim_wave_graph_stats(tfim_ctx *, int64_t *g, int64_t *l) { if(g)*g=0; if(l)*l=0; return 0; }
 int tfim_mlp_create(tfim_ctx *, uint64_t, tfim_mlp **) { return -2; }
