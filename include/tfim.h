@@ -123,6 +123,31 @@ int tfim_count_nodes_submit(tfim_ctx *ctx, const int32_t *item_ids, const int32_
                             int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
                             int32_t path, int64_t *ticket);
 int tfim_count_nodes_wait(tfim_ctx *ctx, int64_t ticket, int32_t *out);
+/* Complement rows, for maximality checks over many survivors.  As
+ * tfim_count_nodes_submit with col_off required (col_ids may be NULL when
+ * col_off[n_rows] == 0), plus n_compl complement rows: compl_rows lists their
+ * row indices in ascending order, each with an empty column range and a
+ * first prefix item x0 < n_cols, and excl holds one exclusion bitmap of
+ * ceil(n_cols / 64) u64 words per complement row (bit e of word e >> 6 set =
+ * survivor e excluded).  Complement row j is counted on the device against
+ * every survivor e whose bit is clear, and the wave returns for it two ints
+ * after the explicit counts (out[col_off[n_rows] + 2j ...]):
+ *   - popcount(AND of the prefix rows), the support of the prefix;
+ *   - #{e != x0 : bit e clear, popcount(AND prefix & row e) >= min_count}.
+ * The reference's maximality test counts every survivor and compares the
+ * number of frequent ones on the host (kernel_bmma.cu:857-877,
+ * kernel_bitmap.cu:861-866); here the threshold step runs on the device and
+ * only the two numbers travel.  tfim_count_nodes_wait returns
+ * col_off[n_rows] + 2 * n_compl ints.  tfim_count_nodes_ex is the blocking
+ * form. */
+int tfim_count_nodes_ex_submit(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item_off,
+                               int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
+                               const int32_t *compl_rows, int32_t n_compl, const uint64_t *excl,
+                               int32_t min_count, int32_t path, int64_t *ticket);
+int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item_off,
+                        int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
+                        const int32_t *compl_rows, int32_t n_compl, const uint64_t *excl,
+                        int32_t min_count, int32_t *out, int32_t path);
 
 /* Tuning knobs (ablation and autotuning; the reference's equivalents are
  * environment variables, kernel_bmma.cu:351-368):

@@ -49,6 +49,9 @@ _SIGS = {
     "tfim_count_node": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "tfim_count_nodes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
+    "tfim_count_nodes_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]),
     "tfim_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "tfim_wave_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64),
                                         _p(ctypes.c_int64), _p(ctypes.c_double)]),
@@ -236,6 +239,43 @@ class Context:
                                                _ptr(col_ids) if len(col_ids) else _ptr(np.zeros(1, np.int32)),
                                                _ptr(col_off), _ptr(out), _PATHS[path]))
         return [out[col_off[i]:col_off[i + 1]] for i in range(n)]
+
+    def count_nodes_ex(self, prefixes, tails, excl, min_count, path="auto"):
+        """Batched path with complement rows.  tails[i] is a column list, or
+        None for a complement row whose exclusion set is excl[i] (survivor
+        ids).  Returns, per row, its counts (explicit rows) or the pair
+        (prefix support, number of non-excluded survivors other than the
+        first prefix item with count >= min_count)."""
+        n = len(prefixes)
+        item_off = np.zeros(n + 1, dtype=np.int32)
+        item_off[1:] = np.cumsum([len(p) for p in prefixes])
+        item_ids = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.int32) for p in prefixes])
+                                        if n else np.zeros(0, np.int32), dtype=np.int32)
+        col_off = np.zeros(n + 1, dtype=np.int32)
+        col_off[1:] = np.cumsum([0 if t is None else len(t) for t in tails])
+        cols = [np.asarray(t, dtype=np.int32) for t in tails if t is not None]
+        col_ids = np.ascontiguousarray(np.concatenate(cols) if cols else np.zeros(0, np.int32), dtype=np.int32)
+        crows = np.asarray([i for i, t in enumerate(tails) if t is None], dtype=np.int32)
+        words = (self.n_cols + 63) // 64
+        ex = np.zeros((max(1, len(crows)), words), dtype=np.uint64)
+        for j, i in enumerate(crows):
+            for e in excl[i]:
+                ex[j, e >> 6] |= np.uint64(1) << np.uint64(e & 63)
+        n_exp = int(col_off[-1])
+        out = np.empty(max(1, n_exp + 2 * len(crows)), dtype=np.int32)
+        one = np.zeros(1, np.int32)
+        self._check(self._lib.tfim_count_nodes_ex(
+            self._h, _ptr(item_ids), _ptr(item_off), n, _ptr(col_ids) if len(col_ids) else None, _ptr(col_off),
+            _ptr(crows) if len(crows) else _ptr(one), len(crows), _ptr(ex), int(min_count), _ptr(out),
+            _PATHS[path]))
+        res, j = [], 0
+        for i, t in enumerate(tails):
+            if t is None:
+                res.append(out[n_exp + 2 * j:n_exp + 2 * j + 2])
+                j += 1
+            else:
+                res.append(out[col_off[i]:col_off[i + 1]])
+        return res
 
     def config(self, key, value):
         self._check(self._lib.tfim_config(self._h, key.encode(), int(value)))

@@ -3,6 +3,7 @@
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
 //             [--workers N] [--host-threads N] [--learned] [--rank] [--gate indep|always]
+//             [--compl auto|on|off]
 //             [--wave-kernel 0|1|2|3] [--graphs 0|1] [--parse-threads N]
 //             [--prep gpu|host] [--model PATH] [--trace PATH] [--verbose]
 //
@@ -50,6 +51,7 @@ int main(int argc, char **argv) {
         else if (a == "--device") device = atoi(next().c_str());
         else if (a == "--learned") opt.learned = true;
         else if (a == "--gate") opt.gate = next();
+        else if (a == "--compl") opt.compl_rows = next();
         else if (a == "--workers") opt.workers = atoi(next().c_str());
         else if (a == "--host-threads") opt.host_threads = atoi(next().c_str());
         else if (a == "--rank") opt.rank = true;
@@ -82,6 +84,11 @@ int main(int argc, char **argv) {
     }
     if (wave_kernel && tfim_config(ctx, "wave_kernel", wave_kernel) != TFIM_OK) {
         fprintf(stderr, "%s\n", tfim_last_error(ctx));
+        tfim_close(ctx);
+        return 1;
+    }
+    if (opt.compl_rows != "auto" && opt.compl_rows != "on" && opt.compl_rows != "off") {
+        fprintf(stderr, "--compl must be auto, on or off\n");
         tfim_close(ctx);
         return 1;
     }

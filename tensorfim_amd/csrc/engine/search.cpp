@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <chrono>
 #include <functional>
 #include <condition_variable>
@@ -30,26 +31,52 @@ class WaveBatch {
         col_ids_.clear();
         item_off_.assign(1, 0);
         col_off_.assign(1, 0);
+        compl_rows_.clear();
+        compl_of_.clear();
+        excl_.clear();
+        compl_cols_ = 0;
     }
     int add(const int32_t *prefix, int k, const int32_t *cols, int nc) {
         item_ids_.insert(item_ids_.end(), prefix, prefix + k);
         item_off_.push_back((int32_t)item_ids_.size());
         col_ids_.insert(col_ids_.end(), cols, cols + nc);
         col_off_.push_back((int32_t)col_ids_.size());
+        compl_of_.push_back(-1);
+        return rows() - 1;
+    }
+    // A complement row (tfim_count_nodes_ex_submit): counted on the device
+    // against every survivor whose bit in `excl` is clear; its result is
+    // (support of the prefix, number of those survivors other than prefix[0]
+    // reaching the threshold).  `live` = columns it counts, for statistics.
+    int add_complement(const int32_t *prefix, int k, const uint64_t *excl, int words, int64_t live) {
+        item_ids_.insert(item_ids_.end(), prefix, prefix + k);
+        item_off_.push_back((int32_t)item_ids_.size());
+        col_off_.push_back((int32_t)col_ids_.size());
+        compl_of_.push_back((int32_t)compl_rows_.size());
+        compl_rows_.push_back(rows() - 1);
+        excl_.insert(excl_.end(), excl, excl + words);
+        compl_cols_ += live;
         return rows() - 1;
     }
     int rows() const { return (int)item_off_.size() - 1; }
-    int64_t cols_total() const { return (int64_t)col_ids_.size(); }
-    // Launches the wave asynchronously (tfim_count_nodes_submit); wait()
-    // makes its counts available.  A wave with no columns submits nothing.
-    int submit(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, std::string &err) {
-        out_.resize(std::max<size_t>(1, col_ids_.size()));
+    int64_t cols_total() const { return (int64_t)col_ids_.size() + compl_cols_; }
+    // Launches the wave asynchronously (tfim_count_nodes_submit, or the _ex
+    // form when it holds complement rows); wait() makes its counts
+    // available.  A wave with no columns submits nothing.
+    int submit(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, int32_t min_count, std::string &err) {
+        out_.resize(std::max<size_t>(1, col_ids_.size() + 2 * compl_rows_.size()));
         st.nodes += rows();
         ticket_ = -1;
-        if (col_ids_.empty()) return TFIM_OK;   // nothing to count (e.g. a leaf holding every survivor)
+        if (col_ids_.empty() && compl_rows_.empty()) return TFIM_OK;   // nothing to count
         const double t0 = now();
-        const int rc = tfim_count_nodes_submit(ctx, item_ids_.data(), item_off_.data(), rows(), col_ids_.data(),
-                                               col_off_.data(), TFIM_PATH_AUTO, &ticket_);
+        const int32_t one = 0;
+        const int rc = compl_rows_.empty()
+                           ? tfim_count_nodes_submit(ctx, item_ids_.data(), item_off_.data(), rows(), col_ids_.data(),
+                                                     col_off_.data(), TFIM_PATH_AUTO, &ticket_)
+                           : tfim_count_nodes_ex_submit(ctx, item_ids_.data(), item_off_.data(), rows(),
+                                                        col_ids_.empty() ? &one : col_ids_.data(), col_off_.data(),
+                                                        compl_rows_.data(), (int32_t)compl_rows_.size(),
+                                                        excl_.data(), min_count, TFIM_PATH_AUTO, &ticket_);
         st.count_seconds += now() - t0;
         st.submit_seconds += now() - t0;
         if (rc != TFIM_OK) {
@@ -68,17 +95,27 @@ class WaveBatch {
         if (rc != TFIM_OK) err = std::string("tfim_count_nodes_wait: ") + tfim_last_error(ctx);
         return rc;
     }
-    int run(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, std::string &err) {
-        if (int rc = submit(ctx, st, n_bits, err)) return rc;
-        return wait(ctx, st, err);
+    // Counts of row r (CSR-aligned with its columns), or for a complement
+    // row its two results.
+    const int32_t *result(int r) const {
+        const int32_t j = compl_of_[r];
+        return j < 0 ? out_.data() + col_off_[r] : out_.data() + col_ids_.size() + 2 * (size_t)j;
     }
-    const int32_t *result(int r) const { return out_.data() + col_off_[r]; }
     int ncols(int r) const { return col_off_[r + 1] - col_off_[r]; }
 
    private:
-    std::vector<int32_t> item_ids_, item_off_{0}, col_ids_, col_off_{0}, out_;
+    std::vector<int32_t> item_ids_, item_off_{0}, col_ids_, col_off_{0}, out_, compl_rows_, compl_of_;
+    std::vector<uint64_t> excl_;
+    int64_t compl_cols_ = 0;
     int64_t ticket_ = -1;
 };
+
+// Smallest integer count that passes the reference's `sup >= thrAbs`
+// (double compare, kernel_bmma.cu:864): for integer s, s >= t <=> s >= ceil(t).
+int32_t min_count_of(const Database &db) {
+    if (!(db.thr_abs > 0)) return 0;
+    return (int32_t)std::min<double>(std::ceil(db.thr_abs), (double)INT32_MAX);
+}
 
 bool frequent(const Database &db, int32_t s) { return !((double)s < db.thr_abs); }
 
@@ -239,7 +276,7 @@ int run_wave(tfim_ctx *ctx, const Database &db, int32_t cap, std::vector<Mfi> &o
         f.wb.clear();
         for (auto &nd : f.wave) f.wb.add(nd.items.data(), (int)nd.items.size(), all.data(), db.n_surv);
         st.ref_rows += (int64_t)f.wave.size();
-        return f.wb.submit(ctx, st, db.n_bits(), err);
+        return f.wb.submit(ctx, st, db.n_bits(), 0, err);
     };
     int cur = 0;
     if (int rc = launch(fl[cur])) return rc;
@@ -474,6 +511,12 @@ class Mafia {
     int run(std::vector<Mfi> &out, std::string &err) {
         const int32_t F = db_.n_surv;
         cap_ = opt_.wave_cap > 0 ? opt_.wave_cap : auto_wave_cap(db_);
+        // maximality rows (HUT, LEAF) as complement rows reduced on the
+        // device when the survivors are too many for listing them per row
+        // to be cheap (the same bound as the LDS kernel's)
+        compl_ = opt_.compl_rows == "on" || (opt_.compl_rows == "auto" && !lds_shape(db_));
+        min_count_ = min_count_of(db_);
+        excl_words_ = (F + 63) / 64;
         // item-0 subtree: the reference enumeration, reproduced node for node
         if (db_.d_count > F && db_.is_root[F]) {
             for (auto &r : ref_roots(db_, true)) {
@@ -553,6 +596,7 @@ class Mafia {
         // (added to the run's after it finishes) and status
         std::vector<uint8_t> mark;
         std::vector<int32_t> cols;
+        std::vector<uint64_t> excl;
         SearchStats st;
         std::string err;
         int rc = 0;
@@ -631,6 +675,18 @@ class Mafia {
             if (!mark[i]) cols.push_back(i);
     }
 
+    // Exclusion bitmap of a complement row: x and the survivors known
+    // infrequent with its head; returns how many survivors stay to count.
+    int64_t exclusion(const std::vector<int32_t> &x, const std::vector<uint64_t> &known,
+                      std::vector<uint64_t> &ex) const {
+        ex.assign((size_t)excl_words_, 0ull);
+        for (size_t w = 0; w < known.size() && w < ex.size(); w++) ex[w] = known[w];
+        for (int32_t i : x) ex[(size_t)i >> 6] |= 1ull << (i & 63);
+        int64_t n = 0;
+        for (uint64_t w : ex) n += __builtin_popcountll(w);
+        return db_.n_surv - n;
+    }
+
     // Packs a wave and issues it; touches only the flight, the database and
     // the library (so it may run beside process_wave).
     int count_wave(Flight &f, std::string &err) {
@@ -646,14 +702,24 @@ class Mafia {
                 case MNode::HUT: {
                     std::vector<int32_t> hut = n.head;
                     hut.insert(hut.end(), n.tail.begin(), n.tail.end());
-                    complement(hut, n.known, cols, true, f.mark);
-                    wb_.add(hut.data(), (int)hut.size(), cols.data(), (int)cols.size());
+                    if (compl_) {
+                        const int64_t live = exclusion(hut, n.known, f.excl);
+                        wb_.add_complement(hut.data(), (int)hut.size(), f.excl.data(), excl_words_, live + 1);
+                    } else {
+                        complement(hut, n.known, cols, true, f.mark);
+                        wb_.add(hut.data(), (int)hut.size(), cols.data(), (int)cols.size());
+                    }
                     f.st.hut_rows++;
                     break;
                 }
                 case MNode::LEAF:
-                    complement(n.head, n.known, cols, false, f.mark);
-                    wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
+                    if (compl_) {
+                        const int64_t live = exclusion(n.head, n.known, f.excl);
+                        wb_.add_complement(n.head.data(), (int)n.head.size(), f.excl.data(), excl_words_, live + 1);
+                    } else {
+                        complement(n.head, n.known, cols, false, f.mark);
+                        wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
+                    }
                     f.st.leaf_rows++;
                     break;
                 case MNode::REF:
@@ -664,7 +730,7 @@ class Mafia {
                     break;
             }
         }
-        return wb_.submit(ctx_, f.st, db_.n_bits(), err);
+        return wb_.submit(ctx_, f.st, db_.n_bits(), min_count_, err);
     }
 
     void emit(std::vector<int32_t> items, int32_t sup, std::vector<Mfi> &out) {
@@ -802,11 +868,14 @@ class Mafia {
                 }
                 decisions.push_back(std::move(d));
             } else if (n.kind == MNode::HUT) {
+                // explicit: res = {sup(HUT), counts of the complement};
+                // complement row: res = {sup(HUT), frequent extensions}
                 const bool hut_freq = frequent(db_, res[0]);
                 pol_.observe(n.decision, hut_freq);
                 if (hut_freq) {
                     st_.hut_success++;
                     bool maximal = true;
+                    if (compl_) maximal = res[1] == 0;
                     for (int j = 1; j < nc && maximal; j++) maximal = !frequent(db_, res[j]);
                     std::vector<int32_t> hut = n.head;
                     hut.insert(hut.end(), n.tail.begin(), n.tail.end());
@@ -825,6 +894,7 @@ class Mafia {
                 }
             } else if (n.kind == MNode::LEAF) {
                 bool maximal = true;
+                if (compl_) maximal = res[1] == 0;
                 for (int j = 0; j < nc && maximal; j++) maximal = !frequent(db_, res[j]);
                 if (maximal) emit(n.head, n.sup, out);
             } else {   // REF: reference rule, own output path
@@ -928,6 +998,9 @@ class Mafia {
     AsyncWorker helper_;
     int32_t cap_ = 256;
     int64_t budget_ = INT64_MAX;
+    bool compl_ = false;         // HUT and LEAF rows are complement rows
+    int32_t min_count_ = 0;
+    int excl_words_ = 0;
 };
 
 }  // namespace

@@ -31,6 +31,7 @@ struct SearchOptions {
     int32_t host_threads = 0;     // mafia: threads for the subsumption checks (0 = up to 8)
     bool learned = false;         // mafia: use the MLP ranker/gate
     std::string gate = "indep";   // mafia plain lookahead rule: indep | always
+    std::string compl_rows = "auto";   // mafia: HUT/LEAF rows as device-reduced complement rows: auto | on | off
     bool rank = false;            // mafia learned: reorder tails with the ranker head
     std::string model_path;       // mafia: load MLP weights (else trained online)
     std::string trace_path;       // mafia: write (features, outcome) records

@@ -258,13 +258,25 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint4 *__restrict__ bm
             }
         }
         const int ob = d.col_off[r];
+        // complement tile: the columns are survivor ids; live = exclusion bit
+        // clear, or the row's first prefix item (its count is the prefix's
+        // support).  Wave-uniform, so dead groups and columns are skipped by
+        // scalar branches.
+        uint64_t live = ~0ull;
+        if (tile.compl_) {
+            live = ~d.excl[(int64_t)(tile.compl_ - 1) * d.excl_words + (tile.col_begin >> 6)];
+            const int x0 = d.item_ids[ib] - tile.col_begin;
+            if (x0 >= 0 && x0 < 64) live |= 1ull << x0;
+        }
         for (int j = tile.col_begin; j < tile.col_end; j += 8) {
+            const uint32_t live8 = tile.compl_ ? (uint32_t)(live >> (j - tile.col_begin)) & 0xffu : 0xffu;
+            if (!live8) continue;
             int v8[8];
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 uint32_t acc = 0;
-                if (j + q < tile.col_end) {
-                    const int col = d.col_ids ? d.col_ids[ob + j + q] : j + q;
+                if (j + q < tile.col_end && ((live8 >> q) & 1u)) {
+                    const int col = (d.col_ids && !tile.compl_) ? d.col_ids[ob + j + q] : j + q;
                     const uint4 *row = bm + (int64_t)col * pitch4;
 #pragma unroll
                     for (int qq = 0; qq < RW; qq++) {
@@ -284,6 +296,47 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint4 *__restrict__ bm
 }
 
 int wave_chunk_words(int rw) { return 64 * rw * 2; }
+
+// One wave per complement row: lanes walk the survivors 64 at a time (one
+// exclusion word per step, read once for the wave), count the live columns
+// that reach min_count, and reduce over the wave.
+__global__ __launch_bounds__(256) void k_compl_reduce(WaveDesc d, const int32_t *__restrict__ compl_rows,
+                                                      int32_t n_compl, int32_t n_cols, int32_t min_count,
+                                                      int32_t *__restrict__ res) {
+    const int lane = threadIdx.x & 63;
+    const int j = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (j >= n_compl) return;
+    const int r = compl_rows[j];
+    const int x0 = d.item_ids[d.item_off[r]];
+    const int32_t *full = d.out + d.col_off[r];
+    const uint64_t *ex = d.excl + (int64_t)j * d.excl_words;
+    int n = 0;
+    for (int w = 0; w < d.excl_words; w++) {
+        const int e = w * 64 + lane;
+        const bool live = e < n_cols && e != x0 && !((ex[w] >> lane) & 1ull);
+        if (live && full[e] >= min_count) n++;
+    }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if (lane == 0) {
+        res[2 * j] = full[x0];
+        res[2 * j + 1] = n;
+    }
+}
+
+void plan_compl_reduce(const WaveDesc &d, const int32_t *compl_rows, int32_t n_compl, int32_t n_cols,
+                       int32_t min_count, int32_t *res, KernelLaunch &L) {
+    L.func = (const void *)k_compl_reduce;
+    L.grid = dim3((unsigned)std::max(1, (n_compl + 3) / 4));
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.n_args = L.used = 0;
+    L.arg(d);
+    L.arg(compl_rows);
+    L.arg(n_compl);
+    L.arg(n_cols);
+    L.arg(min_count);
+    L.arg(res);
+}
 
 void plan_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d, int grid_blocks, int rw,
                      KernelLaunch &L) {

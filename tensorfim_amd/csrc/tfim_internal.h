@@ -54,12 +54,17 @@ struct KernelLaunch {
 void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
                        int32_t n_cols, int32_t *out, int n_cu, int shape, hipStream_t s);
 
-// One tile of a wave: a row's prefix, and a run of its columns.
+// One tile of a wave: a row's prefix, and a run of its columns.  compl == 0:
+// the columns are entries [col_begin, col_end) of the row's column list.
+// compl == j + 1: the row is complement row j, the columns are the survivor
+// ids [col_begin, col_end) (one 64-aligned word of its exclusion bitmap) and
+// only those whose exclusion bit is clear, plus the row's first prefix item,
+// are counted.
 struct WaveTile {
     int32_t row;        // node index
-    int32_t col_begin;  // index into the row's column list
+    int32_t col_begin;
     int32_t col_end;
-    int32_t pad;
+    int32_t compl_;
 };
 
 // Wave descriptor consumed by the fused wave kernel (device pointers).
@@ -69,10 +74,19 @@ struct WaveDesc {
     const int32_t *col_ids;    // columns, CSR by col_off; nullptr = identity over [0,n_cols)
     const int32_t *col_off;    // [n_rows+1] output offsets (also when col_ids is null)
     const WaveTile *tiles;     // [n_tiles]
-    int32_t *out;              // counts, indexed by col_off[row] + j
+    int32_t *out;              // counts, indexed by col_off[row] + j (complement rows: + survivor id)
+    const uint64_t *excl;      // complement rows' exclusion bitmaps, excl_words each
     int32_t n_tiles;
     int32_t n_chunks;          // word chunks of the wave kernel's chunk size
+    int32_t excl_words;
 };
+
+// Complement-row reduction after the wave kernel: for complement row j (row
+// compl_rows[j], first prefix item x0, full counts at out[col_off[row] + e]):
+// res[2j] = count of x0 (= support of the prefix) and res[2j+1] = number of
+// survivors e != x0 with a clear exclusion bit and count >= min_count.
+void plan_compl_reduce(const WaveDesc &d, const int32_t *compl_rows, int32_t n_compl, int32_t n_cols,
+                       int32_t min_count, int32_t *res, KernelLaunch &L);
 
 // Fused wave kernel: prefix AND + column counts + split-k accumulation
 // (atomics).  `out` must be zeroed.  grid_blocks must be a multiple of 8.

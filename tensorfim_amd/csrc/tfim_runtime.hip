@@ -69,13 +69,16 @@ struct tfim_ctx {
         int64_t ticket = -1, n_out = 0, n_rows = 0;
         int tuned = -1;
         double work = 0;
-        // one executable graph per count-kernel function: H2D descriptor
-        // copy -> output fill -> ev0 -> count kernel -> ev1 -> D2H copy
+        // one executable graph per (count-kernel function, reduction): H2D
+        // descriptor copy -> output fill -> ev0 -> count kernel [-> complement
+        // reduction] -> ev1 -> D2H copy
         struct Graph {
             const void *func = nullptr;
+            bool reduce = false;   // complement-row reduction after the count kernel
             hipGraph_t g = nullptr;
             hipGraphExec_t x = nullptr;
-            hipGraphNode_t h2d = nullptr, fill = nullptr, e0 = nullptr, kern = nullptr, e1 = nullptr, d2h = nullptr;
+            hipGraphNode_t h2d = nullptr, fill = nullptr, e0 = nullptr, kern = nullptr, red = nullptr, e1 = nullptr,
+                           d2h = nullptr;
         };
         std::vector<Graph> graphs;
     };
@@ -395,27 +398,31 @@ hipKernelNodeParams node_params(KernelLaunch &L) {
 // operations form one graph per (slot, kernel function), instantiated on
 // first use and replayed with this wave's node parameters; the slot is idle
 // (its last wave waited on) whenever it is issued again.
-int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_out, KernelLaunch &K) {
+int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_fill, int64_t n_out, KernelLaunch &K,
+               KernelLaunch *R) {
     KernelLaunch Z;
-    plan_zero_i32(sl.d_out, n_out, Z);
+    plan_zero_i32(sl.d_out, n_fill, Z);
     const size_t out_bytes = (size_t)n_out * 4;
     if (!c->use_graphs) {
         HIPCHK(c, hipMemcpyAsync(sl.d_desc, sl.h_desc, (size_t)desc_bytes, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, Z.launch(c->stream));
         HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
         HIPCHK(c, K.launch(c->stream));
+        if (R) HIPCHK(c, R->launch(c->stream));
         HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
         HIPCHK(c, hipMemcpyAsync(sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost, c->stream));
         return TFIM_OK;
     }
-    hipKernelNodeParams zp = node_params(Z), kp = node_params(K);
+    hipKernelNodeParams zp = node_params(Z), kp = node_params(K), rp{};
+    if (R) rp = node_params(*R);
     tfim_ctx::Slot::Graph *G = nullptr;
     for (auto &g : sl.graphs)
-        if (g.func == K.func) G = &g;
+        if (g.func == K.func && g.reduce == (R != nullptr)) G = &g;
     if (!G) {
         // build and instantiate; only a complete graph is kept
         tfim_ctx::Slot::Graph g;
         g.func = K.func;
+        g.reduce = R != nullptr;
         hipError_t e = hipGraphCreate(&g.g, 0);
         if (e == hipSuccess)
             e = hipGraphAddMemcpyNode1D(&g.h2d, g.g, nullptr, 0, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
@@ -423,7 +430,8 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_ou
         if (e == hipSuccess) e = hipGraphAddKernelNode(&g.fill, g.g, &g.h2d, 1, &zp);
         if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e0, g.g, &g.fill, 1, sl.ev0);
         if (e == hipSuccess) e = hipGraphAddKernelNode(&g.kern, g.g, &g.e0, 1, &kp);
-        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e1, g.g, &g.kern, 1, sl.ev1);
+        if (e == hipSuccess && R) e = hipGraphAddKernelNode(&g.red, g.g, &g.kern, 1, &rp);
+        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e1, g.g, R ? &g.red : &g.kern, 1, sl.ev1);
         if (e == hipSuccess)
             e = hipGraphAddMemcpyNode1D(&g.d2h, g.g, &g.e1, 1, sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipGraphInstantiate(&g.x, g.g, nullptr, nullptr, 0);
@@ -439,6 +447,7 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_ou
                                                     hipMemcpyHostToDevice));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->fill, &zp));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->kern, &kp));
+        if (R) HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->red, &rp));
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->d2h, sl.h_out, sl.d_out, out_bytes,
                                                     hipMemcpyDeviceToHost));
     }
@@ -449,13 +458,25 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_ou
 
 }  // namespace
 
-int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
-                            const int32_t *col_ids, const int32_t *col_off, int32_t path, int64_t *ticket) {
-    if (!c || !item_ids || !item_off || !ticket || n_rows < 0)
+namespace {
+
+// Common body of tfim_count_nodes_submit and tfim_count_nodes_ex_submit.
+// Complement rows (n_compl > 0) need explicit column offsets (col_off), and
+// their waves run on the L2 kernel followed by the complement reduction.
+int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
+                const int32_t *col_ids, const int32_t *col_off, const int32_t *compl_rows, int32_t n_compl,
+                const uint64_t *excl, int32_t min_count, int32_t path, int64_t *ticket) {
+    if (!c || !item_ids || !item_off || !ticket || n_rows < 0 || n_compl < 0)
         return fail(c, TFIM_E_INVALID, "count_nodes: null argument");
     std::lock_guard<std::mutex> wave_lock(c->wave_mu);
     *ticket = -1;
     if (col_ids && !col_off) return fail(c, TFIM_E_INVALID, "count_nodes: col_ids without col_off");
+    if (n_compl > 0) {
+        if (!col_off || !compl_rows || !excl)
+            return fail(c, TFIM_E_INVALID, "count_nodes_ex: complement rows need col_off, compl_rows and excl");
+        if (!col_ids && col_off[n_rows] != 0)
+            return fail(c, TFIM_E_INVALID, "count_nodes_ex: explicit columns without col_ids");
+    }
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_nodes: no bitmaps loaded");
     if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED)
         return fail(c, TFIM_E_INVALID, "count_nodes: path %d not available for node waves", path);
@@ -472,7 +493,8 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         if (item_ids[i] < 0 || item_ids[i] >= c->n_items)
             return fail(c, TFIM_E_INVALID, "count_nodes: item %d out of range", item_ids[i]);
     int64_t n_out;
-    if (col_ids) {
+    const bool csr = col_ids || n_compl > 0;
+    if (csr) {
         if (col_off[0] != 0) return fail(c, TFIM_E_INVALID, "count_nodes: col_off[0] != 0");
         for (int32_t r = 0; r < n_rows; r++)
             if (col_off[r + 1] < col_off[r]) return fail(c, TFIM_E_INVALID, "count_nodes: col_off decreasing at %d", r);
@@ -483,7 +505,23 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     } else {
         n_out = (int64_t)n_rows * c->n_cols;
     }
-    if (n_out > INT32_MAX) return fail(c, TFIM_E_RANGE, "count_nodes: output too large");
+    for (int32_t j = 0; j < n_compl; j++) {
+        const int32_t r = compl_rows[j];
+        if (r < 0 || r >= n_rows || (j > 0 && r <= compl_rows[j - 1]))
+            return fail(c, TFIM_E_INVALID, "count_nodes_ex: complement rows must be ascending row indices");
+        if (col_off[r + 1] != col_off[r])
+            return fail(c, TFIM_E_INVALID, "count_nodes_ex: complement row %d has explicit columns", r);
+        if (item_ids[item_off[r]] >= c->n_cols)
+            return fail(c, TFIM_E_INVALID, "count_nodes_ex: complement row %d starts with a non-survivor item", r);
+    }
+    // output: explicit counts | two results per complement row | (device
+    // only) every survivor's count of each complement row
+    const int64_t n_exp = n_out;
+    const int64_t excl_words = ((int64_t)c->n_cols + 63) / 64;
+    const int64_t full_base = (n_exp + 2 * (int64_t)n_compl + 63) / 64 * 64;
+    const int64_t n_dev = n_compl ? full_base + (int64_t)n_compl * c->n_cols : n_exp;
+    n_out = n_exp + 2 * (int64_t)n_compl;
+    if (n_dev > INT32_MAX) return fail(c, TFIM_E_RANGE, "count_nodes: output too large");
     if (n_out == 0) return TFIM_OK;
     if (int r = set_device(c)) return r;
 
@@ -499,7 +537,7 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
             for (int32_t it : rows) c->slot_of[it] = -1;
         }
     } slot_reset{c, slot_rows};
-    bool lds = c->wave_kernel != 1 && n_out <= kLdsMaxOut;
+    bool lds = c->wave_kernel != 1 && n_compl == 0 && n_out <= kLdsMaxOut;
     if (lds) {
         if ((int32_t)c->slot_of.size() < c->n_items) c->slot_of.assign(c->n_items, -1);
         auto slot = [&](int32_t item) {
@@ -519,7 +557,7 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         if (!lds)
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
     }
-    if (c->wave_kernel >= 2 && !lds)
+    if (c->wave_kernel >= 2 && !lds && n_compl == 0)
         return fail(c, TFIM_E_RANGE, "count_nodes: wave does not fit the LDS kernel (%zu items, %lld outputs)",
                     slot_rows.size(), (long long)n_out);
     // autotune over (kernel, columns per tile) for waves both kernels accept:
@@ -549,8 +587,20 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     std::vector<WaveTile> tiles;
     tiles.reserve((size_t)n_rows * 2);
     for (int32_t r = 0; r < n_rows; r++) {
-        const int32_t nc = col_ids ? col_off[r + 1] - col_off[r] : c->n_cols;
+        const int32_t nc = csr ? col_off[r + 1] - col_off[r] : c->n_cols;
         for (int32_t b = 0; b < nc; b += tile_cols) tiles.push_back({r, b, std::min(nc, b + tile_cols), 0});
+    }
+    // complement rows: one tile per 64 survivor ids holding a column to count
+    for (int32_t j = 0; j < n_compl; j++) {
+        const int32_t r = compl_rows[j], x0 = item_ids[item_off[r]];
+        const uint64_t *ex = excl + (int64_t)j * excl_words;
+        for (int64_t w = 0; w < excl_words; w++) {
+            uint64_t live = ~ex[w];
+            if (x0 >> 6 == w) live |= 1ull << (x0 & 63);
+            const int32_t b = (int32_t)(w * 64), e = std::min<int32_t>(c->n_cols, b + 64);
+            if (e - b < 64) live &= (1ull << (e - b)) - 1;
+            if (live) tiles.push_back({r, b, e, j + 1});
+        }
     }
     if (tiles.empty()) {   // only empty tails: n_out == 0, handled above
         for (int32_t it : slot_rows) c->slot_of[it] = -1;
@@ -565,10 +615,12 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     const int64_t o_coff = align16(o_ioff + (int64_t)(n_rows + 1) * 4);
     const int64_t o_cids = align16(o_coff + (int64_t)(n_rows + 1) * 4);
     const bool need_cols = col_ids != nullptr || lds;
-    const int64_t o_tiles = align16(o_cids + (need_cols ? n_out * 4 : 0));
-    const int64_t total = o_tiles + (int64_t)tiles.size() * (int64_t)sizeof(WaveTile);
+    const int64_t o_tiles = align16(o_cids + (need_cols ? n_exp * 4 : 0));
+    const int64_t o_excl = align16(o_tiles + (int64_t)tiles.size() * (int64_t)sizeof(WaveTile));
+    const int64_t o_crows = align16(o_excl + (int64_t)n_compl * excl_words * 8);
+    const int64_t total = o_crows + (int64_t)n_compl * 4;
     if (int r = ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, total)) return r;
-    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, n_out)) return r;
+    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, n_dev)) return r;
     char *h = sl.h_desc;
     int32_t *hi = reinterpret_cast<int32_t *>(h + o_items);
     int32_t *hc = reinterpret_cast<int32_t *>(h + o_cids);
@@ -583,14 +635,20 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         for (int32_t it : slot_rows) c->slot_of[it] = -1;
     } else {
         memcpy(hi, item_ids, (size_t)n_item_ids * 4);
-        if (col_ids) memcpy(hc, col_ids, (size_t)n_out * 4);
+        if (col_ids) memcpy(hc, col_ids, (size_t)n_exp * 4);
     }
     memcpy(h + o_ioff, item_off, (size_t)(n_rows + 1) * 4);
     int32_t *coff = reinterpret_cast<int32_t *>(h + o_coff);
-    if (col_ids) {
+    if (csr) {
         memcpy(coff, col_off, (size_t)(n_rows + 1) * 4);
     } else {
         for (int32_t r = 0; r <= n_rows; r++) coff[r] = r * c->n_cols;
+    }
+    // a complement row's counts go to its block of the device-only region
+    for (int32_t j = 0; j < n_compl; j++) coff[compl_rows[j]] = (int32_t)(full_base + (int64_t)j * c->n_cols);
+    if (n_compl) {
+        memcpy(h + o_excl, excl, (size_t)(n_compl * excl_words * 8));
+        memcpy(h + o_crows, compl_rows, (size_t)n_compl * 4);
     }
     if (lds) {
         // the LDS kernel takes tiles resolved to prefix and output ranges
@@ -606,7 +664,7 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     } else {
         memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
     }
-    KernelLaunch K;
+    KernelLaunch K, R;
     if (lds) {
         WaveLdsDesc d;
         d.slot_rows = reinterpret_cast<const int32_t *>(sl.d_desc + o_slots);
@@ -631,6 +689,8 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         d.col_ids = col_ids ? reinterpret_cast<const int32_t *>(sl.d_desc + o_cids) : nullptr;
         d.tiles = reinterpret_cast<const WaveTile *>(sl.d_desc + o_tiles);
         d.out = sl.d_out;
+        d.excl = reinterpret_cast<const uint64_t *>(sl.d_desc + o_excl);
+        d.excl_words = (int32_t)excl_words;
         d.n_tiles = (int32_t)tiles.size();
         const int rw = c->pitch >= 2048 ? 4 : (c->pitch >= 512 ? 2 : 1);
         const int64_t cw = wave_chunk_words(rw);
@@ -640,8 +700,11 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
         blocks = std::min<int64_t>(blocks, (int64_t)c->n_cu * 8);
         blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
         plan_count_wave(c->d_bm, c->pitch, d, (int)blocks, rw, K);
+        if (n_compl)
+            plan_compl_reduce(d, reinterpret_cast<const int32_t *>(sl.d_desc + o_crows), n_compl, c->n_cols,
+                              min_count, sl.d_out + n_exp, R);
     }
-    if (int r = issue_wave(c, sl, total, n_out, K)) return r;
+    if (int r = issue_wave(c, sl, total, n_dev, n_out, K, n_compl ? &R : nullptr)) return r;
     HIPCHK(c, hipEventRecord(sl.done, c->stream));
     sl.busy = true;
     sl.ticket = c->next_ticket++;
@@ -651,6 +714,38 @@ int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t 
     sl.work = work;
     *ticket = sl.ticket;
     return TFIM_OK;
+}
+
+}  // namespace
+
+int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
+                            const int32_t *col_ids, const int32_t *col_off, int32_t path, int64_t *ticket) {
+    return wave_submit(c, item_ids, item_off, n_rows, col_ids, col_off, nullptr, 0, nullptr, 0, path, ticket);
+}
+
+int tfim_count_nodes_ex_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
+                               const int32_t *col_ids, const int32_t *col_off, const int32_t *compl_rows,
+                               int32_t n_compl, const uint64_t *excl, int32_t min_count, int32_t path,
+                               int64_t *ticket) {
+    if (!col_off || n_rows < 0) return fail(c, TFIM_E_INVALID, "count_nodes_ex: col_off is required");
+    static const int32_t no_cols = 0;
+    if (!col_ids) {
+        if (col_off[n_rows] != 0) return fail(c, TFIM_E_INVALID, "count_nodes_ex: explicit columns without col_ids");
+        col_ids = &no_cols;   // rows are CSR either way
+    }
+    return wave_submit(c, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl, excl, min_count, path,
+                       ticket);
+}
+
+int tfim_count_nodes_ex(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
+                        const int32_t *col_ids, const int32_t *col_off, const int32_t *compl_rows, int32_t n_compl,
+                        const uint64_t *excl, int32_t min_count, int32_t *out, int32_t path) {
+    if (!out) return fail(c, TFIM_E_INVALID, "count_nodes_ex: null output");
+    int64_t ticket = -1;
+    if (int r = tfim_count_nodes_ex_submit(c, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl,
+                                           excl, min_count, path, &ticket))
+        return r;
+    return tfim_count_nodes_wait(c, ticket, out);
 }
 
 int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {

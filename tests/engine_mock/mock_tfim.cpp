@@ -49,6 +49,17 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t
 static std::vector<int32_t> g_pending[3]; static int64_t g_next=0;
 int tfim_count_nodes_submit(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co, int32_t p, int64_t *t) {
   int64_t tk=g_next++; auto &v=g_pending[tk%3]; v.assign(cids?co[n]:(size_t)n*c->n_cols,0); tfim_count_nodes(c,ids,io,n,cids,co,v.data(),p); *t=tk; return 0; }
+int tfim_count_nodes_ex(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co,
+                        const int32_t *cr, int32_t nc, const uint64_t *ex, int32_t mc, int32_t *out, int32_t) {
+  for (int r=0;r<n;r++) if (co[r+1]>co[r]) cnt(c,ids+io[r],io[r+1]-io[r],cids+co[r],co[r+1]-co[r],out+co[r]);
+  const int64_t W=(c->n_cols+63)/64; std::vector<int32_t> all(c->n_cols);
+  for (int j=0;j<nc;j++){ const int r=cr[j]; cnt(c,ids+io[r],io[r+1]-io[r],nullptr,c->n_cols,all.data()); const int x0=ids[io[r]];
+    int f=0; for (int e=0;e<c->n_cols;e++) if (e!=x0 && !(ex[j*W+(e>>6)]>>(e&63)&1) && all[e]>=mc) f++;
+    out[co[n]+2*j]=all[x0]; out[co[n]+2*j+1]=f; }
+  return 0; }
+int tfim_count_nodes_ex_submit(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co,
+                               const int32_t *cr, int32_t nc, const uint64_t *ex, int32_t mc, int32_t p, int64_t *t) {
+  int64_t tk=g_next++; auto &v=g_pending[tk%3]; v.assign((size_t)co[n]+2*nc,0); tfim_count_nodes_ex(c,ids,io,n,cids,co,cr,nc,ex,mc,v.data(),p); *t=tk; return 0; }
 int tfim_count_nodes_wait(tfim_ctx *, int64_t t, int32_t *out) { if(t<0) return 0; auto &v=g_pending[t%3]; memcpy(out,v.data(),v.size()*4); return 0; }
 int tfim_wave_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *r, double *ms) { if(a)*a=0; if(b)*b=0; if(r)*r=0; if(ms)*ms=0; return 0; }
 int tfim_wave_graph_stats(tfim_ctx *, int64_t *g, int64_t *l) { if(g)*g=0; if(l)*l=0; return 0; }

@@ -17,13 +17,14 @@ from conftest import GOLD_PAIRS, gold_path
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = [("plain", []), ("wave", []), ("mafia", []), ("mafia", ["--learned"])]
+ENGINES = [("plain", []), ("wave", []), ("mafia", []), ("mafia", ["--learned"]), ("mafia", ["--compl", "on"]),
+           ("mafia", ["--compl", "off"])]
 HEADER_KEYS = ["transNum", "dimension", "itemIDmax", "dimensionReduced", "transNumReduced", "MFIsNumber",
                "transLengthMean", "transLengthReducedMean"]
 
 
 def _id(e):
-    return e[0] + ("-learned" if e[1] else "")
+    return "-".join([e[0]] + [x.strip("-") for x in e[1]])
 
 
 @pytest.mark.parametrize("engine", ENGINES, ids=_id)
@@ -62,7 +63,7 @@ def test_synthetic_configs_match_oracle(tmp_path, cfg, thr, rows):
     for name, extra in ENGINES:
         if name == "plain" and cfg == "gene":
             continue   # 19k launches; the wave engine runs the same enumeration
-        out = tmp_path / ("%s%s.res" % (name, "-l" if extra else ""))
+        out = tmp_path / ("%s.res" % _id((name, extra)))
         path, stats = tf.mine(str(data), thr, engine=name, out=str(out), extra=extra, timeout=600)
         print(cfg, name, extra, {k: stats.get(k) for k in ("MFIsNumber", "rows", "computation_time(s)")}, flush=True)
         assert tf.load_results(path) == exp, (name, extra)

@@ -26,12 +26,13 @@ def mock_engine(tmp_path_factory):
 
 
 ENGINES = [["--engine", "mafia"], ["--engine", "mafia", "--gate", "always"], ["--engine", "wave"],
-           ["--engine", "plain", "--workers", "3"]]
+           ["--engine", "plain", "--workers", "3"], ["--engine", "mafia", "--compl", "on"],
+           ["--engine", "mafia", "--gate", "always", "--compl", "on"]]
 
 
 @pytest.mark.parametrize("ds,thr", GOLD_PAIRS[:5])
 def test_engines_reproduce_golds(mock_engine, fimi_dir, tmp_path, ds, thr):
-    for eng in ENGINES[:3]:
+    for eng in ENGINES[:3] + ENGINES[4:]:
         out = tmp_path / "m.res"
         subprocess.run([mock_engine, str(fimi_dir / ds), repr(thr), "--out", str(out)] + eng, check=True,
                        capture_output=True)
