@@ -15,7 +15,9 @@ per-node count kernel's HBM fraction on two shapes (unless --no-per-node), and u
 --no-e2e the end-to-end mining time of the dense replicated workload (49,046-row base x256
 = 12,555,776 transactions, support 0.85) for the plain per-node GPU engine
 (one and eight host workers) and the batched MAFIA engine with and without the
-learned policy, with their MFI sets compared.
+learned policy, with their MFI sets compared, and the same comparison on the
+two wide workloads (clickstream: 990,000 transactions at support 0.1 %; gene:
+1,100 x 20,000 at support 0.12), checked against their golden MFI digests.
 """
 import argparse
 import json
@@ -212,6 +214,46 @@ def bench_e2e(args):
     return out
 
 
+def bench_e2e_wide(args):
+    """End to end on the two wide workloads, each checked against its golden
+    MFI digest (tests/golden/synthetic): clickstream (990,000 transactions,
+    41,270 items, support 0.1 %: 1,758 survivors) and gene (1,100 x 20,000,
+    support 0.12: 3,876 survivors), batched+learned and batched MAFIA next to
+    the plain per-node engine."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import mfi_digest
+    tmp = tempfile.mkdtemp(prefix="tfim_bench_wide_")
+    res = {}
+    for name, support, runs in (
+            ("clickstream", 0.001, [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
+                                    ("plain_w8", "plain", ["--workers", "8"])]),
+            ("gene", 0.12, [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
+                            ("plain_w8", "plain", ["--workers", "8"]), ("plain", "plain", [])])):
+        with open(os.path.join(ROOT, "tests", "golden", "synthetic", name + ".json")) as f:
+            fx = json.load(f)
+        data = os.path.join(tmp, name + ".txt")
+        subprocess.run([tf.binary("tfim_gen"), name, data], check=True)
+        out = {}
+        for run, engine, extra in runs:
+            log("%s e2e: %s ..." % (name, run))
+            t0 = time.perf_counter()
+            path, st = tf.mine(data, support, engine=engine, out=os.path.join(tmp, name + run + ".res"), extra=extra,
+                               timeout=600)
+            wall = time.perf_counter() - t0
+            got = tf.load_results(path)
+            out[run] = {"wall_s": wall, "computation_s": float(st["computation_time(s)"]), "rows": int(st["rows"]),
+                        "waves": int(st["waves"]), "mfis": len(got), "kernel_ms": float(st.get("kernel_ms", 0)),
+                        "matches_golden": mfi_digest(got) == fx["digest"]}
+            log("  %s: %.3f s computation, %d rows, golden=%s" % (run, out[run]["computation_s"], out[run]["rows"],
+                                                                  out[run]["matches_golden"]))
+            if not out[run]["matches_golden"]:
+                raise SystemExit("%s %s: MFI set differs from the golden set" % (name, run))
+        best_plain = min(out[r]["computation_s"] for r in out if r.startswith("plain"))
+        out["speedup_computation_vs_best_plain"] = best_plain / out["batched_learned"]["computation_s"]
+        res[name] = out
+    return res
+
+
 def cpu_engine_baseline(data, total_nodes, args):
     """The reference bitmap engine's logic on the host cores: the oracle's
     worker-pool DFS (kernel_bitmap.cu:853-894 semantics, 64-bit popcount
@@ -292,6 +334,7 @@ def main():
                          "(%.1f s)" % (reps, cpu_s)}
 
     e2e = None if args.no_e2e else bench_e2e(args)
+    e2e_wide = None if args.no_e2e else bench_e2e_wide(args)
 
     kernel_s = head["kernel_ms"] * 1e-3
     if head_path == "mfma":
@@ -329,6 +372,9 @@ def main():
         line["per_node_kernel"] = per_node
     if e2e is not None:
         line["e2e_dense_replicated"] = e2e
+    if e2e_wide is not None:
+        line["e2e_clickstream"] = e2e_wide["clickstream"]
+        line["e2e_gene"] = e2e_wide["gene"]
     print(json.dumps(line), flush=True)
 
 

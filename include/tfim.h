@@ -157,7 +157,13 @@ int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *i
  *                  block
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel (two
  *                  workgroups per CU), 3 LDS-staged with wide staging (one
- *                  workgroup per CU, up to 512 bits per lane per column)
+ *                  workgroup per CU, up to 512 bits per lane per column),
+ *                  4 matrix cores (prefix masks, int8-MFMA count of every
+ *                  (row, survivor) pair, gather; waves of up to 8 GiB of
+ *                  masks).  In auto mode waves outside the LDS kernel go to
+ *                  the L2 kernel or the matrix cores by a second tuner that
+ *                  predicts each one's time from its measured cost per unit
+ *                  of work
  *   "tile_cols"    columns per work tile of the batched path: 8..64, or 0
  *                  (default) to let the wave tuner choose
  *   "mfma_tile"    workgroup tile of the int8-MFMA matrix count: 0 auto
@@ -181,6 +187,12 @@ int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
 int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,
                     int64_t *rows, double *kernel_ms);
 
+/* Waves since tfim_open by kernel: the L2-resident kernel, the LDS-staged
+ * kernel and the matrix-core path (prefix masks, int8-MFMA count of every
+ * (row, survivor) pair with split-k over word ranges, gather of the wave's
+ * outputs) -- the three forms of the reference's batchCountBMMA dispatch
+ * (naive / smem / split-k, kernel_bmma.cu:772-872) on MI355X. */
+int tfim_wave_path_stats(tfim_ctx *ctx, int64_t *l2, int64_t *lds, int64_t *mfma);
 /* Batched-path graphs instantiated and graph launches since tfim_open.  The
  * reference launches each wave directly (runWaveEngine, kernel_bmma.cu:896-910);
  * no counterpart. */

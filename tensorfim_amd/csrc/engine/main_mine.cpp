@@ -3,7 +3,7 @@
 //   tfim_mine <transactionFile> <relativeSupport> [--engine plain|wave|mafia]
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
 //             [--workers N] [--host-threads N] [--learned] [--rank] [--gate indep|always]
-//             [--compl auto|on|off]
+//             [--compl auto|on|off] [--spec on|off]
 //             [--wave-kernel 0|1|2|3] [--graphs 0|1] [--parse-threads N]
 //             [--prep gpu|host] [--model PATH] [--trace PATH] [--verbose]
 //
@@ -52,6 +52,7 @@ int main(int argc, char **argv) {
         else if (a == "--learned") opt.learned = true;
         else if (a == "--gate") opt.gate = next();
         else if (a == "--compl") opt.compl_rows = next();
+        else if (a == "--spec") opt.spec = next();
         else if (a == "--workers") opt.workers = atoi(next().c_str());
         else if (a == "--host-threads") opt.host_threads = atoi(next().c_str());
         else if (a == "--rank") opt.rank = true;
@@ -84,6 +85,11 @@ int main(int argc, char **argv) {
     }
     if (wave_kernel && tfim_config(ctx, "wave_kernel", wave_kernel) != TFIM_OK) {
         fprintf(stderr, "%s\n", tfim_last_error(ctx));
+        tfim_close(ctx);
+        return 1;
+    }
+    if (opt.spec != "on" && opt.spec != "off") {
+        fprintf(stderr, "--spec must be on or off\n");
         tfim_close(ctx);
         return 1;
     }
@@ -170,8 +176,13 @@ int main(int argc, char **argv) {
     int64_t n_graphs = 0, graph_launches = 0;
     tfim_wave_graph_stats(ctx, &n_graphs, &graph_launches);
     printf("graphs: %lld\ngraph_launches: %lld\n", (long long)n_graphs, (long long)graph_launches);
+    int64_t l2_waves = 0, mfma_waves = 0;
+    tfim_wave_path_stats(ctx, &l2_waves, nullptr, &mfma_waves);
+    printf("l2_waves: %lld\nmfma_waves: %lld\n", (long long)l2_waves, (long long)mfma_waves);
     printf("expand_rows: %lld\nhut_rows: %lld\nleaf_rows: %lld\nref_rows: %lld\n", (long long)st.expand_rows,
            (long long)st.hut_rows, (long long)st.leaf_rows, (long long)st.ref_rows);
+    printf("spec_rows: %lld\nspec_hits: %lld\nspec_misses: %lld\n", (long long)st.spec_rows, (long long)st.spec_hits,
+           (long long)st.spec_misses);
     printf("hut_success: %lld\nhut_skipped: %lld\npep_moves: %lld\nhutmfi_pruned: %lld\nleaf_subsumed: %lld\n",
            (long long)st.hut_success, (long long)st.hut_skipped, (long long)st.pep_moves,
            (long long)st.hutmfi_pruned, (long long)st.leaf_subsumed);

@@ -302,13 +302,21 @@ int run_wave(tfim_ctx *ctx, const Database &db, int32_t cap, std::vector<Mfi> &o
 //                                 -> sup(HUT) in col 0; HUT maximal iff the
 //                                    other columns are all below threshold
 //   LEAF   (H)                   cols = U \ H     -> H maximal iff all below
+//   SPEC   (head, tail)          cols = U \ H     -> an EXPAND node predicted to
+//                                 be a maximal leaf (its tail expected to die):
+//                                 its maximality is checked at once; the tail's
+//                                 counts are among the columns, so the node is
+//                                 expanded from the same row when it lives on
+//                                 (complement rows carry no per-column counts:
+//                                 a SPEC row that fails is counted again as
+//                                 EXPAND)
 //   REF    (reference node)      cols = U         -> only in the subtree of a
 //                                                    frequent item 0 (quirk A.3)
 // U is the survivor set [0, F).  The reference reports the MFIs of U whose
 // largest dense id is a root (strict-threshold roots, quirk A.2), plus its
 // own enumeration under a frequent item-0 root; both are reproduced.
 struct MNode {
-    enum Kind : uint8_t { EXPAND, HUT, LEAF, REF } kind = EXPAND;
+    enum Kind : uint8_t { EXPAND, HUT, LEAF, REF, SPEC } kind = EXPAND;
     std::vector<int32_t> head;       // EXPAND/HUT/LEAF head; REF items
     int32_t sup = 0;                 // sup(head) (REF: node frequency)
     std::vector<int32_t> tail;       // EXPAND: candidates; HUT: ordered tail''
@@ -516,6 +524,7 @@ class Mafia {
         // to be cheap (the same bound as the LDS kernel's)
         compl_ = opt_.compl_rows == "on" || (opt_.compl_rows == "auto" && !lds_shape(db_));
         min_count_ = min_count_of(db_);
+        spec_ = opt_.spec != "off";
         excl_words_ = (F + 63) / 64;
         // item-0 subtree: the reference enumeration, reproduced node for node
         if (db_.d_count > F && db_.is_root[F]) {
@@ -644,6 +653,7 @@ class Mafia {
         st_.expand_rows += f.st.expand_rows;
         st_.hut_rows += f.st.hut_rows;
         st_.leaf_rows += f.st.leaf_rows;
+        st_.spec_rows += f.st.spec_rows;
         st_.ref_rows += f.st.ref_rows;
         st_.counted_bitops += f.st.counted_bitops;
         st_.count_seconds += f.st.count_seconds;
@@ -656,7 +666,8 @@ class Mafia {
         switch (n.kind) {
             case MNode::EXPAND: return (int64_t)n.tail.size();
             case MNode::HUT: return 1 + db_.n_surv - (int64_t)(n.head.size() + n.tail.size());
-            case MNode::LEAF: return db_.n_surv - (int64_t)n.head.size();
+            case MNode::LEAF:
+            case MNode::SPEC: return db_.n_surv - (int64_t)n.head.size();
             default: return db_.n_surv;
         }
     }
@@ -713,6 +724,7 @@ class Mafia {
                     break;
                 }
                 case MNode::LEAF:
+                case MNode::SPEC:
                     if (compl_) {
                         const int64_t live = exclusion(n.head, n.known, f.excl);
                         wb_.add_complement(n.head.data(), (int)n.head.size(), f.excl.data(), excl_words_, live + 1);
@@ -720,7 +732,8 @@ class Mafia {
                         complement(n.head, n.known, cols, false, f.mark);
                         wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
                     }
-                    f.st.leaf_rows++;
+                    if (n.kind == MNode::LEAF) f.st.leaf_rows++;
+                    else f.st.spec_rows++;
                     break;
                 case MNode::REF:
                     cols.resize(db_.n_surv);
@@ -781,7 +794,17 @@ class Mafia {
             c.depth = d.depth + 1;
             c.known = d.known;
             if (!d.item_records.empty()) c.item_record = d.item_records[d.order[q]];
-            for (size_t r = q + 1; r < k; r++) c.tail.push_back(d.tail[d.order[r]]);
+            int32_t tail_max = 0;   // largest sup(head + t) over the child's tail
+            for (size_t r = q + 1; r < k; r++) {
+                c.tail.push_back(d.tail[d.order[r]]);
+                tail_max = std::max(tail_max, d.tail_sup[d.order[r]]);
+            }
+            // speculate that the child is a maximal leaf when even its best
+            // tail item is expected to drop below the threshold if it were
+            // independent of the added item: sup(H+t_q) * sup(H+t) / sup(H)
+            if (spec_ && !c.tail.empty() &&
+                (double)c.sup * (double)tail_max / (double)std::max(1, d.sup) < db_.thr_abs)
+                c.kind = MNode::SPEC;
             if (c.tail.empty()) {
                 // no candidates left: the child is a leaf
                 const bool sub = d.leaf_sub >= 0 ? d.leaf_sub == 1 || index_.subsumed_from(c.head, d.leaf_n)
@@ -804,13 +827,45 @@ class Mafia {
         // (the leaf head after PEP when no other item stays frequent, else
         // its HUT); the loop below adds a check against the MFIs that rows
         // before it emit, so decisions equal a row-by-row pass.
+        // SPEC rows counted against explicit complement columns: the tail's
+        // counts read out of the complement's, how many frequent columns lie
+        // outside the tail, and the infrequent ones (known for the subtree)
+        spec_tres_.resize(wave_.size());
+        spec_other_.assign(wave_.size(), 0);
+        if (!compl_) {
+            if ((int32_t)spec_pos_.size() < db_.n_surv) spec_pos_.assign(db_.n_surv, -1);
+            for (size_t r = 0; r < wave_.size(); r++) {
+                MNode &n = wave_[r];
+                if (n.kind != MNode::SPEC) continue;
+                complement(n.head, n.known, spec_cols_, false, spec_mark_);   // the row's columns
+                const int32_t *res = wb_.result((int)r);
+                int freq_all = 0, freq_tail = 0;
+                for (size_t j = 0; j < spec_cols_.size(); j++) {
+                    spec_pos_[spec_cols_[j]] = (int32_t)j;
+                    freq_all += frequent(db_, res[j]);
+                }
+                auto &tres = spec_tres_[r];
+                tres.resize(n.tail.size());
+                for (size_t j = 0; j < n.tail.size(); j++) {
+                    tres[j] = res[spec_pos_[n.tail[j]]];
+                    freq_tail += frequent(db_, tres[j]);
+                }
+                spec_other_[r] = freq_all - freq_tail;
+                if (n.known.empty()) n.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
+                for (size_t j = 0; j < spec_cols_.size(); j++)
+                    if (!frequent(db_, res[j])) n.known[(size_t)spec_cols_[j] >> 6] |= 1ull << (spec_cols_[j] & 63);
+            }
+        }
+        auto expands = [&](const MNode &n) { return n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_); };
+        auto tail_res = [&](int r) { return wave_[r].kind == MNode::SPEC ? spec_tres_[r].data() : wb_.result(r); };
+        auto tail_n = [&](int r) { return wave_[r].kind == MNode::SPEC ? (int)spec_tres_[r].size() : wb_.ncols(r); };
         const int64_t n_start = index_.size();
         pre_sub_.assign(wave_.size(), 0);
         pool_.run((int64_t)wave_.size(), [&](int64_t r) {
             const MNode &n = wave_[r];
-            if (n.kind != MNode::EXPAND) return;
-            const int32_t *res = wb_.result((int)r);
-            const int nc = wb_.ncols((int)r);
+            if (!expands(n)) return;
+            const int32_t *res = tail_res((int)r);
+            const int nc = tail_n((int)r);
             std::vector<int32_t> q = n.head;
             for (int j = 0; j < nc; j++)
                 if (frequent(db_, res[j])) q.push_back(n.tail[j]);
@@ -823,7 +878,20 @@ class Mafia {
             MNode &n = wave_[r];
             const int32_t *res = wb_.result(r);
             const int nc = wb_.ncols(r);
-            if (n.kind == MNode::EXPAND) {
+            if (n.kind == MNode::SPEC && compl_) {
+                // complement row: res = {sup(head), frequent extensions}
+                if (res[1] == 0) {
+                    st_.spec_hits++;
+                    emit(n.head, n.sup, out);
+                } else {
+                    st_.spec_misses++;
+                    n.kind = MNode::EXPAND;   // lives on: count its tail
+                    pending_.push_back(std::move(n));
+                }
+            } else if (expands(n)) {
+                const bool spec = n.kind == MNode::SPEC;
+                res = tail_res(r);
+                const int nc = tail_n(r);
                 Decision d;
                 d.head = n.head;
                 d.sup = n.sup;
@@ -846,6 +914,15 @@ class Mafia {
                         d.tail.push_back(t);
                         d.tail_sup.push_back(s);
                     }
+                }
+                if (d.tail.empty() && spec) {
+                    // the complement's counts already decide the leaf: no
+                    // frequent column outside the (equivalent) tail items
+                    if (spec_other_[r] == 0) {
+                        st_.spec_hits++;
+                        emit(d.head, d.sup, out);
+                    }
+                    continue;
                 }
                 if (d.tail.empty()) {
                     if (subsumed_row(r, d.head)) {
@@ -989,6 +1066,9 @@ class Mafia {
     MfiIndex index_;
     HostPool pool_;
     std::vector<uint8_t> pre_sub_;
+    std::vector<std::vector<int32_t>> spec_tres_;   // SPEC rows (explicit): the tail's counts
+    std::vector<int32_t> spec_other_, spec_pos_, spec_cols_;
+    std::vector<uint8_t> spec_mark_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
     std::unordered_set<uint64_t> seen64_;
     std::vector<MNode> stack_, pending_;
@@ -998,7 +1078,8 @@ class Mafia {
     AsyncWorker helper_;
     int32_t cap_ = 256;
     int64_t budget_ = INT64_MAX;
-    bool compl_ = false;         // HUT and LEAF rows are complement rows
+    bool compl_ = false;         // HUT, LEAF and SPEC rows are complement rows
+    bool spec_ = true;           // predicted maximal leaves are checked at once (SPEC rows)
     int32_t min_count_ = 0;
     int excl_words_ = 0;
 };

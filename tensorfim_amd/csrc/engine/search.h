@@ -32,6 +32,7 @@ struct SearchOptions {
     bool learned = false;         // mafia: use the MLP ranker/gate
     std::string gate = "indep";   // mafia plain lookahead rule: indep | always
     std::string compl_rows = "auto";   // mafia: HUT/LEAF rows as device-reduced complement rows: auto | on | off
+    std::string spec = "on";      // mafia: check nodes predicted to be maximal leaves at once: on | off
     bool rank = false;            // mafia learned: reorder tails with the ranker head
     std::string model_path;       // mafia: load MLP weights (else trained online)
     std::string trace_path;       // mafia: write (features, outcome) records
@@ -42,6 +43,9 @@ struct SearchStats {
     int64_t nodes = 0;            // count rows issued (all kinds)
     int64_t waves = 0;            // count launches
     int64_t expand_rows = 0, hut_rows = 0, leaf_rows = 0, ref_rows = 0;
+    int64_t spec_rows = 0;        // rows of nodes predicted to be maximal leaves
+    int64_t spec_hits = 0;        // of those, found maximal by the same row
+    int64_t spec_misses = 0;      // complement SPEC rows that had to be counted again as EXPAND
     int64_t hut_success = 0;      // HUT lookaheads that found HUT frequent
     int64_t hut_skipped = 0;      // lookaheads the gate declined
     int64_t pep_moves = 0;        // items moved into heads by PEP

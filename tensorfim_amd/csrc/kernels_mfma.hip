@@ -313,6 +313,111 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
             }
 }
 
+int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                            int64_t pitch_words, int split_k, KernelLaunch &L) {
+    if (split_k < 1) split_k = 1;
+    int64_t kps = (pitch_words + split_k - 1) / split_k;
+    kps = (kps + 7) / 8 * 8;                                     // whole 512-bit stages
+    kps = std::min<int64_t>(kps, (int64_t)1 << 18);              // <= 2^24 bits per split
+    const int64_t zs = (pitch_words + kps - 1) / kps;
+    constexpr int RA = 256, RB = 128;
+    const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
+    L.func = (const void *)k_popmat_mfma_rp<4, 2>;
+    L.grid = dim3((unsigned)(tm * tn), (unsigned)zs);
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.n_args = L.used = 0;
+    L.arg(A);
+    L.arg(B);
+    L.arg(C);
+    L.arg(b);
+    L.arg(d);
+    L.arg(pitch_words);
+    L.arg(kps);
+    L.arg(zs > 1 ? 1 : 0);
+    L.arg((int)tn);
+    return zs;
+}
+
+// A[r] = AND of the prefix rows of wave row r, one uint4 per thread.
+__global__ __launch_bounds__(256) void k_wave_masks(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                    const int32_t *__restrict__ item_ids,
+                                                    const int32_t *__restrict__ item_off, uint4 *__restrict__ A) {
+    const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int r = blockIdx.y;
+    if (w >= pitch4) return;
+    const int ib = item_off[r], ie = item_off[r + 1];
+    uint4 m = bm[(int64_t)item_ids[ib] * pitch4 + w];
+    for (int i = ib + 1; i < ie; i++) {
+        const uint4 v = bm[(int64_t)item_ids[i] * pitch4 + w];
+        m.x &= v.x;
+        m.y &= v.y;
+        m.z &= v.z;
+        m.w &= v.w;
+    }
+    A[(int64_t)r * pitch4 + w] = m;
+}
+
+void plan_wave_masks(const uint64_t *bm, int64_t pitch_words, const int32_t *item_ids, const int32_t *item_off,
+                     int32_t n_rows, uint64_t *masks, KernelLaunch &L) {
+    const int64_t pitch4 = pitch_words / 2;
+    L.func = (const void *)k_wave_masks;
+    L.grid = dim3((unsigned)((pitch4 + 255) / 256), (unsigned)n_rows);
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.n_args = L.used = 0;
+    L.arg(reinterpret_cast<const uint4 *>(bm));
+    L.arg(pitch4);
+    L.arg(item_ids);
+    L.arg(item_off);
+    L.arg(reinterpret_cast<uint4 *>(masks));
+}
+
+// One 64-lane wave per wave row: an explicit row copies its listed columns
+// out of C, a complement row counts its live columns that reach min_count.
+__global__ __launch_bounds__(256) void k_wave_gather(WaveDesc d, const int32_t *__restrict__ C, int32_t n_rows,
+                                                     int32_t n_cols, const int32_t *__restrict__ row_compl,
+                                                     int32_t min_count, int32_t *__restrict__ res) {
+    const int lane = threadIdx.x & 63;
+    const int r = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (r >= n_rows) return;
+    const int32_t *Cr = C + (int64_t)r * n_cols;
+    const int j = row_compl[r];
+    if (j < 0) {
+        for (int q = d.col_off[r] + lane; q < d.col_off[r + 1]; q += 64) d.out[q] = Cr[d.col_ids[q]];
+        return;
+    }
+    const int x0 = d.item_ids[d.item_off[r]];
+    const uint64_t *ex = d.excl + (int64_t)j * d.excl_words;
+    int n = 0;
+    for (int w = 0; w < d.excl_words; w++) {
+        const int e = w * 64 + lane;
+        const bool live = e < n_cols && e != x0 && !((ex[w] >> lane) & 1ull);
+        if (live && Cr[e] >= min_count) n++;
+    }
+    for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+    if (lane == 0) {
+        res[2 * j] = Cr[x0];
+        res[2 * j + 1] = n;
+    }
+}
+
+void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32_t n_cols, const int32_t *row_compl,
+                      int32_t min_count, int32_t *res, KernelLaunch &L) {
+    L.func = (const void *)k_wave_gather;
+    L.grid = dim3((unsigned)std::max(1, (n_rows + 3) / 4));
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.n_args = L.used = 0;
+    L.arg(d);
+    L.arg(C);
+    L.arg(n_rows);
+    L.arg(n_cols);
+    L.arg(row_compl);
+    L.arg(min_count);
+    L.arg(res);
+}
+
 void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                            int64_t pitch_words, int split_k, hipStream_t s) {
     if (split_k < 1) split_k = 1;

@@ -24,16 +24,18 @@ struct KernelLaunch {
     const void *func = nullptr;
     dim3 grid{1, 1, 1}, block{1, 1, 1};
     uint32_t shmem = 0;
-    alignas(16) unsigned char buf[192];
-    int off[6] = {0, 0, 0, 0, 0, 0};
+    static constexpr int kMaxArgs = 12;
+    alignas(16) unsigned char buf[256];
+    int off[kMaxArgs] = {};
     int n_args = 0, used = 0;
-    void *ptrs[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *ptrs[kMaxArgs] = {};
 
     template <class T>
     void arg(const T &v) {
         const int a = (int)alignof(T) < 8 ? 8 : (int)alignof(T);
         used = (used + a - 1) / a * a;
         static_assert(sizeof(T) <= sizeof(buf), "argument too large");
+        if (n_args >= kMaxArgs || used + (int)sizeof(T) > (int)sizeof(buf)) __builtin_trap();
         __builtin_memcpy(buf + used, &v, sizeof(T));
         off[n_args++] = used;
         used += (int)sizeof(T);
@@ -117,6 +119,9 @@ struct WaveLdsDesc {
 };
 // Largest slot count / output count the LDS kernel accepts.
 constexpr int kLdsMaxSlots = 96;
+// Largest prefix-mask block of one wave on the matrix-core path (each of the
+// three wave slots keeps its own; 288 GB of HBM holds them beside the bitmaps).
+constexpr int64_t kMfmaMaskBytes = (int64_t)8 << 30;
 constexpr int kLdsMaxOut = 8192;
 // Returns false if the shape does not fit LDS (caller uses launch_count_wave).
 bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu,
@@ -146,6 +151,22 @@ void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int6
 // expanded in registers from packed LDS.  C must be zeroed when split_k>1.
 void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                            int64_t pitch_words, int split_k, hipStream_t s);
+// The same as a launch description (graph node); `atomic` must be set when
+// C is zeroed and the split is not 1.  Returns the number of word splits.
+int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                            int64_t pitch_words, int split_k, KernelLaunch &L);
+
+// Matrix-core wave path: the prefix masks of a wave's rows are built in
+// `masks` ([n_rows][pitch] u64), C = masks AND-popcount survivors is counted
+// by k_popmat_mfma_rp ([n_rows][n_cols] int32), and k_wave_gather writes the
+// wave's outputs from C: explicit rows their listed columns
+// (out[col_off[r] + j] = C[r][col_ids[col_off[r] + j]]), complement row j its
+// two results at res[2j] (as k_compl_reduce).  row_compl[r] = j or -1.
+void plan_wave_masks(const uint64_t *bm, int64_t pitch_words, const int32_t *item_ids, const int32_t *item_off,
+                     int32_t n_rows, uint64_t *masks, KernelLaunch &L);
+void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32_t n_cols, const int32_t *row_compl,
+                      int32_t min_count, int32_t *res, KernelLaunch &L);
+
 // Matrix AND-popcount, int8 MFMA path, 128x128 tiles with the operand bytes
 // expanded into LDS.  C must be zeroed when split_k>1.
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,

@@ -69,16 +69,20 @@ struct tfim_ctx {
         int64_t ticket = -1, n_out = 0, n_rows = 0;
         int tuned = -1;
         double work = 0;
-        // one executable graph per (count-kernel function, reduction): H2D
-        // descriptor copy -> output fill -> ev0 -> count kernel [-> complement
-        // reduction] -> ev1 -> D2H copy
+        int mf_arm = -1;       // second tuner's arm for this wave (-1: not tuned)
+        double mf_work = 0;
+        // one executable graph per kernel sequence: H2D descriptor copy ->
+        // fill -> ev0 -> kernels -> ev1 -> D2H copy
+        uint64_t *d_mat = nullptr;   // matrix-core path: prefix masks, then the count matrix
+        int64_t mat_cap = 0;
         struct Graph {
-            const void *func = nullptr;
-            bool reduce = false;   // complement-row reduction after the count kernel
+            static constexpr int kMaxKernels = 3;
+            const void *funcs[kMaxKernels] = {};
+            int nk = 0;
             hipGraph_t g = nullptr;
             hipGraphExec_t x = nullptr;
-            hipGraphNode_t h2d = nullptr, fill = nullptr, e0 = nullptr, kern = nullptr, red = nullptr, e1 = nullptr,
-                           d2h = nullptr;
+            hipGraphNode_t h2d = nullptr, fill = nullptr, e0 = nullptr, e1 = nullptr, d2h = nullptr;
+            hipGraphNode_t kern[kMaxKernels] = {};
         };
         std::vector<Graph> graphs;
     };
@@ -89,7 +93,12 @@ struct tfim_ctx {
 
     int64_t wave_launches = 0, wave_graph_replays = 0, wave_graphs = 0, wave_rows = 0, wave_lds_launches = 0;
     double wave_ms = 0;
-    int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel, 3 LDS kernel with wide staging
+    int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel, 3 LDS kernel with wide staging, 4 matrix cores
+    // second tuner, for waves outside the LDS kernel: arm 0 = L2 kernel, arm 1
+    // = matrix cores; EMA of kernel milliseconds per unit of each arm's work
+    double mf_cost[2] = {};
+    int64_t mf_samples[2] = {};
+    int64_t mf_calls = 0, wave_mfma_launches = 0;
     // wave tuner: EMA of kernel milliseconds per (row prefix + column) word,
     // per arm (kernel, columns per tile)
     double tune_cost[kTuneArms] = {};
@@ -200,6 +209,7 @@ void tfim_close(tfim_ctx *c) {
     for (auto &sl : c->slots) {
         hipFree(sl.d_desc); hipHostFree(sl.h_desc);
         hipFree(sl.d_out); hipHostFree(sl.h_out);
+        hipFree(sl.d_mat);
         if (sl.ev0) hipEventDestroy(sl.ev0);
         if (sl.ev1) hipEventDestroy(sl.ev1);
         if (sl.done) hipEventDestroy(sl.done);
@@ -393,45 +403,52 @@ hipKernelNodeParams node_params(KernelLaunch &L) {
     return p;
 }
 
-// Issues one wave on a slot: descriptor copy, output fill, the count kernel
-// between the slot's timing events, output copy.  With graphs on, the six
-// operations form one graph per (slot, kernel function), instantiated on
-// first use and replayed with this wave's node parameters; the slot is idle
-// (its last wave waited on) whenever it is issued again.
-int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_fill, int64_t n_out, KernelLaunch &K,
-               KernelLaunch *R) {
+// Issues one wave on a slot: descriptor copy, fill of the accumulated
+// buffer, the wave's kernels (1 to 3: count kernel, or count kernel and
+// complement reduction, or masks, matrix count and gather) between the
+// slot's timing events, output copy.  With graphs on, these form one graph
+// per (slot, kernel sequence), instantiated on first use and replayed with
+// this wave's node parameters; the slot is idle (its last wave waited on)
+// whenever it is issued again.
+int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fill, int64_t n_fill, int64_t n_out,
+               KernelLaunch *const *ks, int nk) {
     KernelLaunch Z;
-    plan_zero_i32(sl.d_out, n_fill, Z);
+    plan_zero_i32(fill, n_fill, Z);
     const size_t out_bytes = (size_t)n_out * 4;
     if (!c->use_graphs) {
         HIPCHK(c, hipMemcpyAsync(sl.d_desc, sl.h_desc, (size_t)desc_bytes, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, Z.launch(c->stream));
         HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
-        HIPCHK(c, K.launch(c->stream));
-        if (R) HIPCHK(c, R->launch(c->stream));
+        for (int i = 0; i < nk; i++) HIPCHK(c, ks[i]->launch(c->stream));
         HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
         HIPCHK(c, hipMemcpyAsync(sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost, c->stream));
         return TFIM_OK;
     }
-    hipKernelNodeParams zp = node_params(Z), kp = node_params(K), rp{};
-    if (R) rp = node_params(*R);
+    hipKernelNodeParams zp = node_params(Z), kp[tfim_ctx::Slot::Graph::kMaxKernels];
+    for (int i = 0; i < nk; i++) kp[i] = node_params(*ks[i]);
     tfim_ctx::Slot::Graph *G = nullptr;
-    for (auto &g : sl.graphs)
-        if (g.func == K.func && g.reduce == (R != nullptr)) G = &g;
+    for (auto &g : sl.graphs) {
+        bool same = g.nk == nk;
+        for (int i = 0; i < nk && same; i++) same = g.funcs[i] == ks[i]->func;
+        if (same) G = &g;
+    }
     if (!G) {
         // build and instantiate; only a complete graph is kept
         tfim_ctx::Slot::Graph g;
-        g.func = K.func;
-        g.reduce = R != nullptr;
+        g.nk = nk;
+        for (int i = 0; i < nk; i++) g.funcs[i] = ks[i]->func;
         hipError_t e = hipGraphCreate(&g.g, 0);
         if (e == hipSuccess)
             e = hipGraphAddMemcpyNode1D(&g.h2d, g.g, nullptr, 0, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
                                         hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipGraphAddKernelNode(&g.fill, g.g, &g.h2d, 1, &zp);
         if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e0, g.g, &g.fill, 1, sl.ev0);
-        if (e == hipSuccess) e = hipGraphAddKernelNode(&g.kern, g.g, &g.e0, 1, &kp);
-        if (e == hipSuccess && R) e = hipGraphAddKernelNode(&g.red, g.g, &g.kern, 1, &rp);
-        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e1, g.g, R ? &g.red : &g.kern, 1, sl.ev1);
+        hipGraphNode_t prev = g.e0;
+        for (int i = 0; i < nk && e == hipSuccess; i++) {
+            e = hipGraphAddKernelNode(&g.kern[i], g.g, &prev, 1, &kp[i]);
+            prev = g.kern[i];
+        }
+        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e1, g.g, &prev, 1, sl.ev1);
         if (e == hipSuccess)
             e = hipGraphAddMemcpyNode1D(&g.d2h, g.g, &g.e1, 1, sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost);
         if (e == hipSuccess) e = hipGraphInstantiate(&g.x, g.g, nullptr, nullptr, 0);
@@ -446,8 +463,7 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_fi
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->h2d, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
                                                     hipMemcpyHostToDevice));
         HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->fill, &zp));
-        HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->kern, &kp));
-        if (R) HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->red, &rp));
+        for (int i = 0; i < nk; i++) HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->kern[i], &kp[i]));
         HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->d2h, sl.h_out, sl.d_out, out_bytes,
                                                     hipMemcpyDeviceToHost));
     }
@@ -459,6 +475,80 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int64_t n_fi
 }  // namespace
 
 namespace {
+
+// Matrix-core path of a wave (arguments already checked by wave_submit):
+// descriptor = item ids | item_off | col_off | col ids | exclusion bitmaps |
+// row -> complement index; d_mat = prefix masks [n_rows][pitch] then the
+// count matrix [n_rows][n_cols].  Marks the slot busy with its ticket.
+int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, const int32_t *item_off,
+                     int32_t n_rows, const int32_t *col_ids, const int32_t *col_off, const int32_t *compl_rows,
+                     int32_t n_compl, const uint64_t *excl, int32_t min_count, int64_t n_exp, int64_t n_out) {
+    const int64_t n_item_ids = item_off[n_rows];
+    const int64_t excl_words = ((int64_t)c->n_cols + 63) / 64;
+    auto align16 = [](int64_t x) { return (x + 15) / 16 * 16; };
+    const int64_t o_items = 0;
+    const int64_t o_ioff = align16(o_items + n_item_ids * 4);
+    const int64_t o_coff = align16(o_ioff + (int64_t)(n_rows + 1) * 4);
+    const int64_t o_cids = align16(o_coff + (int64_t)(n_rows + 1) * 4);
+    const int64_t o_excl = align16(o_cids + n_exp * 4);
+    const int64_t o_rc = align16(o_excl + (int64_t)n_compl * excl_words * 8);
+    const int64_t total = o_rc + (int64_t)n_rows * 4;
+    if (int r = ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, total)) return r;
+    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, std::max<int64_t>(1, n_out))) return r;
+    const int64_t mask_words = (int64_t)n_rows * c->pitch;
+    const int64_t c_ints = (int64_t)n_rows * c->n_cols;
+    const int64_t c_off = (mask_words + 31) / 32 * 32;   // 256-byte aligned
+    if (int r = ensure_pair<uint64_t>(c, &sl.d_mat, nullptr, &sl.mat_cap, c_off + (c_ints + 1) / 2)) return r;
+    char *h = sl.h_desc;
+    memcpy(h + o_items, item_ids, (size_t)n_item_ids * 4);
+    memcpy(h + o_ioff, item_off, (size_t)(n_rows + 1) * 4);
+    int32_t *coff = reinterpret_cast<int32_t *>(h + o_coff);
+    int32_t *cids = reinterpret_cast<int32_t *>(h + o_cids);
+    if (col_ids) {
+        memcpy(coff, col_off, (size_t)(n_rows + 1) * 4);
+        if (n_exp) memcpy(cids, col_ids, (size_t)n_exp * 4);
+    } else {   // every row against all survivors: out = the count matrix itself
+        for (int32_t r = 0; r <= n_rows; r++) coff[r] = r * c->n_cols;
+        for (int32_t r = 0; r < n_rows; r++)
+            for (int32_t j = 0; j < c->n_cols; j++) cids[(int64_t)r * c->n_cols + j] = j;
+    }
+    if (n_compl) memcpy(h + o_excl, excl, (size_t)(n_compl * excl_words * 8));
+    int32_t *rc = reinterpret_cast<int32_t *>(h + o_rc);
+    for (int32_t r = 0; r < n_rows; r++) rc[r] = -1;
+    for (int32_t j = 0; j < n_compl; j++) rc[compl_rows[j]] = j;
+
+    uint64_t *masks = sl.d_mat;
+    int32_t *C = reinterpret_cast<int32_t *>(sl.d_mat + c_off);
+    const int32_t *d_items = reinterpret_cast<const int32_t *>(sl.d_desc + o_items);
+    const int32_t *d_ioff = reinterpret_cast<const int32_t *>(sl.d_desc + o_ioff);
+    KernelLaunch M, P, G;
+    plan_wave_masks(c->d_bm, c->pitch, d_items, d_ioff, n_rows, masks, M);
+    // split the words so that about three workgroups per CU run, keeping at
+    // least 16 stages of 512 bits per split (short rows are not split)
+    const int64_t tiles = ((n_rows + 255) / 256) * ((c->n_cols + 127) / 128);
+    const int64_t stages = c->pitch / 8;
+    const int64_t split = std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
+    const int64_t zs = plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P);
+    WaveDesc d{};
+    d.item_ids = d_items;
+    d.item_off = d_ioff;
+    d.col_off = reinterpret_cast<const int32_t *>(sl.d_desc + o_coff);
+    d.col_ids = reinterpret_cast<const int32_t *>(sl.d_desc + o_cids);
+    d.out = sl.d_out;
+    d.excl = reinterpret_cast<const uint64_t *>(sl.d_desc + o_excl);
+    d.excl_words = (int32_t)excl_words;
+    plan_wave_gather(d, C, n_rows, c->n_cols, reinterpret_cast<const int32_t *>(sl.d_desc + o_rc), min_count,
+                     sl.d_out + n_exp, G);
+    KernelLaunch *ks[3] = {&M, &P, &G};
+    if (int r = issue_wave(c, sl, total, C, zs > 1 ? c_ints : 0, n_out, ks, 3)) return r;
+    HIPCHK(c, hipEventRecord(sl.done, c->stream));
+    sl.busy = true;
+    sl.ticket = c->next_ticket++;
+    sl.n_out = n_out;
+    sl.n_rows = n_rows;
+    c->wave_mfma_launches++;
+    return TFIM_OK;
+}
 
 // Common body of tfim_count_nodes_submit and tfim_count_nodes_ex_submit.
 // Complement rows (n_compl > 0) need explicit column offsets (col_off), and
@@ -537,7 +627,8 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             for (int32_t it : rows) c->slot_of[it] = -1;
         }
     } slot_reset{c, slot_rows};
-    bool lds = c->wave_kernel != 1 && n_compl == 0 && n_out <= kLdsMaxOut;
+    bool lds = (c->wave_kernel == 0 || c->wave_kernel == 2 || c->wave_kernel == 3) && n_compl == 0 &&
+               n_out <= kLdsMaxOut;
     if (lds) {
         if ((int32_t)c->slot_of.size() < c->n_items) c->slot_of.assign(c->n_items, -1);
         auto slot = [&](int32_t item) {
@@ -557,7 +648,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
         if (!lds)
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
     }
-    if (c->wave_kernel >= 2 && !lds && n_compl == 0)
+    if ((c->wave_kernel == 2 || c->wave_kernel == 3) && !lds && n_compl == 0)
         return fail(c, TFIM_E_RANGE, "count_nodes: wave does not fit the LDS kernel (%zu items, %lld outputs)",
                     slot_rows.size(), (long long)n_out);
     // autotune over (kernel, columns per tile) for waves both kernels accept:
@@ -579,6 +670,60 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             lds = false;
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
             slot_rows.clear();
+        }
+    }
+    // Waves outside the LDS kernel go to the L2 kernel (counts only the
+    // listed pairs) or to the matrix cores (masks + int8 MFMA count of every
+    // (row, survivor) pair + gather).  wave_kernel 4 forces the second; in
+    // auto mode a second tuner predicts each arm's time as its work times its
+    // measured cost per unit (each arm tried once first, the other one
+    // re-tried every 16 waves) and takes the cheaper.
+    int mf_arm = -1;
+    double mf_work = 0;
+    if (!lds && (c->wave_kernel == 0 || c->wave_kernel == 4)) {
+        const int64_t mask_bytes = (int64_t)n_rows * c->pitch * 8;
+        // (the mask kernel's grid has one row per y index, at most 65535)
+        const bool fits = c->n_cols > 0 && n_rows <= 65535 && mask_bytes <= kMfmaMaskBytes &&
+                          (int64_t)n_rows * c->n_cols <= INT32_MAX;
+        if (c->wave_kernel == 4 && !fits)
+            return fail(c, TFIM_E_RANGE, "count_nodes: wave too large for the matrix-core path (%d rows)", n_rows);
+        if (fits) {
+            double pairs = (double)n_exp;
+            for (int32_t j = 0; j < n_compl; j++) {
+                const uint64_t *ex = excl + (int64_t)j * excl_words;
+                int64_t dead = 0;
+                for (int64_t w = 0; w < excl_words; w++) dead += __builtin_popcountll(ex[w]);
+                pairs += (double)std::max<int64_t>(1, c->n_cols - dead);
+            }
+            const double w_l2 = ((double)n_item_ids + pairs) * (double)c->pitch;
+            // padded matrix pairs, plus the mask build's traffic weighted by
+            // the ratio of int8-MFMA word-pair rate to HBM word rate (~39)
+            const double w_mf = ((double)((n_rows + 255) / 256 * 256) * (double)((c->n_cols + 127) / 128 * 128) +
+                                 39.0 * (double)(n_item_ids + n_rows)) *
+                                (double)c->pitch;
+            int arm = 1;
+            if (c->wave_kernel == 0) {
+                c->mf_calls++;
+                if (c->mf_samples[0] < 1 || c->mf_samples[1] < 1) {
+                    arm = c->mf_samples[1] < c->mf_samples[0] ? 1 : 0;
+                } else {
+                    arm = w_mf * c->mf_cost[1] < w_l2 * c->mf_cost[0] ? 1 : 0;
+                    if (c->mf_calls % 16 == 0) arm = 1 - arm;
+                }
+                mf_arm = arm;
+                mf_work = arm ? w_mf : w_l2;
+            }
+            if (arm == 1) {
+                int r = wave_submit_mfma(c, sl, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl,
+                                         excl, min_count, n_exp, n_out);
+                if (r) return r;
+                sl.tuned = -1;
+                sl.work = 0;
+                sl.mf_arm = mf_arm;
+                sl.mf_work = mf_work;
+                *ticket = sl.ticket;
+                return TFIM_OK;
+            }
         }
     }
     const int tile_cols = c->tile_cols ? c->tile_cols : (tuned >= 0 ? kArmTile[tuned] : 32);
@@ -704,7 +849,8 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             plan_compl_reduce(d, reinterpret_cast<const int32_t *>(sl.d_desc + o_crows), n_compl, c->n_cols,
                               min_count, sl.d_out + n_exp, R);
     }
-    if (int r = issue_wave(c, sl, total, n_dev, n_out, K, n_compl ? &R : nullptr)) return r;
+    KernelLaunch *ks[2] = {&K, &R};
+    if (int r = issue_wave(c, sl, total, sl.d_out, n_dev, n_out, ks, n_compl ? 2 : 1)) return r;
     HIPCHK(c, hipEventRecord(sl.done, c->stream));
     sl.busy = true;
     sl.ticket = c->next_ticket++;
@@ -712,6 +858,8 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     sl.n_rows = n_rows;
     sl.tuned = tuned;
     sl.work = work;
+    sl.mf_arm = mf_arm;
+    sl.mf_work = mf_work;
     *ticket = sl.ticket;
     return TFIM_OK;
 }
@@ -781,6 +929,12 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     c->wave_ms += ms;
     c->wave_launches++;
     c->wave_rows += sl.n_rows;
+    if (sl.mf_arm >= 0 && sl.mf_work > 0) {
+        const double per = (double)ms / sl.mf_work;
+        double &cst = c->mf_cost[sl.mf_arm];
+        cst = c->mf_samples[sl.mf_arm] == 0 ? per : 0.8 * cst + 0.2 * per;
+        c->mf_samples[sl.mf_arm]++;
+    }
     if (sl.tuned >= 0 && sl.work > 0) {
         const double per = (double)ms / sl.work;
         double &cst = c->tune_cost[sl.tuned];
@@ -801,7 +955,7 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
 int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
     if (!c || !key) return TFIM_E_INVALID;
     const std::string k(key);
-    if (k == "wave_kernel" && value >= 0 && value <= 3) {
+    if (k == "wave_kernel" && value >= 0 && value <= 4) {
         c->wave_kernel = (int)value;
         return TFIM_OK;
     }
@@ -833,6 +987,15 @@ int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *lds_launches, int64
     if (launches) *launches = c->wave_launches;
     if (rows) *rows = c->wave_rows;
     if (ms) *ms = c->wave_ms;
+    return TFIM_OK;
+}
+
+int tfim_wave_path_stats(tfim_ctx *c, int64_t *l2, int64_t *lds, int64_t *mfma) {
+    if (!c) return TFIM_E_INVALID;
+    std::lock_guard<std::mutex> g(c->wave_mu);
+    if (l2) *l2 = c->wave_launches - c->wave_lds_launches - c->wave_mfma_launches;
+    if (lds) *lds = c->wave_lds_launches;
+    if (mfma) *mfma = c->wave_mfma_launches;
     return TFIM_OK;
 }
 

@@ -29,5 +29,13 @@ def fimi_dir(tmp_path_factory):
     return d
 
 
+def mfi_digest(results):
+    """SHA-256 of a {frozenset(items): frequency} MFI set: one line
+    "sorted ids:frequency" per MFI, lines sorted."""
+    import hashlib
+    lines = sorted(" ".join(map(str, sorted(s))) + ":" + str(f) for s, f in results.items())
+    return hashlib.sha256("\n".join(lines).encode()).hexdigest()
+
+
 def gold_path(ds, thr):
     return os.path.join(FIMI, "%s-%f=Results.txt" % (ds, thr))

@@ -27,7 +27,8 @@ def mock_engine(tmp_path_factory):
 
 ENGINES = [["--engine", "mafia"], ["--engine", "mafia", "--gate", "always"], ["--engine", "wave"],
            ["--engine", "plain", "--workers", "3"], ["--engine", "mafia", "--compl", "on"],
-           ["--engine", "mafia", "--gate", "always", "--compl", "on"]]
+           ["--engine", "mafia", "--gate", "always", "--compl", "on"], ["--engine", "mafia", "--spec", "off"],
+           ["--engine", "mafia", "--compl", "on", "--spec", "off"]]
 
 
 @pytest.mark.parametrize("ds,thr", GOLD_PAIRS[:5])

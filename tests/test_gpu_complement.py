@@ -61,12 +61,23 @@ def make_wave(rng, n_items, n_cols, n_rows, p_compl):
     return prefixes, tails, excl
 
 
+@pytest.mark.parametrize("kernel", [1, 4])
 @pytest.mark.parametrize("n_bits,n_items,n_cols", [(70, 9, 8), (3200, 131, 130), (2048 * 5 + 17, 200, 199),
                                                    (300_017, 150, 128), (1_000_003, 70, 65)])
-def test_count_nodes_ex_matches_oracle(ctx, n_bits, n_items, n_cols):
+def test_count_nodes_ex_matches_oracle(ctx, n_bits, n_items, n_cols, kernel):
+    """Kernel 1: L2 kernel + complement reduction; kernel 4: matrix cores
+    (masks, int8-MFMA count of every pair, gather and reduction)."""
     rng = np.random.default_rng(n_bits + n_items)
     rows = rand_rows(rng, n_items, n_bits, 0.6)
     ctx.load_bitmaps(rows, n_bits, n_cols)
+    ctx.config("wave_kernel", kernel)
+    try:
+        _check_ex_waves(ctx, rng, rows, n_bits, n_items, n_cols)
+    finally:
+        ctx.config("wave_kernel", 0)
+
+
+def _check_ex_waves(ctx, rng, rows, n_bits, n_items, n_cols):
     for p_compl in (1.0, 0.5):
         prefixes, tails, excl = make_wave(rng, n_items, n_cols, 40, p_compl)
         sups = [int(oracle_lib.node_counts(rows, p, n_cols)[p[0]]) if t is None else 0
@@ -87,7 +98,8 @@ def test_count_nodes_ex_matches_oracle(ctx, n_bits, n_items, n_cols):
 @pytest.mark.parametrize("graphs", [1, 0])
 def test_complement_waves_replayed(ctx, graphs):
     """Waves of changing size alternate with and without complement rows (two
-    graph variants per slot), with eager checks between them."""
+    graph variants per slot), with eager checks between them; the automatic
+    choice between the L2 kernel and the matrix cores tries both."""
     rng = np.random.default_rng(11 + graphs)
     n_bits, n_items, n_cols = 40_000, 300, 300
     rows = rand_rows(rng, n_items, n_bits, 0.7)
@@ -106,6 +118,9 @@ def test_complement_waves_replayed(ctx, graphs):
                 exp = expected(rows, n_cols, p, ex, min_count) if t is None else \
                     oracle_lib.node_counts(rows, p, n_cols)[t]
                 np.testing.assert_array_equal(got[i], exp, err_msg="wave %d row %d" % (it, i))
+        st = ctx.wave_stats()
+        print("paths: l2 %d mfma %d" % (st["l2_launches"], st["mfma_launches"]), flush=True)
+        assert st["mfma_launches"] > 0 and st["l2_launches"] > 0
     finally:
         ctx.config("graphs", 1)
 

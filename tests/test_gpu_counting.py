@@ -165,11 +165,12 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
 
 
 @pytest.mark.parametrize("graphs", [1, 0])
-@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
 @pytest.mark.parametrize("n_bits,n_items", [(1100, 30), (3200, 75), (400_003, 90), (2_000_000, 38)])
 def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
-    """Both batched kernels (1 = L2-resident, 2 = LDS-staged, 3 = LDS-staged
-    with wide staging) on the same waves,
+    """Every batched path (1 = L2-resident kernel, 2 = LDS-staged, 3 =
+    LDS-staged with wide staging, 4 = matrix cores: masks, int8-MFMA count of
+    all (row, survivor) pairs, gather) on the same waves,
     replayed from graphs or issued directly: uneven prefixes, uneven and empty
     tails, full-column rows."""
     rng = np.random.default_rng(kernel * 100 + n_items)

@@ -1,6 +1,8 @@
 """CPU tests: the oracle pinned against the reference's own golden data and
 known answers, plus closed-form checks a plausible mistake would fail."""
 import itertools
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -109,3 +111,21 @@ def test_bernoulli_rng_density_and_tail():
     assert abs(bits[:100_003].mean() - 0.3) < 0.01
     other = oracle_lib.bernoulli_row(1, 1, 5, 100_003, 1563, thr32)
     assert (row != other).any()
+
+
+@pytest.mark.parametrize("name", ["dense-small", "gene"])
+def test_synthetic_fixture_reproduces(tmp_path, name):
+    """The committed golden digests of the small synthetic workloads are what
+    the generator and the oracle give now (the larger ones take minutes and
+    are checked on the GPU against the engines)."""
+    import json
+    from conftest import mfi_digest
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "synthetic", name + ".json")) as f:
+        fx = json.load(f)
+    data = tmp_path / (name + ".txt")
+    subprocess.run([tf.binary("tfim_gen"), fx["workload"], str(data)], check=True)
+    res = tmp_path / "o.res"
+    oracle_lib.mine(data, fx["support"], threads=4, out=res)
+    got = {s: f * fx["replicate"] for s, f in tf.load_results(str(res)).items()}
+    assert len(got) == fx["n_mfi"]
+    assert mfi_digest(got) == fx["digest"]
