@@ -313,6 +313,8 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
             }
 }
 
+int rp_rows_per_tile(int64_t b) { return b <= 64 ? 64 : b <= 128 ? 128 : 256; }
+
 int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                             int64_t pitch_words, int split_k, KernelLaunch &L) {
     if (split_k < 1) split_k = 1;
@@ -320,9 +322,12 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
     kps = (kps + 7) / 8 * 8;                                     // whole 512-bit stages
     kps = std::min<int64_t>(kps, (int64_t)1 << 18);              // <= 2^24 bits per split
     const int64_t zs = (pitch_words + kps - 1) / kps;
-    constexpr int RA = 256, RB = 128;
+    // row tile by the row count: 64, 128 or 256 rows x 128 columns (waves of
+    // the engine often have few rows; a 256-row tile would count padding)
+    const int RA = rp_rows_per_tile(b), RB = 128;
     const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
-    L.func = (const void *)k_popmat_mfma_rp<4, 2>;
+    L.func = RA == 64 ? (const void *)k_popmat_mfma_rp<1, 2>
+                      : RA == 128 ? (const void *)k_popmat_mfma_rp<2, 2> : (const void *)k_popmat_mfma_rp<4, 2>;
     L.grid = dim3((unsigned)(tm * tn), (unsigned)zs);
     L.block = dim3(256);
     L.shmem = 0;
@@ -420,17 +425,10 @@ void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32
 
 void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                            int64_t pitch_words, int split_k, hipStream_t s) {
-    if (split_k < 1) split_k = 1;
-    int64_t kps = (pitch_words + split_k - 1) / split_k;
-    kps = (kps + 7) / 8 * 8;                                     // whole 512-bit stages
-    kps = std::min<int64_t>(kps, (int64_t)1 << 18);              // <= 2^24 bits per split
-    const int64_t zs = (pitch_words + kps - 1) / kps;
-    if (zs > 1 && split_k == 1) launch_zero_i32(C, b * d, s);    // the caller zeroes C only for split_k > 1
-    constexpr int RA = 256, RB = 128;
-    const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
-    dim3 grid((unsigned)(tm * tn), (unsigned)zs);
-    hipLaunchKernelGGL((k_popmat_mfma_rp<4, 2>), grid, dim3(256), 0, s, A, B, C, b, d, pitch_words, kps,
-                       zs > 1 ? 1 : 0, (int)tn);
+    KernelLaunch L;
+    const int64_t zs = plan_popmat_mfma_rp(A, B, C, b, d, pitch_words, split_k, L);
+    if (zs > 1 && split_k <= 1) launch_zero_i32(C, b * d, s);   // the caller zeroes C only for split_k > 1
+    L.launch(s);
 }
 
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,

@@ -151,10 +151,12 @@ void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int6
 // expanded in registers from packed LDS.  C must be zeroed when split_k>1.
 void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                            int64_t pitch_words, int split_k, hipStream_t s);
-// The same as a launch description (graph node); `atomic` must be set when
-// C is zeroed and the split is not 1.  Returns the number of word splits.
+// The same as a launch description (graph node), with a row tile of 64, 128
+// or 256 rows (rp_rows_per_tile(b)) x 128 columns; C must be zeroed when the
+// returned number of word splits is above 1.
 int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                             int64_t pitch_words, int split_k, KernelLaunch &L);
+int rp_rows_per_tile(int64_t b);
 
 // Matrix-core wave path: the prefix masks of a wave's rows are built in
 // `masks` ([n_rows][pitch] u64), C = masks AND-popcount survivors is counted

@@ -525,7 +525,8 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     plan_wave_masks(c->d_bm, c->pitch, d_items, d_ioff, n_rows, masks, M);
     // split the words so that about three workgroups per CU run, keeping at
     // least 16 stages of 512 bits per split (short rows are not split)
-    const int64_t tiles = ((n_rows + 255) / 256) * ((c->n_cols + 127) / 128);
+    const int ra = rp_rows_per_tile(n_rows);
+    const int64_t tiles = ((n_rows + ra - 1) / ra) * ((c->n_cols + 127) / 128);
     const int64_t stages = c->pitch / 8;
     const int64_t split = std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
     const int64_t zs = plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P);
@@ -698,7 +699,8 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             const double w_l2 = ((double)n_item_ids + pairs) * (double)c->pitch;
             // padded matrix pairs, plus the mask build's traffic weighted by
             // the ratio of int8-MFMA word-pair rate to HBM word rate (~39)
-            const double w_mf = ((double)((n_rows + 255) / 256 * 256) * (double)((c->n_cols + 127) / 128 * 128) +
+            const int ra = rp_rows_per_tile(n_rows);
+            const double w_mf = ((double)((n_rows + ra - 1) / ra * ra) * (double)((c->n_cols + 127) / 128 * 128) +
                                  39.0 * (double)(n_item_ids + n_rows)) *
                                 (double)c->pitch;
             int arm = 1;
