@@ -94,11 +94,14 @@ class LearnedPolicy : public Policy {
             // lookaheads (their outcomes keep the labels balanced) and the
             // ranker's nodes
             bool need = !trained_ || (opt_.rank && d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail);
-            if (!need) need = d->lookahead ? rand01() < kRecordAccepted : d->tail.size() <= kMaxGateTail;
+            // the model is asked about a share ask_p_ of the declined nodes
+            // (all of them while its additions pay off)
+            const bool ask = !d->lookahead && d->tail.size() <= kMaxGateTail && (ask_p_ >= 1.0f || rand01() < ask_p_);
+            if (!need) need = d->lookahead ? rand01() < kRecordAccepted : ask;
             if (!need) continue;
             node_features(*d);
             node_row_[q] = (int64_t)(feats_.size() / NF) - 1;
-            if (use_model && !d->lookahead && d->tail.size() <= kMaxGateTail) {
+            if (use_model && ask) {
                 infer_row_[q] = (int64_t)(infer_.size() / NF);
                 infer_.insert(infer_.end(), feats_.end() - NF, feats_.end());
             }
@@ -142,10 +145,14 @@ class LearnedPolicy : public Policy {
             const bool rule = d->lookahead;
             if (!trained_) {
                 // warm-up: the rule's lookaheads plus a sample of the
-                // declined bounded tails, so that both outcomes get labelled
-                d->lookahead = rule || (d->tail.size() <= kMaxGateTail && rand01() < kWarmSample);
+                // declined bounded tails, so that both outcomes get labelled;
+                // the sample share calibrates itself like the exploration
+                // share (it shrinks where hardly any sampled lookahead succeeds)
+                const bool warm = !rule && d->tail.size() <= kMaxGateTail && rand01() < warm_p_;
+                d->lookahead = rule || warm;
+                if (warm) gate_src_ = kSrcWarm;
             } else if (!rule && d->tail.size() <= kMaxGateTail) {
-                if (deferred) {
+                if (deferred && infer_row_[q] >= 0) {
                     d->pending = true;   // settled by resolve() from the queued inference
                     continue;
                 }
@@ -346,14 +353,29 @@ class LearnedPolicy : public Policy {
     // kExploreMinHits in a window) the share halves, down to kExploreMin;
     // where many do it doubles, up to kExplore.
     void calibrate(uint8_t src, bool success) {
+        if (src == kSrcWarm) {
+            warm_taken_++;
+            warm_success_ += success;
+            if (warm_taken_ < kGateWindow) return;
+            if (warm_success_ < kExploreMinHits) warm_p_ = std::max(kExploreMin, warm_p_ * 0.5f);
+            warm_taken_ = warm_success_ = 0;
+            return;
+        }
         if (src == kSrcModel) {
             gate_taken_++;
             gate_success_ += success;
             model_success_ += success;
             if (gate_taken_ < kGateWindow) return;
             const double prec = (double)gate_success_ / (double)gate_taken_;
-            if (prec < kGateLow) gate_p_ = std::min(kGateMax, gate_p_ + 0.1f);
-            else if (prec > kGateHigh) gate_p_ = std::max(kGateP, gate_p_ - 0.05f);
+            if (prec < kGateLow) {
+                // far below: jump; at the ceiling already: ask the model about
+                // fewer nodes (its features and inference cost host time)
+                if (gate_p_ >= kGateMax) ask_p_ = std::max(kAskMin, ask_p_ * 0.5f);
+                gate_p_ = std::min(kGateMax, gate_p_ + (prec < kGateLow / 4 ? 0.2f : 0.1f));
+            } else if (prec > kGateHigh) {
+                gate_p_ = std::max(kGateP, gate_p_ - 0.05f);
+                ask_p_ = std::min(1.0f, ask_p_ * 2.0f);
+            }
             gate_taken_ = gate_success_ = 0;
             return;
         }
@@ -414,7 +436,8 @@ class LearnedPolicy : public Policy {
     static constexpr float kWarmSample = 0.25f;    // declined tails looked ahead before the first training
     static constexpr float kExploreMin = 0.0025f;  // floor of the self-calibrated exploration share
     static constexpr int64_t kExploreMinHits = 2;  // per window, below which exploration halves
-    static constexpr uint8_t kSrcRule = 0, kSrcModel = 1, kSrcExplore = 2;
+    static constexpr uint8_t kSrcRule = 0, kSrcModel = 1, kSrcExplore = 2, kSrcWarm = 3;
+    static constexpr float kAskMin = 1.0f / 16;     // floor of the share of declined nodes the model is asked about
     static constexpr float kRecordAccepted = 0.25f;  // share of the rule's lookaheads kept as training records
 
     tfim_ctx *ctx_;
@@ -434,6 +457,9 @@ class LearnedPolicy : public Policy {
     float gate_p_ = kGateP;
     int64_t gate_taken_ = 0, gate_success_ = 0;
     float explore_p_ = kExplore;
+    float warm_p_ = kWarmSample;     // warm-up sample share (self-calibrated)
+    int64_t warm_taken_ = 0, warm_success_ = 0;
+    float ask_p_ = 1.0f;             // share of declined nodes the model is asked about
     int64_t expl_taken_ = 0, expl_success_ = 0;
     uint8_t gate_src_ = kSrcRule;
     double extra_seconds_ = 0, forward_seconds_ = 0;
