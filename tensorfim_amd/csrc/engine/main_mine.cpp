@@ -178,7 +178,8 @@ int main(int argc, char **argv) {
     printf("graphs: %lld\ngraph_launches: %lld\n", (long long)n_graphs, (long long)graph_launches);
     int64_t l2_waves = 0, mfma_waves = 0;
     tfim_wave_path_stats(ctx, &l2_waves, nullptr, &mfma_waves);
-    printf("l2_waves: %lld\nmfma_waves: %lld\n", (long long)l2_waves, (long long)mfma_waves);
+    printf("l2_waves: %lld\nmfma_waves: %lld\nwave_size: %d\n", (long long)l2_waves, (long long)mfma_waves,
+           st.wave_size);
     printf("expand_rows: %lld\nhut_rows: %lld\nleaf_rows: %lld\nref_rows: %lld\n", (long long)st.expand_rows,
            (long long)st.hut_rows, (long long)st.leaf_rows, (long long)st.ref_rows);
     printf("spec_rows: %lld\nspec_hits: %lld\nspec_misses: %lld\n", (long long)st.spec_rows, (long long)st.spec_hits,

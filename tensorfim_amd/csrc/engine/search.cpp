@@ -5,6 +5,7 @@
 #include <cmath>
 #include <chrono>
 #include <functional>
+#include <memory>
 #include <condition_variable>
 #include <mutex>
 #include <cstdio>
@@ -236,7 +237,10 @@ int run_plain(tfim_ctx *ctx, const Database &db, int workers, std::vector<Mfi> &
 // Waves whose rows touch few distinct items fit the LDS-staged kernel when
 // their total output stays within its accumulator budget; such waves are
 // capped by outputs.  Otherwise a wave's column traffic is kept near 4 GiB of
-// bitmap words so it amortises the launch while the DFS frontier stays small.
+// bitmap words so it amortises the launch while the DFS frontier stays small,
+// with at least 128 rows (the matrix-core path counts 64-, 128- or 256-row
+// tiles; a sweep on the clickstream workload put its best cap at 128).  The
+// MAFIA engine tunes around this base (WaveSizeTuner).
 constexpr int64_t kLdsOutBudget = 8192;   // = the library's LDS accumulator limit
 constexpr int32_t kLdsItemBudget = 90;    // survivors (+ item 0) that fit its slots
 
@@ -246,8 +250,69 @@ int32_t auto_wave_cap(const Database &db) {
     if (lds_shape(db)) return 2048;
     const double row_bytes = (double)db.n_bits() / 8.0 * std::max(1, db.n_surv);
     const double cap = 4.0 * (1ull << 30) / std::max(1.0, row_bytes);
-    return (int32_t)std::min(2048.0, std::max(64.0, cap));
+    return (int32_t)std::min(2048.0, std::max(128.0, cap));
 }
+
+// Online choice of the MAFIA engine's wave size among base/2, base and
+// 2*base rows: each arm runs a window of kWindow waves in turn (after the
+// first kSkip waves, while the stack fills), then the arm with the lowest
+// wall time per counted row runs, another one being tried every fourth
+// window.  The time of a wave is the interval between the ends of two
+// consecutive wave processings, so it covers the device and the host sides
+// of the pipeline.  Wave size changes which rows are counted together and
+// how much pruning later rows see, never the output.
+class WaveSizeTuner {
+   public:
+    explicit WaveSizeTuner(int32_t base) {
+        caps_[0] = std::max(1, base / 2);
+        caps_[1] = base;
+        caps_[2] = base * 2;
+    }
+    // the arm for the next wave to pack
+    int next() {
+        if (skip_ < kSkip) return 1;
+        if (in_window_ >= kWindow) {
+            in_window_ = 0;
+            windows_++;
+            if (windows_ < kArms) {
+                arm_ = windows_;
+            } else {
+                int best = 0;
+                for (int a = 1; a < kArms; a++)
+                    if (cost_[a] < cost_[best]) best = a;
+                arm_ = windows_ % 4 == 0 ? (best + 1 + (windows_ / 4) % (kArms - 1)) % kArms : best;
+            }
+        }
+        in_window_++;
+        return arm_;
+    }
+    int32_t cap(int arm) const { return caps_[arm]; }
+    // a processed wave of `rows` rows packed with `arm`, `dt` seconds since
+    // the previous wave's processing ended
+    void record(int arm, int64_t rows, double dt) {
+        if (skip_ < kSkip) {
+            skip_++;
+            return;
+        }
+        if (arm < 0 || rows <= 0) return;
+        const double per = dt / (double)rows;
+        cost_[arm] = n_[arm] == 0 ? per : 0.7 * cost_[arm] + 0.3 * per;
+        n_[arm]++;
+    }
+    int32_t chosen() const {
+        int best = 1;
+        for (int a = 0; a < kArms; a++)
+            if (n_[a] && cost_[a] < cost_[best]) best = a;
+        return caps_[best];
+    }
+
+   private:
+    static constexpr int kArms = 3, kWindow = 4, kSkip = 6;
+    int32_t caps_[kArms];
+    double cost_[kArms] = {1e30, 1e30, 1e30};
+    int64_t n_[kArms] = {};
+    int arm_ = 0, in_window_ = 0, windows_ = 0, skip_ = 0;
+};
 
 int64_t out_budget(const Database &db) { return lds_shape(db) ? kLdsOutBudget : INT64_MAX; }
 
@@ -519,6 +584,7 @@ class Mafia {
     int run(std::vector<Mfi> &out, std::string &err) {
         const int32_t F = db_.n_surv;
         cap_ = opt_.wave_cap > 0 ? opt_.wave_cap : auto_wave_cap(db_);
+        if (opt_.wave_cap <= 0) sizer_.reset(new WaveSizeTuner(cap_));
         // maximality rows (HUT, LEAF) as complement rows reduced on the
         // device when the survivors are too many for listing them per row
         // to be cheap (the same bound as the LDS kernel's)
@@ -578,7 +644,10 @@ class Mafia {
             if (int rc = f.wb.wait(ctx_, st_, err)) return rc;
             const double tp = now();
             if (int rc = process_wave(f, out)) return rc;
-            st_.process_seconds += now() - tp;
+            const double te = now();
+            st_.process_seconds += te - tp;
+            if (sizer_ && last_end_ > 0) sizer_->record(f.cap_arm, (int64_t)f.wave.size(), te - last_end_);
+            last_end_ = te;
             if (join.on) {
                 helper_.join();
                 join.on = false;
@@ -593,6 +662,7 @@ class Mafia {
             if (!fl_[cur].live)
                 if (int rc = launch(fl_[cur], false, err)) return rc;
         }
+        st_.wave_size = sizer_ ? sizer_->chosen() : cap_;
         return TFIM_OK;
     }
 
@@ -606,6 +676,7 @@ class Mafia {
         std::vector<uint8_t> mark;
         std::vector<int32_t> cols;
         std::vector<uint64_t> excl;
+        int cap_arm = -1;   // wave-size tuner arm it was packed with
         SearchStats st;
         std::string err;
         int rc = 0;
@@ -617,8 +688,10 @@ class Mafia {
         if (stack_.empty()) settle_undecided();   // nothing else left to count
         const double t0 = now();
         f.wave.clear();
+        f.cap_arm = sizer_ ? sizer_->next() : -1;
+        const int32_t cap = sizer_ ? sizer_->cap(f.cap_arm) : cap_;
         int64_t outs = 0;
-        while ((int)f.wave.size() < cap_ && !stack_.empty()) {
+        while ((int)f.wave.size() < cap && !stack_.empty()) {
             const int64_t o = row_outputs(stack_.back());
             if (!f.wave.empty() && outs + o > budget_) break;
             outs += o;
@@ -1078,6 +1151,8 @@ class Mafia {
     AsyncWorker helper_;
     int32_t cap_ = 256;
     int64_t budget_ = INT64_MAX;
+    std::unique_ptr<WaveSizeTuner> sizer_;   // wave size (unless --wave-cap is given)
+    double last_end_ = 0;
     bool compl_ = false;         // HUT, LEAF and SPEC rows are complement rows
     bool spec_ = true;           // predicted maximal leaves are checked at once (SPEC rows)
     int32_t min_count_ = 0;

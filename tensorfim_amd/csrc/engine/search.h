@@ -59,6 +59,7 @@ struct SearchStats {
     double build_seconds = 0;     // host time popping nodes into waves and issuing them (includes submit)
     double kernel_ms = 0;         // device time of count kernels (HIP events)
     int64_t lds_waves = 0;        // waves counted by the LDS-staged kernel
+    int32_t wave_size = 0;        // mafia: rows per wave the wave-size tuner settled on (or the fixed cap)
     double learn_seconds = 0;     // host time in the learned part
     int64_t mlp_train_steps = 0;
     int64_t model_lookaheads = 0;  // lookaheads run because the model (not the rule) asked

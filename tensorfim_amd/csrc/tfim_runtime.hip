@@ -71,6 +71,8 @@ struct tfim_ctx {
         double work = 0;
         int mf_arm = -1;       // second tuner's arm for this wave (-1: not tuned)
         double mf_work = 0;
+        int sk_arm = -1;       // third tuner's arm (matrix-core waves)
+        double sk_work = 0;
         // one executable graph per kernel sequence: H2D descriptor copy ->
         // fill -> ev0 -> kernels -> ev1 -> D2H copy
         uint64_t *d_mat = nullptr;   // matrix-core path: prefix masks, then the count matrix
@@ -99,6 +101,11 @@ struct tfim_ctx {
     double mf_cost[2] = {};
     int64_t mf_samples[2] = {};
     int64_t mf_calls = 0, wave_mfma_launches = 0;
+    // third tuner: the matrix-core path's word split, as 1/2, 1 or 2 times
+    // the occupancy rule's split; EMA of milliseconds per unit of work
+    double sk_cost[3] = {};
+    int64_t sk_samples[3] = {};
+    int64_t sk_calls = 0;
     // wave tuner: EMA of kernel milliseconds per (row prefix + column) word,
     // per arm (kernel, columns per tile)
     double tune_cost[kTuneArms] = {};
@@ -524,11 +531,27 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     KernelLaunch M, P, G;
     plan_wave_masks(c->d_bm, c->pitch, d_items, d_ioff, n_rows, masks, M);
     // split the words so that about three workgroups per CU run, keeping at
-    // least 16 stages of 512 bits per split (short rows are not split)
+    // least 16 stages of 512 bits per split (short rows are not split); the
+    // third tuner scales that split by 1/2, 1 or 2 (each tried three times,
+    // then the cheapest per unit of work, another re-tried every 16 waves)
     const int ra = rp_rows_per_tile(n_rows);
     const int64_t tiles = ((n_rows + ra - 1) / ra) * ((c->n_cols + 127) / 128);
     const int64_t stages = c->pitch / 8;
-    const int64_t split = std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
+    const int64_t base = std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
+    int sk_arm = -1;
+    int64_t split = base;
+    if (base > 1) {
+        c->sk_calls++;
+        for (int a = 0; a < 3 && sk_arm < 0; a++)
+            if (c->sk_samples[a] < 3) sk_arm = a;
+        if (sk_arm < 0) {
+            int best = 0;
+            for (int a = 1; a < 3; a++)
+                if (c->sk_cost[a] < c->sk_cost[best]) best = a;
+            sk_arm = c->sk_calls % 16 == 0 ? (best + 1 + (int)((c->sk_calls / 16) % 2)) % 3 : best;
+        }
+        split = sk_arm == 0 ? std::max<int64_t>(1, base / 2) : sk_arm == 2 ? std::min<int64_t>(stages, base * 2) : base;
+    }
     const int64_t zs = plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P);
     WaveDesc d{};
     d.item_ids = d_items;
@@ -547,6 +570,8 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
     sl.n_rows = n_rows;
+    sl.sk_arm = sk_arm;
+    sl.sk_work = (double)((n_rows + ra - 1) / ra * ra) * (double)((c->n_cols + 127) / 128 * 128) * (double)c->pitch;
     c->wave_mfma_launches++;
     return TFIM_OK;
 }
@@ -862,6 +887,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     sl.work = work;
     sl.mf_arm = mf_arm;
     sl.mf_work = mf_work;
+    sl.sk_arm = -1;
     *ticket = sl.ticket;
     return TFIM_OK;
 }
@@ -931,6 +957,12 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     c->wave_ms += ms;
     c->wave_launches++;
     c->wave_rows += sl.n_rows;
+    if (sl.sk_arm >= 0 && sl.sk_work > 0) {
+        const double per = (double)ms / sl.sk_work;
+        double &cst = c->sk_cost[sl.sk_arm];
+        cst = c->sk_samples[sl.sk_arm] == 0 ? per : 0.8 * cst + 0.2 * per;
+        c->sk_samples[sl.sk_arm]++;
+    }
     if (sl.mf_arm >= 0 && sl.mf_work > 0) {
         const double per = (double)ms / sl.mf_work;
         double &cst = c->mf_cost[sl.mf_arm];
