@@ -93,7 +93,12 @@ class LearnedPolicy : public Policy {
             // afterwards the model's candidates, a sample of the rule's own
             // lookaheads (their outcomes keep the labels balanced) and the
             // ranker's nodes
-            bool need = !trained_ || (opt_.rank && d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail);
+            // ranker rows: every ranked node's items; without --rank a
+            // sample of nodes still records them, so head 1 keeps learning
+            // (and small runs collect enough labels for training rounds)
+            const bool items = d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail &&
+                               (opt_.rank || rand01() < kRankRecord);
+            bool need = !trained_ || items;
             // the model is asked about a share ask_p_ of the declined nodes
             // (all of them while its additions pay off)
             const bool ask = !d->lookahead && d->tail.size() <= kMaxGateTail && (ask_p_ >= 1.0f || rand01() < ask_p_);
@@ -105,13 +110,13 @@ class LearnedPolicy : public Policy {
                 infer_row_[q] = (int64_t)(infer_.size() / NF);
                 infer_.insert(infer_.end(), feats_.end() - NF, feats_.end());
             }
-            if (opt_.rank && d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail) {
+            if (items) {
                 // rank of each item in the plain (increasing support) order
                 rank_.assign(d->tail.size(), 0);
                 for (size_t j = 0; j < d->order.size(); j++) rank_[d->order[j]] = (int32_t)j;
                 item_row0_[q] = (int64_t)(feats_.size() / NF);
                 for (size_t i = 0; i < d->tail.size(); i++) item_features(*d, i);
-                if (use_model) {
+                if (use_model && opt_.rank) {
                     const int64_t first = (int64_t)(infer_.size() / NF);
                     infer_.insert(infer_.end(), feats_.end() - (ptrdiff_t)(d->tail.size() * NF), feats_.end());
                     item_infer0_[q] = first;
@@ -343,7 +348,11 @@ class LearnedPolicy : public Policy {
         bm_[b * 2] = head == 0 ? 1.0f : 0.0f;
         bm_[b * 2 + 1] = head == 1 ? 1.0f : 0.0f;
         n_labelled_++;
-        if (n_labelled_ - last_train_ >= (last_train_ == 0 ? kFirstTrain : kTrainEvery)) train();
+        // training rounds after 256, 1,024 and 4,096 labels (small runs get
+        // a trained network), then every kTrainEvery new labels
+        const int64_t due = last_train_ == 0 ? kFirstTrain : last_train_ < kTrainEvery / 2 ? last_train_ * 3
+                                                                                          : kTrainEvery;
+        if (n_labelled_ - last_train_ >= due) train();
     }
     // Self-calibration from the outcomes of the lookaheads the rule declined.
     // Model-added: over each window of kGateWindow, a success share below
@@ -421,7 +430,7 @@ class LearnedPolicy : public Policy {
     }
 
     static constexpr int64_t kTrainEvery = 8192;
-    static constexpr int64_t kFirstTrain = 1024;
+    static constexpr int64_t kFirstTrain = 256;
     static constexpr int64_t kBatch = 8192;
     static constexpr int64_t kHalf = kBatch / 2;
     static constexpr int64_t kRing = 1 << 16;   // labels arrive within a few waves
@@ -439,6 +448,7 @@ class LearnedPolicy : public Policy {
     static constexpr uint8_t kSrcRule = 0, kSrcModel = 1, kSrcExplore = 2, kSrcWarm = 3;
     static constexpr float kAskMin = 1.0f / 16;     // floor of the share of declined nodes the model is asked about
     static constexpr float kRecordAccepted = 0.25f;  // share of the rule's lookaheads kept as training records
+    static constexpr float kRankRecord = 0.125f;     // without --rank: share of nodes whose items are recorded
 
     tfim_ctx *ctx_;
     const SearchOptions &opt_;

@@ -955,6 +955,8 @@ class Mafia {
                 // complement row: res = {sup(head), frequent extensions}
                 if (res[1] == 0) {
                     st_.spec_hits++;
+                    // no survivor extends it: its whole tail dropped out
+                    if (n.item_record >= 0) pol_.observe_value(n.item_record, 0.0f);
                     emit(n.head, n.sup, out);
                 } else {
                     st_.spec_misses++;

@@ -47,3 +47,25 @@ def test_learned_gate_gain_dense_replicated(tmp_path):
     assert int(sl["hut_success"]) >= 0.9 * int(sc["hut_success"])
     # ... and counts fewer rows in total
     assert int(sl["rows"]) < int(sc["rows"])
+
+
+def test_learned_policy_trains_on_gene(tmp_path):
+    """The wide gene-expression-like workload (1,100 x 20,000, support 0.12)
+    has few lookahead decisions and none of them succeeds: the network must
+    still be trained online (gate labels from the warm-up sample, ranker
+    labels from sampled nodes' children), and the output must equal the
+    search without it."""
+    data = tmp_path / "gene.txt"
+    subprocess.run([tf.binary("tfim_gen"), "gene", str(data)], check=True)
+    res = {}
+    for name, extra in (("learned", ["--learned"]), ("indep", []), ("always", ["--gate", "always"])):
+        out = str(tmp_path / (name + ".res"))
+        path, st = tf.mine(str(data), 0.12, engine="mafia", out=out, extra=extra, timeout=600)
+        print(name, {k: st.get(k) for k in ("MFIsNumber", "rows", "hut_rows", "hut_success", "mlp_train_steps",
+                                            "model_lookaheads", "computation_time(s)")}, flush=True)
+        res[name] = (tf.load_results(path), st)
+    assert res["learned"][0] == res["indep"][0] == res["always"][0]
+    assert int(res["learned"][1]["mlp_train_steps"]) > 0, "the network was never trained"
+    # the trained gate looks ahead far less than the classical rule, whose
+    # lookaheads all fail here
+    assert int(res["learned"][1]["hut_rows"]) * 2 <= int(res["always"][1]["hut_rows"])
