@@ -183,7 +183,11 @@ def bench_e2e(args):
     subprocess.run([tf.binary("tfim_gen"), "dense-replicated", data], check=True)
     out = {}
     sets = {}
+    # batched+learned and batched MAFIA; the reference TensorFIM engine's
+    # enumeration in waves, autotuned and with every wave on the matrix cores
+    # (its int8-MFMA split-k form); the plain per-node engine
     runs = [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
+            ("wave_autotuned", "wave", []), ("wave_mfma_splitk", "wave", ["--wave-kernel", "4"]),
             ("plain_w8", "plain", ["--workers", "8"]), ("plain", "plain", [])]
     for name, engine, extra in runs:
         log("dense replicated e2e: %s ..." % name)
@@ -192,7 +196,8 @@ def bench_e2e(args):
         out[name] = {"wall_s": wall, "computation_s": float(st["computation_time(s)"]),
                      "rows": int(st["rows"]), "waves": int(st["waves"]), "mfis": int(st["MFIsNumber"]),
                      "kernel_ms": float(st.get("kernel_ms", 0)), "counted_bitops": float(st["counted_bitops"])}
-        for k in ("hut_rows", "hut_success", "model_lookaheads", "mlp_train_steps", "graph_launches"):
+        for k in ("hut_rows", "hut_success", "model_lookaheads", "mlp_train_steps", "graph_launches", "lds_waves",
+                  "l2_waves", "mfma_waves", "wave_size", "spec_hits"):
             if k in st:
                 out[name][k] = int(st[k])
         if out[name]["kernel_ms"] > 0:
@@ -203,14 +208,18 @@ def bench_e2e(args):
         log("  %s: %.2f s wall, %.3f s computation, %d rows" % (name, wall, out[name]["computation_s"],
                                                                 out[name]["rows"]))
     out["identical_mfi_sets"] = all(sets[n] == sets["plain"] for n, _, _ in runs)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import mfi_digest
+    with open(os.path.join(ROOT, "tests", "golden", "synthetic", "dense-replicated.json")) as f:
+        out["matches_golden"] = mfi_digest(sets["batched_learned"]) == json.load(f)["digest"]
     if not args.no_cpu:
         out["cpu_engine"] = cpu_engine_baseline(data, out["plain"]["rows"], args)
     best_plain = min(out["plain"]["computation_s"], out["plain_w8"]["computation_s"])
     out["speedup_computation_vs_plain_1worker"] = out["plain"]["computation_s"] / out["batched_learned"]["computation_s"]
     out["speedup_computation_vs_best_plain"] = best_plain / out["batched_learned"]["computation_s"]
     out["speedup_wall_vs_plain_1worker"] = out["plain"]["wall_s"] / out["batched_learned"]["wall_s"]
-    if not out["identical_mfi_sets"]:
-        raise SystemExit("a batched MFI set differs from the plain engine")
+    if not out["identical_mfi_sets"] or not out["matches_golden"]:
+        raise SystemExit("a batched MFI set differs from the plain engine or from the golden set")
     return out
 
 
@@ -243,7 +252,14 @@ def bench_e2e_wide(args):
             got = tf.load_results(path)
             out[run] = {"wall_s": wall, "computation_s": float(st["computation_time(s)"]), "rows": int(st["rows"]),
                         "waves": int(st["waves"]), "mfis": len(got), "kernel_ms": float(st.get("kernel_ms", 0)),
+                        "counted_bitops": float(st["counted_bitops"]),
                         "matches_golden": mfi_digest(got) == fx["digest"]}
+            for k in ("l2_waves", "mfma_waves", "wave_size", "mlp_train_steps"):
+                if k in st:
+                    out[run][k] = int(st[k])
+            if out[run]["kernel_ms"] > 0:
+                # listed (row, column) pairs' bit-ops over the count kernels' time
+                out[run]["count_kernels_tbitops"] = out[run]["counted_bitops"] / (out[run]["kernel_ms"] * 1e-3) / 1e12
             log("  %s: %.3f s computation, %d rows, golden=%s" % (run, out[run]["computation_s"], out[run]["rows"],
                                                                   out[run]["matches_golden"]))
             if not out[run]["matches_golden"]:

@@ -394,6 +394,17 @@ struct MNode {
     // stay infrequent for every superset of head, so HUT and LEAF rows skip
     // them (a pruning of columns only; decisions are unchanged)
     std::vector<uint64_t> known;
+
+    // back to a fresh node, keeping the vectors' storage (node recycling)
+    void reset() {
+        kind = EXPAND;
+        head.clear();
+        tail.clear();
+        tail_sup.clear();
+        known.clear();
+        sup = progress = depth = 0;
+        decision = item_record = -1;
+    }
 };
 
 // Found MFIs, sliced by item: bits_[i] has bit m set when MFI m contains
@@ -687,6 +698,8 @@ class Mafia {
     int launch(Flight &f, bool async, std::string &err) {
         if (stack_.empty()) settle_undecided();   // nothing else left to count
         const double t0 = now();
+        if (free_.size() < kMaxFree)
+            for (auto &n : f.wave) free_.push_back(std::move(n));
         f.wave.clear();
         f.cap_arm = sizer_ ? sizer_->next() : -1;
         const int32_t cap = sizer_ ? sizer_->cap(f.cap_arm) : cap_;
@@ -757,6 +770,40 @@ class Mafia {
         if (lead) cols.push_back(x[0]);
         for (int32_t i = 0; i < db_.n_surv; i++)
             if (!mark[i]) cols.push_back(i);
+    }
+
+    // A SPEC row counted against explicit complement columns (narrow
+    // databases): reads the tail's counts out of the complement's into
+    // spec_tres_[r], counts the frequent columns outside the tail into
+    // spec_other_[r] and adds the infrequent ones to the node's known set.
+    // Touches only row r's data (runs on the host pool).
+    void spec_prepare(int r, MNode &n, const int32_t *res) {
+        const int W = (db_.n_surv + 63) / 64;
+        thread_local std::vector<uint64_t> live;
+        thread_local std::vector<int32_t> cum;
+        live.assign((size_t)W, ~0ull);
+        cum.assign((size_t)W + 1, 0);
+        if (db_.n_surv % 64) live[(size_t)W - 1] = (1ull << (db_.n_surv % 64)) - 1;
+        for (size_t w = 0; w < n.known.size() && w < (size_t)W; w++) live[w] &= ~n.known[w];
+        for (int32_t i : n.head) live[(size_t)i >> 6] &= ~(1ull << (i & 63));
+        for (int w = 0; w < W; w++) cum[(size_t)w + 1] = cum[(size_t)w] + __builtin_popcountll(live[(size_t)w]);
+        auto &tres = spec_tres_[(size_t)r];
+        tres.resize(n.tail.size());
+        int freq_tail = 0;
+        for (size_t j = 0; j < n.tail.size(); j++) {
+            const int32_t t = n.tail[j];
+            const uint64_t below = live[(size_t)t >> 6] & ((1ull << (t & 63)) - 1);
+            tres[j] = res[cum[(size_t)t >> 6] + __builtin_popcountll(below)];
+            freq_tail += ok_(tres[j]);
+        }
+        if (n.known.empty()) n.known.assign((size_t)W, 0ull);
+        int freq_all = 0, j = 0;
+        for (int w = 0; w < W; w++)
+            for (uint64_t b = live[(size_t)w]; b; b &= b - 1, j++) {
+                if (ok_(res[j])) freq_all++;
+                else n.known[(size_t)w] |= b & (~b + 1);
+            }
+        spec_other_[(size_t)r] = freq_all - freq_tail;
     }
 
     // Exclusion bitmap of a complement row: x and the survivors known
@@ -859,13 +906,13 @@ class Mafia {
         // first child (largest tail) is counted first
         const size_t k = d.order.size();
         for (size_t q = k; q-- > 0;) {
-            MNode c;
+            MNode c = take_node();
             c.kind = MNode::EXPAND;
-            c.head = d.head;
+            c.head.assign(d.head.begin(), d.head.end());
             c.head.push_back(d.tail[d.order[q]]);
             c.sup = d.tail_sup[d.order[q]];
             c.depth = d.depth + 1;
-            c.known = d.known;
+            c.known.assign(d.known.begin(), d.known.end());
             if (!d.item_records.empty()) c.item_record = d.item_records[d.order[q]];
             int32_t tail_max = 0;   // largest sup(head + t) over the child's tail
             for (size_t r = q + 1; r < k; r++) {
@@ -905,45 +952,28 @@ class Mafia {
         // outside the tail, and the infrequent ones (known for the subtree)
         spec_tres_.resize(wave_.size());
         spec_other_.assign(wave_.size(), 0);
-        if (!compl_) {
-            if ((int32_t)spec_pos_.size() < db_.n_surv) spec_pos_.assign(db_.n_surv, -1);
-            for (size_t r = 0; r < wave_.size(); r++) {
-                MNode &n = wave_[r];
-                if (n.kind != MNode::SPEC) continue;
-                complement(n.head, n.known, spec_cols_, false, spec_mark_);   // the row's columns
-                const int32_t *res = wb_.result((int)r);
-                int freq_all = 0, freq_tail = 0;
-                for (size_t j = 0; j < spec_cols_.size(); j++) {
-                    spec_pos_[spec_cols_[j]] = (int32_t)j;
-                    freq_all += frequent(db_, res[j]);
-                }
-                auto &tres = spec_tres_[r];
-                tres.resize(n.tail.size());
-                for (size_t j = 0; j < n.tail.size(); j++) {
-                    tres[j] = res[spec_pos_[n.tail[j]]];
-                    freq_tail += frequent(db_, tres[j]);
-                }
-                spec_other_[r] = freq_all - freq_tail;
-                if (n.known.empty()) n.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
-                for (size_t j = 0; j < spec_cols_.size(); j++)
-                    if (!frequent(db_, res[j])) n.known[(size_t)spec_cols_[j] >> 6] |= 1ull << (spec_cols_[j] & 63);
-            }
-        }
+        const int64_t n_start = index_.size();
+        pre_sub_.assign(wave_.size(), 0);
+        // Per row, in parallel: the SPEC preparation above (the row's columns
+        // are the survivors outside head and known, ascending, so a column's
+        // index is a prefix popcount of that live mask), then the
+        // subsumption check of the rows that expand.
+        pool_.run((int64_t)wave_.size(), [&](int64_t r) {
+            MNode &n = wave_[r];
+            if (n.kind == MNode::SPEC && !compl_) spec_prepare((int)r, n, wb_.result((int)r));
+            if (!(n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_))) return;
+            const bool spec = n.kind == MNode::SPEC;
+            const int32_t *res = spec ? spec_tres_[r].data() : wb_.result((int)r);
+            const int nc = spec ? (int)spec_tres_[r].size() : wb_.ncols((int)r);
+            thread_local std::vector<int32_t> q;
+            q.assign(n.head.begin(), n.head.end());
+            for (int j = 0; j < nc; j++)
+                if (ok_(res[j])) q.push_back(n.tail[j]);
+            pre_sub_[r] = index_.subsumed(q);
+        });
         auto expands = [&](const MNode &n) { return n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_); };
         auto tail_res = [&](int r) { return wave_[r].kind == MNode::SPEC ? spec_tres_[r].data() : wb_.result(r); };
         auto tail_n = [&](int r) { return wave_[r].kind == MNode::SPEC ? (int)spec_tres_[r].size() : wb_.ncols(r); };
-        const int64_t n_start = index_.size();
-        pre_sub_.assign(wave_.size(), 0);
-        pool_.run((int64_t)wave_.size(), [&](int64_t r) {
-            const MNode &n = wave_[r];
-            if (!expands(n)) return;
-            const int32_t *res = tail_res((int)r);
-            const int nc = tail_n((int)r);
-            std::vector<int32_t> q = n.head;
-            for (int j = 0; j < nc; j++)
-                if (frequent(db_, res[j])) q.push_back(n.tail[j]);
-            pre_sub_[r] = index_.subsumed(q);
-        });
         auto subsumed_row = [&](int r, const std::vector<int32_t> &x) {
             return pre_sub_[r] || index_.subsumed_from(x, n_start);
         };
@@ -968,17 +998,17 @@ class Mafia {
                 res = tail_res(r);
                 const int nc = tail_n(r);
                 Decision d;
-                d.head = n.head;
+                d.head = std::move(n.head);   // the row's node is spent after this
                 d.sup = n.sup;
                 d.depth = n.depth;
                 d.known = std::move(n.known);
                 if (d.known.empty()) d.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
                 int kept = 0;
-                for (int j = 0; j < nc; j++) kept += frequent(db_, res[j]);
+                for (int j = 0; j < nc; j++) kept += ok_(res[j]);
                 if (n.item_record >= 0) pol_.observe_value(n.item_record, nc > 0 ? (float)kept / nc : 0.0f);
                 for (int j = 0; j < nc; j++) {
                     const int32_t t = n.tail[j], s = res[j];
-                    if (!frequent(db_, s)) {
+                    if (!ok_(s)) {
                         d.known[(size_t)t >> 6] |= 1ull << (t & 63);
                         continue;
                     }
@@ -1003,16 +1033,17 @@ class Mafia {
                     if (subsumed_row(r, d.head)) {
                         st_.leaf_subsumed++;
                     } else {
-                        MNode leaf;
+                        MNode leaf = take_node();
                         leaf.kind = MNode::LEAF;
-                        leaf.head = d.head;
+                        leaf.head = std::move(d.head);
                         leaf.sup = d.sup;
                         leaf.known = std::move(d.known);
                         pending_.push_back(std::move(leaf));
                     }
                     continue;
                 }
-                std::vector<int32_t> hut = d.head;
+                std::vector<int32_t> &hut = hut_scratch_;
+                hut.assign(d.head.begin(), d.head.end());
                 hut.insert(hut.end(), d.tail.begin(), d.tail.end());
                 if (subsumed_row(r, hut)) {
                     st_.hutmfi_pruned++;
@@ -1022,13 +1053,13 @@ class Mafia {
             } else if (n.kind == MNode::HUT) {
                 // explicit: res = {sup(HUT), counts of the complement};
                 // complement row: res = {sup(HUT), frequent extensions}
-                const bool hut_freq = frequent(db_, res[0]);
+                const bool hut_freq = ok_(res[0]);
                 pol_.observe(n.decision, hut_freq);
                 if (hut_freq) {
                     st_.hut_success++;
                     bool maximal = true;
                     if (compl_) maximal = res[1] == 0;
-                    for (int j = 1; j < nc && maximal; j++) maximal = !frequent(db_, res[j]);
+                    for (int j = 1; j < nc && maximal; j++) maximal = !ok_(res[j]);
                     std::vector<int32_t> hut = n.head;
                     hut.insert(hut.end(), n.tail.begin(), n.tail.end());
                     if (maximal) emit(hut, res[0], out);
@@ -1047,16 +1078,16 @@ class Mafia {
             } else if (n.kind == MNode::LEAF) {
                 bool maximal = true;
                 if (compl_) maximal = res[1] == 0;
-                for (int j = 0; j < nc && maximal; j++) maximal = !frequent(db_, res[j]);
+                for (int j = 0; j < nc && maximal; j++) maximal = !ok_(res[j]);
                 if (maximal) emit(n.head, n.sup, out);
             } else {   // REF: reference rule, own output path
                 int32_t cnt = 0;
-                for (int j = 0; j < nc; j++) cnt += frequent(db_, res[j]);
+                for (int j = 0; j < nc; j++) cnt += ok_(res[j]);
                 if (cnt == (int32_t)n.head.size()) {
                     out.push_back({n.head, n.sup});
                 } else {
                     for (int32_t i = 0; i <= n.progress - 1 && i < db_.n_surv; i++) {
-                        if (!frequent(db_, res[i])) continue;
+                        if (!ok_(res[i])) continue;
                         if (std::find(n.head.begin(), n.head.end(), i) != n.head.end()) continue;
                         MNode c;
                         c.kind = MNode::REF;
@@ -1142,17 +1173,29 @@ class Mafia {
     HostPool pool_;
     std::vector<uint8_t> pre_sub_;
     std::vector<std::vector<int32_t>> spec_tres_;   // SPEC rows (explicit): the tail's counts
-    std::vector<int32_t> spec_other_, spec_pos_, spec_cols_;
-    std::vector<uint8_t> spec_mark_;
+    std::vector<int32_t> spec_other_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
     std::unordered_set<uint64_t> seen64_;
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;
     static constexpr int kFlights = 3;
+    static constexpr size_t kMaxFree = 1 << 15;
     Flight fl_[kFlights];
     AsyncWorker helper_;
     int32_t cap_ = 256;
     int64_t budget_ = INT64_MAX;
+    // frequent(db, s) as an integer compare: s >= ceil(thr_abs)
+    bool ok_(int32_t s) const { return s >= min_count_; }
+    // spent nodes whose vectors are reused for new ones
+    std::vector<MNode> free_;
+    std::vector<int32_t> hut_scratch_;
+    MNode take_node() {
+        if (free_.empty()) return MNode();
+        MNode n = std::move(free_.back());
+        free_.pop_back();
+        n.reset();
+        return n;
+    }
     std::unique_ptr<WaveSizeTuner> sizer_;   // wave size (unless --wave-cap is given)
     double last_end_ = 0;
     bool compl_ = false;         // HUT, LEAF and SPEC rows are complement rows
