@@ -212,32 +212,46 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
     const int64_t row0 = (lin / tiles_n) * RA, col0 = (lin % tiles_n) * RB;
     const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
     const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
-    const uint4 z4 = make_uint4(0, 0, 0, 0);
 
-    // loader: load j of thread t is uint4 part (idx & 3) of row (idx >> 2), idx = 256 j + t
-    uint4 ra[LA], rb[LB];
+    // loader: load j of thread t is uint4 part (idx & 3) of row (idx >> 2), idx = 256 j + t.
+    // Rows past the edge load the last row and zero the value: a load that is
+    // always from the global buffer stays a global load (a choice between the
+    // buffer and a zero constant would make it a flat load through a private
+    // copy of the constant, and waits on LDS traffic would then also wait for
+    // the loads in flight).
+    // The edge mask is applied when the stage is written to LDS, so nothing
+    // reads the loaded registers before then and the loads stay in flight
+    // during the MFMAs of the current stage.
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(1))) u4v *gptr4;
+    u4v ra[LA], rb[LB];
+    uint32_t ma[LA], mb[LB];
+#pragma unroll
+    for (int j = 0; j < LA; j++) ma[j] = row0 + ((j * 256 + tid) >> 2) < b ? ~0u : 0u;
+#pragma unroll
+    for (int j = 0; j < LB; j++) mb[j] = col0 + ((j * 256 + tid) >> 2) < d ? ~0u : 0u;
     auto gload = [&](int64_t kw) {
 #pragma unroll
         for (int j = 0; j < LA; j++) {
             const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
-            ra[j] = row0 + r < b ? *reinterpret_cast<const uint4 *>(A + (row0 + r) * pitch + kw + 2 * part) : z4;
+            ra[j] = *(gptr4)(A + std::min<int64_t>(row0 + r, b - 1) * pitch + kw + 2 * part);
         }
 #pragma unroll
         for (int j = 0; j < LB; j++) {
             const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
-            rb[j] = col0 + r < d ? *reinterpret_cast<const uint4 *>(B + (col0 + r) * pitch + kw + 2 * part) : z4;
+            rb[j] = *(gptr4)(B + std::min<int64_t>(col0 + r, d - 1) * pitch + kw + 2 * part);
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
         for (int j = 0; j < LA; j++) {
             const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
-            *reinterpret_cast<uint4 *>(&As[buf][r * RP_STRIDE + 4 * part]) = ra[j];
+            *reinterpret_cast<u4v *>(&As[buf][r * RP_STRIDE + 4 * part]) = ra[j] & ma[j];
         }
 #pragma unroll
         for (int j = 0; j < LB; j++) {
             const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
-            *reinterpret_cast<uint4 *>(&Bs[buf][r * RP_STRIDE + 4 * part]) = rb[j];
+            *reinterpret_cast<u4v *>(&Bs[buf][r * RP_STRIDE + 4 * part]) = rb[j] & mb[j];
         }
     };
 
@@ -293,6 +307,10 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
                         acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[i], eb[j], acc[i][j], 0, 0, 0);
             }
         }
+        // keep the stores of the next stage after this stage's LDS reads, so
+        // its global loads stay in flight during the MFMAs instead of being
+        // waited on at the top of the loop
+        asm volatile("" ::: "memory");
         if (more) lstore(buf ^ 1);
         __syncthreads();
         buf ^= 1;
