@@ -50,9 +50,9 @@ PACKED_TRAFFIC_BYTES = 141876016.75 * 1024.0
 # reading for device traffic of known size (`bench.py --copy-calibration 1024`:
 # WRITE_SIZE exact; FETCH_SIZE exact for 64-byte reads, the pattern of this
 # kernel); profiles/popmat_mfma_rp_65536x512x4M_traffic.md holds the readings.
-# Read 37,393,180 KiB + written 23,137,734 KiB (mean over dispatches).
+# Read 44,790,465 KiB + written 1,048,576 KiB (mean over dispatches).
 TRAFFIC_KERNEL = "k_popmat_mfma_rp"
-TRAFFIC_BYTES = (37393180 + 23137734) * 1024.0
+TRAFFIC_BYTES = (44790465 + 1048576) * 1024.0
 
 
 def log(*a):
