@@ -396,8 +396,9 @@ void plan_wave_masks(const uint64_t *bm, int64_t pitch_words, const int32_t *ite
     L.arg(reinterpret_cast<uint4 *>(masks));
 }
 
-// One 64-lane wave per wave row: an explicit row copies its listed columns
-// out of C, a complement row counts its live columns that reach min_count.
+// One 64-lane wave per wave row and slice: an explicit row copies its listed
+// columns out of C (slice y of gridDim.y takes every gridDim.y-th group of 64),
+// a complement row counts its live columns that reach min_count (slice 0).
 __global__ __launch_bounds__(256) void k_wave_gather(WaveDesc d, const int32_t *__restrict__ C, int32_t n_rows,
                                                      int32_t n_cols, const int32_t *__restrict__ row_compl,
                                                      int32_t min_count, int32_t *__restrict__ res) {
@@ -407,9 +408,11 @@ __global__ __launch_bounds__(256) void k_wave_gather(WaveDesc d, const int32_t *
     const int32_t *Cr = C + (int64_t)r * n_cols;
     const int j = row_compl[r];
     if (j < 0) {
-        for (int q = d.col_off[r] + lane; q < d.col_off[r + 1]; q += 64) d.out[q] = Cr[d.col_ids[q]];
+        for (int q = d.col_off[r] + (int)blockIdx.y * 64 + lane; q < d.col_off[r + 1]; q += 64 * (int)gridDim.y)
+            d.out[q] = Cr[d.col_ids[q]];
         return;
     }
+    if (blockIdx.y != 0) return;
     const int x0 = d.item_ids[d.item_off[r]];
     const uint64_t *ex = d.excl + (int64_t)j * d.excl_words;
     int n = 0;
@@ -426,9 +429,11 @@ __global__ __launch_bounds__(256) void k_wave_gather(WaveDesc d, const int32_t *
 }
 
 void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32_t n_cols, const int32_t *row_compl,
-                      int32_t min_count, int32_t *res, KernelLaunch &L) {
+                      int32_t min_count, int32_t *res, int32_t max_row_cols, KernelLaunch &L) {
     L.func = (const void *)k_wave_gather;
-    L.grid = dim3((unsigned)std::max(1, (n_rows + 3) / 4));
+    // slices per row: about 256 explicit outputs each, at most 32
+    const int slices = std::max(1, std::min(32, (max_row_cols + 255) / 256));
+    L.grid = dim3((unsigned)std::max(1, (n_rows + 3) / 4), (unsigned)slices);
     L.block = dim3(256);
     L.shmem = 0;
     L.n_args = L.used = 0;

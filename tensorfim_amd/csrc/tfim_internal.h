@@ -167,7 +167,7 @@ int rp_rows_per_tile(int64_t b);
 void plan_wave_masks(const uint64_t *bm, int64_t pitch_words, const int32_t *item_ids, const int32_t *item_off,
                      int32_t n_rows, uint64_t *masks, KernelLaunch &L);
 void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32_t n_cols, const int32_t *row_compl,
-                      int32_t min_count, int32_t *res, KernelLaunch &L);
+                      int32_t min_count, int32_t *res, int32_t max_row_cols, KernelLaunch &L);
 
 // Matrix AND-popcount, int8 MFMA path, 128x128 tiles with the operand bytes
 // expanded into LDS.  C must be zeroed when split_k>1.

@@ -561,8 +561,10 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     d.out = sl.d_out;
     d.excl = reinterpret_cast<const uint64_t *>(sl.d_desc + o_excl);
     d.excl_words = (int32_t)excl_words;
+    int32_t max_row_cols = 0;
+    for (int32_t r = 0; r < n_rows; r++) max_row_cols = std::max(max_row_cols, coff[r + 1] - coff[r]);
     plan_wave_gather(d, C, n_rows, c->n_cols, reinterpret_cast<const int32_t *>(sl.d_desc + o_rc), min_count,
-                     sl.d_out + n_exp, G);
+                     sl.d_out + n_exp, max_row_cols, G);
     KernelLaunch *ks[3] = {&M, &P, &G};
     if (int r = issue_wave(c, sl, total, C, zs > 1 ? c_ints : 0, n_out, ks, 3)) return r;
     HIPCHK(c, hipEventRecord(sl.done, c->stream));
