@@ -266,29 +266,40 @@ def bench_e2e_wide(args):
                 raise SystemExit("%s %s: MFI set differs from the golden set" % (name, run))
         best_plain = min(out[r]["computation_s"] for r in out if r.startswith("plain"))
         out["speedup_computation_vs_best_plain"] = best_plain / out["batched_learned"]["computation_s"]
+        if not args.no_cpu:
+            # gene's whole DFS (19,044 nodes) fits the sample; clickstream's
+            # (94,602 nodes over 1,758 survivors of 989,270 bits) is sampled
+            out["cpu_engine"] = cpu_engine_baseline(data, out["plain_w8"]["rows"], args, support=support, replicate=1,
+                                                    first=20000 if name == "gene" else 150)
         res[name] = out
     return res
 
 
-def cpu_engine_baseline(data, total_nodes, args):
+def cpu_engine_baseline(data, total_nodes, args, support=0.85, replicate=256, first=4000):
     """The reference bitmap engine's logic on the host cores: the oracle's
     worker-pool DFS (kernel_bitmap.cu:853-894 semantics, 64-bit popcount
-    counter) on the same replicated table, timed over its first M nodes at 1
-    thread and at all usable threads, extrapolated to the full node count of
-    the plain GPU run (same enumeration)."""
+    counter) on the same table, timed over its first `first` nodes at 1
+    thread and `first` x T nodes at all usable threads T, extrapolated to the
+    full node count of the plain GPU run (same enumeration); a run that ends
+    within its sample is its full time."""
     exe = os.path.join(ROOT, "oracle", "_build", "tfim_oracle")
     res = []
     nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    for threads, nodes in ((1, 4000), (nthreads, 4000 * nthreads)):
+    for threads, nodes in ((1, first), (nthreads, first * nthreads)):
         log("cpu engine baseline: %d threads, first %d DFS nodes ..." % (threads, nodes))
-        r = subprocess.run([exe, data, "0.85", "--replicate", "256", "--threads", str(threads), "--max-nodes",
-                            str(nodes), "--out", data + ".cpu.res"], capture_output=True, text=True, check=True)
+        cmd = [exe, data, repr(float(support)), "--threads", str(threads), "--max-nodes", str(nodes), "--out",
+               data + ".cpu.res"]
+        if replicate > 1:
+            cmd += ["--replicate", str(replicate)]
+        r = subprocess.run(cmd, capture_output=True, text=True, check=True)
         st = dict(ln.split(":", 1) for ln in r.stdout.splitlines() if ":" in ln)
         secs, done = float(st["computation_time(s)"]), int(st["nodes"])
+        full = done >= total_nodes
         res.append({"threads": threads, "nodes_timed": done, "seconds": secs,
-                    "extrapolated_computation_s": secs * total_nodes / max(1, done), "nodes_total": total_nodes,
-                    "kind": "port", "note": "extrapolated from the first nodes of the reference DFS"})
-        log("  %.2f s for %d nodes -> %.0f s extrapolated" % (secs, done, res[-1]["extrapolated_computation_s"]))
+                    "extrapolated_computation_s": secs if full else secs * total_nodes / max(1, done),
+                    "nodes_total": total_nodes, "kind": "port",
+                    "note": "full run" if full else "extrapolated from the first nodes of the reference DFS"})
+        log("  %.2f s for %d nodes -> %.2f s" % (secs, done, res[-1]["extrapolated_computation_s"]))
     return res
 
 
