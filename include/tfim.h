@@ -106,7 +106,10 @@ int tfim_time_count_node(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t
  * row r is counted against all survivors and out is [n_rows][n_cols];
  * otherwise row r is counted against its tail col_ids[col_off[r] ..
  * col_off[r+1]) and out is CSR-aligned with col_ids.  Host memory in and out;
- * synchronous. */
+ * synchronous.  path: TFIM_PATH_AUTO lets the library choose the kernel per
+ * wave (its tuners); TFIM_PATH_PACKED keeps the wave on the packed (AND +
+ * v_bcnt) kernels; TFIM_PATH_MFMA_I8 counts it on the matrix cores (prefix
+ * masks, int8-MFMA count with split-k, gather). */
 int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item_off,
                      int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
                      int32_t *out, int32_t path);

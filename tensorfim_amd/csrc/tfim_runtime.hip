@@ -580,7 +580,9 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
 
 // Common body of tfim_count_nodes_submit and tfim_count_nodes_ex_submit.
 // Complement rows (n_compl > 0) need explicit column offsets (col_off), and
-// their waves run on the L2 kernel followed by the complement reduction.
+// their waves run on the L2 kernel followed by the complement reduction, or
+// on the matrix-core path.  path TFIM_PATH_PACKED keeps a wave on the packed
+// kernels, TFIM_PATH_MFMA_I8 sends it to the matrix cores.
 int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, int32_t n_rows,
                 const int32_t *col_ids, const int32_t *col_off, const int32_t *compl_rows, int32_t n_compl,
                 const uint64_t *excl, int32_t min_count, int32_t path, int64_t *ticket) {
@@ -596,8 +598,12 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             return fail(c, TFIM_E_INVALID, "count_nodes_ex: explicit columns without col_ids");
     }
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_nodes: no bitmaps loaded");
-    if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED)
-        return fail(c, TFIM_E_INVALID, "count_nodes: path %d not available for node waves", path);
+    if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED && path != TFIM_PATH_MFMA_I8)
+        return fail(c, TFIM_E_INVALID, "count_nodes: unknown path %d", path);
+    // the path argument overrides the configured wave kernel for this wave
+    const int wave_kernel = path == TFIM_PATH_MFMA_I8 ? 4
+                            : path == TFIM_PATH_PACKED && c->wave_kernel == 4 ? 0 : c->wave_kernel;
+    const bool allow_mfma = path != TFIM_PATH_PACKED;
     if (n_rows == 0) return TFIM_OK;
     tfim_ctx::Slot &sl = c->slots[c->next_ticket % tfim_ctx::kWaveSlots];
     if (sl.busy)
@@ -655,7 +661,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             for (int32_t it : rows) c->slot_of[it] = -1;
         }
     } slot_reset{c, slot_rows};
-    bool lds = (c->wave_kernel == 0 || c->wave_kernel == 2 || c->wave_kernel == 3) && n_compl == 0 &&
+    bool lds = (wave_kernel == 0 || wave_kernel == 2 || wave_kernel == 3) && n_compl == 0 &&
                n_out <= kLdsMaxOut;
     if (lds) {
         if ((int32_t)c->slot_of.size() < c->n_items) c->slot_of.assign(c->n_items, -1);
@@ -676,14 +682,14 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
         if (!lds)
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
     }
-    if ((c->wave_kernel == 2 || c->wave_kernel == 3) && !lds && n_compl == 0)
+    if ((wave_kernel == 2 || wave_kernel == 3) && !lds && n_compl == 0)
         return fail(c, TFIM_E_RANGE, "count_nodes: wave does not fit the LDS kernel (%zu items, %lld outputs)",
                     slot_rows.size(), (long long)n_out);
     // autotune over (kernel, columns per tile) for waves both kernels accept:
     // each arm is sampled three times, then the cheapest per row-word is
     // taken, one other arm being re-sampled every 16 calls (round robin)
     int tuned = -1;
-    if (lds && c->wave_kernel == 0) {
+    if (lds && wave_kernel == 0) {
         c->tune_calls++;
         for (int a = 0; a < kTuneArms && tuned < 0; a++)
             if (c->tune_samples[a] < 3) tuned = a;
@@ -708,12 +714,12 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     // re-tried every 16 waves) and takes the cheaper.
     int mf_arm = -1;
     double mf_work = 0;
-    if (!lds && (c->wave_kernel == 0 || c->wave_kernel == 4)) {
+    if (!lds && allow_mfma && (wave_kernel == 0 || wave_kernel == 4)) {
         const int64_t mask_bytes = (int64_t)n_rows * c->pitch * 8;
         // (the mask kernel's grid has one row per y index, at most 65535)
         const bool fits = c->n_cols > 0 && n_rows <= 65535 && mask_bytes <= kMfmaMaskBytes &&
                           (int64_t)n_rows * c->n_cols <= INT32_MAX;
-        if (c->wave_kernel == 4 && !fits)
+        if (wave_kernel == 4 && !fits)
             return fail(c, TFIM_E_RANGE, "count_nodes: wave too large for the matrix-core path (%d rows)", n_rows);
         if (fits) {
             double pairs = (double)n_exp;
@@ -731,7 +737,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
                                  39.0 * (double)(n_item_ids + n_rows)) *
                                 (double)c->pitch;
             int arm = 1;
-            if (c->wave_kernel == 0) {
+            if (wave_kernel == 0) {
                 c->mf_calls++;
                 if (c->mf_samples[0] < 1 || c->mf_samples[1] < 1) {
                     arm = c->mf_samples[1] < c->mf_samples[0] ? 1 : 0;
@@ -851,7 +857,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
         d.n_slots = (int32_t)slot_rows.size();
         d.n_tiles = (int32_t)tiles.size();
         d.n_out = (int32_t)n_out;
-        const bool wide = c->wave_kernel == 3 || (tuned >= 0 && kArmWide[tuned]);
+        const bool wide = wave_kernel == 3 || (tuned >= 0 && kArmWide[tuned]);
         if (!plan_count_wave_lds(c->d_bm, c->pitch, d, c->n_cu, wide, K))
             return fail(c, TFIM_E_RANGE, "count_nodes: LDS kernel rejected %d slots", d.n_slots);
         c->wave_lds_launches++;

@@ -133,3 +133,25 @@ def test_complement_rows_rejected_when_malformed(ctx):
         ctx.count_nodes_ex([[11, 2]], [None], [set()], 3)   # first item is not a survivor
     ok = ctx.count_nodes_ex([[3, 11]], [None], [{3}], 3)
     np.testing.assert_array_equal(ok[0], expected(rows, 10, [3, 11], {3}, 3))
+
+
+def test_matrix_core_split_tuner_arms(ctx):
+    """Matrix-core waves on long rows (split over the words) run every arm of
+    the split tuner (1/2, 1 and 2 times the occupancy rule's split, three
+    samples each, then re-tries): every wave stays exact."""
+    rng = np.random.default_rng(23)
+    n_bits, n_items, n_cols = 1_500_007, 60, 57
+    rows = rand_rows(rng, n_items, n_bits, 0.55)
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    ctx.config("wave_kernel", 4)
+    try:
+        for it in range(14):
+            prefixes, tails, excl = make_wave(rng, n_items, n_cols, int(rng.integers(1, 70)), 0.4)
+            min_count = int(n_bits * 0.2) + it
+            got = ctx.count_nodes_ex(prefixes, tails, excl, min_count)
+            for i, (p, t, ex) in enumerate(zip(prefixes, tails, excl)):
+                exp = expected(rows, n_cols, p, ex, min_count) if t is None else \
+                    oracle_lib.node_counts(rows, p, n_cols)[t]
+                np.testing.assert_array_equal(got[i], exp, err_msg="wave %d row %d" % (it, i))
+    finally:
+        ctx.config("wave_kernel", 0)

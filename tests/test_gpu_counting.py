@@ -241,3 +241,24 @@ def test_popcount_matrix_tuned_split(ctx, path):
     assert tile in ((64,) if path == "packed" else (128, 256))
     for x in (dA, dB, dC):
         x.free()
+
+
+@pytest.mark.parametrize("path", ["packed", "mfma"])
+def test_count_nodes_path_argument(ctx, path):
+    """The path argument of a wave: packed kernels only, or the matrix cores;
+    both exact, on rows off multiples of 64 and 2048 bits."""
+    rng = np.random.default_rng(5 if path == "packed" else 6)
+    n_bits, n_items = 2048 * 11 + 77, 140
+    rows = rand_rows(rng, n_items, n_bits, 0.6)
+    ctx.load_bitmaps(rows, n_bits, n_items - 1)
+    before = ctx.wave_stats()
+    prefixes = [rng.choice(n_items, size=int(rng.integers(1, 6)), replace=False) for _ in range(90)]
+    tails = [np.sort(rng.choice(n_items - 1, size=int(rng.integers(0, 60)), replace=False)) for _ in range(90)]
+    got = ctx.count_nodes(prefixes, tails, path=path)
+    for p, t, g in zip(prefixes, tails, got):
+        np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_items - 1)[t])
+    after = ctx.wave_stats()
+    if path == "mfma":
+        assert after["mfma_launches"] == before["mfma_launches"] + 1
+    else:
+        assert after["mfma_launches"] == before["mfma_launches"]
