@@ -154,6 +154,10 @@ int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *i
 
 /* Tuning knobs (ablation and autotuning; the reference's equivalents are
  * environment variables, kernel_bmma.cu:351-368):
+ *   "node_spread"  per-node counts: 0 auto (8 copies for at most 512
+ *                  survivors), 1 one copy, 8 spread over 8 copies 4 KB
+ *                  apart (the same-address atomics of one column then come
+ *                  from 1/8 of the word chunks; the library adds the copies)
  *   "node_shape"   launch shape of the per-node kernel (tfim_count_node):
  *                  0 auto, else R * 100 + cpb with R in {1, 2, 4} 128-bit
  *                  words per lane and cpb in {8, 16, 32, 64} columns per

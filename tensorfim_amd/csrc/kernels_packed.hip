@@ -126,7 +126,7 @@ __device__ __forceinline__ uint32_t row_reduce4_packed(const uint32_t v[4]) {
 template <int R, int G>
 __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm, int64_t pitch4,
                                                     const int32_t *__restrict__ items, int k, int n_cols,
-                                                    int cpb, int32_t *__restrict__ out) {
+                                                    int cpb, int32_t *__restrict__ out, int64_t spread_stride) {
     __shared__ int red[4][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t base = (int64_t)blockIdx.x * (256 * R) + tid;
@@ -176,7 +176,11 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
     __syncthreads();
     if (tid < c1 - c0) {
         const int s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-        if (s) atomicAdd(&out[c0 + tid], s);
+        // with a spread stride, blocks of different word chunks add into 8
+        // copies of the counts (4 KB apart), so that the same-address atomics
+        // of one column come from 1/8 of the chunks; the caller sums them
+        int32_t *o = spread_stride ? out + (int64_t)(blockIdx.x & 7) * spread_stride : out;
+        if (s) atomicAdd(&o[c0 + tid], s);
     }
 }
 
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
 // else (1, 16); short, narrow nodes are latency-bound, and more blocks with
 // fewer columns each keep more loads in flight.
 void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
-                       int32_t n_cols, int32_t *out, int n_cu, int shape, hipStream_t s) {
+                       int32_t n_cols, int32_t *out, int64_t spread_stride, int n_cu, int shape, hipStream_t s) {
     const int64_t pitch4 = pitch_words / 2;
     auto blocks = [&](int R, int cpb) { return ((pitch4 + 256 * R - 1) / (256 * R)) * ((n_cols + cpb - 1) / cpb); };
     int R = shape / 100, cpb = shape % 100;
@@ -201,17 +205,17 @@ void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *i
     const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
     if (cpb == 8) {
         if (R == 4)
-            hipLaunchKernelGGL((k_count_node<4, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+            hipLaunchKernelGGL((k_count_node<4, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out, spread_stride);
         else if (R == 2)
-            hipLaunchKernelGGL((k_count_node<2, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+            hipLaunchKernelGGL((k_count_node<2, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out, spread_stride);
         else
-            hipLaunchKernelGGL((k_count_node<1, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+            hipLaunchKernelGGL((k_count_node<1, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out, spread_stride);
     } else if (R == 4) {
-        hipLaunchKernelGGL((k_count_node<4, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+        hipLaunchKernelGGL((k_count_node<4, 1>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out, spread_stride);
     } else if (R == 2) {
-        hipLaunchKernelGGL((k_count_node<2, 2>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+        hipLaunchKernelGGL((k_count_node<2, 2>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out, spread_stride);
     } else {
-        hipLaunchKernelGGL((k_count_node<1, 2>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out);
+        hipLaunchKernelGGL((k_count_node<1, 2>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out, spread_stride);
     }
 }
 

@@ -52,9 +52,11 @@ struct KernelLaunch {
 // ---- launchers (defined in kernels_*.hip) --------------------------------
 
 // Plain per-node count: out[c] += popcount(AND rows[items] & rows[c]) for
-// c < n_cols; `out` must be zeroed.  items are device ids.
+// c < n_cols; `out` must be zeroed.  items are device ids.  With
+// spread_stride > 0 the counts are spread over 8 copies out[j * spread_stride
+// + c], j < 8, which the caller adds up.
 void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
-                       int32_t n_cols, int32_t *out, int n_cu, int shape, hipStream_t s);
+                       int32_t n_cols, int32_t *out, int64_t spread_stride, int n_cu, int shape, hipStream_t s);
 
 // One tile of a wave: a row's prefix, and a run of its columns.  compl == 0:
 // the columns are entries [col_begin, col_end) of the row's column list.

@@ -114,6 +114,7 @@ struct tfim_ctx {
     int tile_cols = 0;      // 0: chosen by the tuner (32 when it does not run)
     int mfma_tile = 0;      // 0 auto (= 256), 128 or 256
     int node_shape = 0;     // per-node kernel: 0 auto, else R * 100 + columns per block
+    int node_spread = 0;    // per-node counts: 0 auto, 1 one copy, 8 spread over 8 copies
     int use_graphs = 1;     // waves replayed as instantiated graphs
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
     std::map<std::string, std::pair<int, int32_t>> popmat_tuned;   // shape -> (tile, split_k)
@@ -345,6 +346,14 @@ int tfim_item_supports(tfim_ctx *c, int32_t *out) {
 
 namespace {
 // Checks a node's items and stages them on the device for k_count_node.
+// Spread stride of the per-node counts (0 = one copy): 8 copies 4 KB apart
+// for nodes of at most 512 survivors (the copy back and the host sum stay
+// small), unless the "node_spread" knob says otherwise.
+int64_t node_spread_stride(const tfim_ctx *c) {
+    const bool on = c->node_spread == 8 || (c->node_spread == 0 && c->n_cols <= 512);
+    return on ? ((int64_t)c->n_cols + 1023) / 1024 * 1024 : 0;
+}
+
 int node_stage(tfim_ctx *c, const char *what, const int32_t *items, int32_t k) {
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "%s: no bitmaps loaded", what);
     for (int32_t i = 0; i < k; i++)
@@ -352,7 +361,9 @@ int node_stage(tfim_ctx *c, const char *what, const int32_t *items, int32_t k) {
             return fail(c, TFIM_E_INVALID, "%s: item %d out of range [0,%d)", what, items[i], c->n_items);
     if (int r = set_device(c)) return r;
     if (int r = ensure_pair(c, &c->d_node_items, &c->h_node_items, &c->node_items_cap, k)) return r;
-    if (int r = ensure_pair(c, &c->d_node_out, &c->h_node_out, &c->node_out_cap, std::max(1, c->n_cols))) return r;
+    if (int r = ensure_pair(c, &c->d_node_out, &c->h_node_out, &c->node_out_cap, 8 * node_spread_stride(c) +
+                                                                                    std::max(1, c->n_cols)))
+        return r;
     memcpy(c->h_node_items, items, (size_t)k * 4);
     HIPCHK(c, hipMemcpyAsync(c->d_node_items, c->h_node_items, (size_t)k * 4, hipMemcpyHostToDevice, c->stream));
     return TFIM_OK;
@@ -364,12 +375,28 @@ int tfim_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t *out) 
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_node: no bitmaps loaded");
     if (c->n_cols == 0) return TFIM_OK;
     if (int r = node_stage(c, "count_node", items, k)) return r;
-    launch_zero_i32(c->d_node_out, c->n_cols, c->stream);
-    launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->node_shape, c->stream);
+    const int64_t ss = node_spread_stride(c);
+    launch_zero_i32(c->d_node_out, ss ? 8 * ss : c->n_cols, c->stream);
+    launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, ss, c->n_cu, c->node_shape,
+                      c->stream);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->h_node_out, c->d_node_out, (size_t)c->n_cols * 4, hipMemcpyDeviceToHost, c->stream));
+    if (ss) {   // the 8 copies, packed, then summed here
+        HIPCHK(c, hipMemcpy2DAsync(c->h_node_out, (size_t)c->n_cols * 4, c->d_node_out, (size_t)ss * 4,
+                                   (size_t)c->n_cols * 4, 8, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->h_node_out, c->d_node_out, (size_t)c->n_cols * 4, hipMemcpyDeviceToHost,
+                                 c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    memcpy(out, c->h_node_out, (size_t)c->n_cols * 4);
+    if (ss) {
+        for (int32_t e = 0; e < c->n_cols; e++) {
+            int32_t t = 0;
+            for (int j = 0; j < 8; j++) t += c->h_node_out[(size_t)j * c->n_cols + e];
+            out[e] = t;
+        }
+    } else {
+        memcpy(out, c->h_node_out, (size_t)c->n_cols * 4);
+    }
     return TFIM_OK;
 }
 
@@ -383,11 +410,13 @@ int tfim_time_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t w
     // the launches accumulate into the library's node buffer, whose contents
     // are not returned
     for (int i = 0; i < warmup; i++)
-        launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->node_shape, c->stream);
+        launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, node_spread_stride(c), c->n_cu,
+                          c->node_shape, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     for (int i = 0; i < reps; i++)
-        launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, c->n_cu, c->node_shape, c->stream);
+        launch_count_node(c->d_bm, c->pitch, c->d_node_items, k, c->n_cols, c->d_node_out, node_spread_stride(c), c->n_cu,
+                          c->node_shape, c->stream);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -1005,6 +1034,10 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
                                              (value % 100 == 8 || value % 100 == 16 || value % 100 == 32 ||
                                               value % 100 == 64)))) {
         c->node_shape = (int)value;
+        return TFIM_OK;
+    }
+    if (k == "node_spread" && (value == 0 || value == 1 || value == 8)) {
+        c->node_spread = (int)value;
         return TFIM_OK;
     }
     if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256)) {
