@@ -394,7 +394,16 @@ struct MNode {
     // stay infrequent for every superset of head, so HUT and LEAF rows skip
     // them (a pruning of columns only; decisions are unchanged)
     std::vector<uint64_t> known;
+    // SPEC rows of wide databases: the tail is (*tail_ref)[tail_off ..),
+    // copied into `tail` only if the node lives on (materialize_tail)
+    std::shared_ptr<const std::vector<int32_t>> tail_ref;
+    int32_t tail_off = 0;
 
+    void materialize_tail() {
+        if (!tail_ref) return;
+        tail.assign(tail_ref->begin() + tail_off, tail_ref->end());
+        tail_ref.reset();
+    }
     // back to a fresh node, keeping the vectors' storage (node recycling)
     void reset() {
         kind = EXPAND;
@@ -402,6 +411,8 @@ struct MNode {
         tail.clear();
         tail_sup.clear();
         known.clear();
+        tail_ref.reset();
+        tail_off = 0;
         sup = progress = depth = 0;
         decision = item_record = -1;
     }
@@ -905,6 +916,14 @@ class Mafia {
         // child i: head + t_i with tail t_{i+1..}; pushed last-first so the
         // first child (largest tail) is counted first
         const size_t k = d.order.size();
+        // largest sup(head + t) over each suffix of the ordered tail
+        suffix_max_.assign(k + 1, 0);
+        for (size_t r = k; r-- > 0;) suffix_max_[r] = std::max(suffix_max_[r + 1], d.tail_sup[d.order[r]]);
+        // the ordered tail, shared by the speculative children of a wide
+        // database (their complement rows do not list it; a child that lives
+        // on copies its suffix then), so a node with a long tail does not copy
+        // a quadratic number of items
+        std::shared_ptr<std::vector<int32_t>> ordered;
         for (size_t q = k; q-- > 0;) {
             MNode c = take_node();
             c.kind = MNode::EXPAND;
@@ -914,18 +933,24 @@ class Mafia {
             c.depth = d.depth + 1;
             c.known.assign(d.known.begin(), d.known.end());
             if (!d.item_records.empty()) c.item_record = d.item_records[d.order[q]];
-            int32_t tail_max = 0;   // largest sup(head + t) over the child's tail
-            for (size_t r = q + 1; r < k; r++) {
-                c.tail.push_back(d.tail[d.order[r]]);
-                tail_max = std::max(tail_max, d.tail_sup[d.order[r]]);
-            }
+            const bool has_tail = q + 1 < k;
             // speculate that the child is a maximal leaf when even its best
             // tail item is expected to drop below the threshold if it were
             // independent of the added item: sup(H+t_q) * sup(H+t) / sup(H)
-            if (spec_ && !c.tail.empty() &&
-                (double)c.sup * (double)tail_max / (double)std::max(1, d.sup) < db_.thr_abs)
+            if (spec_ && has_tail &&
+                (double)c.sup * (double)suffix_max_[q + 1] / (double)std::max(1, d.sup) < db_.thr_abs)
                 c.kind = MNode::SPEC;
-            if (c.tail.empty()) {
+            if (c.kind == MNode::SPEC && compl_) {
+                if (!ordered) {
+                    ordered = std::make_shared<std::vector<int32_t>>(k);
+                    for (size_t r = 0; r < k; r++) (*ordered)[r] = d.tail[d.order[r]];
+                }
+                c.tail_ref = ordered;
+                c.tail_off = (int32_t)(q + 1);
+            } else {
+                for (size_t r = q + 1; r < k; r++) c.tail.push_back(d.tail[d.order[r]]);
+            }
+            if (!has_tail) {
                 // no candidates left: the child is a leaf
                 const bool sub = d.leaf_sub >= 0 ? d.leaf_sub == 1 || index_.subsumed_from(c.head, d.leaf_n)
                                                  : index_.subsumed(c.head);
@@ -991,6 +1016,7 @@ class Mafia {
                 } else {
                     st_.spec_misses++;
                     n.kind = MNode::EXPAND;   // lives on: count its tail
+                    n.materialize_tail();
                     pending_.push_back(std::move(n));
                 }
             } else if (expands(n)) {
@@ -1188,7 +1214,7 @@ class Mafia {
     bool ok_(int32_t s) const { return s >= min_count_; }
     // spent nodes whose vectors are reused for new ones
     std::vector<MNode> free_;
-    std::vector<int32_t> hut_scratch_;
+    std::vector<int32_t> hut_scratch_, suffix_max_;
     MNode take_node() {
         if (free_.empty()) return MNode();
         MNode n = std::move(free_.back());

@@ -98,7 +98,7 @@ struct tfim_ctx {
     int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel, 3 LDS kernel with wide staging, 4 matrix cores
     // second tuner, for waves outside the LDS kernel: arm 0 = L2 kernel, arm 1
     // = matrix cores; EMA of kernel milliseconds per unit of each arm's work
-    double mf_cost[2] = {};
+    double mf_cost[2] = {64.0 / 300e12 * 1e3, 64.0 / 1500e12 * 1e3};   // ms per u64 word of work (priors)
     int64_t mf_samples[2] = {};
     int64_t mf_calls = 0, wave_mfma_launches = 0;
     // third tuner: the matrix-core path's word split, as 1/2, 1 or 2 times
@@ -739,8 +739,9 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     // listed pairs) or to the matrix cores (masks + int8 MFMA count of every
     // (row, survivor) pair + gather).  wave_kernel 4 forces the second; in
     // auto mode a second tuner predicts each arm's time as its work times its
-    // measured cost per unit (each arm tried once first, the other one
-    // re-tried every 16 waves) and takes the cheaper.
+    // measured cost per unit (priors from the two paths' rates on MI355X
+    // until measured; the other arm re-tried every 16 waves) and takes the
+    // cheaper.
     int mf_arm = -1;
     double mf_work = 0;
     if (!lds && allow_mfma && (wave_kernel == 0 || wave_kernel == 4)) {
@@ -768,12 +769,12 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             int arm = 1;
             if (wave_kernel == 0) {
                 c->mf_calls++;
-                if (c->mf_samples[0] < 1 || c->mf_samples[1] < 1) {
-                    arm = c->mf_samples[1] < c->mf_samples[0] ? 1 : 0;
-                } else {
-                    arm = w_mf * c->mf_cost[1] < w_l2 * c->mf_cost[0] ? 1 : 0;
-                    if (c->mf_calls % 16 == 0) arm = 1 - arm;
-                }
+                // the costs start from priors (the L2 kernel's and the matrix
+                // cores' rates on MI355X, about 300 and 1,500 Tbit-op/s over
+                // their units of work) and become measured EMAs; the other
+                // arm is tried on every 16th wave
+                arm = w_mf * c->mf_cost[1] < w_l2 * c->mf_cost[0] ? 1 : 0;
+                if (c->mf_calls % 16 == 0) arm = 1 - arm;
                 mf_arm = arm;
                 mf_work = arm ? w_mf : w_l2;
             }
