@@ -97,7 +97,8 @@ class LearnedPolicy : public Policy {
             // sample of nodes still records them, so head 1 keeps learning
             // (and small runs collect enough labels for training rounds)
             const bool items = d->tail.size() >= 3 && d->tail.size() <= kMaxRankTail &&
-                               (opt_.rank || rand01() < kRankRecord);
+                               (opt_.rank || (rank_recorded_ < kRankRecordMax && rand01() < kRankRecord));
+            if (items && !opt_.rank) rank_recorded_ += (int64_t)d->tail.size();
             bool need = !trained_ || items;
             // the model is asked about a share ask_p_ of the declined nodes
             // (all of them while its additions pay off)
@@ -449,6 +450,7 @@ class LearnedPolicy : public Policy {
     static constexpr float kAskMin = 1.0f / 16;     // floor of the share of declined nodes the model is asked about
     static constexpr float kRecordAccepted = 0.25f;  // share of the rule's lookaheads kept as training records
     static constexpr float kRankRecord = 0.125f;     // without --rank: share of nodes whose items are recorded
+    static constexpr int64_t kRankRecordMax = 8192;  // ... until this many item records exist
 
     tfim_ctx *ctx_;
     const SearchOptions &opt_;
@@ -481,6 +483,7 @@ class LearnedPolicy : public Policy {
     std::vector<uint8_t> rec_src_ = std::vector<uint8_t>((size_t)kRing, 0);
     std::vector<float> bx_, by_, bm_, tx_, ty_, tm_;
     int64_t next_id_ = 0, n_labelled_ = 0, last_train_ = 0, steps_ = 0;
+    int64_t rank_recorded_ = 0;   // item records made without --rank
     int64_t n_head_[2] = {0, 0};
     uint64_t rng_ = 0xC0FFEEull;
 };
