@@ -588,6 +588,41 @@ class AsyncWorker {
     std::thread th_;
 };
 
+// Set of 64-bit keys (itemsets of at most 64 survivors as bitmasks): open
+// addressing with linear probing, doubled when half full.  Key 0 (the empty
+// set) is never inserted.
+class KeySet64 {
+   public:
+    // true if the key was not there yet
+    bool insert(uint64_t key) {
+        if ((n_ + 1) * 2 > slots_.size()) grow();
+        size_t i = mix(key) & (slots_.size() - 1);
+        while (slots_[i]) {
+            if (slots_[i] == key) return false;
+            i = (i + 1) & (slots_.size() - 1);
+        }
+        slots_[i] = key;
+        n_++;
+        return true;
+    }
+
+   private:
+    static uint64_t mix(uint64_t z) {
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    void grow() {
+        std::vector<uint64_t> old(std::max<size_t>(1024, slots_.size() * 2), 0ull);
+        old.swap(slots_);
+        n_ = 0;
+        for (uint64_t k : old)
+            if (k) insert(k);
+    }
+    std::vector<uint64_t> slots_;
+    size_t n_ = 0;
+};
+
 struct VecHash {
     size_t operator()(const std::vector<int32_t> &v) const {
         uint64_t h = 1469598103934665603ull;
@@ -877,12 +912,12 @@ class Mafia {
         return wb_.submit(ctx_, f.st, db_.n_bits(), min_count_, err);
     }
 
-    void emit(std::vector<int32_t> items, int32_t sup, std::vector<Mfi> &out) {
+    void emit(std::vector<int32_t> &&items, int32_t sup, std::vector<Mfi> &out) {
         std::sort(items.begin(), items.end(), std::greater<int32_t>());   // insertion order: root first
         if (db_.n_surv <= 64) {   // the set itself as a 64-bit key
             uint64_t key = 0;
             for (int32_t i : items) key |= 1ull << i;
-            if (!seen64_.insert(key).second) return;
+            if (!seen64_.insert(key)) return;
         } else if (!seen_.insert(items).second) {
             return;
         }
@@ -892,7 +927,7 @@ class Mafia {
             // a single-item MFI carries the root's occurrence count, as the
             // reference reports roots with freqPerItem (kernel_bmma.cu:1227)
             const int32_t f = items.size() == 1 ? db_.freq_dense[items[0]] : sup;
-            out.push_back({items, f});
+            out.push_back({std::move(items), f});
         }
     }
 
@@ -983,6 +1018,14 @@ class Mafia {
         // are the survivors outside head and known, ascending, so a column's
         // index is a prefix popcount of that live mask), then the
         // subsumption check of the rows that expand.
+        // The rows that expand also build their decision here (frequent
+        // tail, PEP moves into the head, infrequent items into known): it
+        // depends on the row alone.  What depends on the order of the rows
+        // (MFIs emitted by earlier rows, the stack, the policy) is done by
+        // the serial loop below.
+        if (pre_dec_.size() < wave_.size()) pre_dec_.resize(wave_.size());
+        pre_kept_.assign(wave_.size(), 0);
+        pre_pep_.assign(wave_.size(), 0);
         pool_.run((int64_t)wave_.size(), [&](int64_t r) {
             MNode &n = wave_[r];
             if (n.kind == MNode::SPEC && !compl_) spec_prepare((int)r, n, wb_.result((int)r));
@@ -990,14 +1033,39 @@ class Mafia {
             const bool spec = n.kind == MNode::SPEC;
             const int32_t *res = spec ? spec_tres_[r].data() : wb_.result((int)r);
             const int nc = spec ? (int)spec_tres_[r].size() : wb_.ncols((int)r);
+            Decision &d = pre_dec_[r];
+            d = Decision();
+            d.head = std::move(n.head);   // the row's node is spent after this
+            d.sup = n.sup;
+            d.depth = n.depth;
+            d.known = std::move(n.known);
+            if (d.known.empty()) d.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
+            int kept = 0, pep = 0;
+            for (int j = 0; j < nc; j++) {
+                const int32_t t = n.tail[j], sv = res[j];
+                if (!ok_(sv)) {
+                    d.known[(size_t)t >> 6] |= 1ull << (t & 63);
+                    continue;
+                }
+                kept++;
+                if (sv == n.sup) {   // PEP: every MFI above head contains t
+                    d.head.push_back(t);
+                    pep++;
+                } else {
+                    d.tail.push_back(t);
+                    d.tail_sup.push_back(sv);
+                }
+            }
+            pre_kept_[r] = kept;
+            pre_pep_[r] = pep;
+            // head + frequent tail: the leaf head when the tail is empty,
+            // else the node's head-union-tail
             thread_local std::vector<int32_t> q;
-            q.assign(n.head.begin(), n.head.end());
-            for (int j = 0; j < nc; j++)
-                if (ok_(res[j])) q.push_back(n.tail[j]);
+            q.assign(d.head.begin(), d.head.end());
+            q.insert(q.end(), d.tail.begin(), d.tail.end());
             pre_sub_[r] = index_.subsumed(q);
         });
         auto expands = [&](const MNode &n) { return n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_); };
-        auto tail_res = [&](int r) { return wave_[r].kind == MNode::SPEC ? spec_tres_[r].data() : wb_.result(r); };
         auto tail_n = [&](int r) { return wave_[r].kind == MNode::SPEC ? (int)spec_tres_[r].size() : wb_.ncols(r); };
         auto subsumed_row = [&](int r, const std::vector<int32_t> &x) {
             return pre_sub_[r] || index_.subsumed_from(x, n_start);
@@ -1012,7 +1080,7 @@ class Mafia {
                     st_.spec_hits++;
                     // no survivor extends it: its whole tail dropped out
                     if (n.item_record >= 0) pol_.observe_value(n.item_record, 0.0f);
-                    emit(n.head, n.sup, out);
+                    emit(std::move(n.head), n.sup, out);
                 } else {
                     st_.spec_misses++;
                     n.kind = MNode::EXPAND;   // lives on: count its tail
@@ -1021,37 +1089,16 @@ class Mafia {
                 }
             } else if (expands(n)) {
                 const bool spec = n.kind == MNode::SPEC;
-                res = tail_res(r);
                 const int nc = tail_n(r);
-                Decision d;
-                d.head = std::move(n.head);   // the row's node is spent after this
-                d.sup = n.sup;
-                d.depth = n.depth;
-                d.known = std::move(n.known);
-                if (d.known.empty()) d.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
-                int kept = 0;
-                for (int j = 0; j < nc; j++) kept += ok_(res[j]);
-                if (n.item_record >= 0) pol_.observe_value(n.item_record, nc > 0 ? (float)kept / nc : 0.0f);
-                for (int j = 0; j < nc; j++) {
-                    const int32_t t = n.tail[j], s = res[j];
-                    if (!ok_(s)) {
-                        d.known[(size_t)t >> 6] |= 1ull << (t & 63);
-                        continue;
-                    }
-                    if (s == n.sup) {   // PEP: every MFI above head contains t
-                        d.head.push_back(t);
-                        st_.pep_moves++;
-                    } else {
-                        d.tail.push_back(t);
-                        d.tail_sup.push_back(s);
-                    }
-                }
+                Decision &d = pre_dec_[r];
+                if (n.item_record >= 0) pol_.observe_value(n.item_record, nc > 0 ? (float)pre_kept_[r] / nc : 0.0f);
+                st_.pep_moves += pre_pep_[r];
                 if (d.tail.empty() && spec) {
                     // the complement's counts already decide the leaf: no
                     // frequent column outside the (equivalent) tail items
                     if (spec_other_[r] == 0) {
                         st_.spec_hits++;
-                        emit(d.head, d.sup, out);
+                        emit(std::move(d.head), d.sup, out);
                     }
                     continue;
                 }
@@ -1088,7 +1135,7 @@ class Mafia {
                     for (int j = 1; j < nc && maximal; j++) maximal = !ok_(res[j]);
                     std::vector<int32_t> hut = n.head;
                     hut.insert(hut.end(), n.tail.begin(), n.tail.end());
-                    if (maximal) emit(hut, res[0], out);
+                    if (maximal) emit(std::move(hut), res[0], out);
                 } else {
                     Decision d;
                     d.head = n.head;
@@ -1105,7 +1152,7 @@ class Mafia {
                 bool maximal = true;
                 if (compl_) maximal = res[1] == 0;
                 for (int j = 0; j < nc && maximal; j++) maximal = !ok_(res[j]);
-                if (maximal) emit(n.head, n.sup, out);
+                if (maximal) emit(std::move(n.head), n.sup, out);
             } else {   // REF: reference rule, own output path
                 int32_t cnt = 0;
                 for (int j = 0; j < nc; j++) cnt += ok_(res[j]);
@@ -1198,10 +1245,12 @@ class Mafia {
     MfiIndex index_;
     HostPool pool_;
     std::vector<uint8_t> pre_sub_;
+    std::vector<Decision> pre_dec_;             // per wave row: its decision, built on the host pool
+    std::vector<int32_t> pre_kept_, pre_pep_;   // per wave row: frequent tail items, PEP moves
     std::vector<std::vector<int32_t>> spec_tres_;   // SPEC rows (explicit): the tail's counts
     std::vector<int32_t> spec_other_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
-    std::unordered_set<uint64_t> seen64_;
+    KeySet64 seen64_;
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;
     static constexpr int kFlights = 3;
