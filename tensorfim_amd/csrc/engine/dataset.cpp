@@ -25,9 +25,12 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <numeric>
 #include <thread>
+#include <utility>
 
 namespace tfim {
 
@@ -37,19 +40,86 @@ double now() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// std::allocator whose resize leaves new elements uninitialised: the
+// parser's threads write every element of the gigabyte id and offset arrays,
+// and their first touch spreads the page faults over those threads instead
+// of one serial zero fill.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U> &) {}
+    template <class U>
+    void construct(U *q) noexcept {
+        ::new ((void *)q) U;
+    }
+    template <class U, class... A>
+    void construct(U *q, A &&...a) {
+        ::new ((void *)q) U(std::forward<A>(a)...);
+    }
+};
+using IdVec = std::vector<int32_t, UninitAlloc<int32_t>>;
+using OffVec = std::vector<int64_t, UninitAlloc<int64_t>>;
+
+inline bool is_digit(unsigned char c) { return (unsigned char)(c - '0') < 10; }
+
+// Ids and rows of bytes [p, end) under the rules of scan_piece below, without
+// parsing: an id starts at every digit whose previous byte is neither a digit
+// nor '-' (a '-' drops the digit run after it), and every '\n' ends a row.
+// The byte before p is a '\n' or the start of the file.  Branch-free over
+// blocks so that the compiler vectorises it.
+void count_piece(const char *p, const char *end, int final_row, int64_t &n_ids, int64_t &n_rows) {
+    const unsigned char *u = reinterpret_cast<const unsigned char *>(p);
+    const size_t n = (size_t)(end - p);
+    int64_t ni = 0, nr = 0;
+    if (n > 0) ni = is_digit(u[0]);
+    for (size_t b = 1; b < n; b += 4096) {
+        const size_t e = std::min(n, b + 4096);
+        uint32_t si = 0, sr = 0;
+        for (size_t i = b; i < e; i++) {
+            const unsigned char c = u[i], q = u[i - 1];
+            si += (uint32_t)(is_digit(c) & !is_digit(q) & (q != '-'));
+            sr += (uint32_t)(c == '\n');
+        }
+        ni += si;
+        nr += sr;
+    }
+    if (n > 0) nr += u[0] == '\n';
+    if (final_row == 1) {
+        nr++;
+    } else if (final_row == 2) {
+        // a final row without '\n' counts if it holds an id
+        size_t t = n;
+        while (t > 0 && u[t - 1] != '\n') t--;
+        bool any = false;
+        for (size_t i = t; i < n && !any; i++)
+            any = is_digit(u[i]) && (i == 0 || (!is_digit(u[i - 1]) && u[i - 1] != '-'));
+        nr += any;
+    }
+    n_ids = ni;
+    n_rows = nr;
+}
+
 // The reader's rules over bytes [p, end): a digit run is an id, '-' and its
 // digits are dropped, any other byte separates, '\n' ends a row.  The bytes
 // after the last '\n' form a final row when `final_row` is 1 (always) or 2
 // (only if it holds an id); 0 ignores them (a piece that ends right after a
-// '\n' has none).  With ids == nullptr only counts ids and rows; otherwise
-// writes ids and row lengths from the given positions.
-void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int32_t *lens, int64_t &n_ids,
-                int64_t &n_rows, int32_t &idmax) {
+// '\n' has none).  Writes the ids from ids[0] and the end offset of each row,
+// id_base plus the ids so far, from ends[0]; returns the largest id through
+// idmax.  At most max_ids ids and max_rows rows are written (the counts of
+// count_piece); n_ids and n_rows report what the bytes hold.
+void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int64_t max_ids, int64_t *ends,
+                int64_t max_rows, int64_t id_base, int64_t &n_ids, int64_t &n_rows, int32_t &idmax) {
     int64_t ni = 0, nr = 0, row_start = 0;
+    int32_t mx = idmax;
     while (p < end) {
         const char c = *p;
         if (c == '\n') {
-            if (lens) lens[nr] = (int32_t)(ni - row_start);
+            if (nr < max_rows) ends[nr] = id_base + ni;
             nr++;
             row_start = ni;
             ++p;
@@ -58,8 +128,8 @@ void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int
         if (c >= '0' && c <= '9') {
             long long v = 0;
             while (p < end && *p >= '0' && *p <= '9') v = v * 10 + (*p++ - '0');
-            if (ids) ids[ni] = (int32_t)v;
-            idmax = std::max(idmax, (int32_t)v);
+            if (ni < max_ids) ids[ni] = (int32_t)v;
+            mx = std::max(mx, (int32_t)v);
             ni++;
             continue;
         }
@@ -71,21 +141,24 @@ void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int
         ++p;
     }
     if (final_row == 1 || (final_row == 2 && ni > row_start)) {
-        if (lens) lens[nr] = (int32_t)(ni - row_start);
+        if (nr < max_rows) ends[nr] = id_base + ni;
         nr++;
     }
     n_ids = ni;
     n_rows = nr;
+    idmax = mx;
 }
 
 // Parses a whole file image into CSR rows with up to `threads` threads: the
 // bytes are cut right after '\n' bytes into one piece per thread; a first
-// pass counts each piece's ids and rows, a second writes them straight into
-// the joined arrays, so the rows are those of one sequential pass and no
-// thread allocates memory of its own.  `newline_at_end` treats the image as if
-// a '\n' followed it (the replicated database appends one when missing).
-void parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, std::vector<int32_t> &ids,
-                    std::vector<int64_t> &off, int32_t &idmax) {
+// pass counts each piece's ids and rows, a second parses them straight into
+// the joined arrays (ids and row offsets), so the rows are those of one
+// sequential pass and no thread allocates memory of its own.
+// `newline_at_end` treats the image as if a '\n' followed it (the replicated
+// database appends one when missing).  Returns false if the two passes
+// disagree (a bug; nothing is written beyond the arrays either way).
+bool parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, IdVec &ids, OffVec &off,
+                    int32_t &idmax) {
     threads = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, threads), n / 2 + 1));
     std::vector<size_t> cut(threads + 1, n);
     cut[0] = 0;
@@ -102,19 +175,17 @@ void parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, s
     };
     std::vector<int64_t> pids(threads + 1, 0), prows(threads + 1, 0);
     std::vector<int32_t> pmax(threads, 0);
+    std::vector<uint8_t> agree(threads, 1);
     auto run = [&](auto &&fn) {
         std::vector<std::thread> pool;
         for (int i = 1; i < threads; i++) pool.emplace_back(fn, i);
         fn(0);
         for (auto &t : pool) t.join();
     };
-    run([&](int i) {
-        scan_piece(p + cut[i], p + cut[i + 1], final_row(i), nullptr, nullptr, pids[i + 1], prows[i + 1], pmax[i]);
-    });
+    run([&](int i) { count_piece(p + cut[i], p + cut[i + 1], final_row(i), pids[i + 1], prows[i + 1]); });
     for (int i = 0; i < threads; i++) {
         pids[i + 1] += pids[i];
         prows[i + 1] += prows[i];
-        idmax = std::max(idmax, pmax[i]);
     }
     // large arrays: ask for transparent huge pages before the first touch
     // (page faults of 4 KB pages dominate filling gigabytes)
@@ -126,15 +197,22 @@ void parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, s
     ids.reserve((size_t)pids[threads]);
     huge(ids.data(), (size_t)pids[threads] * 4);
     ids.resize((size_t)pids[threads]);
-    std::vector<int32_t> lens((size_t)prows[threads]);
-    run([&](int i) {
-        int64_t a, b;
-        int32_t mx = 0;
-        scan_piece(p + cut[i], p + cut[i + 1], final_row(i), ids.data() + pids[i], lens.data() + prows[i], a, b, mx);
-    });
-    off.resize(lens.size() + 1);
+    off.reserve((size_t)prows[threads] + 1);
+    huge(off.data(), ((size_t)prows[threads] + 1) * 8);
+    off.resize((size_t)prows[threads] + 1);
     off[0] = 0;
-    for (size_t r = 0; r < lens.size(); r++) off[r + 1] = off[r] + lens[r];
+    run([&](int i) {
+        const int64_t want_ids = pids[i + 1] - pids[i], want_rows = prows[i + 1] - prows[i];
+        int64_t a = 0, b = 0;
+        scan_piece(p + cut[i], p + cut[i + 1], final_row(i), ids.data() + pids[i], want_ids,
+                   off.data() + prows[i] + 1, want_rows, pids[i], a, b, pmax[i]);
+        agree[i] = a == want_ids && b == want_rows;
+    });
+    for (int i = 0; i < threads; i++) {
+        idmax = std::max(idmax, pmax[i]);
+        if (!agree[i]) return false;
+    }
+    return true;
 }
 
 // Dense namespace from occurrence counts: survivors (id >= 1, ascending),
@@ -209,15 +287,19 @@ bool load_database(const std::string &path, double support, int replicate, Datab
     }
     // the replicated database is the file plus '\n' when it lacks one
     const bool newline_at_end = replicate >= 1 && (n == 0 || img[n - 1] != '\n');
-    std::vector<int32_t> ids;
-    std::vector<int64_t> off;
+    IdVec ids;
+    OffVec off;
     int32_t parsed_max = 0;
     const int threads = parse_threads > 0 ? parse_threads
                         : n < ((size_t)4 << 20) ? 1
                                                 : (int)std::min(32u, std::max(1u, std::thread::hardware_concurrency()));
-    parse_parallel(img, n, newline_at_end, threads, ids, off, parsed_max);
+    const bool parsed = parse_parallel(img, n, newline_at_end, threads, ids, off, parsed_max);
     if (map != MAP_FAILED) munmap(map, n);
     close(fd);
+    if (!parsed) {
+        err = path + ": parser passes disagree on the id or row count";
+        return false;
+    }
     const int64_t base_rows = (int64_t)off.size() - 1;
     const int64_t rep = replicate >= 1 ? replicate : 1;
     db.replicate = (int32_t)rep;
@@ -242,8 +324,8 @@ bool load_database(const std::string &path, double support, int replicate, Datab
             err = std::string("device preprocessing: ") + tfim_last_error(ctx);
             return false;
         }
-        std::vector<int32_t>().swap(ids);
-        std::vector<int64_t>().swap(off);
+        IdVec().swap(ids);
+        OffVec().swap(off);
         int32_t idmin = INT_MAX, dim = 0;
         for (int32_t x = 0; x <= parsed_max; x++)
             if (freq[x] > 0) {
@@ -338,7 +420,7 @@ bool load_database(const std::string &path, double support, int replicate, Datab
             rsum += (double)len;
         }
     }
-    std::vector<int32_t>().swap(ids);
+    IdVec().swap(ids);
     const int64_t base_red = (int64_t)roff.size() - 1;
     db.trans_num_reduced = (int32_t)(base_red * rep);
     db.red_len_min = rmin;

@@ -123,6 +123,7 @@ extern "C" {
 
 int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out) {
     if (!ctx || !out) return TFIM_E_INVALID;
+    *out = nullptr;
     tfim_mlp *m = new tfim_mlp();
     m->ctx = ctx;
     const int np = mlp_nparam();
@@ -145,8 +146,12 @@ int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out) {
     init(0, H, IN);                       // W1
     init(H * IN + H, H, H);               // W2
     init(H * IN + H + H * H + H, OUT, H); // W3
+    if (int r = tfim_mlp_set_params(m, p.data())) {
+        tfim_mlp_destroy(m);
+        return r;
+    }
     *out = m;
-    return tfim_mlp_set_params(m, p.data());
+    return TFIM_OK;
 }
 
 void tfim_mlp_destroy(tfim_mlp *m) {

@@ -198,11 +198,12 @@ int tfim_open(int device, tfim_ctx **out) {
     else snprintf(c->name, sizeof c->name, "%s, %d CUs", p.gcnArchName, c->n_cu);   // the runtime gave no marketing name
     c->hbm = (int64_t)p.totalGlobalMem;
     c->clock_mhz = p.clockRate / 1000;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return TFIM_E_HIP; }
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { delete c; return TFIM_E_HIP; }
+    // from here on a failure releases what was created (tfim_close skips null handles)
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     for (auto &sl : c->slots)
         if (hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) { delete c; return TFIM_E_HIP; }
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     *out = c;
     return TFIM_OK;
 }
