@@ -1,6 +1,8 @@
 #include "results.h"
 
 #include <algorithm>
+#include <charconv>
+#include <string>
 #include <cstdio>
 #include <cstring>
 #include <thread>
@@ -67,12 +69,41 @@ bool write_results(const std::string &path, const std::string &trans_set_file, c
     fprintf(f, "wall_time(s): %g\n", info.wall_time);
     fprintf(f, "\nMFIsNumber: %zu\n", mfis.size());
     fprintf(f, "============================List of MFIs:=============================\n");
+    // the list, formatted into one buffer per block of lines (integers by
+    // to_chars; the support text changes only with the frequency, which the
+    // order groups)
+    std::string buf;
+    buf.reserve(1 << 22);
+    char tail[96] = {0};
+    int tail_len = 0;
+    int32_t tail_freq = -1;
+    auto put = [&](long long v) {
+        char t[24];
+        const auto r = std::to_chars(t, t + sizeof t, v);
+        buf.append(t, r.ptr);
+    };
     for (size_t i = 0; i < mfis.size(); i++) {
         const Mfi &m = mfis[i];
-        fprintf(f, "%zuth: %zu { ", i + 1, m.items.size());
-        for (int32_t x : m.items) fprintf(f, "%d ", db.dense_to_orig[x]);
-        fprintf(f, "}   support: %g  frequency: %d\n", (double)m.frequency / (double)db.trans_num, m.frequency);
+        put((long long)(i + 1));
+        buf += "th: ";
+        put((long long)m.items.size());
+        buf += " { ";
+        for (int32_t x : m.items) {
+            put(db.dense_to_orig[x]);
+            buf += ' ';
+        }
+        if (m.frequency != tail_freq) {
+            tail_freq = m.frequency;
+            tail_len = snprintf(tail, sizeof tail, "}   support: %g  frequency: %d\n",
+                                (double)m.frequency / (double)db.trans_num, m.frequency);
+        }
+        buf.append(tail, (size_t)tail_len);
+        if (buf.size() >= (1 << 22) - 4096) {
+            fwrite(buf.data(), 1, buf.size(), f);
+            buf.clear();
+        }
     }
+    fwrite(buf.data(), 1, buf.size(), f);
     fprintf(f, "%s\n", bar);
     fclose(f);
     return true;
