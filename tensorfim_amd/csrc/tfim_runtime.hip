@@ -1262,6 +1262,63 @@ void prep_release(tfim_ctx *c) {
 }
 }  // namespace
 
+namespace {
+// Host -> device copy of a large pageable block on the ctx stream.  HIP's
+// pageable copy stages the data through pinned memory from one thread; here
+// up to 8 threads each take a contiguous share and move it through two
+// pinned buffers of their own, so host copies run in parallel and overlap
+// the DMA of the previous piece.  Returns after the copy has completed.
+// Blocks below 64 MB take hipMemcpyAsync directly.
+int upload_staged(tfim_ctx *c, void *dst, const void *src, size_t bytes) {
+    constexpr size_t kPiece = (size_t)8 << 20;
+    if (bytes < ((size_t)64 << 20)) {
+        HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return TFIM_OK;
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int T = (int)std::min<unsigned>(8u, hw);
+    std::vector<hipError_t> err((size_t)T, hipSuccess);
+    auto worker = [&](int t) {
+        const size_t share = (bytes / T + kPiece - 1) / kPiece * kPiece;
+        const size_t lo = std::min(bytes, (size_t)t * share), hi = std::min(bytes, lo + share);
+        char *stage[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        hipError_t e = hipSetDevice(c->device);
+        for (int b = 0; b < 2 && e == hipSuccess; b++) {
+            e = hipHostMalloc((void **)&stage[b], kPiece, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[b], hipEventDisableTiming);
+        }
+        size_t k = 0;
+        for (size_t off = lo; off < hi && e == hipSuccess; off += kPiece, k++) {
+            const int b = (int)(k & 1);
+            const size_t n = std::min(kPiece, hi - off);
+            if (k >= 2) e = hipEventSynchronize(ev[b]);   // the piece that last used this buffer has landed
+            if (e != hipSuccess) break;
+            memcpy(stage[b], static_cast<const char *>(src) + off, n);
+            e = hipMemcpyAsync(static_cast<char *>(dst) + off, stage[b], n, hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[b], c->stream);
+        }
+        for (int b = 0; b < 2; b++) {
+            if (ev[b]) {
+                const hipError_t w = hipEventSynchronize(ev[b]);
+                if (e == hipSuccess && k > (size_t)b) e = w;
+                hipEventDestroy(ev[b]);
+            }
+            if (stage[b]) hipHostFree(stage[b]);
+        }
+        err[(size_t)t] = e;
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; t++) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto &th : pool) th.join();
+    for (hipError_t e : err)
+        if (e != hipSuccess) return fail(c, TFIM_E_HIP, "staged upload failed: %s", hipGetErrorString(e));
+    return TFIM_OK;
+}
+}  // namespace
+
 int tfim_prep_begin(tfim_ctx *c, const int32_t *ids, int64_t n_ids, const int64_t *row_off, int64_t n_rows,
                     int32_t id_max, int32_t *freq) {
     if (!c || !row_off || !freq || n_ids < 0 || n_rows < 0 || id_max < 0 || (n_ids > 0 && !ids))
@@ -1284,8 +1341,8 @@ int tfim_prep_begin(tfim_ctx *c, const int32_t *ids, int64_t n_ids, const int64_
     int rc = TFIM_OK;
     int32_t bad = 0;
     do {
-        if (n_ids > 0 && hipMemcpyAsync(c->p_ids, ids, (size_t)n_ids * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
-        if (hipMemcpyAsync(c->p_off, row_off, (size_t)(n_rows + 1) * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
+        if (n_ids > 0 && upload_staged(c, c->p_ids, ids, (size_t)n_ids * 4) != TFIM_OK) { rc = TFIM_E_HIP; break; }
+        if (upload_staged(c, c->p_off, row_off, (size_t)(n_rows + 1) * 8) != TFIM_OK) { rc = TFIM_E_HIP; break; }
         if (hipMemsetAsync(d_freq, 0, (size_t)(id_max + 2) * 4, c->stream) != hipSuccess) { rc = TFIM_E_HIP; break; }
         launch_prep_hist(c->p_ids, n_ids, id_max + 1, d_freq, d_bad, c->n_cu, c->stream);
         if (hipGetLastError() != hipSuccess) { rc = TFIM_E_HIP; break; }

@@ -118,3 +118,47 @@ def test_device_preprocessing_edge_rows(tmp_path):
         hg, hr = tf.load_header(path), tf.load_header(str(ref))
         for k in HEADER_KEYS:
             assert hg[k] == hr[k], (prep, k)
+
+
+def test_device_preprocessing_staged_upload():
+    """Blocks of 64 MB and more go up through pinned buffers filled by several
+    host threads (tfim_prep_begin): about 24 M ids (96 MB, per-thread shares
+    that are not whole 8 MB pieces) reach the device intact.  Checked: the
+    occurrence counts equal numpy's bincount, the kept row count equals the
+    non-empty rows, and for a few items the counts against every item equal
+    the rows' co-occurrence counts (a misplaced piece moves ids between
+    rows)."""
+    import ctypes
+
+    import numpy as np
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(11)
+    n_rows, id_max = 600_001, 1999
+    lens = rng.integers(0, 81, n_rows)
+    off = np.zeros(n_rows + 1, np.int64)
+    np.cumsum(lens, out=off[1:])
+    n_ids = int(off[-1])
+    assert n_ids * 4 >= 64 << 20
+    ids = rng.integers(0, id_max + 1, n_ids, dtype=np.int32)
+    with tf.Context() as ctx:
+        lib = ctx._lib
+        freq = np.zeros(id_max + 1, np.int32)
+        ctx._check(lib.tfim_prep_begin(ctx._h, ids.ctypes.data, n_ids, off.ctypes.data, n_rows, id_max,
+                                       freq.ctypes.data))
+        assert np.array_equal(freq, np.bincount(ids, minlength=id_max + 1))
+        dense_of = np.arange(id_max + 1, dtype=np.int32)
+        n_red, lmin, lmax, lsum, dup = (ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(),
+                                        ctypes.c_int32())
+        ctx._check(lib.tfim_prep_finish(ctx._h, dense_of.ctypes.data, id_max + 1, id_max + 1, ctypes.byref(n_red),
+                                        ctypes.byref(lmin), ctypes.byref(lmax), ctypes.byref(lsum),
+                                        ctypes.byref(dup)))
+        ctx.n_items = ctx.n_cols = id_max + 1
+        assert n_red.value == int((lens > 0).sum())
+        m = sp.csr_matrix((np.ones(n_ids, np.int32), ids, off), shape=(n_rows, id_max + 1))
+        m.sum_duplicates()
+        m.data[:] = 1
+        mt = m.T.tocsr()
+        for x in (0, 7, 1024, id_max):
+            exp = np.asarray((mt[x] @ m).todense()).ravel()
+            assert np.array_equal(ctx.count_node([x]), exp), x
