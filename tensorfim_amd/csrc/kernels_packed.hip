@@ -13,6 +13,8 @@
 //   k_popmat_packed -- isolated C = A AND-popcount B^T (bitmap_kernel of
 //                    code/microbench/bmma_bench.cu:159-174), LDS tiled 64x64,
 //                    4x4 register micro-tiles.
+#include <mutex>
+
 #include "tfim_internal.h"
 
 namespace tfim {
@@ -494,13 +496,22 @@ bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsD
     // one workgroup per CU, so each lane counts more bits per column between
     // reductions, at half the occupancy.
     constexpr size_t kHalf = 80 * 1024, kFull = 160 * 1024;
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipFuncSetAttribute((const void *)k_count_wave_lds<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
-        hipFuncSetAttribute((const void *)k_count_wave_lds<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
-        hipFuncSetAttribute((const void *)k_count_wave_lds<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
-        hipFuncSetAttribute((const void *)k_count_wave_lds<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull);
-        attr_set = true;
+    // the kernels may take up to the full 160 KB: set once per device
+    // (contexts on several threads or devices may plan at the same time)
+    {
+        static std::mutex mu;
+        static uint64_t done = 0;   // bit per device id below 64
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return false;
+        std::lock_guard<std::mutex> g(mu);
+        if (dev >= 64 || !((done >> dev) & 1)) {
+            const void *fn[4] = {(const void *)k_count_wave_lds<1>, (const void *)k_count_wave_lds<2>,
+                                 (const void *)k_count_wave_lds<3>, (const void *)k_count_wave_lds<4>};
+            for (const void *f : fn)
+                if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull) != hipSuccess)
+                    return false;
+            if (dev < 64) done |= 1ull << dev;
+        }
     }
     int rw;
     if (wide) {
