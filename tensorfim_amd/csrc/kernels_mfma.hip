@@ -371,7 +371,19 @@ __global__ __launch_bounds__(256) void k_wave_masks(const uint4 *__restrict__ bm
     if (w >= pitch4) return;
     const int ib = item_off[r], ie = item_off[r + 1];
     uint4 m = bm[(int64_t)item_ids[ib] * pitch4 + w];
-    for (int i = ib + 1; i < ie; i++) {
+    // four rows per step, so that their loads are in flight together
+    int i = ib + 1;
+    for (; i + 3 < ie; i += 4) {
+        const uint4 v0 = bm[(int64_t)item_ids[i] * pitch4 + w];
+        const uint4 v1 = bm[(int64_t)item_ids[i + 1] * pitch4 + w];
+        const uint4 v2 = bm[(int64_t)item_ids[i + 2] * pitch4 + w];
+        const uint4 v3 = bm[(int64_t)item_ids[i + 3] * pitch4 + w];
+        m.x &= v0.x & v1.x & v2.x & v3.x;
+        m.y &= v0.y & v1.y & v2.y & v3.y;
+        m.z &= v0.z & v1.z & v2.z & v3.z;
+        m.w &= v0.w & v1.w & v2.w & v3.w;
+    }
+    for (; i < ie; i++) {
         const uint4 v = bm[(int64_t)item_ids[i] * pitch4 + w];
         m.x &= v.x;
         m.y &= v.y;

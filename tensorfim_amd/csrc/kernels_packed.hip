@@ -255,7 +255,19 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint4 *__restrict__ bm
                 m[q] = idx < pitch4 ? row[idx] : make_uint4(0, 0, 0, 0);
             }
         }
-        for (int i = ib + 1; i < ie; i++) {
+        // prefix rows two at a time, so that both rows' loads are in flight
+        // together
+        int i = ib + 1;
+        for (; i + 1 < ie; i += 2) {
+            const uint4 *r0 = bm + (int64_t)d.item_ids[i] * pitch4;
+            const uint4 *r1 = bm + (int64_t)d.item_ids[i + 1] * pitch4;
+#pragma unroll
+            for (int q = 0; q < RW; q++) {
+                const int64_t idx = base + q * 64;
+                if (idx < pitch4) m[q] = and4(m[q], and4(r0[idx], r1[idx]));
+            }
+        }
+        for (; i < ie; i++) {
             const uint4 *row = bm + (int64_t)d.item_ids[i] * pitch4;
 #pragma unroll
             for (int q = 0; q < RW; q++) {
@@ -427,7 +439,22 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
 #pragma unroll
                 for (int q = 0; q < RW; q++) m[q] = row[q * 64 + lane];
             }
-            for (int i = 1; i < k; i++) {
+            // prefix rows two at a time, so that both rows' reads are in
+            // flight together (the two ANDs fold into one 3-input op)
+            int i = 1;
+            for (; i + 1 < k && i + 1 < 64; i += 2) {
+                const uint4 *r0 = lds + __builtin_amdgcn_readlane(pslot, i) * C4;
+                const uint4 *r1 = lds + __builtin_amdgcn_readlane(pslot, i + 1) * C4;
+                uint4 a0[RW], a1[RW];
+#pragma unroll
+                for (int q = 0; q < RW; q++) {
+                    a0[q] = r0[q * 64 + lane];
+                    a1[q] = r1[q * 64 + lane];
+                }
+#pragma unroll
+                for (int q = 0; q < RW; q++) m[q] = and4(m[q], and4(a0[q], a1[q]));
+            }
+            for (; i < k; i++) {
                 const int sl = i < 64 ? __builtin_amdgcn_readlane(pslot, i) : d.item_slots[tl.ib + i];
                 const uint4 *row = lds + sl * C4;
 #pragma unroll
