@@ -461,9 +461,7 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                 for (int q = 0; q < RW; q++) m[q] = and4(m[q], row[q * 64 + lane]);
             }
             // full groups of eight columns, then groups of four for the rest
-            // (branch-free: past the tile's end the last column is read
-            // again and its count dropped), so at most three columns of a
-            // tile are counted in vain
+            // (the last one stops at the tile's end)
             int j = 0;
             for (; j + 8 <= nc; j += 8) {
                 uint32_t v[8];
@@ -484,15 +482,15 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                 }
             }
             for (; j < nc; j += 4) {
-                uint32_t v[4];
+                uint32_t v[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    const int col = j + q < nc ? j + q : nc - 1;
-                    const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, col) * C4;
+                    if (j + q >= nc) break;   // wave-uniform: columns past the tile's end are skipped
+                    const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, j + q) * C4;
                     uint32_t a = 0;
 #pragma unroll
                     for (int qq = 0; qq < RW; qq++) a = popc_and4_acc(m[qq], row[qq * 64 + lane], a);
-                    v[q] = j + q < nc ? a : 0u;
+                    v[q] = a;
                 }
                 const uint32_t r = row_reduce4_packed(v);
                 if ((lane & 7) == 0) {
