@@ -148,15 +148,15 @@ __global__ void k_mlp_to_bf16(const float *__restrict__ P, uint16_t *__restrict_
     Pb[p] = (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
 }
 
-// Inference: one wave per sample; weights from the bf16 copy (staged in LDS),
-// or the fp32 master weights when Pb is null.  A block handles groups of 4
-// samples, every wave looping the same number of times.
+// Inference: one wave per sample; weights from the bf16 copy, or the fp32
+// master weights when Pb is null, read straight from memory (21 KB, cached):
+// with no weight block in LDS the kernel fits beside the LDS-heavy count
+// kernels and runs while they do.  A block handles groups of 4 samples,
+// every wave looping the same number of times.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, const uint16_t *__restrict__ Pb,
                                                    const float *__restrict__ X, int n, float *__restrict__ out) {
-    __shared__ float W[MLP_NPARAM];
     __shared__ float sh[4][MLP_H];
-    for (int p = threadIdx.x; p < MLP_NPARAM; p += blockDim.x) W[p] = Pb ? bf16_to_f32(Pb[p]) : P[p];
-    __syncthreads();
+    auto W = [&](int p) { return Pb ? bf16_to_f32(Pb[p]) : P[p]; };
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int groups = (n + 3) / 4;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -164,20 +164,20 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float *__restrict__ P, 
         const bool valid = s_raw < n;
         const int s = valid ? s_raw : n - 1;
         const float *x = X + (int64_t)s * MLP_IN;
-        float z1 = W[OFF_B1 + lane];
+        float z1 = W(OFF_B1 + lane);
 #pragma unroll
-        for (int i = 0; i < MLP_IN; i++) z1 += W[OFF_W1 + lane * MLP_IN + i] * x[i];
+        for (int i = 0; i < MLP_IN; i++) z1 += W(OFF_W1 + lane * MLP_IN + i) * x[i];
         __syncthreads();
         sh[w][lane] = fmaxf(z1, 0.0f);
         __syncthreads();
-        float z2 = W[OFF_B2 + lane];
-        for (int i = 0; i < MLP_H; i++) z2 += W[OFF_W2 + lane * MLP_H + i] * sh[w][i];
+        float z2 = W(OFF_B2 + lane);
+        for (int i = 0; i < MLP_H; i++) z2 += W(OFF_W2 + lane * MLP_H + i) * sh[w][i];
         const float h2 = fmaxf(z2, 0.0f);
 #pragma unroll
         for (int o = 0; o < MLP_OUT; o++) {
-            float v = W[OFF_W3 + o * MLP_H + lane] * h2;
+            float v = W(OFF_W3 + o * MLP_H + lane) * h2;
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0 && valid) out[(int64_t)s * MLP_OUT + o] = v + W[OFF_B3 + o];
+            if (lane == 0 && valid) out[(int64_t)s * MLP_OUT + o] = v + W(OFF_B3 + o);
         }
     }
 }
