@@ -783,8 +783,11 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
                 int r = wave_submit_mfma(c, sl, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl,
                                          excl, min_count, n_exp, n_out);
                 if (r) return r;
-                sl.tuned = -1;
-                sl.work = 0;
+                // a wave the first tuner sent off the LDS kernel is timed
+                // for that arm as well, at what leaving the LDS kernel
+                // actually cost (here the matrix cores)
+                sl.tuned = tuned;
+                sl.work = tuned >= 0 ? work : 0;
                 sl.mf_arm = mf_arm;
                 sl.mf_work = mf_work;
                 *ticket = sl.ticket;
