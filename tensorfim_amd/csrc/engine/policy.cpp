@@ -471,7 +471,7 @@ class LearnedPolicy : public Policy {
     float explore_p_ = kExplore;
     float warm_p_ = kWarmSample;     // warm-up sample share (self-calibrated)
     int64_t warm_taken_ = 0, warm_success_ = 0;
-    float ask_p_ = 0.25f;            // share of declined nodes the model is asked about (grows while it pays)
+    float ask_p_ = kAskMin;          // share of declined nodes the model is asked about (grows while it pays)
     int64_t expl_taken_ = 0, expl_success_ = 0;
     uint8_t gate_src_ = kSrcRule;
     double extra_seconds_ = 0, forward_seconds_ = 0;
