@@ -110,10 +110,12 @@ void count_piece(const char *p, const char *end, int final_row, int64_t &n_ids, 
 // (only if it holds an id); 0 ignores them (a piece that ends right after a
 // '\n' has none).  Writes the ids from ids[0] and the end offset of each row,
 // id_base plus the ids so far, from ends[0]; returns the largest id through
-// idmax.  At most max_ids ids and max_rows rows are written (the counts of
-// count_piece); n_ids and n_rows report what the bytes hold.
+// idmax, and sets too_big if an id exceeds INT32_MAX.  At most max_ids ids
+// and max_rows rows are written (the counts of count_piece); n_ids and n_rows
+// report what the bytes hold.
 void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int64_t max_ids, int64_t *ends,
-                int64_t max_rows, int64_t id_base, int64_t &n_ids, int64_t &n_rows, int32_t &idmax) {
+                int64_t max_rows, int64_t id_base, int64_t &n_ids, int64_t &n_rows, int32_t &idmax,
+                bool &too_big) {
     int64_t ni = 0, nr = 0, row_start = 0;
     int32_t mx = idmax;
     while (p < end) {
@@ -126,8 +128,17 @@ void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int
             continue;
         }
         if (c >= '0' && c <= '9') {
-            long long v = 0;
-            while (p < end && *p >= '0' && *p <= '9') v = v * 10 + (*p++ - '0');
+            // ids past the int32 range are refused (saturated here, reported
+            // through too_big)
+            int64_t v = 0;
+            while (p < end && *p >= '0' && *p <= '9') {
+                v = v * 10 + (*p++ - '0');
+                if (v > INT32_MAX) v = (int64_t)INT32_MAX + 1;
+            }
+            if (v > INT32_MAX) {
+                too_big = true;
+                v = INT32_MAX;
+            }
             if (ni < max_ids) ids[ni] = (int32_t)v;
             mx = std::max(mx, (int32_t)v);
             ni++;
@@ -155,10 +166,11 @@ void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int
 // the joined arrays (ids and row offsets), so the rows are those of one
 // sequential pass and no thread allocates memory of its own.
 // `newline_at_end` treats the image as if a '\n' followed it (the replicated
-// database appends one when missing).  Returns false if the two passes
-// disagree (a bug; nothing is written beyond the arrays either way).
+// database appends one when missing).  Returns false, with the reason in
+// err, if an id exceeds the int32 range or the two passes disagree (a bug;
+// nothing is written beyond the arrays either way).
 bool parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, IdVec &ids, OffVec &off,
-                    int32_t &idmax) {
+                    int32_t &idmax, std::string &err) {
     threads = (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(1, threads), n / 2 + 1));
     std::vector<size_t> cut(threads + 1, n);
     cut[0] = 0;
@@ -175,7 +187,7 @@ bool parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, I
     };
     std::vector<int64_t> pids(threads + 1, 0), prows(threads + 1, 0);
     std::vector<int32_t> pmax(threads, 0);
-    std::vector<uint8_t> agree(threads, 1);
+    std::vector<uint8_t> agree(threads, 1), big(threads, 0);
     auto run = [&](auto &&fn) {
         std::vector<std::thread> pool;
         for (int i = 1; i < threads; i++) pool.emplace_back(fn, i);
@@ -204,13 +216,22 @@ bool parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, I
     run([&](int i) {
         const int64_t want_ids = pids[i + 1] - pids[i], want_rows = prows[i + 1] - prows[i];
         int64_t a = 0, b = 0;
+        bool too_big = false;
         scan_piece(p + cut[i], p + cut[i + 1], final_row(i), ids.data() + pids[i], want_ids,
-                   off.data() + prows[i] + 1, want_rows, pids[i], a, b, pmax[i]);
+                   off.data() + prows[i] + 1, want_rows, pids[i], a, b, pmax[i], too_big);
         agree[i] = a == want_ids && b == want_rows;
+        big[i] = too_big;
     });
     for (int i = 0; i < threads; i++) {
         idmax = std::max(idmax, pmax[i]);
-        if (!agree[i]) return false;
+        if (!agree[i]) {
+            err = "parser passes disagree on the id or row count";
+            return false;
+        }
+        if (big[i]) {
+            err = "an id exceeds 2147483647";
+            return false;
+        }
     }
     return true;
 }
@@ -293,11 +314,12 @@ bool load_database(const std::string &path, double support, int replicate, Datab
     const int threads = parse_threads > 0 ? parse_threads
                         : n < ((size_t)4 << 20) ? 1
                                                 : (int)std::min(32u, std::max(1u, std::thread::hardware_concurrency()));
-    const bool parsed = parse_parallel(img, n, newline_at_end, threads, ids, off, parsed_max);
+    std::string perr;
+    const bool parsed = parse_parallel(img, n, newline_at_end, threads, ids, off, parsed_max, perr);
     if (map != MAP_FAILED) munmap(map, n);
     close(fd);
     if (!parsed) {
-        err = path + ": parser passes disagree on the id or row count";
+        err = path + ": " + perr;
         return false;
     }
     const int64_t base_rows = (int64_t)off.size() - 1;
