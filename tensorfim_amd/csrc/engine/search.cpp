@@ -1185,10 +1185,12 @@ class Mafia {
         for (auto &n : pending_) stack_.push_back(std::move(n));
         pending_.clear();
         {
+            // pending decisions too: their leaf check then needs no pass of
+            // its own when they are settled (push_children checks the MFIs
+            // found in between; a lookahead taken after all leaves it unused)
             std::vector<Decision *> leaves;
             for (auto &d : hut_children_) leaves.push_back(&d);
-            for (auto &d : decisions)
-                if (!d.pending) leaves.push_back(&d);
+            for (auto &d : decisions) leaves.push_back(&d);
             check_leaves(leaves);
         }
         for (auto &d : hut_children_) push_children(d);
@@ -1232,7 +1234,6 @@ class Mafia {
         const double t0 = now();
         pol_.resolve(ptrs);
         st_.learn_seconds += now() - t0;
-        check_leaves(ptrs);
         for (auto &d : undecided_) apply(d);
         undecided_.clear();
     }
