@@ -110,7 +110,7 @@ void count_piece(const char *p, const char *end, int final_row, int64_t &n_ids, 
 // (only if it holds an id); 0 ignores them (a piece that ends right after a
 // '\n' has none).  Writes the ids from ids[0] and the end offset of each row,
 // id_base plus the ids so far, from ends[0]; returns the largest id through
-// idmax, and sets too_big if an id exceeds INT32_MAX.  At most max_ids ids
+// idmax, and sets too_big if an id exceeds kMaxItemId.  At most max_ids ids
 // and max_rows rows are written (the counts of count_piece); n_ids and n_rows
 // report what the bytes hold.
 void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int64_t max_ids, int64_t *ends,
@@ -128,16 +128,16 @@ void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int
             continue;
         }
         if (c >= '0' && c <= '9') {
-            // ids past the int32 range are refused (saturated here, reported
+            // ids past kMaxItemId are refused (saturated here, reported
             // through too_big)
             int64_t v = 0;
             while (p < end && *p >= '0' && *p <= '9') {
                 v = v * 10 + (*p++ - '0');
-                if (v > INT32_MAX) v = (int64_t)INT32_MAX + 1;
+                if (v > kMaxItemId) v = (int64_t)kMaxItemId + 1;
             }
-            if (v > INT32_MAX) {
+            if (v > kMaxItemId) {
                 too_big = true;
-                v = INT32_MAX;
+                v = kMaxItemId;
             }
             if (ni < max_ids) ids[ni] = (int32_t)v;
             mx = std::max(mx, (int32_t)v);
@@ -167,7 +167,7 @@ void scan_piece(const char *p, const char *end, int final_row, int32_t *ids, int
 // sequential pass and no thread allocates memory of its own.
 // `newline_at_end` treats the image as if a '\n' followed it (the replicated
 // database appends one when missing).  Returns false, with the reason in
-// err, if an id exceeds the int32 range or the two passes disagree (a bug;
+// err, if an id exceeds kMaxItemId or the two passes disagree (a bug;
 // nothing is written beyond the arrays either way).
 bool parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, IdVec &ids, OffVec &off,
                     int32_t &idmax, std::string &err) {
@@ -229,7 +229,7 @@ bool parse_parallel(const char *p, size_t n, bool newline_at_end, int threads, I
             return false;
         }
         if (big[i]) {
-            err = "an id exceeds 2147483647";
+            err = "an id exceeds " + std::to_string(kMaxItemId);
             return false;
         }
     }

@@ -3,6 +3,7 @@
 // 388-426, 1075-1086, 1186-1215); see the parity quirks in dataset.cpp.
 #pragma once
 
+#include <climits>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -10,6 +11,12 @@
 struct tfim_ctx;
 
 namespace tfim {
+
+// Largest item id the reader accepts.  Per-id tables are sized id_max + 1 and
+// the device histogram id_max + 2 entries, counted in int32, so two ids below
+// INT32_MAX stay free and every loop `x <= id_max` ends.  Larger ids make the
+// load fail with a message.
+constexpr int32_t kMaxItemId = INT32_MAX - 2;
 
 struct Database {
     // raw statistics (readTransSetFile)

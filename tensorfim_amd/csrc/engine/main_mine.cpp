@@ -189,8 +189,8 @@ int main(int argc, char **argv) {
            (long long)st.hutmfi_pruned, (long long)st.leaf_subsumed);
     printf("counted_bitops: %.6g\nlearn_time(s): %g\nmlp_train_steps: %lld\nmodel_lookaheads: %lld\n",
            st.counted_bitops, st.learn_seconds, (long long)st.mlp_train_steps, (long long)st.model_lookaheads);
-    printf("model_lookahead_hits: %lld\ngate_threshold: %g\nexplore_share: %g\n", (long long)st.model_lookahead_hits,
-           st.gate_threshold, st.explore_share);
+    printf("model_lookahead_hits: %lld\ngate_threshold: %g\nexplore_share: %g\nmodel_failures: %lld\n",
+           (long long)st.model_lookahead_hits, st.gate_threshold, st.explore_share, (long long)st.model_failures);
     tfim_close(ctx);
     return 0;
 }

@@ -50,7 +50,10 @@ class LearnedPolicy : public Policy {
    public:
     LearnedPolicy(tfim_ctx *ctx, const SearchOptions &opt) : ctx_(ctx), opt_(opt), plain_(false) {
         const auto t0 = std::chrono::steady_clock::now();
-        if (tfim_mlp_create(ctx, 0x5EEDull, &mlp_) != TFIM_OK) mlp_ = nullptr;
+        if (tfim_mlp_create(ctx, 0x5EEDull, &mlp_) != TFIM_OK) {
+            mlp_ = nullptr;
+            create_error_ = std::string("the learned policy's network could not be created: ") + tfim_last_error(ctx);
+        }
         if (mlp_ && !opt.model_path.empty()) {
             std::vector<float> p(TFIM_MLP_PARAMS);
             FILE *f = fopen(opt.model_path.c_str(), "rb");
@@ -129,7 +132,7 @@ class LearnedPolicy : public Policy {
         if (deferred) {
             if (n > 0) {
                 if (tfim_mlp_forward_submit(mlp_, infer_.data(), (int32_t)n, 1) != TFIM_OK) {
-                    trained_ = false;   // no model: the rule decides from here on
+                    model_failed("inference");   // no model: the rule decides from here on
                 } else {
                     pend_feats_ = infer_;
                     pend_n_ = n;
@@ -140,7 +143,7 @@ class LearnedPolicy : public Policy {
         } else if (use_model && n > 0) {
             const auto tf0 = std::chrono::steady_clock::now();
             logits_.resize((size_t)n * 2);
-            if (tfim_mlp_forward(mlp_, infer_.data(), (int32_t)n, logits_.data(), 1) != TFIM_OK) trained_ = false;
+            if (tfim_mlp_forward(mlp_, infer_.data(), (int32_t)n, logits_.data(), 1) != TFIM_OK) model_failed("inference");
             forward_seconds_ += secs_since(tf0);
             forward_calls_++;
             forward_rows_ += n;
@@ -197,7 +200,10 @@ class LearnedPolicy : public Policy {
         if (pend_n_ > 0) {
             const auto t0 = std::chrono::steady_clock::now();
             logits_.resize((size_t)pend_n_ * 2);
-            ok = tfim_mlp_forward_wait(mlp_, logits_.data()) == TFIM_OK && ok;
+            if (tfim_mlp_forward_wait(mlp_, logits_.data()) != TFIM_OK) {
+                model_failed("inference");
+                ok = false;
+            }
             forward_seconds_ += secs_since(t0);
         }
         for (size_t q = 0; q < ds.size(); q++) {
@@ -227,6 +233,7 @@ class LearnedPolicy : public Policy {
         st.model_lookahead_hits = model_success_;
         st.gate_threshold = gate_p_;
         st.explore_share = explore_p_;
+        st.model_failures = failures_;
         st.learn_seconds += extra_seconds_;   // set-up and training (decide() is timed by the engine)
         if (opt_.verbose)
             fprintf(stderr, "learned policy: set-up+training %.4f s (%lld steps), forward %.4f s in %lld calls (%lld rows)\n",
@@ -246,6 +253,14 @@ class LearnedPolicy : public Policy {
     }
 
    private:
+    // A failed model call leaves the rule in charge from then on; it is
+    // reported on stderr (once) and counted in the run statistics.
+    void model_failed(const char *what) {
+        if (failures_++ == 0) fprintf(stderr, "learned policy: %s failed (%s); the rule decides from here on\n", what,
+                                      tfim_last_error(ctx_));
+        trained_ = false;
+        disabled_ = true;
+    }
     static double secs_since(std::chrono::steady_clock::time_point t) {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
     }
@@ -398,7 +413,7 @@ class LearnedPolicy : public Policy {
     }
 
     void train() {
-        if (!mlp_) return;
+        if (!mlp_ || disabled_) return;
         const auto t0 = std::chrono::steady_clock::now();
         struct Acc {
             double &s;
@@ -423,11 +438,14 @@ class LearnedPolicy : public Policy {
         for (int e = 0; e < kEpochs; e++) {
             // the last step reports the loss (and waits); the others stay queued
             float *lp = (e == kEpochs - 1 && opt_.verbose) ? &loss : nullptr;
-            if (tfim_mlp_train_step(mlp_, tx_.data(), ty_.data(), tm_.data(), take, 2e-3f, lp) != TFIM_OK) return;
+            if (tfim_mlp_train_step(mlp_, tx_.data(), ty_.data(), tm_.data(), take, 2e-3f, lp) != TFIM_OK) {
+                model_failed("training");
+                return;
+            }
             steps_++;
         }
         if (opt_.verbose) fprintf(stderr, "mlp train: %d records, loss %.4f\n", take, loss);
-        trained_ = true;
+        trained_ = !disabled_;
     }
 
     static constexpr int64_t kTrainEvery = 8192;
@@ -455,9 +473,16 @@ class LearnedPolicy : public Policy {
     tfim_ctx *ctx_;
     const SearchOptions &opt_;
     tfim_mlp *mlp_ = nullptr;
+
+   public:
+    std::string create_error_;   // non-empty: no network (the run must not be labelled learned)
+
+   private:
     FILE *trace_ = nullptr;
     PlainPolicy plain_;
     bool trained_ = false;
+    bool disabled_ = false;   // a model call failed: no further training or inference
+    int64_t failures_ = 0;
     double N_ = 1, thr_ = 1;
     const std::vector<int32_t> *freq_ = nullptr;
     NodeAgg cur_{};
@@ -490,8 +515,15 @@ class LearnedPolicy : public Policy {
 
 }  // namespace
 
-std::unique_ptr<Policy> make_policy(tfim_ctx *ctx, const SearchOptions &opt) {
-    if (opt.learned) return std::unique_ptr<Policy>(new LearnedPolicy(ctx, opt));
+std::unique_ptr<Policy> make_policy(tfim_ctx *ctx, const SearchOptions &opt, std::string &err) {
+    if (opt.learned) {
+        std::unique_ptr<LearnedPolicy> p(new LearnedPolicy(ctx, opt));
+        if (!p->create_error_.empty()) {
+            err = p->create_error_;
+            return nullptr;
+        }
+        return std::unique_ptr<Policy>(p.release());
+    }
     return std::unique_ptr<Policy>(new PlainPolicy(opt.gate == "always"));
 }
 

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "dataset.h"
@@ -63,6 +64,8 @@ class PlainPolicy : public Policy {
     bool always_;
 };
 
-std::unique_ptr<Policy> make_policy(tfim_ctx *ctx, const SearchOptions &opt);
+// Returns nullptr, with the reason in err, when --learned is asked for and the
+// network cannot be created (the run then fails rather than mining without it).
+std::unique_ptr<Policy> make_policy(tfim_ctx *ctx, const SearchOptions &opt, std::string &err);
 
 }  // namespace tfim

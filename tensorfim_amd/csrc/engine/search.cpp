@@ -1299,7 +1299,8 @@ int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std:
             const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
             return run_wave(ctx, db, cap, out, st, err);
         }
-        std::unique_ptr<Policy> pol = make_policy(ctx, opt);
+        std::unique_ptr<Policy> pol = make_policy(ctx, opt, err);
+        if (!pol) return TFIM_E_STATE;
         Mafia m(ctx, db, opt, *pol, st);
         const int rc = m.run(out, err);
         pol->finish(st);

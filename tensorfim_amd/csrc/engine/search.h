@@ -66,6 +66,7 @@ struct SearchStats {
     int64_t model_lookahead_hits = 0;   // of those (exploration excluded), the ones whose HUT was frequent
     double gate_threshold = 0;          // learned gate's final self-calibrated threshold
     double explore_share = 0;           // and its final exploration share
+    int64_t model_failures = 0;         // model calls that failed (the rule decided from then on)
 };
 
 // Runs the engine; returns MFIs (dense ids, insertion order) with frequencies.

@@ -349,7 +349,7 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
     L.grid = dim3((unsigned)(tm * tn), (unsigned)zs);
     L.block = dim3(256);
     L.shmem = 0;
-    L.n_args = L.used = 0;
+    L.reset();
     L.arg(A);
     L.arg(B);
     L.arg(C);
@@ -400,7 +400,7 @@ void plan_wave_masks(const uint64_t *bm, int64_t pitch_words, const int32_t *ite
     L.grid = dim3((unsigned)((pitch4 + 255) / 256), (unsigned)n_rows);
     L.block = dim3(256);
     L.shmem = 0;
-    L.n_args = L.used = 0;
+    L.reset();
     L.arg(reinterpret_cast<const uint4 *>(bm));
     L.arg(pitch4);
     L.arg(item_ids);
@@ -448,7 +448,7 @@ void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32
     L.grid = dim3((unsigned)std::max(1, (n_rows + 3) / 4), (unsigned)slices);
     L.block = dim3(256);
     L.shmem = 0;
-    L.n_args = L.used = 0;
+    L.reset();
     L.arg(d);
     L.arg(C);
     L.arg(n_rows);

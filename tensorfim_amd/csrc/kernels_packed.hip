@@ -347,7 +347,7 @@ void plan_compl_reduce(const WaveDesc &d, const int32_t *compl_rows, int32_t n_c
     L.grid = dim3((unsigned)std::max(1, (n_compl + 3) / 4));
     L.block = dim3(256);
     L.shmem = 0;
-    L.n_args = L.used = 0;
+    L.reset();
     L.arg(d);
     L.arg(compl_rows);
     L.arg(n_compl);
@@ -367,7 +367,7 @@ void plan_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &d,
     L.grid = dim3((unsigned)grid_blocks);
     L.block = dim3(256);
     L.shmem = 0;
-    L.n_args = L.used = 0;
+    L.reset();
     L.arg(reinterpret_cast<const uint4 *>(bm));
     L.arg((int64_t)(pitch_words / 2));
     L.arg(d);
@@ -559,7 +559,7 @@ bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsD
     L.grid = dim3((unsigned)grid);
     L.block = dim3(kLdsThreads);
     L.shmem = (uint32_t)lds_bytes;
-    L.n_args = L.used = 0;
+    L.reset();
     L.arg(reinterpret_cast<const uint4 *>(bm));
     L.arg(pitch4);
     L.arg(d);
@@ -587,7 +587,7 @@ void plan_zero_i32(int32_t *p, int64_t n, KernelLaunch &L) {
     L.grid = dim3((unsigned)blocks);
     L.block = dim3(256);
     L.shmem = 0;
-    L.n_args = L.used = 0;
+    L.reset();
     L.arg(p);
     L.arg(n);
 }

@@ -19,7 +19,9 @@ inline int64_t pitch_for_bits(int64_t n_bits) {
 
 // One kernel launch, described so that it can be issued directly or set as
 // the parameters of a graph kernel node.  Arguments live in `buf`; params()
-// returns the pointer array HIP expects (valid while this object lives).
+// returns the pointer array HIP expects (valid while this object lives).  An
+// argument list that overflows `buf` marks the launch invalid (`overflow`):
+// launch() then returns hipErrorInvalidValue and the graph path refuses it.
 struct KernelLaunch {
     const void *func = nullptr;
     dim3 grid{1, 1, 1}, block{1, 1, 1};
@@ -28,23 +30,32 @@ struct KernelLaunch {
     alignas(16) unsigned char buf[256];
     int off[kMaxArgs] = {};
     int n_args = 0, used = 0;
+    bool overflow = false;
     void *ptrs[kMaxArgs] = {};
 
+    void reset() {
+        n_args = used = 0;
+        overflow = false;
+    }
     template <class T>
     void arg(const T &v) {
         const int a = (int)alignof(T) < 8 ? 8 : (int)alignof(T);
-        used = (used + a - 1) / a * a;
+        const int at = (used + a - 1) / a * a;
         static_assert(sizeof(T) <= sizeof(buf), "argument too large");
-        if (n_args >= kMaxArgs || used + (int)sizeof(T) > (int)sizeof(buf)) __builtin_trap();
-        __builtin_memcpy(buf + used, &v, sizeof(T));
-        off[n_args++] = used;
-        used += (int)sizeof(T);
+        if (n_args >= kMaxArgs || at + (int)sizeof(T) > (int)sizeof(buf)) {
+            overflow = true;
+            return;
+        }
+        __builtin_memcpy(buf + at, &v, sizeof(T));
+        off[n_args++] = at;
+        used = at + (int)sizeof(T);
     }
     void **params() {
         for (int i = 0; i < n_args; i++) ptrs[i] = buf + off[i];
         return ptrs;
     }
     hipError_t launch(hipStream_t s) {
+        if (overflow || !func) return hipErrorInvalidValue;
         return hipLaunchKernel(func, grid, block, params(), shmem, s);
     }
 };

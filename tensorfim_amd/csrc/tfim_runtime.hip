@@ -451,6 +451,8 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fil
                KernelLaunch *const *ks, int nk) {
     KernelLaunch Z;
     plan_zero_i32(fill, n_fill, Z);
+    for (int i = 0; i < nk; i++)
+        if (ks[i]->overflow || !ks[i]->func) return fail(c, TFIM_E_INVALID, "wave kernel %d: bad launch description", i);
     const size_t out_bytes = (size_t)n_out * 4;
     if (!c->use_graphs) {
         HIPCHK(c, hipMemcpyAsync(sl.d_desc, sl.h_desc, (size_t)desc_bytes, hipMemcpyHostToDevice, c->stream));
@@ -1329,6 +1331,8 @@ int tfim_prep_begin(tfim_ctx *c, const int32_t *ids, int64_t n_ids, const int64_
     if (row_off[0] != 0 || row_off[n_rows] != n_ids)
         return fail(c, TFIM_E_INVALID, "prep_begin: row offsets must run from 0 to n_ids");
     if (n_rows > (int64_t)INT32_MAX) return fail(c, TFIM_E_RANGE, "prep_begin: %lld rows exceed int32", (long long)n_rows);
+    // the histogram holds id_max + 2 int32 entries (counts and a flag)
+    if (id_max > INT32_MAX - 2) return fail(c, TFIM_E_RANGE, "prep_begin: id_max %d above %d", id_max, INT32_MAX - 2);
     if (!c->owns_bm) return fail(c, TFIM_E_STATE, "prep_begin: context shares its parent's bitmaps");
     if (int r = set_device(c)) return r;
     prep_release(c);

@@ -43,8 +43,9 @@ int main() {
             if (ok1 != ok2 || (ok1 && (i1 != i2 || o1 != o2 || m1 != m2))) { if (bad++ < 10) fprintf(stderr, "parallel mismatch th=%d len=%d\n", th, len); }
         }
     }
-    // ids up to INT32_MAX parse; one past it (or a longer run) is refused
-    struct { const char *text; bool ok; } lim[] = {{"1 2147483647\n", true}, {"1 2147483648\n", false},
+    // ids up to kMaxItemId (INT32_MAX - 2) parse; one past it (or a longer run) is refused
+    struct { const char *text; bool ok; } lim[] = {{"1 2147483645\n", true}, {"1 2147483646\n", false},
+                                                  {"1 2147483647\n", false}, {"1 2147483648\n", false},
                                                   {"5 99999999999999999999999\n", false}, {"-99999999999999999999 3\n", true}};
     for (auto &t : lim) {
         tfim::IdVec i3; tfim::OffVec o3; int32_t m3 = 0; std::string e3;

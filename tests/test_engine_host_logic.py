@@ -125,3 +125,17 @@ def test_device_preprocessing_flow_matches_host(mock_engine, tmp_path):
             res.append((r.returncode, tf.load_results(str(out)) if r.returncode == 0 else None,
                         {k: tf.load_header(str(out)).get(k) for k in keys} if r.returncode == 0 else None))
         assert res[0] == res[1], (it, txt[:200], sup, res)
+
+
+def test_ids_past_the_limit_are_refused(mock_engine, tmp_path):
+    """Ids above 2,147,483,645 (INT32_MAX - 2: per-id tables hold id_max + 2
+    int32 entries) make the load fail with a message on both preprocessing
+    paths instead of wrapping or looping."""
+    for big in ("2147483646", "2147483647", "99999999999"):
+        db = tmp_path / "big.txt"
+        db.write_text("1 2 3\n2 %s\n" % big)
+        for prep in ("host", "gpu"):
+            r = subprocess.run([mock_engine, str(db), "0.5", "--out", str(tmp_path / "x.res"), "--prep", prep],
+                               capture_output=True, text=True, timeout=60)
+            assert r.returncode != 0, (big, prep, r.stdout, r.stderr)
+            assert "exceeds 2147483645" in (r.stdout + r.stderr), (big, prep, r.stdout, r.stderr)
