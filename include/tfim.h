@@ -91,6 +91,21 @@ int tfim_item_supports(tfim_ctx *ctx, int32_t *out);
  * for every survivor e < n_cols; synchronous, `out` is host memory. */
 int tfim_count_node(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): tfim_count_node without any allocation.  items is host
+ * memory (1 <= k <= TFIM_NOALLOC_MAX_ITEMS ids, checked here and passed to
+ * the kernel by value); out is DEVICE memory owned by the caller, n_cols
+ * int32, written asynchronously on the ctx stream (tfim_sync waits).  The
+ * kernel's partial counts live in a fixed per-device scratch array of the
+ * library (8 x TFIM_NOALLOC_MAX_COLS int32 = 256 KiB, kept until the process
+ * ends); more than TFIM_NOALLOC_MAX_COLS survivors is TFIM_E_RANGE.  Every
+ * call on one GPU shares that array, so calls that overlap on one GPU (other
+ * contexts or streams) race; calls on one ctx are ordered by its stream. */
+#define TFIM_NOALLOC_MAX_COLS 8192
+#define TFIM_NOALLOC_MAX_ITEMS 60
+int tfim_count_node_noalloc(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t *out);
+
 /* Times `reps` back-to-back launches of tfim_count_node's kernel for one node
  * (after `warmup` untimed ones) with HIP events on the ctx stream; avg_ms =
  * mean per launch.  Measurement only: no counts are returned.  Stands beside

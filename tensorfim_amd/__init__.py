@@ -47,6 +47,7 @@ _SIGS = {
                                                      ctypes.c_int32]),
     "tfim_item_supports": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "tfim_count_node": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
+    "tfim_count_node_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]),
     "tfim_count_nodes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]),
     "tfim_count_nodes_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
@@ -210,6 +211,14 @@ class Context:
         out = np.empty(self.n_cols, dtype=np.int32)
         self._check(self._lib.tfim_count_node(self._h, _ptr(it), len(it), _ptr(out)))
         return out
+
+    def count_node_noalloc(self, items, out_dev_ptr):
+        """tfim_count_node_noalloc: the plain path's counts written to the
+        caller's device buffer at `out_dev_ptr` (n_cols int32), asynchronous
+        on the context's stream, without allocating; call sync() before
+        reading the buffer."""
+        it = np.ascontiguousarray(items, dtype=np.int32)
+        self._check(self._lib.tfim_count_node_noalloc(self._h, _ptr(it), len(it), ctypes.c_void_p(int(out_dev_ptr))))
 
     def time_count_node(self, items, warmup=3, reps=100):
         """Mean kernel ms of the plain per-node count for one node (HIP events)."""

@@ -125,10 +125,12 @@ __device__ __forceinline__ uint32_t row_reduce4_packed(const uint32_t v[4]) {
 // blocks of cpb <= 64 columns.  Every block ANDs the k prefix rows over its
 // chunk once (registers), then streams its columns 8*G at a time: the R*8*G
 // loads of a batch are independent, so they are all in flight together.
-template <int R, int G>
-__global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm, int64_t pitch4,
-                                                    const int32_t *__restrict__ items, int k, int n_cols,
-                                                    int cpb, int32_t *__restrict__ out, int64_t spread_stride) {
+// `item(i)` gives prefix item i (from device memory, or from the kernel's
+// argument block).
+template <int R, int G, class ItemFn>
+__device__ __forceinline__ void count_node_body(const uint4 *__restrict__ bm, int64_t pitch4, ItemFn item, int k,
+                                                int n_cols, int cpb, int32_t *__restrict__ out,
+                                                int64_t spread_stride) {
     __shared__ int red[4][64];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t base = (int64_t)blockIdx.x * (256 * R) + tid;
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
     const int c1 = min(n_cols, c0 + cpb);
     uint4 m[R];
     {
-        const uint4 *row = bm + (int64_t)items[0] * pitch4;
+        const uint4 *row = bm + (int64_t)item(0) * pitch4;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int64_t idx = base + r * 256;
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
         }
     }
     for (int i = 1; i < k; i++) {
-        const uint4 *row = bm + (int64_t)items[i] * pitch4;
+        const uint4 *row = bm + (int64_t)item(i) * pitch4;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int64_t idx = base + r * 256;
@@ -186,15 +188,31 @@ __global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm
     }
 }
 
+template <int R, int G>
+__global__ __launch_bounds__(256) void k_count_node(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                    const int32_t *__restrict__ items, int k, int n_cols,
+                                                    int cpb, int32_t *__restrict__ out, int64_t spread_stride) {
+    count_node_body<R, G>(bm, pitch4, [&](int i) { return items[i]; }, k, n_cols, cpb, out, spread_stride);
+}
+
+// The same with the prefix passed in the kernel's argument block (nothing to
+// copy to device memory first).
+template <int R, int G>
+__global__ __launch_bounds__(256) void k_count_node_args(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                         NodeItems items, int n_cols, int cpb,
+                                                         int32_t *__restrict__ out, int64_t spread_stride) {
+    count_node_body<R, G>(bm, pitch4, [&](int i) { return items.id[i]; }, items.n, n_cols, cpb, out,
+                          spread_stride);
+}
+
 // shape = R * 100 + cpb (R in {1, 2, 4}, cpb in {8, 16, 32, 64}).  0 picks
 // from measurements on MI355X: (2, 32) once that gives >= 8 blocks per CU,
 // else (1, 16); short, narrow nodes are latency-bound, and more blocks with
 // fewer columns each keep more loads in flight.
-void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
-                       int32_t n_cols, int32_t *out, int64_t spread_stride, int n_cu, int shape, hipStream_t s) {
-    const int64_t pitch4 = pitch_words / 2;
-    auto blocks = [&](int R, int cpb) { return ((pitch4 + 256 * R - 1) / (256 * R)) * ((n_cols + cpb - 1) / cpb); };
-    int R = shape / 100, cpb = shape % 100;
+static void node_shape_of(int64_t pitch4, int32_t n_cols, int n_cu, int shape, int &R, int &cpb) {
+    auto blocks = [&](int r, int c) { return ((pitch4 + 256 * r - 1) / (256 * r)) * ((n_cols + c - 1) / c); };
+    R = shape / 100;
+    cpb = shape % 100;
     if (shape == 0) {
         R = 1;
         cpb = 16;
@@ -203,6 +221,13 @@ void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *i
             cpb = 32;
         }
     }
+}
+
+void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
+                       int32_t n_cols, int32_t *out, int64_t spread_stride, int n_cu, int shape, hipStream_t s) {
+    const int64_t pitch4 = pitch_words / 2;
+    int R, cpb;
+    node_shape_of(pitch4, n_cols, n_cu, shape, R, cpb);
     const dim3 grid((unsigned)((pitch4 + 256 * R - 1) / (256 * R)), (unsigned)((n_cols + cpb - 1) / cpb));
     const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
     if (cpb == 8) {
@@ -219,6 +244,43 @@ void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *i
     } else {
         hipLaunchKernelGGL((k_count_node<1, 2>), grid, dim3(256), 0, s, b4, pitch4, items, k, n_cols, cpb, out, spread_stride);
     }
+}
+
+void launch_count_node_args(const uint64_t *bm, int64_t pitch_words, const NodeItems &items, int32_t n_cols,
+                            int32_t *out, int64_t spread_stride, int n_cu, int shape, hipStream_t s) {
+    const int64_t pitch4 = pitch_words / 2;
+    int R, cpb;
+    node_shape_of(pitch4, n_cols, n_cu, shape, R, cpb);
+    const dim3 grid((unsigned)((pitch4 + 256 * R - 1) / (256 * R)), (unsigned)((n_cols + cpb - 1) / cpb));
+    const uint4 *b4 = reinterpret_cast<const uint4 *>(bm);
+    if (cpb == 8 || R == 4) {
+        if (R == 4)
+            hipLaunchKernelGGL((k_count_node_args<4, 1>), grid, dim3(256), 0, s, b4, pitch4, items, n_cols, cpb, out, spread_stride);
+        else if (R == 2)
+            hipLaunchKernelGGL((k_count_node_args<2, 1>), grid, dim3(256), 0, s, b4, pitch4, items, n_cols, cpb, out, spread_stride);
+        else
+            hipLaunchKernelGGL((k_count_node_args<1, 1>), grid, dim3(256), 0, s, b4, pitch4, items, n_cols, cpb, out, spread_stride);
+    } else if (R == 2) {
+        hipLaunchKernelGGL((k_count_node_args<2, 2>), grid, dim3(256), 0, s, b4, pitch4, items, n_cols, cpb, out, spread_stride);
+    } else {
+        hipLaunchKernelGGL((k_count_node_args<1, 2>), grid, dim3(256), 0, s, b4, pitch4, items, n_cols, cpb, out, spread_stride);
+    }
+}
+
+// out[e] = sum over j < 8 of copies[j * stride + e], e < n.
+__global__ __launch_bounds__(256) void k_sum_copies8(const int32_t *__restrict__ copies, int64_t stride, int32_t n,
+                                                     int32_t *__restrict__ out) {
+    const int e = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (e >= n) return;
+    int32_t t = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) t += copies[(int64_t)j * stride + e];
+    out[e] = t;
+}
+
+void launch_sum_copies8(const int32_t *copies, int64_t stride, int32_t n, int32_t *out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sum_copies8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, copies, stride, n, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -69,6 +69,18 @@ struct KernelLaunch {
 void launch_count_node(const uint64_t *bm, int64_t pitch_words, const int32_t *items, int32_t k,
                        int32_t n_cols, int32_t *out, int64_t spread_stride, int n_cu, int shape, hipStream_t s);
 
+// A node's prefix passed by value in the kernel's argument block.
+constexpr int kNodeArgItems = TFIM_NOALLOC_MAX_ITEMS;
+struct NodeItems {
+    int32_t n;
+    int32_t id[kNodeArgItems];
+};
+// launch_count_node with the prefix in the argument block.
+void launch_count_node_args(const uint64_t *bm, int64_t pitch_words, const NodeItems &items, int32_t n_cols,
+                            int32_t *out, int64_t spread_stride, int n_cu, int shape, hipStream_t s);
+// out[e] = sum of the 8 copies copies[j * stride + e], e < n.
+void launch_sum_copies8(const int32_t *copies, int64_t stride, int32_t n, int32_t *out, hipStream_t s);
+
 // One tile of a wave: a row's prefix, and a run of its columns.  compl == 0:
 // the columns are entries [col_begin, col_end) of the row's column list.
 // compl == j + 1: the row is complement row j, the columns are the survivor

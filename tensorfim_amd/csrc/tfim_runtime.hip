@@ -119,6 +119,8 @@ struct tfim_ctx {
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
     std::map<std::string, std::pair<int, int32_t>> popmat_tuned;   // shape -> (tile, split_k)
 
+    int32_t *d_scratch = nullptr;   // this device's `scratch` (tfim_count_node_noalloc)
+
     // device preprocessing state between tfim_prep_begin and tfim_prep_finish
     int32_t *p_ids = nullptr;
     int64_t *p_off = nullptr;
@@ -174,6 +176,34 @@ int set_device(tfim_ctx *c) {
 
 }  // namespace
 
+// Scratch of tfim_count_node_noalloc: the 8 spread copies of one node's
+// counts (partial results that its kernel's atomics are spread over), for at
+// most TFIM_NOALLOC_MAX_COLS survivors: 256 KiB per device, held until the
+// process ends.  Host code must not pass `scratch` itself to a kernel (that
+// name is a host-side stand-in): it uses the device address
+// hipGetSymbolAddress returns, looked up once per device in tfim_open.
+static __device__ __align__(256) int32_t scratch[8 * TFIM_NOALLOC_MAX_COLS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: the device address of `scratch` on it (once per
+// device; later calls return the kept address).
+int scratch_setup(int device, int32_t **ptr) {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return TFIM_E_RANGE;
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return TFIM_E_HIP;
+        g_scratch_ptr[device] = static_cast<int32_t *>(p);
+    }
+    *ptr = g_scratch_ptr[device];
+    return TFIM_OK;
+}
+}  // namespace
+
 namespace tfim {
 hipStream_t ctx_stream(tfim_ctx *c) { return c->stream; }
 int ctx_device(tfim_ctx *c) { return c->device; }
@@ -198,6 +228,8 @@ int tfim_open(int device, tfim_ctx **out) {
     else snprintf(c->name, sizeof c->name, "%s, %d CUs", p.gcnArchName, c->n_cu);   // the runtime gave no marketing name
     c->hbm = (int64_t)p.totalGlobalMem;
     c->clock_mhz = p.clockRate / 1000;
+    // device address of the no-allocation entry point's scratch (once per device)
+    if (device < kMaxDevices && scratch_setup(device, &c->d_scratch) != TFIM_OK) { delete c; return TFIM_E_HIP; }
     // from here on a failure releases what was created (tfim_close skips null handles)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
@@ -398,6 +430,34 @@ int tfim_count_node(tfim_ctx *c, const int32_t *items, int32_t k, int32_t *out) 
     } else {
         memcpy(out, c->h_node_out, (size_t)c->n_cols * 4);
     }
+    return TFIM_OK;
+}
+
+int tfim_count_node_noalloc(tfim_ctx *c, const int32_t *items, int32_t k, int32_t *out) {
+    if (!c || !items || !out) return fail(c, TFIM_E_INVALID, "count_node_noalloc: null argument");
+    if (k < 1 || k > TFIM_NOALLOC_MAX_ITEMS)
+        return fail(c, TFIM_E_RANGE, "count_node_noalloc: %d prefix items (1..%d accepted)", k, TFIM_NOALLOC_MAX_ITEMS);
+    if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_node_noalloc: no bitmaps loaded");
+    if (c->n_cols > TFIM_NOALLOC_MAX_COLS)
+        return fail(c, TFIM_E_RANGE, "count_node_noalloc: %d survivors exceed the scratch limit of %d", c->n_cols,
+                    TFIM_NOALLOC_MAX_COLS);
+    if (!c->d_scratch) return fail(c, TFIM_E_STATE, "count_node_noalloc: no scratch on device %d", c->device);
+    NodeItems it{};
+    it.n = k;
+    for (int32_t i = 0; i < k; i++) {
+        if (items[i] < 0 || items[i] >= c->n_items)
+            return fail(c, TFIM_E_INVALID, "count_node_noalloc: item %d out of range [0,%d)", items[i], c->n_items);
+        it.id[i] = items[i];
+    }
+    if (c->n_cols == 0) return TFIM_OK;
+    if (int r = set_device(c)) return r;
+    // the 8 copies, 1024-int aligned; cleared by a fill kernel on every call
+    // (the array keeps the previous call's partial counts)
+    const int64_t stride = ((int64_t)c->n_cols + 1023) / 1024 * 1024;
+    launch_zero_i32(c->d_scratch, 8 * stride, c->stream);
+    launch_count_node_args(c->d_bm, c->pitch, it, c->n_cols, c->d_scratch, stride, c->n_cu, c->node_shape, c->stream);
+    launch_sum_copies8(c->d_scratch, stride, c->n_cols, out, c->stream);
+    HIPCHK(c, hipGetLastError());
     return TFIM_OK;
 }
 

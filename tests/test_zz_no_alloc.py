@@ -1,0 +1,61 @@
+"""GPU: tfim_count_node_noalloc (the plain per-node count without any device
+allocation, its partial counts in the library's fixed scratch array) gives
+exactly what tfim_count_node gives, on two calls in a row -- the second on a
+smaller database, so counts left in the scratch array by the first call would
+show -- and refuses sizes beyond its limit with a clear error.  The output
+buffer is caller-owned device memory (a torch tensor here)."""
+import numpy as np
+import pytest
+
+import oracle_lib
+import tensorfim_amd as tf
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_rows(rng, n_items, n_bits, density):
+    words = (n_bits + 63) // 64
+    bits = rng.random((n_items, words * 64)) < density
+    bits[:, n_bits:] = False
+    return np.packbits(bits, axis=1, bitorder="little").view(np.uint64).reshape(n_items, words)
+
+
+def noalloc_counts(ctx, torch, items, n_cols):
+    out = torch.full((n_cols,), -7, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    ctx.count_node_noalloc(items, out.data_ptr())
+    ctx.sync()
+    return out.cpu().numpy()
+
+
+def test_noalloc_matches_count_node_twice():
+    torch = pytest.importorskip("torch")
+    with tf.Context(0) as ctx:
+        rng = np.random.default_rng(31)
+        cases = [(3000, 1_000_003, [17, 400, 2999, 5]),   # 8 copies of 3,072 counts
+                 (70, 5_001, [3, 69])]                      # smaller: stale partial counts would show
+        for n_cols, n_bits, prefix in cases:
+            print("case: %d survivors, %d transactions, prefix %s" % (n_cols, n_bits, prefix), flush=True)
+            rows = rand_rows(rng, n_cols + 1, n_bits, 0.5)    # one extra prefix-only row
+            ctx.load_bitmaps(rows, n_bits, n_cols)
+            for p in (prefix, prefix + [n_cols]):
+                ref = ctx.count_node(p)
+                got = noalloc_counts(ctx, torch, p, n_cols)
+                np.testing.assert_array_equal(got, ref, err_msg="prefix %s" % p)
+                np.testing.assert_array_equal(got, oracle_lib.node_counts(rows, p, n_cols))
+
+
+def test_noalloc_rejects_sizes_beyond_its_limit():
+    torch = pytest.importorskip("torch")
+    with tf.Context(0) as ctx:
+        rng = np.random.default_rng(2)
+        rows = rand_rows(rng, 8193, 300, 0.5)
+        ctx.load_bitmaps(rows, 300, 8193)              # one survivor above TFIM_NOALLOC_MAX_COLS
+        out = torch.zeros(8193, dtype=torch.int32, device="cuda:0")
+        with pytest.raises(tf.TfimError, match="scratch limit"):
+            ctx.count_node_noalloc([1, 2], out.data_ptr())
+        ctx.load_bitmaps(rows[:100], 300, 100)
+        with pytest.raises(tf.TfimError, match="prefix items"):
+            ctx.count_node_noalloc(list(range(61)), out.data_ptr())
+        with pytest.raises(tf.TfimError, match="out of range"):
+            ctx.count_node_noalloc([1, 100], out.data_ptr())
