@@ -1,7 +1,7 @@
 """Benchmark: exact support counting on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--no-cpu] [--no-per-node] [--tuned]
-                    [--paths mfma,packed]
+                    [--paths mfma,packed] [--no-sweep] [--sustain-seconds S]
     python bench.py --copy-calibration MIB     (a plain device copy, for counter scaling)
 
 A step is one pass of the hot path over one batch: the isolated count
@@ -18,6 +18,12 @@ per-node count kernel's HBM fraction on two shapes (unless --no-per-node), and u
 learned policy, with their MFI sets compared, and the same comparison on the
 two wide workloads (clickstream: 990,000 transactions at support 0.1 %; gene:
 1,100 x 20,000 at support 0.12), checked against their golden MFI digests.
+Unless --no-sweep: the reference microbench's eight shapes
+(code/microbench/bmma_bench.cu:438-447) on both paths beside its published
+RTX 3060 Ti figures.  --sustain-seconds S (default 30, 0 = off): the int8
+path run back to back on the headline shape for S seconds, per-launch times
+before and after the first 3 s (bmma_bench.cu:216-289), counts re-checked
+afterwards.
 """
 import argparse
 import json
@@ -59,6 +65,51 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# Host CPUs a one-GPU job may use: the GPU boxes grant each GPU a share of 16
+# host CPUs, whatever the machine's thread count.
+HOST_SHARE_PER_GPU = 16
+
+
+def host_cpu_share():
+    """Threads for the CPU baselines: the CPUs this process may run on (its
+    affinity mask), capped by the cgroup CPU quota if one is set, else by
+    the per-GPU host share."""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(-(-int(q) // int(per))))
+    except (OSError, ValueError):
+        pass
+    return min(n, quota) if quota else min(n, HOST_SHARE_PER_GPU)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def isolated_sample_rows(b, pitch):
+    """Rows whose counts are checked against the oracle: the first and the
+    last 256-row tile, the rows on both sides of every 4 GiB boundary of A,
+    and 64 random rows."""
+    rows = set(range(min(256, b))) | set(range(max(0, b - 256), b))
+    per4g = (1 << 32) // (pitch * 8)
+    for k in range(1, b // per4g + 1):
+        rows |= {k * per4g - 1, min(b - 1, k * per4g)}
+    rng = np.random.default_rng(12)
+    rows |= set(int(r) for r in rng.choice(b, size=min(64, b), replace=False))
+    return np.array(sorted(rows), dtype=np.int64)
+
+
 def cpu_baseline(A_rows, B_rows, n_bits, threads, min_seconds=10.0):
     """The oracle's popcount (bmma_bench.cu cpu_ref semantics) on a sample,
     repeated until at least `min_seconds` of CPU work has been timed."""
@@ -90,17 +141,20 @@ def bench_isolated(ctx, args):
     bitops = float(b) * d * n_bits
     alg_bytes = (b + d) * n_bits / 8 + 4.0 * b * d
     res = {}
-    sample_rows = np.array([0, 1, 4097, 33333, b - 1])
+    sample_rows = isolated_sample_rows(b, pitch)
     B_host = B.read(np.uint64, d * pitch).reshape(d, pitch)
     A_sample = np.stack([A.read(np.uint64, pitch, offset=int(r) * pitch * 8) for r in sample_rows])
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    C_exp = oracle_lib.popcount_matrix(A_sample, B_host, threads=8)
+    t0 = time.perf_counter()
+    C_exp = oracle_lib.popcount_matrix(A_sample, B_host, threads=host_cpu_share())
+    log("oracle counts of %d sampled rows x %d items in %.1f s" % (len(sample_rows), d, time.perf_counter() - t0))
     for path in args.paths:
         split = 8 if path == "mfma" else 1
         ms = ctx.time_popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=split, warmup=args.warmup,
                                       reps=args.steps)
-        # exactness on sampled rows (all 512 columns) + full edge tile
+        # exactness on the sampled rows, all 512 columns each: both 256-row
+        # edge tiles, the rows around every 4 GiB boundary of A, 64 random rows
         got = np.stack([C.read(np.int32, d, offset=int(r) * d * 4) for r in sample_rows])
         ok = bool((got == C_exp).all())
         t0 = time.perf_counter()
@@ -122,6 +176,7 @@ def bench_isolated(ctx, args):
                                      "memory_gbs": PACKED_TRAFFIC_BYTES / (ms * 1e-3) / 1e9,
                                      "memory_frac": PACKED_TRAFFIC_BYTES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
         log("%-6s %.3f ms/launch  %.1f Tbit-op/s  exact=%s" % (path, ms, tbit, ok))
+        res[path]["checked_rows"] = int(len(sample_rows))
         if not ok:
             raise SystemExit("support counts differ from the oracle on sampled rows (%s path)" % path)
         if args.tuned:
@@ -132,9 +187,100 @@ def bench_isolated(ctx, args):
             res[path]["tuned"] = {"tile": tile, "split_k": tsplit, "kernel_ms": tms,
                                   "tbitops": bitops / (tms * 1e-3) / 1e12, "exact_on_sample": bool((got == C_exp).all())}
             log("%-6s tuned: tile %d split %d  %.3f ms" % (path, tile, tsplit, tms))
+    if args.sustain_seconds > 0 and "mfma" in args.paths:
+        res["mfma"]["sustain"] = sustain(ctx, A, B, C, b, d, pitch, args.sustain_seconds, sample_rows, C_exp)
     for x in (A, B, C):
         x.free()
-    return res, bitops, alg_bytes, B_host, A_sample, n_bits
+    # the CPU baseline's sample: the 512 distinct rows of the two edge tiles
+    edge = np.concatenate([np.flatnonzero(sample_rows < 256), np.flatnonzero(sample_rows >= b - 256)])
+    return res, bitops, alg_bytes, B_host, A_sample[edge], n_bits
+
+
+def sustain(ctx, A, B, C, b, d, pitch, seconds, sample_rows, C_exp):
+    """The int8 path (split-k 8) back to back for `seconds` of wall time,
+    each launch timed with HIP events; mean time in the first 3 s against
+    the rest, as the reference's throttle probe reports it
+    (bmma_bench.cu:216-289).  The counts are re-checked afterwards."""
+    log("sustain: int8 path, %.0f s ..." % seconds)
+    bitops = float(b) * d * pitch * 64
+    ms_list, walls = [], []
+    t0 = time.perf_counter()
+    next_log = 5.0
+    while True:
+        wall = time.perf_counter() - t0
+        if wall >= seconds:
+            break
+        ms_list.append(ctx.time_popcount_matrix(A, B, C, b, d, pitch, path="mfma", split_k=8, warmup=0, reps=1))
+        walls.append(time.perf_counter() - t0)
+        if walls[-1] >= next_log:
+            log("  %.0f s: %d launches, last %.2f ms" % (walls[-1], len(ms_list), ms_list[-1]))
+            next_log += 5.0
+    got = np.stack([C.read(np.int32, d, offset=int(r) * d * 4) for r in sample_rows])
+    early = [m for m, w in zip(ms_list, walls) if w <= 3.0]
+    late = [m for m, w in zip(ms_list, walls) if w > 3.0]
+    out = {"seconds": walls[-1] if walls else 0.0, "launches": len(ms_list),
+           "mean_ms": float(np.mean(ms_list)), "max_ms": float(np.max(ms_list)), "min_ms": float(np.min(ms_list)),
+           "tbitops_mean": bitops / (float(np.mean(ms_list)) * 1e-3) / 1e12,
+           "exact_after": bool((got == C_exp).all())}
+    if early and late:
+        out["first3s_ms"] = float(np.mean(early))
+        out["after3s_ms"] = float(np.mean(late))
+        out["ratio"] = out["after3s_ms"] / out["first3s_ms"]
+    log("  sustain: %d launches, mean %.2f ms (%.0f Tbit-op/s), after/first 3 s %.3f, exact=%s" % (
+        out["launches"], out["mean_ms"], out["tbitops_mean"], out.get("ratio", float("nan")), out["exact_after"]))
+    if not out["exact_after"]:
+        raise SystemExit("support counts differ from the oracle after the sustained run")
+    return out
+
+
+# The reference microbench's sweep (code/microbench/bmma_bench.cu:438-447):
+# (transactions N, items d, prefix rows b) and its published RTX 3060 Ti
+# Tbit-op/s (code/microbench/sweep_results_v2.txt:15-70): naive BMMA, smem
+# BMMA, bitmap loop; None where the sweep skipped the shape.
+SWEEP = [((1 << 20, 256, 128), (12.7, 8.6, 3.7)), ((1 << 20, 256, 1024), (55.9, 42.7, 1.6)),
+         ((1 << 22, 256, 1024), (46.8, 41.7, 1.1)), ((1 << 22, 1024, 1024), (5.9, 53.0, 1.0)),
+         ((1 << 24, 1024, 1024), (5.4, 53.3, 0.9)), ((1 << 24, 4096, 128), None),
+         ((1 << 26, 256, 128), (12.7, 8.3, 1.3)), ((1 << 20, 16384, 1024), (6.0, 56.2, 1.3))]
+
+
+def bench_sweep(ctx, args):
+    """Both paths on each sweep shape with the library's tuned tile and split
+    (split_k = 0), Bernoulli(0.3) bits; counts checked against the oracle on
+    the first, middle and last rows over the first and last 64 items."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    out = []
+    for (n_bits, d, b), pub in SWEEP:
+        pitch = tf.pitch_words(n_bits)
+        A, B, C = ctx.buffer(b * pitch * 8), ctx.buffer(d * pitch * 8), ctx.buffer(b * d * 4)
+        ctx.fill_bernoulli(A, b, n_bits, pitch, seed=7, matrix=0, p=0.3)
+        ctx.fill_bernoulli(B, d, n_bits, pitch, seed=7, matrix=1, p=0.3)
+        rows = sorted({0, b // 2, b - 1})
+        cols = sorted(set(range(min(64, d))) | set(range(max(0, d - 64), d)))
+        Ar = np.stack([A.read(np.uint64, pitch, offset=r * pitch * 8) for r in rows])
+        Bc = np.stack([B.read(np.uint64, pitch, offset=c * pitch * 8) for c in cols])
+        exp = oracle_lib.popcount_matrix(Ar, Bc, threads=host_cpu_share())
+        bitops = float(b) * d * n_bits
+        ent = {"transactions": n_bits, "items": d, "rows": b}
+        if pub:
+            ent["published_rtx3060ti_tbitops"] = {"bmma_naive": pub[0], "bmma_smem": pub[1], "bitmap": pub[2]}
+        for path in ("mfma", "packed"):
+            tile, split = ctx.popcount_matrix_plan(A, B, C, b, d, pitch, path=path)
+            ms = ctx.time_popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=0, warmup=1, reps=max(3, args.steps // 2))
+            got = np.stack([C.read(np.int32, d, offset=r * d * 4)[cols] for r in rows])
+            ok = bool((got == exp).all())
+            ent[path] = {"kernel_ms": ms, "tbitops": bitops / (ms * 1e-3) / 1e12, "tile": tile, "split_k": split,
+                         "exact_on_sample": ok}
+            if not ok:
+                raise SystemExit("sweep %s: %s path differs from the oracle" % (ent, path))
+        if "mfma" in ent:
+            ent["mfma"]["i8_frac"] = 2 * ent["mfma"]["tbitops"] / I8_PEAK_TOPS
+        log("sweep N=%d d=%d b=%d: mfma %.0f Tbit-op/s (tile %d split %d), packed %.0f" % (
+            n_bits, d, b, ent["mfma"]["tbitops"], ent["mfma"]["tile"], ent["mfma"]["split_k"], ent["packed"]["tbitops"]))
+        out.append(ent)
+        for x in (A, B, C):
+            x.free()
+    return out
 
 
 def bench_per_node(ctx, args):
@@ -186,9 +332,13 @@ def bench_e2e(args):
     # batched+learned and batched MAFIA; the reference TensorFIM engine's
     # enumeration in waves, autotuned and with every wave on the matrix cores
     # (its int8-MFMA split-k form); the plain per-node engine
+    # (the reference's split-k ablation, BMMA_SPLITK_OFF, kernel_bmma.cu:351-368
+    # and README.md:80: the same matrix-core waves with one word range per tile)
+    nodes = os.path.join(tmp, "dense_replicated.nodes")
     runs = [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
             ("wave_autotuned", "wave", []), ("wave_mfma_splitk", "wave", ["--wave-kernel", "4"]),
-            ("plain_w8", "plain", ["--workers", "8"]), ("plain", "plain", [])]
+            ("wave_mfma_splitk_off", "wave", ["--wave-kernel", "4", "--split-k", "off"]),
+            ("plain_w8", "plain", ["--workers", "8", "--dump-nodes", nodes]), ("plain", "plain", [])]
     for name, engine, extra in runs:
         log("dense replicated e2e: %s ..." % name)
         path, st, wall = run_engine(data, engine, extra, os.path.join(tmp, name + ".res"))
@@ -212,8 +362,9 @@ def bench_e2e(args):
     from conftest import mfi_digest
     with open(os.path.join(ROOT, "tests", "golden", "synthetic", "dense-replicated.json")) as f:
         out["matches_golden"] = mfi_digest(sets["batched_learned"]) == json.load(f)["digest"]
+    out["splitk_off_slowdown"] = out["wave_mfma_splitk_off"]["computation_s"] / out["wave_mfma_splitk"]["computation_s"]
     if not args.no_cpu:
-        out["cpu_engine"] = cpu_engine_baseline(data, out["plain"]["rows"], args)
+        out["cpu_engine"] = cpu_engine_baseline(data, nodes, args)
     best_plain = min(out["plain"]["computation_s"], out["plain_w8"]["computation_s"])
     out["speedup_computation_vs_plain_1worker"] = out["plain"]["computation_s"] / out["batched_learned"]["computation_s"]
     out["speedup_computation_vs_best_plain"] = best_plain / out["batched_learned"]["computation_s"]
@@ -235,15 +386,18 @@ def bench_e2e_wide(args):
     res = {}
     for name, support, runs in (
             ("clickstream", 0.001, [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
-                                    ("plain_w8", "plain", ["--workers", "8"])]),
+                                    ("plain_w8", "plain", ["--workers", "8", "--dump-nodes", "NODES"])]),
             ("gene", 0.12, [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
-                            ("plain_w8", "plain", ["--workers", "8"]), ("plain", "plain", [])])):
+                            ("plain_w8", "plain", ["--workers", "8", "--dump-nodes", "NODES"]),
+                            ("plain", "plain", [])])):
         with open(os.path.join(ROOT, "tests", "golden", "synthetic", name + ".json")) as f:
             fx = json.load(f)
         data = os.path.join(tmp, name + ".txt")
+        nodes = os.path.join(tmp, name + ".nodes")
         subprocess.run([tf.binary("tfim_gen"), name, data], check=True)
         out = {}
         for run, engine, extra in runs:
+            extra = [nodes if x == "NODES" else x for x in extra]
             log("%s e2e: %s ..." % (name, run))
             t0 = time.perf_counter()
             path, st = tf.mine(data, support, engine=engine, out=os.path.join(tmp, name + run + ".res"), extra=extra,
@@ -267,39 +421,41 @@ def bench_e2e_wide(args):
         best_plain = min(out[r]["computation_s"] for r in out if r.startswith("plain"))
         out["speedup_computation_vs_best_plain"] = best_plain / out["batched_learned"]["computation_s"]
         if not args.no_cpu:
-            # gene's whole DFS (19,044 nodes) fits the sample; clickstream's
-            # (94,602 nodes over 1,758 survivors of 989,270 bits) is sampled
-            out["cpu_engine"] = cpu_engine_baseline(data, out["plain_w8"]["rows"], args, support=support, replicate=1,
-                                                    first=20000 if name == "gene" else 150)
+            # gene's whole DFS (19,044 nodes) is timed; clickstream's 94,602
+            # nodes (1,758 survivors x 989,270 bits) are sampled
+            out["cpu_engine"] = cpu_engine_baseline(data, nodes, args, support=support, replicate=0,
+                                                    per_thread=20000 if name == "gene" else 250)
         res[name] = out
     return res
 
 
-def cpu_engine_baseline(data, total_nodes, args, support=0.85, replicate=256, first=4000):
-    """The reference bitmap engine's logic on the host cores: the oracle's
-    worker-pool DFS (kernel_bitmap.cu:853-894 semantics, 64-bit popcount
-    counter) on the same table, timed over its first `first` nodes at 1
-    thread and `first` x T nodes at all usable threads T, extrapolated to the
-    full node count of the plain GPU run (same enumeration); a run that ends
-    within its sample is its full time."""
+def cpu_engine_baseline(data, nodes_file, args, support=0.85, replicate=256, per_thread=1000):
+    """The reference bitmap engine's logic on the host cores: the worker's
+    per-node step of the oracle's DFS (kernel_bitmap.cu:853-894 semantics,
+    64-bit popcount counter, children built as the worker builds them) on a
+    random sample of the plain GPU run's nodes (same enumeration; the node
+    list is that run's), `per_thread` nodes per thread (at least 1,000) at 1
+    thread and at the host share, extrapolated to every node of the list."""
     exe = os.path.join(ROOT, "oracle", "_build", "tfim_oracle")
     res = []
-    nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    for threads, nodes in ((1, first), (nthreads, first * nthreads)):
-        log("cpu engine baseline: %d threads, first %d DFS nodes ..." % (threads, nodes))
-        cmd = [exe, data, repr(float(support)), "--threads", str(threads), "--max-nodes", str(nodes), "--out",
-               data + ".cpu.res"]
+    nthreads = args.cpu_threads or host_cpu_share()
+    for threads in (1, nthreads):
+        k = max(1000, per_thread * threads)
+        log("cpu engine baseline: %d threads, %d random DFS nodes ..." % (threads, k))
+        cmd = [exe, data, repr(float(support)), "--threads", str(threads), "--time-nodes", nodes_file,
+               "--sample", str(k), "--seed", "11"]
         if replicate > 1:
             cmd += ["--replicate", str(replicate)]
         r = subprocess.run(cmd, capture_output=True, text=True, check=True)
         st = dict(ln.split(":", 1) for ln in r.stdout.splitlines() if ":" in ln)
-        secs, done = float(st["computation_time(s)"]), int(st["nodes"])
-        full = done >= total_nodes
-        res.append({"threads": threads, "nodes_timed": done, "seconds": secs,
-                    "extrapolated_computation_s": secs if full else secs * total_nodes / max(1, done),
-                    "nodes_total": total_nodes, "kind": "port",
-                    "note": "full run" if full else "extrapolated from the first nodes of the reference DFS"})
-        log("  %.2f s for %d nodes -> %.2f s" % (secs, done, res[-1]["extrapolated_computation_s"]))
+        total, timed, secs = int(st["nodes_total"]), int(st["nodes_timed"]), float(st["seconds"])
+        full = timed >= total
+        res.append({"threads": threads, "nodes_timed": timed, "seconds": secs,
+                    "extrapolated_computation_s": float(st["extrapolated_s"]), "nodes_total": total, "kind": "port",
+                    "cpu_model": cpu_model(),
+                    "note": "every node of the DFS" if full else
+                            "random sample of the DFS nodes, extrapolated to all of them"})
+        log("  %.2f s for %d of %d nodes -> %.2f s" % (secs, timed, total, res[-1]["extrapolated_computation_s"]))
     return res
 
 
@@ -330,6 +486,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-node", action="store_true")
     ap.add_argument("--tuned", action="store_true", help="also time the tuner's tile/split choice")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the reference's eight microbench shapes")
+    ap.add_argument("--sustain-seconds", type=float, default=30.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     args.paths = [p for p in args.paths.split(",") if p]
@@ -345,20 +503,23 @@ def main():
     log("device:", info)
     res, bitops, alg_bytes, B_host, A_sample, n_bits = bench_isolated(ctx, args)
     per_node = None if args.no_per_node else bench_per_node(ctx, args)
+    sweep = None if args.no_sweep else bench_sweep(ctx, args)
     ctx.close()
     head = res.get("mfma") or res[args.paths[0]]
     head_path = "mfma" if "mfma" in res else args.paths[0]
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_cpu_share()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: oracle popcount, %d threads, 512 rows x 512 items x 2^22 ..." % threads)
-        rng_rows = np.repeat(A_sample, 103, axis=0)[:512]   # 512 device-generated rows (5 distinct)
-        _, cpu_tbit, cpu_s, reps = cpu_baseline(rng_rows, B_host, n_bits, threads)
+        # 512 distinct device-generated rows: the two 256-row edge tiles of A
+        _, cpu_tbit, cpu_s, reps = cpu_baseline(A_sample, B_host, n_bits, threads)
         log("cpu: %.3f Tbit-op/s (%d passes, %.1f s)" % (cpu_tbit, reps, cpu_s))
         cpu = {"value": cpu_tbit, "unit": "Tbit-op/s", "cores": threads, "kind": "port",
-               "sample": "oracle AND-popcount of 512 rows x 512 items x 2^22 transactions, %d passes "
-                         "(%.1f s)" % (reps, cpu_s)}
+               "cpu_model": cpu_model(), "hardware_threads": os.cpu_count(),
+               "sample": "oracle AND-popcount of 512 distinct rows (both 256-row edge tiles of A) x 512 items x "
+                         "2^22 transactions, %d passes (%.1f s) on %d threads (the host share of one GPU)"
+                         % (reps, cpu_s, threads)}
 
     e2e = None if args.no_e2e else bench_e2e(args)
     e2e_wide = None if args.no_e2e else bench_e2e_wide(args)
@@ -397,6 +558,8 @@ def main():
     }
     if per_node is not None:
         line["per_node_kernel"] = per_node
+    if sweep is not None:
+        line["sweep"] = sweep
     if e2e is not None:
         line["e2e_dense_replicated"] = e2e
     if e2e_wide is not None:

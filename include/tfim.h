@@ -123,8 +123,9 @@ int tfim_time_count_node(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t
  * col_off[r+1]) and out is CSR-aligned with col_ids.  Host memory in and out;
  * synchronous.  path: TFIM_PATH_AUTO lets the library choose the kernel per
  * wave (its tuners); TFIM_PATH_PACKED keeps the wave on the packed (AND +
- * v_bcnt) kernels; TFIM_PATH_MFMA_I8 counts it on the matrix cores (prefix
- * masks, int8-MFMA count with split-k, gather). */
+ * v_bcnt) kernels; TFIM_PATH_MFMA_I8 counts it on the matrix cores (the fused
+ * kernel; the three-launch form -- prefix masks, int8-MFMA count with split-k,
+ * gather -- for waves outside its limits). */
 int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item_off,
                      int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
                      int32_t *out, int32_t path);
@@ -180,18 +181,27 @@ int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *i
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel (two
  *                  workgroups per CU), 3 LDS-staged with wide staging (one
  *                  workgroup per CU, up to 512 bits per lane per column),
- *                  4 matrix cores (prefix masks, int8-MFMA count of every
- *                  (row, survivor) pair, gather; waves of up to 8 GiB of
- *                  masks).  In auto mode waves outside the LDS kernel go to
- *                  the L2 kernel or the matrix cores by a second tuner that
- *                  predicts each one's time from its measured cost per unit
- *                  of work
+ *                  4 matrix cores in three launches (prefix masks, int8-MFMA
+ *                  count of every (row, survivor) pair, gather; waves of up
+ *                  to 8 GiB of masks), 5 fused matrix-core kernel (one
+ *                  launch: prefix masks built in LDS from the wave's staged
+ *                  item words, int8-MFMA count, split-k reduced in the
+ *                  epilogue; waves of at most 256 distinct prefix items).
+ *                  In auto mode the wave tuner also tries the fused kernel on
+ *                  waves the LDS kernel takes, and waves outside the LDS
+ *                  kernel go to the L2 kernel, the three-launch or the fused
+ *                  matrix-core form by a second tuner that predicts each
+ *                  one's time from its measured cost per unit of work
  *   "tile_cols"    columns per work tile of the batched path: 8..64, or 0
  *                  (default) to let the wave tuner choose
  *   "mfma_tile"    workgroup tile of the int8-MFMA matrix count: 0 auto
  *                  (= 256), 256 (256 x 128 outputs, operand bytes expanded
  *                  in registers from packed bits in LDS) or 128 (128 x 128,
  *                  bytes expanded into LDS)
+ *   "mfma_split"   matrix-core waves: 0 (default) split the word axis by an
+ *                  occupancy rule scaled by a tuner; 1 no split, one word
+ *                  range per tile (the reference's BMMA_SPLITK_OFF ablation,
+ *                  kernel_bmma.cu:351-368)
  *   "graphs"       1 (default): each wave is replayed from an instantiated
  *                  HIP graph (descriptor copy, output fill, count kernel,
  *                  output copy) whose node parameters are updated per wave;
@@ -210,11 +220,13 @@ int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,
                     int64_t *rows, double *kernel_ms);
 
 /* Waves since tfim_open by kernel: the L2-resident kernel, the LDS-staged
- * kernel and the matrix-core path (prefix masks, int8-MFMA count of every
- * (row, survivor) pair with split-k over word ranges, gather of the wave's
- * outputs) -- the three forms of the reference's batchCountBMMA dispatch
- * (naive / smem / split-k, kernel_bmma.cu:772-872) on MI355X. */
-int tfim_wave_path_stats(tfim_ctx *ctx, int64_t *l2, int64_t *lds, int64_t *mfma);
+ * kernel, the matrix-core path in three launches (prefix masks, int8-MFMA
+ * count of every (row, survivor) pair with split-k over word ranges, gather
+ * of the wave's outputs) and the fused matrix-core kernel (the same in one
+ * launch, masks built in LDS, split-k reduced in the epilogue) -- the forms
+ * of the reference's batchCountBMMA dispatch (naive / smem / split-k,
+ * kernel_bmma.cu:772-872) on MI355X. */
+int tfim_wave_path_stats(tfim_ctx *ctx, int64_t *l2, int64_t *lds, int64_t *mfma, int64_t *fused);
 /* Batched-path graphs instantiated and graph launches since tfim_open.  The
  * reference launches each wave directly (runWaveEngine, kernel_bmma.cu:896-910);
  * no counterpart. */

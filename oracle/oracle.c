@@ -383,6 +383,69 @@ int64_t oracle_mine_bounded(const oracle_db *db, int threads, int64_t max_nodes,
     return s.popped;
 }
 
+typedef struct {
+    const oracle_db *db;
+    const int32_t *items;
+    const int64_t *off;
+    int64_t n;
+    int64_t next;          /* shared index, taken under mu */
+    int64_t kids;          /* children generated (keeps the scan's work live) */
+    pthread_mutex_t mu;
+} tn_state;
+
+static void *tn_worker(void *arg) {
+    tn_state *s = (tn_state *)arg;
+    const oracle_db *db = s->db;
+    int32_t *res = (int32_t *)calloc((size_t)db->d_count + 2, sizeof(int32_t));
+    uint64_t *mask = (uint64_t *)malloc((size_t)(db->words > 0 ? db->words : 1) * sizeof(uint64_t));
+    int64_t kids = 0;
+    for (;;) {
+        pthread_mutex_lock(&s->mu);
+        const int64_t i = s->next++;
+        pthread_mutex_unlock(&s->mu);
+        if (i >= s->n) break;
+        const int32_t *it = s->items + s->off[i];
+        const int32_t k = (int32_t)(s->off[i + 1] - s->off[i]);
+        memset(res, 0, ((size_t)db->d_count + 2) * sizeof(int32_t));
+        const int32_t cnt = count_node_buf(db, it, k, res, mask);
+        if (cnt == k) continue;   /* maximal: the worker records it */
+        const int32_t progress = it[k - 1];
+        for (int32_t e = 0; e <= progress - 1; e++) {
+            if ((double)res[e] < db->thr_abs) continue;
+            int outside = 1;
+            for (int32_t j = 0; j < k; j++) if (it[j] == e) { outside = 0; break; }
+            if (!outside) continue;
+            /* the child's item list, as the worker builds it */
+            int32_t *c = (int32_t *)malloc((size_t)(k + 1) * sizeof(int32_t));
+            memcpy(c, it, (size_t)k * sizeof(int32_t));
+            c[k] = e;
+            kids += c[k] >= 0;
+            free(c);
+        }
+    }
+    pthread_mutex_lock(&s->mu);
+    s->kids += kids;
+    pthread_mutex_unlock(&s->mu);
+    free(res); free(mask);
+    return NULL;
+}
+
+double oracle_time_nodes(const oracle_db *db, const int32_t *items, const int64_t *off, int64_t n, int threads) {
+    tn_state s;
+    memset(&s, 0, sizeof(s));
+    s.db = db; s.items = items; s.off = off; s.n = n;
+    pthread_mutex_init(&s.mu, NULL);
+    if (threads < 1) threads = 1;
+    pthread_t *th = (pthread_t *)malloc((size_t)threads * sizeof(pthread_t));
+    const double t0 = now_s();
+    for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, tn_worker, &s);
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    const double dt = now_s() - t0;
+    free(th);
+    pthread_mutex_destroy(&s.mu);
+    return dt;
+}
+
 int oracle_mine(const oracle_db *db, int threads, oracle_result *out) {
     oracle_mine_bounded(db, threads, 0, out);
     return 0;

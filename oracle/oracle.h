@@ -100,6 +100,14 @@ int64_t oracle_mine_bounded(const oracle_db *db, int threads, int64_t max_nodes,
 
 void oracle_result_free(oracle_result *r);
 
+/* The worker's per-node step of the same DFS (kernel_bitmap.cu:853-894:
+ * getResFrequency, the maximality test and the scan for children) on a given
+ * list of nodes (CSR: node i = items[off[i] .. off[i+1]), dense ids in
+ * insertion order, its progress being the last item), `threads` workers
+ * taking nodes from a shared index.  Returns the seconds of the whole pass
+ * (steady clock); used to time a random sample of a run's nodes. */
+double oracle_time_nodes(const oracle_db *db, const int32_t *items, const int64_t *off, int64_t n, int threads);
+
 /* Sort MFIs by frequency desc (ties: item list, for a deterministic file) and
  * write the Results file in the reference layout (kernel_bmma.cu:1034-1073). */
 int oracle_write_results(const oracle_db *db, oracle_result *r, const char *path,

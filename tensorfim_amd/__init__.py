@@ -58,7 +58,7 @@ _SIGS = {
                                         _p(ctypes.c_int64), _p(ctypes.c_double)]),
     "tfim_wave_graph_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64)]),
     "tfim_wave_path_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64),
-                                             _p(ctypes.c_int64)]),
+                                             _p(ctypes.c_int64), _p(ctypes.c_int64)]),
     "tfim_prep_begin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                         ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
     "tfim_prep_finish": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
@@ -297,10 +297,12 @@ class Context:
                                               ctypes.byref(d)))
         g, r = ctypes.c_int64(), ctypes.c_int64()
         self._check(self._lib.tfim_wave_graph_stats(self._h, ctypes.byref(g), ctypes.byref(r)))
-        l2, lds, mf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-        self._check(self._lib.tfim_wave_path_stats(self._h, ctypes.byref(l2), ctypes.byref(lds), ctypes.byref(mf)))
+        l2, lds, mf, fu = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.tfim_wave_path_stats(self._h, ctypes.byref(l2), ctypes.byref(lds), ctypes.byref(mf),
+                                                   ctypes.byref(fu)))
         return {"launches": a.value, "lds_launches": b.value, "rows": c.value, "kernel_ms": d.value,
-                "graphs": g.value, "graph_launches": r.value, "l2_launches": l2.value, "mfma_launches": mf.value}
+                "graphs": g.value, "graph_launches": r.value, "l2_launches": l2.value, "mfma_launches": mf.value,
+                "fused_launches": fu.value}
 
     def buffer(self, nbytes):
         return DeviceBuffer(self, nbytes)

@@ -37,6 +37,9 @@ int main(int argc, char **argv) {
     int parse_threads = 0;   // 0: hardware threads for files of 4 MB and more
     std::string prep = "gpu";   // counts, reduction and bitmaps: gpu | host
     int replicate = 0, device = 0;
+    std::string dump_nodes;        // plain engine: write every counted node (original ids), one per line
+    std::string split_k = "auto";  // matrix-core waves: auto | off (one word range per tile)
+    std::vector<std::vector<int32_t>> node_log;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
@@ -63,6 +66,8 @@ int main(int argc, char **argv) {
         else if (a == "--model") opt.model_path = next();
         else if (a == "--trace") opt.trace_path = next();
         else if (a == "--verbose") opt.verbose = true;
+        else if (a == "--dump-nodes") dump_nodes = next();
+        else if (a == "--split-k") split_k = next();
         else pos.push_back(a);
     }
     if (pos.size() < 2) {
@@ -87,6 +92,20 @@ int main(int argc, char **argv) {
         fprintf(stderr, "%s\n", tfim_last_error(ctx));
         tfim_close(ctx);
         return 1;
+    }
+    if ((split_k != "auto" && split_k != "off") ||
+        (split_k == "off" && tfim_config(ctx, "mfma_split", 1) != TFIM_OK)) {
+        fprintf(stderr, "--split-k must be auto or off\n");
+        tfim_close(ctx);
+        return 1;
+    }
+    if (!dump_nodes.empty()) {
+        if (opt.engine != "plain") {
+            fprintf(stderr, "--dump-nodes needs --engine plain\n");
+            tfim_close(ctx);
+            return 1;
+        }
+        opt.node_log = &node_log;
     }
     if (opt.spec != "on" && opt.spec != "off") {
         fprintf(stderr, "--spec must be on or off\n");
@@ -157,6 +176,19 @@ int main(int argc, char **argv) {
         tfim_close(ctx);
         return 1;
     }
+    if (!dump_nodes.empty()) {   // after the timed search
+        FILE *f = fopen(dump_nodes.c_str(), "w");
+        if (!f) {
+            fprintf(stderr, "cannot write %s\n", dump_nodes.c_str());
+            tfim_close(ctx);
+            return 1;
+        }
+        for (const auto &v : node_log) {
+            for (size_t i = 0; i < v.size(); i++) fprintf(f, i ? " %d" : "%d", db.dense_to_orig[v[i]]);
+            fputc('\n', f);
+        }
+        fclose(f);
+    }
     const double wall = now() - t0;
     printf("results: %s\n", path);
     printf("engine: %s\n", info.engine.c_str());
@@ -176,10 +208,10 @@ int main(int argc, char **argv) {
     int64_t n_graphs = 0, graph_launches = 0;
     tfim_wave_graph_stats(ctx, &n_graphs, &graph_launches);
     printf("graphs: %lld\ngraph_launches: %lld\n", (long long)n_graphs, (long long)graph_launches);
-    int64_t l2_waves = 0, mfma_waves = 0;
-    tfim_wave_path_stats(ctx, &l2_waves, nullptr, &mfma_waves);
-    printf("l2_waves: %lld\nmfma_waves: %lld\nwave_size: %d\n", (long long)l2_waves, (long long)mfma_waves,
-           st.wave_size);
+    int64_t l2_waves = 0, mfma_waves = 0, fused_waves = 0;
+    tfim_wave_path_stats(ctx, &l2_waves, nullptr, &mfma_waves, &fused_waves);
+    printf("l2_waves: %lld\nmfma_waves: %lld\nfused_waves: %lld\nwave_size: %d\n", (long long)l2_waves,
+           (long long)mfma_waves, (long long)fused_waves, st.wave_size);
     printf("expand_rows: %lld\nhut_rows: %lld\nleaf_rows: %lld\nref_rows: %lld\n", (long long)st.expand_rows,
            (long long)st.hut_rows, (long long)st.leaf_rows, (long long)st.ref_rows);
     printf("spec_rows: %lld\nspec_hits: %lld\nspec_misses: %lld\n", (long long)st.spec_rows, (long long)st.spec_hits,

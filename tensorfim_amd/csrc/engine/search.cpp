@@ -169,7 +169,7 @@ std::vector<RefNode> ref_roots(const Database &db, bool only_item0) {
 // its own context (stream + scratch) sharing the resident bitmaps, pops a
 // node, counts it with one launch and pushes its children.
 int run_plain(tfim_ctx *ctx, const Database &db, int workers, std::vector<Mfi> &out, SearchStats &st,
-              std::string &err) {
+              std::string &err, std::vector<std::vector<int32_t>> *node_log) {
     std::vector<RefNode> stack = ref_roots(db, false);
     if (db.n_surv == 0) return TFIM_OK;
     workers = std::max(1, workers);
@@ -185,6 +185,7 @@ int run_plain(tfim_ctx *ctx, const Database &db, int workers, std::vector<Mfi> &
     std::condition_variable cv;
     int active = 0, rc_all = TFIM_OK;
     std::vector<SearchStats> wst(workers);
+    std::vector<std::vector<std::vector<int32_t>>> wlog(node_log ? workers : 0);
     auto worker = [&](int w) {
         std::vector<int32_t> res((size_t)db.n_surv + 1, 0);
         std::vector<RefNode> kids;
@@ -217,6 +218,7 @@ int run_plain(tfim_ctx *ctx, const Database &db, int workers, std::vector<Mfi> &
             }
             cv.notify_all();
             wst[w].nodes++;
+            if (node_log) wlog[w].push_back(nd.items);
         }
         cv.notify_all();
     };
@@ -224,6 +226,9 @@ int run_plain(tfim_ctx *ctx, const Database &db, int workers, std::vector<Mfi> &
     for (int w = 0; w < workers; w++) th.emplace_back(worker, w);
     for (auto &t : th) t.join();
     for (int w = 1; w < workers; w++) tfim_close(ctxs[w]);
+    if (node_log)
+        for (auto &l : wlog)
+            for (auto &v : l) node_log->push_back(std::move(v));
     for (auto &x : wst) {
         st.nodes += x.nodes;
         st.count_seconds += x.count_seconds;
@@ -1284,7 +1289,7 @@ class Mafia {
 
 int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std::vector<Mfi> &out,
                SearchStats &st, std::string &err) {
-    if (opt.engine == "plain") return run_plain(ctx, db, opt.workers, out, st, err);
+    if (opt.engine == "plain") return run_plain(ctx, db, opt.workers, out, st, err, opt.node_log);
     if (opt.engine == "wave") {
         const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
         const int rc = run_wave(ctx, db, cap, out, st, err);

@@ -37,6 +37,9 @@ struct SearchOptions {
     std::string model_path;       // mafia: load MLP weights (else trained online)
     std::string trace_path;       // mafia: write (features, outcome) records
     bool verbose = false;
+    // plain: when set, receives every counted node's items (dense ids,
+    // insertion order), for timing samples of the reference DFS elsewhere
+    std::vector<std::vector<int32_t>> *node_log = nullptr;
 };
 
 struct SearchStats {

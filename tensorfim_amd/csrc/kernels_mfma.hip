@@ -18,6 +18,9 @@
 // (half, byte) pairs, so any consistent assignment of k to (half, byte) is
 // exact.  The accumulator layout is the 32x32 one of the CDNA3/CDNA4 ISA:
 // register q of lane l holds C[8*(q/4) + 4*(l/32) + (q%4)][l % 32].
+#include <cstdio>
+#include <mutex>
+
 #include "tfim_internal.h"
 
 namespace tfim {
@@ -329,6 +332,324 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
                     else C[r * d + c] = v;
                 }
             }
+}
+
+// ---------------------------------------------------------------------------
+// Fused matrix-core wave: one launch per wave does what the reference splits
+// into mask build, BMMA and split-k reduce (kernel_bmma.cu:619-768) and what
+// k_wave_masks -> k_popmat_mfma_rp -> k_wave_gather do in three launches here,
+// without the [rows][pitch] mask buffer and without the count matrix.
+//
+// Workgroup (row tile tm of RA = 64 TI wave rows, column tile tn of RB = 64 TJ
+// survivors, word range blockIdx.y).  Per 512-bit stage:
+//   - the words of every distinct prefix item of the wave ("slots") are
+//     staged in LDS (Ps), two stages ahead of the MFMAs, double-buffered;
+//   - each row's prefix AND is built from Ps into the A operand block (As),
+//     one stage ahead, so row masks never leave LDS;
+//   - the survivor tile's words are staged as the B operand (Bs);
+//   - the MFMA loop of k_popmat_mfma_rp (register-expanded int8 operands,
+//     64 x popcount per accumulator).
+// One barrier per stage.  Epilogue: the tile's counts go to LDS, then
+//   - explicit rows: the wave's listed outputs that fall in this tile
+//     (entries grouped by column tile and row, resolved on the host) are
+//     written, with atomics when the word axis is split;
+//   - complement rows: the live columns' partial counts are added into the
+//     row's full-count accumulator; the last workgroup of the row tile to
+//     arrive (arrival counter) thresholds them: support of the prefix and
+//     the number of non-excluded survivors reaching min_count.
+constexpr int kFusedSlotLoads = 4;   // uint4 slot loads per thread per stage: at most 256 slots
+
+template <int TI, int TJ>
+__global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict__ bm, int64_t pitch,
+                                                      WaveMfmaDesc d) {
+    constexpr int RA = 64 * TI, RB = 64 * TJ, TS = RB + 1;
+    extern __shared__ uint32_t lds_u32[];
+    __shared__ int last_arrival;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
+    const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
+    const int tm = (int)(lin / d.tiles_n), tn = (int)(lin % d.tiles_n);
+    const int row0 = tm * RA, col0 = tn * RB;
+    const int rend = min(d.n_rows, row0 + RA);
+    const int64_t kbeg = (int64_t)blockIdx.y * d.k_per_split;
+    const int64_t kend = std::min<int64_t>(pitch, kbeg + d.k_per_split);
+
+    uint32_t *As = lds_u32;                              // [2][RA][RP_STRIDE]
+    uint32_t *Bs = As + 2 * RA * RP_STRIDE;              // [2][RB][RP_STRIDE]
+    uint32_t *Ps = Bs + 2 * RB * RP_STRIDE;              // [2][n_slots][16]
+    int32_t *loff = reinterpret_cast<int32_t *>(Ps + 2 * d.n_slots * 16);   // [RA + 1]
+    uint16_t *lsl = reinterpret_cast<uint16_t *>(loff + RA + 1);           // prefix slots of the tile's rows
+
+    // the tile's prefix slot lists -> LDS (rows past the wave's end: empty)
+    const int ib0 = d.item_off[row0];
+    for (int r = tid; r <= RA; r += 256) loff[r] = d.item_off[min(row0 + r, rend)] - ib0;
+    const int nl = d.item_off[rend] - ib0;
+    for (int i = tid; i < nl; i += 256) lsl[i] = (uint16_t)d.item_slots[ib0 + i];
+
+    typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(1))) u4v *gptr4;
+    // slot loads: load j of thread t is uint4 part (idx & 3) of slot (idx >> 2), idx = 256 j + t
+    const int ns4 = d.n_slots * 4;
+    int64_t sbase[kFusedSlotLoads];
+#pragma unroll
+    for (int j = 0; j < kFusedSlotLoads; j++) {
+        const int idx = j * 256 + tid;
+        sbase[j] = idx < ns4 ? (int64_t)d.slot_rows[idx >> 2] * pitch + 2 * (idx & 3) : -1;
+    }
+    // B loads: as k_popmat_mfma_rp (rows past the edge read the last row, zeroed at the store)
+    int64_t bbase[TJ];
+    uint32_t mb[TJ];
+#pragma unroll
+    for (int j = 0; j < TJ; j++) {
+        const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
+        mb[j] = col0 + r < d.n_cols ? ~0u : 0u;
+        bbase[j] = (int64_t)min(col0 + r, d.n_cols - 1) * pitch + 2 * part;
+    }
+    u4v rs[kFusedSlotLoads], rb[TJ];
+    auto gload_s = [&](int64_t kw) {
+#pragma unroll
+        for (int j = 0; j < kFusedSlotLoads; j++)
+            if (sbase[j] >= 0) rs[j] = *(gptr4)(bm + sbase[j] + kw);
+    };
+    auto lstore_s = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < kFusedSlotLoads; j++) {
+            const int idx = j * 256 + tid;
+            if (sbase[j] >= 0) *reinterpret_cast<u4v *>(&Ps[(buf * d.n_slots + (idx >> 2)) * 16 + 4 * (idx & 3)]) = rs[j];
+        }
+    };
+    auto gload_b = [&](int64_t kw) {
+#pragma unroll
+        for (int j = 0; j < TJ; j++) rb[j] = *(gptr4)(bm + bbase[j] + kw);
+    };
+    auto lstore_b = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < TJ; j++) {
+            const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
+            *reinterpret_cast<u4v *>(&Bs[(buf * RB + r) * RP_STRIDE + 4 * part]) = rb[j] & mb[j];
+        }
+    };
+    // row masks of the stage in Ps[pbuf] -> As[abuf]; task j of thread t is
+    // uint4 part (x & 3) of tile row (x >> 2), x = 256 j + t
+    auto build = [&](int pbuf, int abuf) {
+        const uint32_t *P = Ps + pbuf * d.n_slots * 16;
+#pragma unroll
+        for (int j = 0; j < TI; j++) {
+            const int x = j * 256 + tid, r = x >> 2, part = x & 3;
+            const int b = loff[r], e = loff[r + 1];
+            u4v m = {0u, 0u, 0u, 0u};
+            if (e > b) {
+                m = *reinterpret_cast<const u4v *>(&P[lsl[b] * 16 + 4 * part]);
+                int i = b + 1;
+                for (; i + 1 < e; i += 2)
+                    m &= *reinterpret_cast<const u4v *>(&P[lsl[i] * 16 + 4 * part]) &
+                         *reinterpret_cast<const u4v *>(&P[lsl[i + 1] * 16 + 4 * part]);
+                if (i < e) m &= *reinterpret_cast<const u4v *>(&P[lsl[i] * 16 + 4 * part]);
+            }
+            *reinterpret_cast<u4v *>(&As[(abuf * RA + r) * RP_STRIDE + 4 * part]) = m;
+        }
+    };
+
+    v16i acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[i][j][q] = 0;
+
+    if (kbeg < kend) {
+        gload_s(kbeg);
+        lstore_s(0);
+        if (kbeg + 8 < kend) {
+            gload_s(kbeg + 8);
+            lstore_s(1);
+        }
+        gload_b(kbeg);
+        lstore_b(0);
+    }
+    __syncthreads();   // Ps, Bs of the first stages, loff, lsl
+    if (kbeg < kend) build(0, 0);
+    __syncthreads();
+    int cur = 0;
+    for (int64_t kw = kbeg; kw < kend; kw += 8) {
+        const bool n1 = kw + 8 < kend, n2 = kw + 16 < kend;
+        if (n2) gload_s(kw + 16);
+        if (n1) gload_b(kw + 8);
+#pragma unroll
+        for (int hk = 0; hk < 2; hk++) {
+            uint4 pa[TI], pb[TJ];
+#pragma unroll
+            for (int i = 0; i < TI; i++) {
+                const int r = wr * (32 * TI) + i * 32 + l32;
+                pa[i] = *reinterpret_cast<const uint4 *>(&As[(cur * RA + r) * RP_STRIDE + 8 * half + 4 * hk]);
+            }
+#pragma unroll
+            for (int j = 0; j < TJ; j++) {
+                const int r = wc * (32 * TJ) + j * 32 + l32;
+                pb[j] = *reinterpret_cast<const uint4 *>(&Bs[(cur * RB + r) * RP_STRIDE + 8 * half + 4 * hk]);
+            }
+#pragma unroll
+            for (int s2 = 0; s2 < 8; s2++) {
+                const int wi = s2 >> 1, par = s2 & 1;
+                v4i ea[TI], eb[TJ];
+#pragma unroll
+                for (int i = 0; i < TI; i++) {
+                    const uint32_t w = wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w;
+                    ea[i] = rp_expand_a(w, par);
+                }
+#pragma unroll
+                for (int j = 0; j < TJ; j++) {
+                    const uint32_t w = wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w;
+                    eb[j] = rp_expand_b(w, par);
+                }
+#pragma unroll
+                for (int i = 0; i < TI; i++)
+#pragma unroll
+                    for (int j = 0; j < TJ; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[i], eb[j], acc[i][j], 0, 0, 0);
+            }
+        }
+        // the next stage's masks from its staged slot words (Ps[cur ^ 1],
+        // written before the last barrier) into the A block the MFMAs of the
+        // previous stage have finished reading
+        if (n1) build(cur ^ 1, cur ^ 1);
+        asm volatile("" ::: "memory");
+        if (n2) lstore_s(cur);        // stage + 2 -> the slot buffer build() read one stage ago
+        if (n1) lstore_b(cur ^ 1);    // stage + 1 -> the B buffer the previous stage's MFMAs read
+        __syncthreads();
+        cur ^= 1;
+    }
+
+    // epilogue: the tile's counts in LDS (the staging buffers are free)
+    int32_t *T = reinterpret_cast<int32_t *>(lds_u32);   // [RA][TS]
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int r = wr * (32 * TI) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
+                const int c = wc * (32 * TJ) + j * 32 + l32;
+                T[r * TS + c] = acc[i][j][q] >> 6;
+            }
+    __syncthreads();
+    const bool split = gridDim.y > 1;
+    if (d.seg_off) {
+        // listed outputs of this tile: entries of column tile tn, rows row0 .. rend
+        const int32_t *seg = d.seg_off + (int64_t)tn * (d.n_rows + 1);
+        const int e1 = seg[rend];
+        for (int e = seg[row0] + tid; e < e1; e += 256) {
+            const int2 en = d.entries[e];
+            const int v = T[((en.y >> 8) - row0) * TS + (en.y & 255)];
+            if (!split) d.out[en.x] = v;
+            else if (v) atomicAdd(&d.out[en.x], v);
+        }
+    } else if (d.dense_out) {
+        for (int i = tid; i < RA * RB; i += 256) {
+            const int r = i / RB, c = i % RB;
+            if (row0 + r < rend && col0 + c < d.n_cols) {
+                const int v = T[r * TS + c];
+                int32_t *o = &d.out[(int64_t)(row0 + r) * d.n_cols + col0 + c];
+                if (!split) *o = v;
+                else if (v) atomicAdd(o, v);
+            }
+        }
+    }
+    if (!d.row_compl) return;
+    // complement rows: partial counts of the live columns into the full-count
+    // accumulator (uniform over the workgroup: every thread walks the rows)
+    bool any = false;
+    for (int r = row0; r < rend; r++) {
+        const int j = d.row_compl[r];
+        if (j < 0) continue;
+        any = true;
+        const int x0 = d.compl_x0[j];
+        for (int c = tid; c < RB && col0 + c < d.n_cols; c += 256) {
+            const int e = col0 + c;
+            const bool live = e == x0 || !((d.excl[(int64_t)j * d.excl_words + (e >> 6)] >> (e & 63)) & 1ull);
+            const int v = T[(r - row0) * TS + c];
+            if (live && v) atomicAdd(&d.acc[(int64_t)j * d.n_cols + e], v);
+        }
+    }
+    if (!any) return;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last_arrival = atomicAdd(&d.arrive[tm], 1) == d.tiles_n * (int)gridDim.y - 1;
+    __syncthreads();
+    if (!last_arrival) return;
+    __threadfence();
+    // the last workgroup of the row tile: one wave per complement row
+    for (int r = row0 + wave; r < rend; r += 4) {
+        const int j = d.row_compl[r];
+        if (j < 0) continue;
+        const int x0 = d.compl_x0[j];
+        const int32_t *full = d.acc + (int64_t)j * d.n_cols;
+        const uint64_t *ex = d.excl + (int64_t)j * d.excl_words;
+        int n = 0;
+        for (int w = 0; w < d.excl_words; w++) {
+            const int e = w * 64 + lane;
+            const bool live = e < d.n_cols && e != x0 && !((ex[w] >> lane) & 1ull);
+            if (live && __hip_atomic_load(&full[e], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= d.min_count) n++;
+        }
+        for (int o = 32; o > 0; o >>= 1) n += __shfl_xor(n, o);
+        if (lane == 0) {
+            d.res[2 * j] = __hip_atomic_load(&full[x0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            d.res[2 * j + 1] = n;
+        }
+    }
+}
+
+size_t wave_mfma_lds_bytes(int ti, int tj, int n_slots, int max_tile_list) {
+    const int RA = 64 * ti, RB = 64 * tj;
+    const size_t stage = (size_t)4 * (2 * RA * RP_STRIDE + 2 * RB * RP_STRIDE + 2 * n_slots * 16 + RA + 1) +
+                         (size_t)2 * max_tile_list + 4;
+    const size_t tile = (size_t)4 * RA * (RB + 1);
+    return (std::max(stage, tile) + 15) / 16 * 16;
+}
+
+bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc &d, int ti, int tj, int tiles_m,
+                    int splits, size_t lds_bytes, KernelLaunch &L) {
+    const void *f = nullptr;
+    if (tj == 1) f = ti == 1 ? (const void *)k_wave_mfma<1, 1> : ti == 2 ? (const void *)k_wave_mfma<2, 1>
+                                                                          : (const void *)k_wave_mfma<4, 1>;
+    else f = ti == 1 ? (const void *)k_wave_mfma<1, 2> : (const void *)k_wave_mfma<2, 2>;
+    if ((tj == 2 && ti == 4) || d.n_slots < 1 || d.n_slots > 64 * kFusedSlotLoads || lds_bytes > kFusedMaxLds)
+        return false;
+    {
+        // the kernels may take up to the full 160 KB: set once per device
+        static std::mutex mu;
+        static uint64_t done = 0;
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return false;
+        std::lock_guard<std::mutex> g(mu);
+        if (dev >= 64 || !((done >> dev) & 1)) {
+            const void *fn[5] = {(const void *)k_wave_mfma<1, 1>, (const void *)k_wave_mfma<2, 1>,
+                                 (const void *)k_wave_mfma<4, 1>, (const void *)k_wave_mfma<1, 2>,
+                                 (const void *)k_wave_mfma<2, 2>};
+            for (const void *fp : fn) {
+                const hipError_t e = hipFuncSetAttribute(fp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFusedMaxLds);
+                if (e != hipSuccess) {
+                    fprintf(stderr, "k_wave_mfma: hipFuncSetAttribute: %s\n", hipGetErrorString(e));
+                    (void)hipGetLastError();
+                    return false;
+                }
+            }
+            if (dev < 64) done |= 1ull << dev;
+        }
+    }
+    L.func = f;
+    L.grid = dim3((unsigned)(tiles_m * d.tiles_n), (unsigned)splits);
+    L.block = dim3(256);
+    L.shmem = (uint32_t)lds_bytes;
+    L.reset();
+    L.arg(bm);
+    L.arg(pitch_words);
+    L.arg(d);
+    return true;
 }
 
 int rp_rows_per_tile(int64_t b) { return b <= 64 ? 64 : b <= 128 ? 128 : 256; }

@@ -194,6 +194,38 @@ void plan_wave_masks(const uint64_t *bm, int64_t pitch_words, const int32_t *ite
 void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32_t n_cols, const int32_t *row_compl,
                       int32_t min_count, int32_t *res, int32_t max_row_cols, KernelLaunch &L);
 
+// Fused matrix-core wave (k_wave_mfma): one launch per wave builds each
+// row's prefix mask from LDS-staged item words, counts it against survivor
+// tiles on the int8 matrix cores with the word axis split over
+// gridDim.y, and writes only the wave's outputs.
+struct WaveMfmaDesc {
+    const int32_t *slot_rows;   // [n_slots] bitmap row of each distinct prefix item
+    const int32_t *item_off;    // [n_rows + 1] prefix CSR
+    const int32_t *item_slots;  // prefix slots, CSR by item_off
+    const int32_t *seg_off;     // [tiles_n][n_rows + 1] listed-output entries per (column tile, row), or null
+    const int2 *entries;        // (output index, row << 8 | column within the tile)
+    const int32_t *row_compl;   // [n_rows] complement index or -1; null when the wave has none
+    const int32_t *compl_x0;    // [n_compl] first prefix item of each complement row
+    const uint64_t *excl;       // [n_compl][excl_words] exclusion bitmaps
+    int32_t *out;               // listed outputs (dense_out: [n_rows][n_cols])
+    int32_t *res;               // two results per complement row
+    int32_t *acc;               // [n_compl][n_cols] full counts of complement rows, zeroed
+    int32_t *arrive;            // [tiles_m] arrival counters, zeroed
+    int64_t k_per_split;        // words per split (a multiple of 8)
+    int32_t n_rows, n_cols, n_slots, excl_words, min_count, tiles_n, dense_out;
+};
+// Dynamic LDS of a launch: the staging buffers (with up to max_tile_list
+// prefix entries per row tile) or the epilogue's count tile, the larger.
+size_t wave_mfma_lds_bytes(int ti, int tj, int n_slots, int max_tile_list);
+// Launch description; false if the shape is outside the kernel's limits
+// (more than 256 slots, more LDS than a CU has, an unsupported tile).
+bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc &d, int ti, int tj, int tiles_m,
+                    int splits, size_t lds_bytes, KernelLaunch &L);
+constexpr int kFusedMaxSlots = 256;
+// Dynamic LDS limit of k_wave_mfma: a CU's 160 KB less room for its static
+// shared variable.
+constexpr size_t kFusedMaxLds = 160 * 1024 - 256;
+
 // Matrix AND-popcount, int8 MFMA path, 128x128 tiles with the operand bytes
 // expanded into LDS.  C must be zeroed when split_k>1.
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,

@@ -23,13 +23,43 @@ struct tfim_buf {
 };
 
 namespace {
+// Word-split tuner of a matrix-core wave path: the split is an occupancy
+// rule's `base` scaled by 1/2, 1 or 2 (arms 0..2); each arm is tried three
+// times, then the cheapest per unit of work runs, another being re-tried
+// every 16 waves.
+struct SplitTuner {
+    double cost[3] = {};
+    int64_t samples[3] = {};
+    int64_t calls = 0;
+    int pick() {
+        calls++;
+        for (int a = 0; a < 3; a++)
+            if (samples[a] < 3) return a;
+        int best = 0;
+        for (int a = 1; a < 3; a++)
+            if (cost[a] < cost[best]) best = a;
+        return calls % 16 == 0 ? (best + 1 + (int)((calls / 16) % 2)) % 3 : best;
+    }
+    static int64_t split_of(int arm, int64_t base, int64_t stages) {
+        return arm == 0 ? std::max<int64_t>(1, base / 2) : arm == 2 ? std::min<int64_t>(stages, base * 2) : base;
+    }
+    void record(int arm, double per) {
+        cost[arm] = samples[arm] == 0 ? per : 0.8 * cost[arm] + 0.2 * per;
+        samples[arm]++;
+    }
+};
+
 // Online tuner arms for waves that both batched kernels accept:
-// (kernel 1 = L2-resident, 2 = LDS-staged; columns per work tile; LDS
-// staging wide = one workgroup per CU with up to 4 x 128 bits per lane).
-constexpr int kTuneArms = 7;
-constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2, 2, 2, 2};
-constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64, 16, 32, 64};
-constexpr bool kArmWide[kTuneArms] = {false, false, false, false, true, true, true};
+// (kernel 1 = L2-resident, 2 = LDS-staged, 5 = fused matrix-core; columns per
+// work tile; LDS staging wide = one workgroup per CU with up to 4 x 128 bits
+// per lane).
+constexpr int kTuneArms = 8;
+constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2, 2, 2, 2, 5};
+constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64, 16, 32, 64, 0};
+constexpr bool kArmWide[kTuneArms] = {false, false, false, false, true, true, true, false};
+// A fused matrix-core wave that does not fit the kernel's limits (the caller
+// takes another path); not an error.
+constexpr int kFusedNoFit = 1;
 }  // namespace
 
 struct tfim_ctx {
@@ -72,6 +102,7 @@ struct tfim_ctx {
         int mf_arm = -1;       // second tuner's arm for this wave (-1: not tuned)
         double mf_work = 0;
         int sk_arm = -1;       // third tuner's arm (matrix-core waves)
+        bool sk_fused = false; // ... of the fused path's split tuner
         double sk_work = 0;
         // one executable graph per kernel sequence: H2D descriptor copy ->
         // fill -> ev0 -> kernels -> ev1 -> D2H copy
@@ -97,15 +128,14 @@ struct tfim_ctx {
     double wave_ms = 0;
     int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel, 3 LDS kernel with wide staging, 4 matrix cores
     // second tuner, for waves outside the LDS kernel: arm 0 = L2 kernel, arm 1
-    // = matrix cores; EMA of kernel milliseconds per unit of each arm's work
-    double mf_cost[2] = {64.0 / 300e12 * 1e3, 64.0 / 1500e12 * 1e3};   // ms per u64 word of work (priors)
-    int64_t mf_samples[2] = {};
-    int64_t mf_calls = 0, wave_mfma_launches = 0;
-    // third tuner: the matrix-core path's word split, as 1/2, 1 or 2 times
+    // = matrix cores in three launches, arm 2 = fused matrix-core launch; EMA
+    // of kernel milliseconds per unit of each arm's work
+    double mf_cost[3] = {64.0 / 300e12 * 1e3, 64.0 / 1500e12 * 1e3, 64.0 / 1500e12 * 1e3};   // ms per u64 word (priors)
+    int64_t mf_samples[3] = {};
+    int64_t mf_calls = 0, wave_mfma_launches = 0, wave_fused_launches = 0;
+    // third tuner: each matrix-core path's word split, as 1/2, 1 or 2 times
     // the occupancy rule's split; EMA of milliseconds per unit of work
-    double sk_cost[3] = {};
-    int64_t sk_samples[3] = {};
-    int64_t sk_calls = 0;
+    SplitTuner sk, fk;
     // wave tuner: EMA of kernel milliseconds per (row prefix + column) word,
     // per arm (kernel, columns per tile)
     double tune_cost[kTuneArms] = {};
@@ -113,10 +143,13 @@ struct tfim_ctx {
     int64_t tune_calls = 0;
     int tile_cols = 0;      // 0: chosen by the tuner (32 when it does not run)
     int mfma_tile = 0;      // 0 auto (= 256), 128 or 256
+    int mfma_split = 0;     // matrix-core waves: 0 split by the occupancy rule and tuner, 1 no split
     int node_shape = 0;     // per-node kernel: 0 auto, else R * 100 + columns per block
     int node_spread = 0;    // per-node counts: 0 auto, 1 one copy, 8 spread over 8 copies
     int use_graphs = 1;     // waves replayed as instantiated graphs
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
+    std::vector<int32_t> fslot_of;  // item -> slot of the fused matrix-core wave being packed
+    const char *fused_nofit = "";   // the limit the last wave refused by the fused kernel exceeded
     std::map<std::string, std::pair<int, int32_t>> popmat_tuned;   // shape -> (tile, split_k)
 
     int32_t *d_scratch = nullptr;   // this device's `scratch` (tfim_count_node_noalloc)
@@ -629,20 +662,14 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     const int ra = rp_rows_per_tile(n_rows);
     const int64_t tiles = ((n_rows + ra - 1) / ra) * ((c->n_cols + 127) / 128);
     const int64_t stages = c->pitch / 8;
-    const int64_t base = std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
+    const int64_t base = c->mfma_split == 1
+                             ? 1
+                             : std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
     int sk_arm = -1;
     int64_t split = base;
     if (base > 1) {
-        c->sk_calls++;
-        for (int a = 0; a < 3 && sk_arm < 0; a++)
-            if (c->sk_samples[a] < 3) sk_arm = a;
-        if (sk_arm < 0) {
-            int best = 0;
-            for (int a = 1; a < 3; a++)
-                if (c->sk_cost[a] < c->sk_cost[best]) best = a;
-            sk_arm = c->sk_calls % 16 == 0 ? (best + 1 + (int)((c->sk_calls / 16) % 2)) % 3 : best;
-        }
-        split = sk_arm == 0 ? std::max<int64_t>(1, base / 2) : sk_arm == 2 ? std::min<int64_t>(stages, base * 2) : base;
+        sk_arm = c->sk.pick();
+        split = SplitTuner::split_of(sk_arm, base, stages);
     }
     const int64_t zs = plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P);
     WaveDesc d{};
@@ -665,8 +692,169 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     sl.n_out = n_out;
     sl.n_rows = n_rows;
     sl.sk_arm = sk_arm;
+    sl.sk_fused = false;
     sl.sk_work = (double)((n_rows + ra - 1) / ra * ra) * (double)((c->n_cols + 127) / 128 * 128) * (double)c->pitch;
     c->wave_mfma_launches++;
+    return TFIM_OK;
+}
+
+// Fused matrix-core path of a wave (arguments already checked by
+// wave_submit): one k_wave_mfma launch.  Descriptor = slot rows | item_off |
+// prefix slots | gather segments | gather entries | row -> complement index |
+// complement x0 | exclusion bitmaps; output block = explicit outputs | two
+// results per complement row | (device only, 64-aligned) full counts of the
+// complement rows | arrival counters of the row tiles, all zeroed by the
+// wave's fill kernel.  Returns kFusedNoFit, with nothing issued, when the
+// wave has more than kFusedMaxSlots distinct prefix items or needs more LDS
+// than a CU has.  Marks the slot busy with its ticket.
+int wave_submit_fused(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, const int32_t *item_off,
+                      int32_t n_rows, const int32_t *col_ids, const int32_t *col_off, const int32_t *compl_rows,
+                      int32_t n_compl, const uint64_t *excl, int32_t min_count, int64_t n_exp, int64_t n_out) {
+    c->fused_nofit = "a wave of 1 to 2^23 - 1 rows over at least one survivor";
+    if (c->n_cols < 1 || n_rows < 1 || (int64_t)n_rows >= (1 << 23)) return kFusedNoFit;
+    const int64_t n_item_ids = item_off[n_rows];
+    // distinct prefix items -> slots (fslot_of is cleared again below)
+    if ((int32_t)c->fslot_of.size() < c->n_items) c->fslot_of.assign(c->n_items, -1);
+    std::vector<int32_t> slot_rows;
+    std::vector<int32_t> islots((size_t)n_item_ids);
+    bool fits = true;
+    for (int64_t i = 0; i < n_item_ids; i++) {
+        int32_t &so = c->fslot_of[item_ids[i]];
+        if (so < 0) {
+            if ((int)slot_rows.size() >= kFusedMaxSlots) {
+                fits = false;
+                break;
+            }
+            so = (int32_t)slot_rows.size();
+            slot_rows.push_back(item_ids[i]);
+        }
+        islots[(size_t)i] = so;
+    }
+    for (int32_t it : slot_rows) c->fslot_of[it] = -1;
+    c->fused_nofit = "at most 256 distinct prefix items";
+    if (!fits) return kFusedNoFit;
+    // tile shape: 64- or 128-column survivor tiles, 64 to 256 rows
+    const int tj = c->n_cols <= 64 ? 1 : 2;
+    const int ti = n_rows <= 64 ? 1 : (n_rows <= 128 || tj == 2) ? 2 : 4;
+    const int RA = 64 * ti, RB = 64 * tj;
+    const int tiles_m = (n_rows + RA - 1) / RA, tiles_n = (c->n_cols + RB - 1) / RB;
+    int max_list = 0;
+    for (int tm = 0; tm < tiles_m; tm++)
+        max_list = std::max(max_list, item_off[std::min(n_rows, (tm + 1) * RA)] - item_off[tm * RA]);
+    const size_t lds = wave_mfma_lds_bytes(ti, tj, (int)slot_rows.size(), max_list);
+    c->fused_nofit = "prefix lists of a row tile that fit the LDS";
+    if (lds > kFusedMaxLds) return kFusedNoFit;
+
+    const bool csr = col_ids || n_compl > 0;
+    const int64_t excl_words = ((int64_t)c->n_cols + 63) / 64;
+    auto align16 = [](int64_t x) { return (x + 15) / 16 * 16; };
+    const int64_t n_seg = csr ? (int64_t)tiles_n * (n_rows + 1) : 0;
+    const int64_t o_slots = 0;
+    const int64_t o_ioff = align16(o_slots + (int64_t)slot_rows.size() * 4);
+    const int64_t o_isl = align16(o_ioff + (int64_t)(n_rows + 1) * 4);
+    const int64_t o_seg = align16(o_isl + n_item_ids * 4);
+    const int64_t o_ent = align16(o_seg + n_seg * 4);
+    const int64_t o_rc = align16(o_ent + (csr ? n_exp : 0) * 8);
+    const int64_t o_x0 = align16(o_rc + (n_compl ? (int64_t)n_rows * 4 : 0));
+    const int64_t o_excl = align16(o_x0 + (int64_t)n_compl * 4);
+    const int64_t total = o_excl + (int64_t)n_compl * excl_words * 8;
+    // outputs: explicit | results | complement full counts | arrival counters
+    const int64_t o_acc = (n_out + 63) / 64 * 64;
+    const int64_t o_arr = o_acc + (int64_t)n_compl * c->n_cols;
+    const int64_t n_dev = n_compl ? o_arr + tiles_m : n_out;
+    c->fused_nofit = "outputs within int32 indices";
+    if (n_dev > INT32_MAX) return kFusedNoFit;
+    if (int r = ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, total)) return r;
+    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, std::max<int64_t>(1, n_dev))) return r;
+    char *h = sl.h_desc;
+    memcpy(h + o_slots, slot_rows.data(), slot_rows.size() * 4);
+    memcpy(h + o_ioff, item_off, (size_t)(n_rows + 1) * 4);
+    memcpy(h + o_isl, islots.data(), (size_t)n_item_ids * 4);
+    if (csr) {
+        // listed outputs grouped by (column tile, row): counting sort
+        int32_t *seg = reinterpret_cast<int32_t *>(h + o_seg);
+        int2 *ent = reinterpret_cast<int2 *>(h + o_ent);
+        std::fill(seg, seg + n_seg, 0);
+        for (int32_t r = 0; r < n_rows; r++)
+            for (int32_t q = col_off[r]; q < col_off[r + 1]; q++) seg[(int64_t)(col_ids[q] / RB) * (n_rows + 1) + r]++;
+        int32_t run = 0;
+        for (int64_t i = 0; i < n_seg; i++) {
+            const int32_t n = seg[i];
+            seg[i] = run;
+            run += n;
+        }
+        // seg[t][r] is the start of (t, r); entries are written in order
+        std::vector<int32_t> fillp(seg, seg + n_seg);
+        for (int32_t r = 0; r < n_rows; r++)
+            for (int32_t q = col_off[r]; q < col_off[r + 1]; q++) {
+                const int32_t col = col_ids[q], t = col / RB;
+                ent[fillp[(size_t)t * (n_rows + 1) + r]++] = make_int2(q, (r << 8) | (col - t * RB));
+            }
+        // (seg[t][n_rows], an empty bucket, is the end of tile t's entries)
+    }
+    if (n_compl) {
+        int32_t *rc = reinterpret_cast<int32_t *>(h + o_rc);
+        int32_t *x0 = reinterpret_cast<int32_t *>(h + o_x0);
+        for (int32_t r = 0; r < n_rows; r++) rc[r] = -1;
+        for (int32_t j = 0; j < n_compl; j++) {
+            rc[compl_rows[j]] = j;
+            x0[j] = item_ids[item_off[compl_rows[j]]];
+        }
+        memcpy(h + o_excl, excl, (size_t)(n_compl * excl_words * 8));
+    }
+    // word split: about three workgroups per CU, at least 16 stages per
+    // split, scaled by the fused path's own split tuner
+    const int64_t tiles = (int64_t)tiles_m * tiles_n;
+    const int64_t stages = c->pitch / 8;
+    const int64_t base = c->mfma_split == 1
+                             ? 1
+                             : std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
+    int sk_arm = -1;
+    int64_t split = base;
+    if (base > 1) {
+        sk_arm = c->fk.pick();
+        split = SplitTuner::split_of(sk_arm, base, stages);
+    }
+    int64_t kps = (c->pitch + split - 1) / split;
+    kps = (kps + 7) / 8 * 8;
+    kps = std::min<int64_t>(kps, (int64_t)1 << 18);   // <= 2^24 bits per split: accumulators stay below 2^30
+    const int64_t zs = (c->pitch + kps - 1) / kps;
+    WaveMfmaDesc d{};
+    const char *dd = sl.d_desc;
+    d.slot_rows = reinterpret_cast<const int32_t *>(dd + o_slots);
+    d.item_off = reinterpret_cast<const int32_t *>(dd + o_ioff);
+    d.item_slots = reinterpret_cast<const int32_t *>(dd + o_isl);
+    d.seg_off = csr ? reinterpret_cast<const int32_t *>(dd + o_seg) : nullptr;
+    d.entries = reinterpret_cast<const int2 *>(dd + o_ent);
+    d.row_compl = n_compl ? reinterpret_cast<const int32_t *>(dd + o_rc) : nullptr;
+    d.compl_x0 = reinterpret_cast<const int32_t *>(dd + o_x0);
+    d.excl = reinterpret_cast<const uint64_t *>(dd + o_excl);
+    d.out = sl.d_out;
+    d.res = sl.d_out + n_exp;
+    d.acc = sl.d_out + o_acc;
+    d.arrive = sl.d_out + o_arr;
+    d.k_per_split = kps;
+    d.n_rows = n_rows;
+    d.n_cols = c->n_cols;
+    d.n_slots = (int32_t)slot_rows.size();
+    d.excl_words = (int32_t)excl_words;
+    d.min_count = min_count;
+    d.tiles_n = tiles_n;
+    d.dense_out = csr ? 0 : 1;
+    KernelLaunch K;
+    c->fused_nofit = "a launch the kernel accepts (tile, slots, LDS attribute)";
+    if (!plan_wave_mfma(c->d_bm, c->pitch, d, ti, tj, tiles_m, (int)zs, lds, K)) return kFusedNoFit;
+    KernelLaunch *ks[1] = {&K};
+    if (int r = issue_wave(c, sl, total, sl.d_out, n_dev, n_out, ks, 1)) return r;
+    HIPCHK(c, hipEventRecord(sl.done, c->stream));
+    sl.busy = true;
+    sl.ticket = c->next_ticket++;
+    sl.n_out = n_out;
+    sl.n_rows = n_rows;
+    sl.sk_arm = sk_arm;
+    sl.sk_fused = true;
+    sl.sk_work = (double)tiles_m * RA * (double)tiles_n * RB * (double)c->pitch;
+    c->wave_fused_launches++;
     return TFIM_OK;
 }
 
@@ -692,9 +880,12 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_nodes: no bitmaps loaded");
     if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED && path != TFIM_PATH_MFMA_I8)
         return fail(c, TFIM_E_INVALID, "count_nodes: unknown path %d", path);
-    // the path argument overrides the configured wave kernel for this wave
-    const int wave_kernel = path == TFIM_PATH_MFMA_I8 ? 4
-                            : path == TFIM_PATH_PACKED && c->wave_kernel == 4 ? 0 : c->wave_kernel;
+    // the path argument overrides the configured wave kernel for this wave:
+    // TFIM_PATH_MFMA_I8 takes the fused matrix-core kernel, or the
+    // three-launch form when the wave is outside the fused kernel's limits
+    const bool mfma_any = path == TFIM_PATH_MFMA_I8;
+    const int wave_kernel = mfma_any ? 5
+                            : path == TFIM_PATH_PACKED && c->wave_kernel >= 4 ? 0 : c->wave_kernel;
     const bool allow_mfma = path != TFIM_PATH_PACKED;
     if (n_rows == 0) return TFIM_OK;
     tfim_ctx::Slot &sl = c->slots[c->next_ticket % tfim_ctx::kWaveSlots];
@@ -796,6 +987,21 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             lds = false;
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
             slot_rows.clear();
+        } else if (kArmKernel[tuned] == 5 && allow_mfma) {
+            // the fused matrix-core kernel on a wave the LDS kernel also takes
+            const int r = wave_submit_fused(c, sl, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl,
+                                            excl, min_count, n_out, n_out);
+            if (r == TFIM_OK) {
+                sl.tuned = tuned;
+                sl.work = work;
+                sl.mf_arm = -1;
+                *ticket = sl.ticket;
+                return TFIM_OK;
+            }
+            if (r != kFusedNoFit) return r;
+            tuned = 2;   // outside the fused kernel's limits: an LDS arm counts it
+        } else if (kArmKernel[tuned] == 5) {
+            tuned = 2;
         }
     }
     // Waves outside the LDS kernel go to the L2 kernel (counts only the
@@ -807,14 +1013,29 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     // cheaper.
     int mf_arm = -1;
     double mf_work = 0;
-    if (!lds && allow_mfma && (wave_kernel == 0 || wave_kernel == 4)) {
+    if (!lds && allow_mfma && wave_kernel == 5) {
+        // the fused matrix-core kernel, forced (or asked for by the path)
+        const int r = wave_submit_fused(c, sl, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl,
+                                        excl, min_count, n_exp, n_out);
+        if (r == TFIM_OK) {
+            sl.tuned = -1;
+            sl.mf_arm = -1;
+            *ticket = sl.ticket;
+            return TFIM_OK;
+        }
+        if (r != kFusedNoFit) return r;
+        if (!mfma_any)
+            return fail(c, TFIM_E_RANGE, "count_nodes: wave outside the fused matrix-core kernel's limits (it needs %s)",
+                        c->fused_nofit);
+    }
+    if (!lds && allow_mfma && (wave_kernel == 0 || wave_kernel == 4 || mfma_any)) {
         const int64_t mask_bytes = (int64_t)n_rows * c->pitch * 8;
         // (the mask kernel's grid has one row per y index, at most 65535)
         const bool fits = c->n_cols > 0 && n_rows <= 65535 && mask_bytes <= kMfmaMaskBytes &&
                           (int64_t)n_rows * c->n_cols <= INT32_MAX;
-        if (wave_kernel == 4 && !fits)
+        if ((wave_kernel == 4 || mfma_any) && !fits)
             return fail(c, TFIM_E_RANGE, "count_nodes: wave too large for the matrix-core path (%d rows)", n_rows);
-        if (fits) {
+        if (fits || wave_kernel == 0) {
             double pairs = (double)n_exp;
             for (int32_t j = 0; j < n_compl; j++) {
                 const uint64_t *ex = excl + (int64_t)j * excl_words;
@@ -829,17 +1050,47 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             const double w_mf = ((double)((n_rows + ra - 1) / ra * ra) * (double)((c->n_cols + 127) / 128 * 128) +
                                  39.0 * (double)(n_item_ids + n_rows)) *
                                 (double)c->pitch;
+            // fused: padded (row, survivor) pairs of its tiles, plus the
+            // prefix ANDs built from LDS (an eighth of a pair per prefix item
+            // and tile column)
+            const int tjf = c->n_cols <= 64 ? 1 : 2;
+            const int raf = 64 * (n_rows <= 64 ? 1 : (n_rows <= 128 || tjf == 2) ? 2 : 4), rbf = 64 * tjf;
+            const double tnf = (double)((c->n_cols + rbf - 1) / rbf);
+            const double w_fu = ((double)((n_rows + raf - 1) / raf * raf) * tnf * rbf +
+                                 8.0 * (double)n_item_ids * tnf) * (double)c->pitch;
+            const double w[3] = {w_l2, w_mf, w_fu};
             int arm = 1;
             if (wave_kernel == 0) {
                 c->mf_calls++;
                 // the costs start from priors (the L2 kernel's and the matrix
                 // cores' rates on MI355X, about 300 and 1,500 Tbit-op/s over
-                // their units of work) and become measured EMAs; the other
-                // arm is tried on every 16th wave
-                arm = w_mf * c->mf_cost[1] < w_l2 * c->mf_cost[0] ? 1 : 0;
-                if (c->mf_calls % 16 == 0) arm = 1 - arm;
+                // their units of work) and become measured EMAs; another arm
+                // is tried on every 16th wave
+                arm = 0;
+                for (int a = 1; a < 3; a++)
+                    if ((a != 1 || fits) && w[a] * c->mf_cost[a] < w[arm] * c->mf_cost[arm]) arm = a;
+                if (c->mf_calls % 16 == 0) {
+                    arm = (arm + 1 + (int)((c->mf_calls / 16) % 2)) % 3;
+                    if (arm == 1 && !fits) arm = 2;
+                }
+            }
+            if (arm == 2) {
+                const int r = wave_submit_fused(c, sl, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl,
+                                                excl, min_count, n_exp, n_out);
+                if (r == TFIM_OK) {
+                    sl.tuned = tuned;
+                    sl.work = tuned >= 0 ? work : 0;
+                    sl.mf_arm = 2;
+                    sl.mf_work = w_fu;
+                    *ticket = sl.ticket;
+                    return TFIM_OK;
+                }
+                if (r != kFusedNoFit) return r;
+                arm = fits && w_mf * c->mf_cost[1] < w_l2 * c->mf_cost[0] ? 1 : 0;
+            }
+            if (wave_kernel == 0) {
                 mf_arm = arm;
-                mf_work = arm ? w_mf : w_l2;
+                mf_work = w[arm];
             }
             if (arm == 1) {
                 int r = wave_submit_mfma(c, sl, item_ids, item_off, n_rows, col_ids, col_off, compl_rows, n_compl,
@@ -1061,12 +1312,7 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     c->wave_ms += ms;
     c->wave_launches++;
     c->wave_rows += sl.n_rows;
-    if (sl.sk_arm >= 0 && sl.sk_work > 0) {
-        const double per = (double)ms / sl.sk_work;
-        double &cst = c->sk_cost[sl.sk_arm];
-        cst = c->sk_samples[sl.sk_arm] == 0 ? per : 0.8 * cst + 0.2 * per;
-        c->sk_samples[sl.sk_arm]++;
-    }
+    if (sl.sk_arm >= 0 && sl.sk_work > 0) (sl.sk_fused ? c->fk : c->sk).record(sl.sk_arm, (double)ms / sl.sk_work);
     if (sl.mf_arm >= 0 && sl.mf_work > 0) {
         const double per = (double)ms / sl.mf_work;
         double &cst = c->mf_cost[sl.mf_arm];
@@ -1093,7 +1339,7 @@ int tfim_count_nodes(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_o
 int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
     if (!c || !key) return TFIM_E_INVALID;
     const std::string k(key);
-    if (k == "wave_kernel" && value >= 0 && value <= 4) {
+    if (k == "wave_kernel" && value >= 0 && value <= 5) {
         c->wave_kernel = (int)value;
         return TFIM_OK;
     }
@@ -1109,6 +1355,10 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
     }
     if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256)) {
         c->mfma_tile = (int)value;
+        return TFIM_OK;
+    }
+    if (k == "mfma_split" && (value == 0 || value == 1)) {
+        c->mfma_split = (int)value;
         return TFIM_OK;
     }
     if (k == "graphs" && (value == 0 || value == 1)) {
@@ -1132,12 +1382,13 @@ int tfim_wave_stats(tfim_ctx *c, int64_t *launches, int64_t *lds_launches, int64
     return TFIM_OK;
 }
 
-int tfim_wave_path_stats(tfim_ctx *c, int64_t *l2, int64_t *lds, int64_t *mfma) {
+int tfim_wave_path_stats(tfim_ctx *c, int64_t *l2, int64_t *lds, int64_t *mfma, int64_t *fused) {
     if (!c) return TFIM_E_INVALID;
     std::lock_guard<std::mutex> g(c->wave_mu);
-    if (l2) *l2 = c->wave_launches - c->wave_lds_launches - c->wave_mfma_launches;
+    if (l2) *l2 = c->wave_launches - c->wave_lds_launches - c->wave_mfma_launches - c->wave_fused_launches;
     if (lds) *lds = c->wave_lds_launches;
     if (mfma) *mfma = c->wave_mfma_launches;
+    if (fused) *fused = c->wave_fused_launches;
     return TFIM_OK;
 }
 

@@ -63,7 +63,7 @@ int tfim_count_nodes_ex_submit(tfim_ctx *c, const int32_t *ids, const int32_t *i
 int tfim_count_nodes_wait(tfim_ctx *, int64_t t, int32_t *out) { if(t<0) return 0; auto &v=g_pending[t%3]; memcpy(out,v.data(),v.size()*4); return 0; }
 int tfim_wave_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *r, double *ms) { if(a)*a=0; if(b)*b=0; if(r)*r=0; if(ms)*ms=0; return 0; }
 int tfim_wave_graph_stats(tfim_ctx *, int64_t *g, int64_t *l) { if(g)*g=0; if(l)*l=0; return 0; }
-int tfim_wave_path_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *m) { if(a)*a=0; if(b)*b=0; if(m)*m=0; return 0; }
+int tfim_wave_path_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *m, int64_t *f) { if(a)*a=0; if(b)*b=0; if(m)*m=0; if(f)*f=0; return 0; }
 int tfim_mlp_create(tfim_ctx *, uint64_t, tfim_mlp **) { return -2; }
 void tfim_mlp_destroy(tfim_mlp *) {}
 int tfim_mlp_get_params(tfim_mlp *, float *) { return -2; }

@@ -32,7 +32,8 @@ def _id(e):
 def test_gold_identity(fimi_dir, tmp_path, ds, thr, engine):
     name, extra = engine
     if name == "plain" and ds == "connect.txt" and thr == 0.85:
-        pytest.skip("142k single-node launches; covered by the wave and mafia engines")
+        # 142k nodes of single-node launches: the reference's worker pool
+        extra = extra + ["--workers", "8"]
     out = str(tmp_path / ("%s-%s.res" % (ds, name)))
     path, stats = tf.mine(str(fimi_dir / ds), thr, engine=name, out=out, extra=extra, timeout=300)
     print(ds, thr, name, extra, {k: stats.get(k) for k in ("MFIsNumber", "rows", "waves", "computation_time(s)")},

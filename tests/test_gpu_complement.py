@@ -61,12 +61,14 @@ def make_wave(rng, n_items, n_cols, n_rows, p_compl):
     return prefixes, tails, excl
 
 
-@pytest.mark.parametrize("kernel", [1, 4])
+@pytest.mark.parametrize("kernel", [1, 4, 5])
 @pytest.mark.parametrize("n_bits,n_items,n_cols", [(70, 9, 8), (3200, 131, 130), (2048 * 5 + 17, 200, 199),
                                                    (300_017, 150, 128), (1_000_003, 70, 65)])
 def test_count_nodes_ex_matches_oracle(ctx, n_bits, n_items, n_cols, kernel):
     """Kernel 1: L2 kernel + complement reduction; kernel 4: matrix cores
-    (masks, int8-MFMA count of every pair, gather and reduction)."""
+    (masks, int8-MFMA count of every pair, gather and reduction); kernel 5:
+    the fused matrix-core kernel (complement rows thresholded by the last
+    workgroup of each row tile)."""
     rng = np.random.default_rng(n_bits + n_items)
     rows = rand_rows(rng, n_items, n_bits, 0.6)
     ctx.load_bitmaps(rows, n_bits, n_cols)
@@ -134,15 +136,17 @@ def test_complement_rows_rejected_when_malformed(ctx):
     np.testing.assert_array_equal(ok[0], expected(rows, 10, [3, 11], {3}, 3))
 
 
-def test_matrix_core_split_tuner_arms(ctx):
+@pytest.mark.parametrize("kernel", [4, 5])
+def test_matrix_core_split_tuner_arms(ctx, kernel):
     """Matrix-core waves on long rows (split over the words) run every arm of
     the split tuner (1/2, 1 and 2 times the occupancy rule's split, three
-    samples each, then re-tries): every wave stays exact."""
+    samples each, then re-tries): every wave stays exact, in the three-launch
+    form (4) and the fused kernel (5)."""
     rng = np.random.default_rng(23)
     n_bits, n_items, n_cols = 1_500_007, 60, 57
     rows = rand_rows(rng, n_items, n_bits, 0.55)
     ctx.load_bitmaps(rows, n_bits, n_cols)
-    ctx.config("wave_kernel", 4)
+    ctx.config("wave_kernel", kernel)
     try:
         for it in range(14):
             prefixes, tails, excl = make_wave(rng, n_items, n_cols, int(rng.integers(1, 70)), 0.4)

@@ -165,12 +165,13 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
 
 
 @pytest.mark.parametrize("graphs", [1, 0])
-@pytest.mark.parametrize("kernel", [1, 2, 3, 4])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("n_bits,n_items", [(1100, 30), (3200, 75), (400_003, 90), (2_000_000, 38)])
 def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
     """Every batched path (1 = L2-resident kernel, 2 = LDS-staged, 3 =
     LDS-staged with wide staging, 4 = matrix cores: masks, int8-MFMA count of
-    all (row, survivor) pairs, gather) on the same waves,
+    all (row, survivor) pairs, gather; 5 = the fused matrix-core kernel) on
+    the same waves,
     replayed from graphs or issued directly: uneven prefixes, uneven and empty
     tails, full-column rows."""
     rng = np.random.default_rng(kernel * 100 + n_items)
@@ -258,7 +259,66 @@ def test_count_nodes_path_argument(ctx, path):
     for p, t, g in zip(prefixes, tails, got):
         np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_items - 1)[t])
     after = ctx.wave_stats()
+    mat = lambda st: st["mfma_launches"] + st["fused_launches"]   # noqa: E731
     if path == "mfma":
-        assert after["mfma_launches"] == before["mfma_launches"] + 1
+        assert after["fused_launches"] == before["fused_launches"] + 1   # the fused kernel takes it
+        assert mat(after) == mat(before) + 1
     else:
-        assert after["mfma_launches"] == before["mfma_launches"]
+        assert mat(after) == mat(before)
+
+
+@pytest.mark.parametrize("n_bits,n_items,n_cols,n_rows,kmax", [
+    (2048 * 3 + 5, 60, 59, 700, 10),        # 64-column tiles, 256-row tiles, three row tiles
+    (300_001, 300, 290, 300, 6),            # 128-column tiles, three of them, 128-row tiles
+    (1_500_007, 40, 39, 150, 25),           # long rows: the word axis split, deep prefixes
+    (5_000, 260, 200, 64, 3)])              # one 64-row tile
+def test_fused_matrix_core_wave_shapes(ctx, n_bits, n_items, n_cols, n_rows, kmax):
+    """The fused matrix-core kernel (wave_kernel 5) over several row and
+    column tiles, split word ranges and deep prefixes; tails and full rows."""
+    rng = np.random.default_rng(n_rows + n_items)
+    rows = rand_rows(rng, n_items, n_bits, 0.7)
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    ctx.config("wave_kernel", 5)
+    try:
+        before = ctx.wave_stats()["fused_launches"]
+        for rep in range(2):
+            pool = rng.choice(n_items, size=min(n_items, 250), replace=False)   # <= 256 distinct prefix items
+            prefixes = [rng.choice(pool, size=int(rng.integers(1, kmax + 1)), replace=False) for _ in range(n_rows)]
+            tails = [np.sort(rng.choice(n_cols, size=int(rng.integers(0, n_cols + 1)), replace=False))
+                     for _ in range(n_rows)]
+            print("rep %d: %d rows, %d distinct prefix items" % (rep, n_rows, len(set(np.concatenate(prefixes)))),
+                  flush=True)
+            got = ctx.count_nodes(prefixes, tails)
+            for i, (p, t, g) in enumerate(zip(prefixes, tails, got)):
+                np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols)[t], err_msg="row %d" % i)
+        full = ctx.count_nodes(prefixes[:100])
+        for p, g in zip(prefixes[:100], full):
+            np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols))
+        assert ctx.wave_stats()["fused_launches"] == before + 3
+    finally:
+        ctx.config("wave_kernel", 0)
+
+
+def test_fused_matrix_core_limits(ctx):
+    """A wave with 257 distinct prefix items is outside the fused kernel:
+    forcing it is an error, and the matrix-core path argument then takes the
+    three-launch form, exactly."""
+    rng = np.random.default_rng(3)
+    n_bits, n_items, n_cols = 9_000, 300, 280
+    rows = rand_rows(rng, n_items, n_bits, 0.8)
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    prefixes = [[i] for i in range(257)]
+    tails = [list(range(0, n_cols, 7))] * 257
+    ctx.config("wave_kernel", 5)
+    try:
+        with pytest.raises(tf.TfimError, match="256 distinct prefix items"):
+            ctx.count_nodes(prefixes, tails)
+    finally:
+        ctx.config("wave_kernel", 0)
+    before = ctx.wave_stats()
+    got = ctx.count_nodes(prefixes, tails, path="mfma")
+    for p, t, g in zip(prefixes, tails, got):
+        np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols)[t])
+    after = ctx.wave_stats()
+    assert after["mfma_launches"] == before["mfma_launches"] + 1
+    assert after["fused_launches"] == before["fused_launches"]
