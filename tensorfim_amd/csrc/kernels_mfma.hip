@@ -18,7 +18,6 @@
 // (half, byte) pairs, so any consistent assignment of k to (half, byte) is
 // exact.  The accumulator layout is the 32x32 one of the CDNA3/CDNA4 ISA:
 // register q of lane l holds C[8*(q/4) + 4*(l/32) + (q%4)][l % 32].
-#include <cstdio>
 #include <mutex>
 
 #include "tfim_internal.h"
@@ -358,13 +357,17 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
 //     arrive (arrival counter) thresholds them: support of the prefix and
 //     the number of non-excluded survivors reaching min_count.
 constexpr int kFusedSlotLoads = 4;   // uint4 slot loads per thread per stage: at most 256 slots
+constexpr int kFusedRegSlots = 16;   // prefix slots per row held in registers
+// u32 per staged slot: 16 data + 4 pad, so that the 16 rows a wave builds at
+// once read 16 different slots from disjoint LDS banks
+constexpr int PS_STRIDE = 20;
 
 template <int TI, int TJ>
 __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict__ bm, int64_t pitch,
                                                       WaveMfmaDesc d) {
     constexpr int RA = 64 * TI, RB = 64 * TJ, TS = RB + 1;
     extern __shared__ uint32_t lds_u32[];
-    __shared__ int last_arrival;
+    __shared__ int last_arrival, any_compl;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, l32 = lane & 31;
@@ -379,25 +382,37 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
 
     uint32_t *As = lds_u32;                              // [2][RA][RP_STRIDE]
     uint32_t *Bs = As + 2 * RA * RP_STRIDE;              // [2][RB][RP_STRIDE]
-    uint32_t *Ps = Bs + 2 * RB * RP_STRIDE;              // [2][n_slots][16]
-    int32_t *loff = reinterpret_cast<int32_t *>(Ps + 2 * d.n_slots * 16);   // [RA + 1]
+    uint32_t *Ps = Bs + 2 * RB * RP_STRIDE;              // [2][n_slots][PS_STRIDE]
+    int32_t *loff = reinterpret_cast<int32_t *>(Ps + 2 * d.n_slots * PS_STRIDE);   // [RA + 1]
     uint16_t *lsl = reinterpret_cast<uint16_t *>(loff + RA + 1);           // prefix slots of the tile's rows
 
     // the tile's prefix slot lists -> LDS (rows past the wave's end: empty)
     const int ib0 = d.item_off[row0];
     for (int r = tid; r <= RA; r += 256) loff[r] = d.item_off[min(row0 + r, rend)] - ib0;
     const int nl = d.item_off[rend] - ib0;
-    for (int i = tid; i < nl; i += 256) lsl[i] = (uint16_t)d.item_slots[ib0 + i];
+    // four loads in flight per thread and round (the last index is read
+    // again past the end, not stored)
+    for (int i0 = tid; i0 < nl; i0 += 1024) {
+        int32_t v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = d.item_slots[ib0 + min(i0 + 256 * q, nl - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (i0 + 256 * q < nl) lsl[i0 + 256 * q] = (uint16_t)v[q];
+    }
 
     typedef uint32_t u4v __attribute__((ext_vector_type(4)));
     typedef const __attribute__((address_space(1))) u4v *gptr4;
-    // slot loads: load j of thread t is uint4 part (idx & 3) of slot (idx >> 2), idx = 256 j + t
+    // slot loads: load j of thread t is uint4 part (idx & 3) of slot (idx >> 2), idx = 256 j + t.
+    // Loads past the last slot read the last slot's last part and are not
+    // stored: loads without a branch around them stay in flight together
+    // (under a per-lane branch each one waited for the one before).
     const int ns4 = d.n_slots * 4;
     int64_t sbase[kFusedSlotLoads];
 #pragma unroll
     for (int j = 0; j < kFusedSlotLoads; j++) {
-        const int idx = j * 256 + tid;
-        sbase[j] = idx < ns4 ? (int64_t)d.slot_rows[idx >> 2] * pitch + 2 * (idx & 3) : -1;
+        const int idx = min(j * 256 + tid, ns4 - 1);
+        sbase[j] = (int64_t)d.slot_rows[idx >> 2] * pitch + 2 * (idx & 3);
     }
     // B loads: as k_popmat_mfma_rp (rows past the edge read the last row, zeroed at the store)
     int64_t bbase[TJ];
@@ -411,14 +426,13 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
     u4v rs[kFusedSlotLoads], rb[TJ];
     auto gload_s = [&](int64_t kw) {
 #pragma unroll
-        for (int j = 0; j < kFusedSlotLoads; j++)
-            if (sbase[j] >= 0) rs[j] = *(gptr4)(bm + sbase[j] + kw);
+        for (int j = 0; j < kFusedSlotLoads; j++) rs[j] = *(gptr4)(bm + sbase[j] + kw);
     };
     auto lstore_s = [&](int buf) {
 #pragma unroll
         for (int j = 0; j < kFusedSlotLoads; j++) {
             const int idx = j * 256 + tid;
-            if (sbase[j] >= 0) *reinterpret_cast<u4v *>(&Ps[(buf * d.n_slots + (idx >> 2)) * 16 + 4 * (idx & 3)]) = rs[j];
+            if (idx < ns4) *reinterpret_cast<u4v *>(&Ps[(buf * d.n_slots + (idx >> 2)) * PS_STRIDE + 4 * (idx & 3)]) = rs[j];
         }
     };
     auto gload_b = [&](int64_t kw) {
@@ -433,22 +447,52 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
         }
     };
     // row masks of the stage in Ps[pbuf] -> As[abuf]; task j of thread t is
-    // uint4 part (x & 3) of tile row (x >> 2), x = 256 j + t
+    // uint4 part (x & 3) of tile row (x >> 2), x = 256 j + t.  Each task's
+    // first kFusedRegSlots prefix slots are held in registers (bytes; a
+    // shorter prefix is padded with its first slot, which the AND absorbs),
+    // so a stage's loads of one task do not wait on each other; groups of
+    // four run up to the largest prefix in the wave.  Longer prefixes read
+    // the rest of their list from LDS.
+    uint32_t sreg[TI][kFusedRegSlots / 4];
+    int kt[TI], kw[TI];
+    auto load_lists = [&]() {
+#pragma unroll
+        for (int j = 0; j < TI; j++) {
+            const int r = (j * 256 + tid) >> 2;
+            const int b = loff[r], k = loff[r + 1] - b;
+            const uint32_t first = k > 0 ? lsl[b] : 0u;
+#pragma unroll
+            for (int q = 0; q < kFusedRegSlots / 4; q++) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int t = 0; t < 4; t++) w |= (4 * q + t < k ? (uint32_t)lsl[b + 4 * q + t] : first) << (8 * t);
+                sreg[j][q] = w;
+            }
+            kt[j] = k;
+            int mx = k;
+            for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+            kw[j] = __builtin_amdgcn_readfirstlane(mx);
+        }
+    };
     auto build = [&](int pbuf, int abuf) {
-        const uint32_t *P = Ps + pbuf * d.n_slots * 16;
+        const uint32_t *P = Ps + pbuf * d.n_slots * PS_STRIDE;
 #pragma unroll
         for (int j = 0; j < TI; j++) {
             const int x = j * 256 + tid, r = x >> 2, part = x & 3;
-            const int b = loff[r], e = loff[r + 1];
-            u4v m = {0u, 0u, 0u, 0u};
-            if (e > b) {
-                m = *reinterpret_cast<const u4v *>(&P[lsl[b] * 16 + 4 * part]);
-                int i = b + 1;
-                for (; i + 1 < e; i += 2)
-                    m &= *reinterpret_cast<const u4v *>(&P[lsl[i] * 16 + 4 * part]) &
-                         *reinterpret_cast<const u4v *>(&P[lsl[i + 1] * 16 + 4 * part]);
-                if (i < e) m &= *reinterpret_cast<const u4v *>(&P[lsl[i] * 16 + 4 * part]);
+            const uint32_t *Pp = P + 4 * part;
+            auto ld = [&](uint32_t slot) { return *reinterpret_cast<const u4v *>(&Pp[slot * PS_STRIDE]); };
+            u4v m = ld(sreg[j][0] & 255u);
+#pragma unroll
+            for (int q = 0; q < kFusedRegSlots / 4; q++) {
+                if (4 * q >= kw[j]) break;   // wave-uniform
+                const uint32_t w = sreg[j][q];
+                m &= (ld(w & 255u) & ld((w >> 8) & 255u)) & (ld((w >> 16) & 255u) & ld(w >> 24));
             }
+            if (kw[j] > kFusedRegSlots) {
+                const int b = loff[r];
+                for (int i = kFusedRegSlots; i < kt[j]; i++) m &= ld(lsl[b + i]);
+            }
+            if (kt[j] == 0) m = u4v{0u, 0u, 0u, 0u};   // rows past the wave's end
             *reinterpret_cast<u4v *>(&As[(abuf * RA + r) * RP_STRIDE + 4 * part]) = m;
         }
     };
@@ -472,13 +516,18 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
         lstore_b(0);
     }
     __syncthreads();   // Ps, Bs of the first stages, loff, lsl
+    load_lists();
     if (kbeg < kend) build(0, 0);
     __syncthreads();
     int cur = 0;
+    // (the last stages load the last stage again, unused: no branch around
+    // the loads, so they are all in flight together)
+    const int64_t klast = kend - 8;
     for (int64_t kw = kbeg; kw < kend; kw += 8) {
         const bool n1 = kw + 8 < kend, n2 = kw + 16 < kend;
-        if (n2) gload_s(kw + 16);
-        if (n1) gload_b(kw + 8);
+        gload_s(std::min<int64_t>(kw + 16, klast));
+        gload_b(std::min<int64_t>(kw + 8, klast));
+        asm volatile("" ::: "memory");   // the loads are issued before this stage's MFMAs
 #pragma unroll
         for (int hk = 0; hk < 2; hk++) {
             uint4 pa[TI], pb[TJ];
@@ -541,12 +590,19 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
     if (d.seg_off) {
         // listed outputs of this tile: entries of column tile tn, rows row0 .. rend
         const int32_t *seg = d.seg_off + (int64_t)tn * (d.n_rows + 1);
-        const int e1 = seg[rend];
-        for (int e = seg[row0] + tid; e < e1; e += 256) {
-            const int2 en = d.entries[e];
-            const int v = T[((en.y >> 8) - row0) * TS + (en.y & 255)];
-            if (!split) d.out[en.x] = v;
-            else if (v) atomicAdd(&d.out[en.x], v);
+        const int e0 = seg[row0], e1 = seg[rend];
+        // four entries per thread and round, their loads in flight together
+        for (int b = e0 + tid; b < e1; b += 1024) {
+            int2 en[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) en[q] = d.entries[min(b + 256 * q, e1 - 1)];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (b + 256 * q >= e1) break;
+                const int v = T[((en[q].y >> 8) - row0) * TS + (en[q].y & 255)];
+                if (!split) d.out[en[q].x] = v;
+                else if (v) atomicAdd(&d.out[en[q].x], v);
+            }
         }
     } else if (d.dense_out) {
         for (int i = tid; i < RA * RB; i += 256) {
@@ -561,20 +617,27 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
     }
     if (!d.row_compl) return;
     // complement rows: partial counts of the live columns into the full-count
-    // accumulator (uniform over the workgroup: every thread walks the rows)
-    bool any = false;
-    for (int r = row0; r < rend; r++) {
-        const int j = d.row_compl[r];
-        if (j < 0) continue;
-        any = true;
-        const int x0 = d.compl_x0[j];
-        for (int c = tid; c < RB && col0 + c < d.n_cols; c += 256) {
-            const int e = col0 + c;
-            const bool live = e == x0 || !((d.excl[(int64_t)j * d.excl_words + (e >> 6)] >> (e & 63)) & 1ull);
-            const int v = T[(r - row0) * TS + c];
+    // accumulator; the tile's complement indices go to LDS first
+    int *rcl = T + RA * TS;
+    bool mine = false;
+    for (int r = tid; r < RA; r += 256) {
+        const int j = row0 + r < rend ? d.row_compl[row0 + r] : -1;
+        rcl[r] = j;
+        mine |= j >= 0;
+    }
+    if (tid == 0) any_compl = 0;
+    __syncthreads();
+    if (mine) any_compl = 1;   // every writer stores 1
+    __syncthreads();
+    const bool any = any_compl != 0;
+    if (any)
+        for (int i = tid; i < RA * RB; i += 256) {
+            const int r = i / RB, c = i % RB, j = rcl[r], e = col0 + c;
+            if (j < 0 || e >= d.n_cols) continue;
+            const bool live = e == d.compl_x0[j] || !((d.excl[(int64_t)j * d.excl_words + (e >> 6)] >> (e & 63)) & 1ull);
+            const int v = T[r * TS + c];
             if (live && v) atomicAdd(&d.acc[(int64_t)j * d.n_cols + e], v);
         }
-    }
     if (!any) return;
     __threadfence();
     __syncthreads();
@@ -605,9 +668,9 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
 
 size_t wave_mfma_lds_bytes(int ti, int tj, int n_slots, int max_tile_list) {
     const int RA = 64 * ti, RB = 64 * tj;
-    const size_t stage = (size_t)4 * (2 * RA * RP_STRIDE + 2 * RB * RP_STRIDE + 2 * n_slots * 16 + RA + 1) +
+    const size_t stage = (size_t)4 * (2 * RA * RP_STRIDE + 2 * RB * RP_STRIDE + 2 * n_slots * PS_STRIDE + RA + 1) +
                          (size_t)2 * max_tile_list + 4;
-    const size_t tile = (size_t)4 * RA * (RB + 1);
+    const size_t tile = (size_t)4 * (RA * (RB + 1) + RA);
     return (std::max(stage, tile) + 15) / 16 * 16;
 }
 
@@ -631,9 +694,13 @@ bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc 
                                  (const void *)k_wave_mfma<4, 1>, (const void *)k_wave_mfma<1, 2>,
                                  (const void *)k_wave_mfma<2, 2>};
             for (const void *fp : fn) {
-                const hipError_t e = hipFuncSetAttribute(fp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFusedMaxLds);
+                // the dynamic limit is the CU's LDS less the kernel's static shared variables
+                hipFuncAttributes fa{};
+                hipError_t e = hipFuncGetAttributes(&fa, fp);
+                if (e == hipSuccess && fa.sharedSizeBytes + kFusedMaxLds > 160 * 1024) e = hipErrorInvalidValue;
+                if (e == hipSuccess)
+                    e = hipFuncSetAttribute(fp, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFusedMaxLds);
                 if (e != hipSuccess) {
-                    fprintf(stderr, "k_wave_mfma: hipFuncSetAttribute: %s\n", hipGetErrorString(e));
                     (void)hipGetLastError();
                     return false;
                 }

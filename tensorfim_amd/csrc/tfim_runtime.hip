@@ -802,13 +802,14 @@ int wave_submit_fused(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, 
         }
         memcpy(h + o_excl, excl, (size_t)(n_compl * excl_words * 8));
     }
-    // word split: about three workgroups per CU, at least 16 stages per
-    // split, scaled by the fused path's own split tuner
+    // word split: the workgroups that fit at once (two per CU), at least 32
+    // stages per split, scaled by the fused path's own split tuner (each
+    // workgroup's set-up and output pass are worth a few stages)
     const int64_t tiles = (int64_t)tiles_m * tiles_n;
     const int64_t stages = c->pitch / 8;
     const int64_t base = c->mfma_split == 1
                              ? 1
-                             : std::max<int64_t>(1, std::min<int64_t>((3 * c->n_cu + tiles - 1) / tiles, stages / 16));
+                             : std::max<int64_t>(1, std::min<int64_t>((2 * c->n_cu + tiles - 1) / tiles, stages / 32));
     int sk_arm = -1;
     int64_t split = base;
     if (base > 1) {
@@ -981,7 +982,14 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             for (int a = 1; a < kTuneArms; a++)
                 if (c->tune_cost[a] < c->tune_cost[best]) best = a;
             tuned = best;
-            if (c->tune_calls % 16 == 0) tuned = (best + 1 + (int)((c->tune_calls / 16) % (kTuneArms - 1))) % kTuneArms;
+            if (c->tune_calls % 16 == 0) {
+                // round robin over the other arms; an arm measured at more
+                // than twice the best one's cost is re-tried only every
+                // eighth time its turn comes
+                const int64_t turn = c->tune_calls / 16;
+                const int cand = (best + 1 + (int)(turn % (kTuneArms - 1))) % kTuneArms;
+                if (c->tune_cost[cand] <= 2.0 * c->tune_cost[best] || (turn / (kTuneArms - 1)) % 8 == 0) tuned = cand;
+            }
         }
         if (kArmKernel[tuned] == 1) {
             lds = false;
@@ -1070,8 +1078,13 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
                 for (int a = 1; a < 3; a++)
                     if ((a != 1 || fits) && w[a] * c->mf_cost[a] < w[arm] * c->mf_cost[arm]) arm = a;
                 if (c->mf_calls % 16 == 0) {
-                    arm = (arm + 1 + (int)((c->mf_calls / 16) % 2)) % 3;
-                    if (arm == 1 && !fits) arm = 2;
+                    // another arm; one predicted at more than twice the
+                    // chosen one's time only every eighth turn
+                    const int64_t turn = c->mf_calls / 16;
+                    int cand = (arm + 1 + (int)(turn % 2)) % 3;
+                    if (cand == 1 && !fits) cand = 2;
+                    if (cand != arm && (w[cand] * c->mf_cost[cand] <= 2.0 * w[arm] * c->mf_cost[arm] || (turn / 2) % 8 == 0))
+                        arm = cand;
                 }
             }
             if (arm == 2) {
