@@ -134,7 +134,7 @@ int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item
  * while the GPU counts another: submit packs and launches the wave
  * asynchronously on the ctx stream and returns a ticket; wait blocks until
  * that wave's counts are done and copies them into `out` (layout as in
- * tfim_count_nodes).  At most three waves may be in flight per ctx; tickets
+ * tfim_count_nodes).  At most four waves may be in flight per ctx; tickets
  * are waited in submission order.  A ticket of -1 (nothing to count) waits
  * trivially.  One thread may call submit while another waits on an earlier
  * ticket of the same ctx; no other calls on a ctx may overlap. */

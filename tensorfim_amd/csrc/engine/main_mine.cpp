@@ -58,6 +58,7 @@ int main(int argc, char **argv) {
         else if (a == "--spec") opt.spec = next();
         else if (a == "--workers") opt.workers = atoi(next().c_str());
         else if (a == "--host-threads") opt.host_threads = atoi(next().c_str());
+        else if (a == "--flights") opt.flights = atoi(next().c_str());
         else if (a == "--rank") opt.rank = true;
         else if (a == "--wave-kernel") wave_kernel = atoi(next().c_str());
         else if (a == "--graphs") graphs = atoi(next().c_str());

@@ -48,6 +48,9 @@ class Policy {
     // ranker label of a child: fraction of its tail that stayed frequent
     virtual void observe_value(int32_t record, float value) { (void)record; (void)value; }
     virtual void finish(SearchStats &st) { (void)st; }
+    // true when decide() of one node depends on that node alone (no state,
+    // no batch): the engine may then call it per node from several threads
+    virtual bool decides_alone() const { return false; }
 };
 
 // MAFIA's plain policy: tail by increasing support (ties: ascending id).
@@ -59,6 +62,7 @@ class PlainPolicy : public Policy {
    public:
     explicit PlainPolicy(bool always = false) : always_(always) {}
     void decide(std::vector<Decision *> &ds, const Database &db) override;
+    bool decides_alone() const override { return true; }
 
    private:
     bool always_;

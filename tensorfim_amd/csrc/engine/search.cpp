@@ -14,6 +14,8 @@
 #include <thread>
 #include <unordered_set>
 
+#include <immintrin.h>
+
 #include "policy.h"
 
 namespace tfim {
@@ -423,9 +425,40 @@ struct MNode {
     }
 };
 
+// Whether the AND of the k >= 2 slices sl[] is non-zero in some word of
+// [w0, w1): portable form, one word at a time with an early exit per word.
+bool any_and_scalar(const uint64_t *const *sl, size_t k, size_t w0, size_t w1) {
+    for (size_t w = w0; w < w1; w++) {
+        uint64_t acc = sl[0][w] & sl[1][w];
+        for (size_t j = 2; j < k && acc; j++) acc &= sl[j][w];
+        if (acc) return true;
+    }
+    return false;
+}
+
+// The same eight words at a time with 512-bit ANDs (the slices are read in
+// rarest-first order, and a block of eight words stops once it is all zero).
+__attribute__((target("avx512f"))) bool any_and_avx512(const uint64_t *const *sl, size_t k, size_t w0,
+                                                        size_t w1) {
+    size_t w = w0;
+    for (; w + 8 <= w1; w += 8) {
+        __m512i a = _mm512_and_si512(_mm512_loadu_si512(sl[0] + w), _mm512_loadu_si512(sl[1] + w));
+        size_t j = 2;
+        for (; j + 1 < k && _mm512_test_epi64_mask(a, a); j += 2)
+            a = _mm512_and_si512(a, _mm512_and_si512(_mm512_loadu_si512(sl[j] + w), _mm512_loadu_si512(sl[j + 1] + w)));
+        if (j < k) a = _mm512_and_si512(a, _mm512_loadu_si512(sl[j] + w));
+        if (_mm512_test_epi64_mask(a, a)) return true;
+    }
+    return any_and_scalar(sl, k, w, w1);
+}
+
+using AnyAndFn = bool (*)(const uint64_t *const *, size_t, size_t, size_t);
+AnyAndFn pick_any_and() { return __builtin_cpu_supports("avx512f") ? any_and_avx512 : any_and_scalar; }
+const AnyAndFn any_and = pick_any_and();
+
 // Found MFIs, sliced by item: bits_[i] has bit m set when MFI m contains
 // item i.  X is inside some found MFI iff the AND of its items' slices is
-// non-zero, scanned word by word with an early exit.
+// non-zero in some word.
 class MfiIndex {
    public:
     explicit MfiIndex(int32_t n) : bits_((size_t)std::max(1, n)), cnt_((size_t)std::max(1, n), 0) {}
@@ -434,20 +467,15 @@ class MfiIndex {
     bool subsumed_from(const std::vector<int32_t> &x, int64_t n0) const {
         if (x.empty()) return n_ > n0;
         // rarest items first, so the AND of a word empties sooner
-        int32_t ord[64] = {};
-        const size_t k = std::min<size_t>(x.size(), 64);
-        std::copy_n(x.begin(), k, ord);
-        std::sort(ord, ord + k, [&](int32_t a, int32_t b) { return cnt_[a] < cnt_[b]; });
+        thread_local std::vector<const uint64_t *> sl;
+        thread_local std::vector<int32_t> ord;
+        ord.assign(x.begin(), x.end());
+        std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return cnt_[a] < cnt_[b]; });
         if (cnt_[ord[0]] == 0) return false;
-        const uint64_t *p0 = bits_[ord[0]].data();
-        const uint64_t *p1 = bits_[ord[k > 1 ? 1 : 0]].data();
-        for (size_t w = (size_t)(n0 >> 6); w < words_; w++) {
-            uint64_t acc = p0[w] & p1[w];
-            for (size_t j = 2; j < k && acc; j++) acc &= bits_[ord[j]][w];
-            for (size_t j = k; j < x.size() && acc; j++) acc &= bits_[x[j]][w];
-            if (acc) return true;
-        }
-        return false;
+        sl.clear();
+        for (int32_t i : ord) sl.push_back(bits_[i].data());
+        if (sl.size() == 1) sl.push_back(sl[0]);
+        return any_and(sl.data(), sl.size(), (size_t)(n0 >> 6), words_);
     }
     void add(const std::vector<int32_t> &x) {
         if (n_ % 64 == 0) {
@@ -646,6 +674,7 @@ class Mafia {
     int run(std::vector<Mfi> &out, std::string &err) {
         const int32_t F = db_.n_surv;
         cap_ = opt_.wave_cap > 0 ? opt_.wave_cap : auto_wave_cap(db_);
+        flights_ = std::max(2, std::min(kFlights, opt_.flights));
         if (opt_.wave_cap <= 0) sizer_.reset(new WaveSizeTuner(cap_));
         // maximality rows (HUT, LEAF) as complement rows reduced on the
         // device when the survivors are too many for listing them per row
@@ -682,18 +711,26 @@ class Mafia {
             push_children(root);
         }
         budget_ = out_budget(db_);
-        // Three waves in flight: while this thread processes wave k, the
-        // device counts k+1 and then k+2, which the helper thread packs and
-        // issues meanwhile (the library allows one submit beside one wait).
-        // Wave k+2 is taken off the stack before k's children are on it;
-        // the answers do not depend on the order of the search.
+        // flights_ waves in flight (four by default): while this thread
+        // processes wave k, the device counts k+1 and the next ones; the
+        // helper thread packs and issues the newest meanwhile (the library
+        // allows one submit beside one wait).  Those waves are taken off the
+        // stack before k's children are on it; the answers do not depend on
+        // the order of the search.
         int cur = 0;
+        const int nf = flights_;
         if (int rc = launch(fl_[cur], false, err)) return rc;
         while (fl_[cur].live) {
-            Flight &n1 = fl_[(cur + 1) % kFlights];
-            if (!n1.live)
-                if (int rc = launch(n1, false, err)) return rc;
-            Flight &n2 = fl_[(cur + 2) % kFlights];
+            // the waves between this one and the newest are in flight (or
+            // launched here if the stack ran dry before); the newest is
+            // packed and issued by the helper
+            for (int j = 1; j + 1 < nf; j++) {
+                Flight &nj = fl_[(cur + j) % nf];
+                if (!nj.live)
+                    if (int rc = launch(nj, false, err)) return rc;
+            }
+            Flight &n1 = fl_[(cur + nf - 2) % nf];
+            Flight &n2 = fl_[(cur + nf - 1) % nf];
             if (int rc = launch(n2, n1.live, err)) return rc;
             struct Join {
                 AsyncWorker &w;
@@ -720,7 +757,7 @@ class Mafia {
                 }
             }
             f.live = false;
-            cur = (cur + 1) % kFlights;
+            cur = (cur + 1) % nf;
             if (!fl_[cur].live)
                 if (int rc = launch(fl_[cur], false, err)) return rc;
         }
@@ -945,28 +982,35 @@ class Mafia {
             Decision &d = *ds[i];
             d.leaf_sub = -1;
             if (d.order.empty() || (d.lookahead && d.tail.size() >= 2)) return;   // a HUT row goes first
-            std::vector<int32_t> leaf = d.head;
+            thread_local std::vector<int32_t> leaf;
+            leaf.assign(d.head.begin(), d.head.end());
             leaf.push_back(d.tail[d.order.back()]);
             d.leaf_sub = index_.subsumed(leaf) ? 1 : 0;
             d.leaf_n = n_now;
         });
     }
 
-    void push_children(const Decision &d) {
-        // child i: head + t_i with tail t_{i+1..}; pushed last-first so the
-        // first child (largest tail) is counted first
+    // The children of d (child i: head + t_i with tail t_{i+1..}), in stacking
+    // order (last child first, so the first child, with the largest tail, is
+    // counted first), into out[0 ..) -- nodes the caller provides, at least
+    // d.order.size() of them.  Returns how many are kept: a leaf child inside
+    // a found MFI is dropped and counted in *subsumed.  Reads only d, the MFI
+    // index and the database (runs on the host pool).
+    size_t build_children(const Decision &d, MNode *out, int64_t *subsumed) const {
         const size_t k = d.order.size();
         // largest sup(head + t) over each suffix of the ordered tail
-        suffix_max_.assign(k + 1, 0);
-        for (size_t r = k; r-- > 0;) suffix_max_[r] = std::max(suffix_max_[r + 1], d.tail_sup[d.order[r]]);
+        thread_local std::vector<int32_t> smax;
+        smax.assign(k + 1, 0);
+        for (size_t r = k; r-- > 0;) smax[r] = std::max(smax[r + 1], d.tail_sup[d.order[r]]);
         // the ordered tail, shared by the speculative children of a wide
         // database (their complement rows do not list it; a child that lives
         // on copies its suffix then), so a node with a long tail does not copy
         // a quadratic number of items
         std::shared_ptr<std::vector<int32_t>> ordered;
+        size_t m = 0;
         for (size_t q = k; q-- > 0;) {
-            MNode c = take_node();
-            c.kind = MNode::EXPAND;
+            MNode &c = out[m];
+            c.reset();
             c.head.assign(d.head.begin(), d.head.end());
             c.head.push_back(d.tail[d.order[q]]);
             c.sup = d.tail_sup[d.order[q]];
@@ -977,8 +1021,7 @@ class Mafia {
             // speculate that the child is a maximal leaf when even its best
             // tail item is expected to drop below the threshold if it were
             // independent of the added item: sup(H+t_q) * sup(H+t) / sup(H)
-            if (spec_ && has_tail &&
-                (double)c.sup * (double)suffix_max_[q + 1] / (double)std::max(1, d.sup) < db_.thr_abs)
+            if (spec_ && has_tail && (double)c.sup * (double)smax[q + 1] / (double)std::max(1, d.sup) < db_.thr_abs)
                 c.kind = MNode::SPEC;
             if (c.kind == MNode::SPEC && compl_) {
                 if (!ordered) {
@@ -994,19 +1037,43 @@ class Mafia {
                 // no candidates left: the child is a leaf
                 const bool sub = d.leaf_sub >= 0 ? d.leaf_sub == 1 || index_.subsumed_from(c.head, d.leaf_n)
                                                  : index_.subsumed(c.head);
-                if (sub) { st_.leaf_subsumed++; continue; }
+                if (sub) {
+                    ++*subsumed;
+                    continue;
+                }
                 c.kind = MNode::LEAF;
             }
-            stack_.push_back(std::move(c));
+            m++;
         }
+        return m;
+    }
+
+    // At least n nodes in kids_, spent ones refilled from the recycled nodes
+    // (their vectors keep their storage).
+    void arena(size_t n) {
+        if (kids_.size() < n) kids_.resize(n);
+        for (size_t i = 0; i < n && !free_.empty(); i++)
+            if (kids_[i].head.capacity() == 0) {
+                kids_[i] = std::move(free_.back());
+                free_.pop_back();
+            }
+    }
+
+    void push_children(const Decision &d) {
+        arena(d.order.size());
+        int64_t sub = 0;
+        const size_t m = build_children(d, kids_.data(), &sub);
+        st_.leaf_subsumed += sub;
+        for (size_t i = 0; i < m; i++) stack_.push_back(std::move(kids_[i]));
     }
 
     int process_wave(Flight &f, std::vector<Mfi> &out) {
         settle_undecided();
         std::vector<MNode> &wave_ = f.wave;
         WaveBatch &wb_ = f.wb;
-        std::vector<Decision> decisions;
-        decisions.reserve(wave_.size());
+        // this wave's decisions: dec_[0 .. n_dec_), slots reused across waves
+        // (their vectors keep their storage)
+        n_dec_ = 0;
         // Subsumption checks of the EXPAND rows, in parallel against the MFIs
         // found so far.  An EXPAND row checks head + every frequent tail item
         // (the leaf head after PEP when no other item stays frequent, else
@@ -1039,12 +1106,19 @@ class Mafia {
             const int32_t *res = spec ? spec_tres_[r].data() : wb_.result((int)r);
             const int nc = spec ? (int)spec_tres_[r].size() : wb_.ncols((int)r);
             Decision &d = pre_dec_[r];
-            d = Decision();
-            d.head = std::move(n.head);   // the row's node is spent after this
+            d.head.assign(n.head.begin(), n.head.end());
             d.sup = n.sup;
             d.depth = n.depth;
-            d.known = std::move(n.known);
-            if (d.known.empty()) d.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
+            if (n.known.empty()) d.known.assign((size_t)(db_.n_surv + 63) / 64, 0ull);
+            else d.known.assign(n.known.begin(), n.known.end());
+            d.tail.clear();
+            d.tail_sup.clear();
+            d.order.clear();
+            d.item_records.clear();
+            d.lookahead = d.pending = false;
+            d.record = -1;
+            d.leaf_sub = -1;
+            d.leaf_n = 0;
             int kept = 0, pep = 0;
             for (int j = 0; j < nc; j++) {
                 const int32_t t = n.tail[j], sv = res[j];
@@ -1072,9 +1146,14 @@ class Mafia {
         });
         auto expands = [&](const MNode &n) { return n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_); };
         auto tail_n = [&](int r) { return wave_[r].kind == MNode::SPEC ? (int)spec_tres_[r].size() : wb_.ncols(r); };
-        auto subsumed_row = [&](int r, const std::vector<int32_t> &x) {
-            return pre_sub_[r] || index_.subsumed_from(x, n_start);
-        };
+        // Rows that expand into a leaf or a decision are checked against the
+        // MFIs found before the wave (pre_sub_) and against those this wave
+        // emits: the first pass below emits and notes these rows (a leaf gets
+        // its place among the pending nodes), the checks then run on the host
+        // pool against every MFI the wave emitted, and a last pass in row
+        // order settles them.  (Pruning by any found MFI is sound, so the
+        // checks need not see only the MFIs of earlier rows.)
+        chk_.clear();
         for (int r = 0; r < (int)wave_.size(); r++) {
             MNode &n = wave_[r];
             const int32_t *res = wb_.result(r);
@@ -1108,26 +1187,13 @@ class Mafia {
                     continue;
                 }
                 if (d.tail.empty()) {
-                    if (subsumed_row(r, d.head)) {
-                        st_.leaf_subsumed++;
-                    } else {
-                        MNode leaf = take_node();
-                        leaf.kind = MNode::LEAF;
-                        leaf.head = std::move(d.head);
-                        leaf.sup = d.sup;
-                        leaf.known = std::move(d.known);
-                        pending_.push_back(std::move(leaf));
-                    }
+                    // a leaf unless a found MFI holds it: its place among the
+                    // pending nodes, settled after the checks
+                    chk_.push_back({r, (int64_t)pending_.size(), false});
+                    pending_.push_back(take_node());
                     continue;
                 }
-                std::vector<int32_t> &hut = hut_scratch_;
-                hut.assign(d.head.begin(), d.head.end());
-                hut.insert(hut.end(), d.tail.begin(), d.tail.end());
-                if (subsumed_row(r, hut)) {
-                    st_.hutmfi_pruned++;
-                    continue;
-                }
-                decisions.push_back(std::move(d));
+                chk_.push_back({r, -1, false});   // a decision unless a found MFI holds head + tail
             } else if (n.kind == MNode::HUT) {
                 // explicit: res = {sup(HUT), counts of the complement};
                 // complement row: res = {sup(HUT), frequent extensions}
@@ -1178,16 +1244,62 @@ class Mafia {
                 }
             }
         }
-        // policy: order tails, choose lookaheads (batched over the wave)
+        pool_.run((int64_t)chk_.size(), [&](int64_t i) {
+            Check &c = chk_[i];
+            const Decision &d = pre_dec_[c.row];
+            if (pre_sub_[c.row]) {
+                c.sub = true;
+            } else if (c.pend >= 0) {
+                c.sub = index_.subsumed_from(d.head, n_start);
+            } else {
+                thread_local std::vector<int32_t> hut;
+                hut.assign(d.head.begin(), d.head.end());
+                hut.insert(hut.end(), d.tail.begin(), d.tail.end());
+                c.sub = index_.subsumed_from(hut, n_start);
+            }
+        });
+        pend_drop_.assign(pending_.size(), 0);
+        for (const Check &c : chk_) {
+            Decision &d = pre_dec_[c.row];
+            if (c.pend >= 0) {
+                if (c.sub) {
+                    st_.leaf_subsumed++;
+                    pend_drop_[(size_t)c.pend] = 1;
+                } else {
+                    MNode &leaf = pending_[(size_t)c.pend];
+                    leaf.reset();
+                    leaf.kind = MNode::LEAF;
+                    leaf.head.swap(d.head);
+                    leaf.sup = d.sup;
+                    leaf.known.swap(d.known);
+                }
+            } else if (c.sub) {
+                st_.hutmfi_pruned++;
+            } else {
+                if (n_dec_ == dec_.size()) dec_.emplace_back();
+                std::swap(dec_[n_dec_++], d);
+            }
+        }
+        // policy: order tails, choose lookaheads (batched over the wave; a
+        // policy that decides each node alone runs on the host pool)
         std::vector<Decision *> ptrs;
-        for (auto &d : decisions) ptrs.push_back(&d);
+        for (size_t q = 0; q < n_dec_; q++) ptrs.push_back(&dec_[q]);
         if (!ptrs.empty()) {
             const double t0 = now();
-            pol_.decide(ptrs, db_);
+            if (pol_.decides_alone()) {
+                pool_.run((int64_t)ptrs.size(), [&](int64_t i) {
+                    thread_local std::vector<Decision *> one(1);
+                    one[0] = ptrs[i];
+                    pol_.decide(one, db_);
+                });
+            } else {
+                pol_.decide(ptrs, db_);
+            }
             st_.learn_seconds += now() - t0;
         }
         // push in reverse so the first decision's work is on top
-        for (auto &n : pending_) stack_.push_back(std::move(n));
+        for (size_t i = 0; i < pending_.size(); i++)
+            if (!pend_drop_[i]) stack_.push_back(std::move(pending_[i]));
         pending_.clear();
         {
             // pending decisions too: their leaf check then needs no pass of
@@ -1195,15 +1307,44 @@ class Mafia {
             // found in between; a lookahead taken after all leaves it unused)
             std::vector<Decision *> leaves;
             for (auto &d : hut_children_) leaves.push_back(&d);
-            for (auto &d : decisions) leaves.push_back(&d);
+            for (size_t q = 0; q < n_dec_; q++) leaves.push_back(&dec_[q]);
             check_leaves(leaves);
         }
         for (auto &d : hut_children_) push_children(d);
         hut_children_.clear();
-        for (size_t q = decisions.size(); q-- > 0;) {
-            Decision &d = decisions[q];
-            if (d.pending) undecided_.push_back(std::move(d));   // settled after the next wave
-            else apply(d);
+        // the decisions' work, stacked in reverse order (the first decision's
+        // on top): their children are built on the host pool into kids_,
+        // then stacked here in that order
+        work_.clear();
+        size_t total = 0;
+        for (size_t q = n_dec_; q-- > 0;) {
+            Decision &d = dec_[q];
+            if (d.pending) {
+                undecided_.push_back(std::move(d));   // settled after the next wave
+                continue;
+            }
+            const bool hut = d.lookahead && d.tail.size() >= 2;
+            work_.push_back({q, hut, total, 0, 0});
+            if (!hut) total += d.order.size();
+        }
+        arena(total);
+        kid_work_.clear();
+        for (size_t i = 0; i < work_.size(); i++)
+            if (!work_[i].hut) kid_work_.push_back(i);
+        pool_.run((int64_t)kid_work_.size(), [&](int64_t i) {
+            Work &w = work_[kid_work_[i]];
+            w.n = build_children(dec_[w.dec], kids_.data() + w.off, &w.subsumed);
+        });
+        for (const Work &w : work_) {
+            Decision &d = dec_[w.dec];
+            if (w.hut) {
+                apply(d);   // its head-union-tail row
+                continue;
+            }
+            if (d.tail.size() >= 2) st_.hut_skipped++;
+            pol_.observe(d.record, -1);
+            st_.leaf_subsumed += w.subsumed;
+            for (size_t i = 0; i < w.n; i++) stack_.push_back(std::move(kids_[w.off + i]));
         }
         return TFIM_OK;
     }
@@ -1259,9 +1400,32 @@ class Mafia {
     KeySet64 seen64_;
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;
-    static constexpr int kFlights = 3;
+    std::vector<Decision> dec_;   // decisions of the wave being processed (first n_dec_)
+    size_t n_dec_ = 0;
+    // a decision's stacking work: its HUT row, or its children (n kept of
+    // those built at kids_[off ..), `subsumed` leaves dropped)
+    struct Work {
+        size_t dec;
+        bool hut;
+        size_t off, n;
+        int64_t subsumed;
+    };
+    std::vector<Work> work_;
+    // a wave row that expands, to check against the found MFIs: a leaf (its
+    // place `pend` among the pending nodes) or a decision (pend < 0)
+    struct Check {
+        int row;
+        int64_t pend;
+        bool sub;
+    };
+    std::vector<Check> chk_;
+    std::vector<uint8_t> pend_drop_;   // pending nodes dropped (leaves inside a found MFI)
+    std::vector<size_t> kid_work_;
+    std::vector<MNode> kids_;   // children under construction
+    static constexpr int kFlights = 4;   // at most; flights_ in use
     static constexpr size_t kMaxFree = 1 << 15;
     Flight fl_[kFlights];
+    int flights_ = 4;
     AsyncWorker helper_;
     int32_t cap_ = 256;
     int64_t budget_ = INT64_MAX;
@@ -1269,7 +1433,6 @@ class Mafia {
     bool ok_(int32_t s) const { return s >= min_count_; }
     // spent nodes whose vectors are reused for new ones
     std::vector<MNode> free_;
-    std::vector<int32_t> hut_scratch_, suffix_max_;
     MNode take_node() {
         if (free_.empty()) return MNode();
         MNode n = std::move(free_.back());

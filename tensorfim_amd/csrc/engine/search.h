@@ -29,6 +29,7 @@ struct SearchOptions {
     int32_t wave_cap = 0;         // rows per wave; 0 = automatic
     int32_t workers = 1;          // plain engine: host workers (one stream each)
     int32_t host_threads = 0;     // mafia: threads for the subsumption checks (0 = up to 8)
+    int32_t flights = 4;          // mafia: waves in flight (2 to 4)
     bool learned = false;         // mafia: use the MLP ranker/gate
     std::string gate = "indep";   // mafia plain lookahead rule: indep | always
     std::string compl_rows = "auto";   // mafia: HUT/LEAF rows as device-reduced complement rows: auto | on | off

@@ -119,7 +119,7 @@ struct tfim_ctx {
         };
         std::vector<Graph> graphs;
     };
-    static constexpr int kWaveSlots = 3;
+    static constexpr int kWaveSlots = 4;
     Slot slots[kWaveSlots];
     int64_t next_ticket = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
