@@ -445,17 +445,39 @@ void launch_count_wave(const uint64_t *bm, int64_t pitch_words, const WaveDesc &
 // ---------------------------------------------------------------------------
 // LDS-staged fused wave kernel.  Workgroup g (1024 threads = 16 waves) handles
 // word chunks g, g+G, ...; per chunk it copies the chunk of every slot (item
-// row) the wave uses into LDS with 128-bit loads, then its waves take the
-// tiles round-robin.  Per tile the prefix and column slot numbers are loaded
-// lane-parallel once (prefetched during the previous tile) and broadcast with
-// readlane, so the column loop issues only LDS reads and VALU work: prefix
-// AND into registers, 8 (then 4) columns at a time without branches, a DPP reduction
-// over each 16-lane row, and one LDS atomic per (row, column).  Partial
-// counts leave the workgroup once, as one global atomic per non-zero output.
-// Global traffic per wave is one pass over the slots' rows whatever the
-// number of rows.
+// row) the wave uses into LDS with 128-bit loads, four in flight per thread,
+// then its waves take tiles from a workgroup counter in LDS (the host orders
+// the tiles by falling cost, so the waves reach the end of a chunk together).
+// A tile's record is fetched two tiles ahead and its prefix and column slot
+// numbers one tile ahead, lane-parallel, then broadcast with readlane, so the
+// column loop issues only LDS reads and VALU work: prefix AND into registers,
+// then 8 (then 4) columns at a time, each column's words read while the one
+// before it is counted, a DPP reduction over each 16-lane row, and one LDS
+// atomic per (row, column).  Partial counts leave the workgroup once, as one
+// global atomic per non-zero output.  Global traffic per wave is one pass over
+// the slots' rows whatever the number of rows.
 constexpr int kLdsThreads = 1024;
 constexpr int kLdsWaves = kLdsThreads / 64;
+
+// LDS beyond the slot chunks: the partial counts (n_out ints, rounded up to
+// 16 bytes), the slots' bitmap rows and the tile counter.
+__host__ __device__ constexpr int lds_acc_ints(int n_out) { return (n_out + 3) / 4 * 4; }
+constexpr int kLdsTailBytes = kLdsMaxSlots * 4 + 16;
+
+template <int RW>
+__device__ __forceinline__ void lds_col_load(uint4 (&x)[RW], const uint4 *lds, int slot, int lane) {
+    const uint4 *row = lds + slot * (64 * RW);
+#pragma unroll
+    for (int q = 0; q < RW; q++) x[q] = row[q * 64 + lane];
+}
+
+template <int RW>
+__device__ __forceinline__ uint32_t lds_col_count(const uint4 (&m)[RW], const uint4 (&x)[RW]) {
+    uint32_t a = 0;
+#pragma unroll
+    for (int q = 0; q < RW; q++) a = popc_and4_acc(m[q], x[q], a);
+    return a;
+}
 
 template <int RW>
 __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__restrict__ bm, int64_t pitch4,
@@ -463,77 +485,105 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
     extern __shared__ uint4 lds[];
     constexpr int C4 = 64 * RW;   // uint4 per slot per chunk
     int *acc = reinterpret_cast<int *>(lds + (size_t)d.n_slots * C4);
+    int *srow = acc + lds_acc_ints(d.n_out);
+    int *counter = srow + kLdsMaxSlots;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int n_tiles = d.n_tiles, n_stage = d.n_slots * C4;
     for (int i = tid; i < d.n_out; i += kLdsThreads) acc[i] = 0;
+    for (int i = tid; i < d.n_slots; i += kLdsThreads) srow[i] = d.slot_rows[i];
+    // the next tile of this chunk for the calling wave (wave-uniform)
+    auto grab = [&]() {
+        int g = 0;
+        if (lane == 0) g = atomicAdd(counter, 1);
+        return __builtin_amdgcn_readfirstlane(g);
+    };
+    auto tile_at = [&](int t) {
+        LdsTile r = {0, 0, 0, 0};
+        if (t < n_tiles) r = d.tiles[t];
+        r.ib = __builtin_amdgcn_readfirstlane(r.ib);
+        r.k = __builtin_amdgcn_readfirstlane(r.k);
+        r.cb = __builtin_amdgcn_readfirstlane(r.cb);
+        r.nc = __builtin_amdgcn_readfirstlane(r.nc);
+        return r;
+    };
     for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
         __syncthreads();
+        if (tid == 0) *counter = kLdsWaves;   // tiles 0 .. 15 go to waves 0 .. 15
         const int64_t base = c * C4;
-        for (int i = tid; i < d.n_slots * C4; i += kLdsThreads) {
-            const int slot = i / C4, off = i % C4;
-            const int64_t g = base + off;
-            lds[i] = g < pitch4 ? bm[(int64_t)d.slot_rows[slot] * pitch4 + g] : make_uint4(0, 0, 0, 0);
+        for (int i0 = tid; i0 < n_stage; i0 += 4 * kLdsThreads) {
+            uint4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * kLdsThreads;
+                v[u] = make_uint4(0, 0, 0, 0);
+                if (i < n_stage) {
+                    const int64_t g = base + i % C4;
+                    if (g < pitch4) v[u] = bm[(int64_t)srow[i / C4] * pitch4 + g];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * kLdsThreads;
+                if (i < n_stage) lds[i] = v[u];
+            }
         }
         __syncthreads();
-        // tiles round-robin over the workgroup's waves; the slot numbers of
-        // the next tile are loaded while this one is counted
+        // t: tile in work (record tl, slots pslot / cslot); t1: the next
+        // (record n1, slots loaded during this tile); t2: the one after
+        // (record loaded during this tile)
         int t = wave;
-        LdsTile tl = {0, 0, 0, 0};
+        LdsTile tl = tile_at(t);
         int pslot = 0, cslot = 0;
-        if (t < d.n_tiles) {
-            tl = d.tiles[t];
+        if (t < n_tiles) {
             pslot = lane < tl.k ? d.item_slots[tl.ib + lane] : 0;
             cslot = lane < tl.nc ? d.col_slots[tl.cb + lane] : 0;
         }
-        for (; t < d.n_tiles; t += kLdsWaves) {
-            const int tn = t + kLdsWaves;
-            LdsTile nx = {0, 0, 0, 0};
+        int t1 = grab();
+        LdsTile n1 = tile_at(t1);
+        while (t < n_tiles) {
+            const int t2 = t1 < n_tiles ? grab() : n_tiles;
+            const LdsTile n2 = tile_at(t2);
             int npslot = 0, ncslot = 0;
-            if (tn < d.n_tiles) {
-                nx = d.tiles[tn];
-                npslot = lane < nx.k ? d.item_slots[nx.ib + lane] : 0;
-                ncslot = lane < nx.nc ? d.col_slots[nx.cb + lane] : 0;
+            if (t1 < n_tiles) {
+                npslot = lane < n1.k ? d.item_slots[n1.ib + lane] : 0;
+                ncslot = lane < n1.nc ? d.col_slots[n1.cb + lane] : 0;
             }
             const int k = tl.k, nc = tl.nc;   // nc <= 64 (tile_cols)
             uint4 m[RW];
-            {
-                const uint4 *row = lds + __builtin_amdgcn_readlane(pslot, 0) * C4;
-#pragma unroll
-                for (int q = 0; q < RW; q++) m[q] = row[q * 64 + lane];
-            }
+            lds_col_load<RW>(m, lds, __builtin_amdgcn_readlane(pslot, 0), lane);
             // prefix rows two at a time, so that both rows' reads are in
             // flight together (the two ANDs fold into one 3-input op)
             int i = 1;
             for (; i + 1 < k && i + 1 < 64; i += 2) {
-                const uint4 *r0 = lds + __builtin_amdgcn_readlane(pslot, i) * C4;
-                const uint4 *r1 = lds + __builtin_amdgcn_readlane(pslot, i + 1) * C4;
                 uint4 a0[RW], a1[RW];
-#pragma unroll
-                for (int q = 0; q < RW; q++) {
-                    a0[q] = r0[q * 64 + lane];
-                    a1[q] = r1[q * 64 + lane];
-                }
+                lds_col_load<RW>(a0, lds, __builtin_amdgcn_readlane(pslot, i), lane);
+                lds_col_load<RW>(a1, lds, __builtin_amdgcn_readlane(pslot, i + 1), lane);
 #pragma unroll
                 for (int q = 0; q < RW; q++) m[q] = and4(m[q], and4(a0[q], a1[q]));
             }
             for (; i < k; i++) {
                 const int sl = i < 64 ? __builtin_amdgcn_readlane(pslot, i) : d.item_slots[tl.ib + i];
-                const uint4 *row = lds + sl * C4;
+                uint4 a0[RW];
+                lds_col_load<RW>(a0, lds, __builtin_amdgcn_readfirstlane(sl), lane);
 #pragma unroll
-                for (int q = 0; q < RW; q++) m[q] = and4(m[q], row[q * 64 + lane]);
+                for (int q = 0; q < RW; q++) m[q] = and4(m[q], a0[q]);
             }
             // full groups of eight columns, then groups of four for the rest
-            // (the last one stops at the tile's end)
+            // (the last one stops at the tile's end); column j + 1's words
+            // are read while column j is counted
+            uint4 x[RW];
+            if (nc > 0) lds_col_load<RW>(x, lds, __builtin_amdgcn_readlane(cslot, 0), lane);
             int j = 0;
             for (; j + 8 <= nc; j += 8) {
                 uint32_t v[8];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
-                    const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, j + q) * C4;
-                    uint32_t a = 0;
+                    uint4 y[RW];   // the next column (the last one again at the tile's end)
+                    lds_col_load<RW>(y, lds, __builtin_amdgcn_readlane(cslot, min(j + q + 1, nc - 1)), lane);
+                    v[q] = lds_col_count<RW>(m, x);
 #pragma unroll
-                    for (int qq = 0; qq < RW; qq++) a = popc_and4_acc(m[qq], row[qq * 64 + lane], a);
-                    v[q] = a;
+                    for (int qq = 0; qq < RW; qq++) x[qq] = y[qq];
                 }
                 const uint32_t r = row_reduce8_packed(v);
                 if ((lane & 3) == 0) {
@@ -548,11 +598,11 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     if (j + q >= nc) break;   // wave-uniform: columns past the tile's end are skipped
-                    const uint4 *row = lds + __builtin_amdgcn_readlane(cslot, j + q) * C4;
-                    uint32_t a = 0;
+                    uint4 y[RW];   // the next column (the last one again at the tile's end)
+                    lds_col_load<RW>(y, lds, __builtin_amdgcn_readlane(cslot, min(j + q + 1, nc - 1)), lane);
+                    v[q] = lds_col_count<RW>(m, x);
 #pragma unroll
-                    for (int qq = 0; qq < RW; qq++) a = popc_and4_acc(m[qq], row[qq * 64 + lane], a);
-                    v[q] = a;
+                    for (int qq = 0; qq < RW; qq++) x[qq] = y[qq];
                 }
                 const uint32_t r = row_reduce4_packed(v);
                 if ((lane & 7) == 0) {
@@ -562,9 +612,12 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                     if (c0 + 1 < nc && hi) atomicAdd(&acc[tl.cb + c0 + 1], hi);
                 }
             }
-            tl = nx;
+            t = t1;
+            tl = n1;
             pslot = npslot;
             cslot = ncslot;
+            t1 = t2;
+            n1 = n2;
         }
     }
     __syncthreads();
@@ -576,8 +629,8 @@ bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsD
                          KernelLaunch &L) {
     if (d.n_slots < 1 || d.n_slots > kLdsMaxSlots || d.n_out > kLdsMaxOut) return false;
     const int64_t pitch4 = pitch_words / 2;
-    const size_t acc_bytes = ((size_t)d.n_out * 4 + 15) / 16 * 16;
-    auto bytes = [&](int rw) { return (size_t)d.n_slots * 64 * rw * 16 + acc_bytes; };
+    const size_t tail_bytes = (size_t)lds_acc_ints(d.n_out) * 4 + kLdsTailBytes;
+    auto bytes = [&](int rw) { return (size_t)d.n_slots * 64 * rw * 16 + tail_bytes; };
     // Narrow: two workgroups per CU (<= 80 KB each), the widest chunk that
     // fits.  Wide: the widest chunk (up to 4 x 128 bits per lane) that fits
     // one workgroup per CU, so each lane counts more bits per column between

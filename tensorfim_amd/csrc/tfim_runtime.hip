@@ -1201,6 +1201,13 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             lt[i].cb = coff[w.row] + w.col_begin;
             lt[i].nc = w.col_end - w.col_begin;
         }
+        // by falling cost (a column reads and counts its words, a prefix
+        // item only reads and ANDs them), so that the waves of a workgroup,
+        // taking tiles in this order, finish a chunk together
+        std::sort(lt, lt + tiles.size(), [](const LdsTile &a, const LdsTile &b) {
+            const int ca = 2 * a.nc + a.k, cb = 2 * b.nc + b.k;
+            return ca != cb ? ca > cb : a.cb < b.cb;
+        });
     } else {
         memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
     }
