@@ -836,12 +836,17 @@ class Mafia {
         f.st = SearchStats();
     }
 
+    // Columns a row lists (what it costs of the wave's output budget): the
+    // complement rows skip the survivors known infrequent with the head.
     int64_t row_outputs(const MNode &n) const {
+        int64_t known = 0;
+        if (n.kind != MNode::EXPAND && n.kind != MNode::REF)
+            for (uint64_t w : n.known) known += __builtin_popcountll(w);
         switch (n.kind) {
             case MNode::EXPAND: return (int64_t)n.tail.size();
-            case MNode::HUT: return 1 + db_.n_surv - (int64_t)(n.head.size() + n.tail.size());
+            case MNode::HUT: return 1 + db_.n_surv - (int64_t)(n.head.size() + n.tail.size()) - known;
             case MNode::LEAF:
-            case MNode::SPEC: return db_.n_surv - (int64_t)n.head.size();
+            case MNode::SPEC: return db_.n_surv - (int64_t)n.head.size() - known;
             default: return db_.n_surv;
         }
     }
@@ -1018,10 +1023,19 @@ class Mafia {
             c.known.assign(d.known.begin(), d.known.end());
             if (!d.item_records.empty()) c.item_record = d.item_records[d.order[q]];
             const bool has_tail = q + 1 < k;
-            // speculate that the child is a maximal leaf when even its best
-            // tail item is expected to drop below the threshold if it were
-            // independent of the added item: sup(H+t_q) * sup(H+t) / sup(H)
-            if (spec_ && has_tail && (double)c.sup * (double)smax[q + 1] / (double)std::max(1, d.sup) < db_.thr_abs)
+            // Narrow databases: every child with a tail is a SPEC row.  It
+            // lists every survivor outside head and known, so its tail's
+            // counts are read out of it when it lives on, its maximality is
+            // settled at once when the tail dies, and its infrequent columns
+            // join the known set of its whole subtree, which shortens every
+            // maximality row below it (on the dense replicated workload this
+            // halves the columns counted against listing tails).  Wide
+            // databases (complement rows, no per-column counts): speculate
+            // that the child is a maximal leaf when even its best tail item
+            // is expected to drop below the threshold if it were independent
+            // of the added item: sup(H+t_q) * sup(H+t) / sup(H) < thr.
+            if (spec_ && has_tail &&
+                (!compl_ || (double)c.sup * (double)smax[q + 1] / (double)std::max(1, d.sup) < db_.thr_abs))
                 c.kind = MNode::SPEC;
             if (c.kind == MNode::SPEC && compl_) {
                 if (!ordered) {
