@@ -132,7 +132,10 @@ int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item
 
 /* The same count split in two, so the host can prepare or process one wave
  * while the GPU counts another: submit packs and launches the wave
- * asynchronously on the ctx stream and returns a ticket; wait blocks until
+ * asynchronously on the ctx stream (every step a kernel: the descriptor is
+ * read from pinned host memory by the wave's first kernel and the counts are
+ * written to pinned host memory by its last) and returns a ticket; wait
+ * blocks until
  * that wave's counts are done and copies them into `out` (layout as in
  * tfim_count_nodes).  At most four waves may be in flight per ctx; tickets
  * are waited in submission order.  A ticket of -1 (nothing to count) waits
@@ -142,6 +145,23 @@ int tfim_count_nodes_submit(tfim_ctx *ctx, const int32_t *item_ids, const int32_
                             int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
                             int32_t path, int64_t *ticket);
 int tfim_count_nodes_wait(tfim_ctx *ctx, int64_t ticket, int32_t *out);
+/* Batch count with rows given as bit masks, for databases of at most 64
+ * dense ids (n_items <= 64 in tfim_load_bitmaps): row r counts the prefix of
+ * the ids set in prefix_mask[r] (at least one) against the columns set in
+ * col_mask[r] (survivor ids < n_cols) in ascending id order; the outputs are
+ * CSR by popcount(col_mask[r]) (row r's first output follows every earlier
+ * row's columns).  The same counts as tfim_count_nodes[_submit] with item and
+ * column lists, but each row is two words: the host neither builds nor
+ * checks id lists, and the kernel finds the ids with scalar bit scans.
+ * Waited on with tfim_count_nodes_wait (the tickets are shared with
+ * tfim_count_nodes_submit); path as there.  Replaces batchCountBMMA
+ * (kernel_bmma.cu:772) for narrow databases; TFIM_E_RANGE if more than 64
+ * ids are loaded. */
+int tfim_count_masks_submit(tfim_ctx *ctx, const uint64_t *prefix_mask, const uint64_t *col_mask,
+                            int32_t n_rows, int32_t path, int64_t *ticket);
+/* Blocking form of tfim_count_masks_submit + tfim_count_nodes_wait. */
+int tfim_count_masks(tfim_ctx *ctx, const uint64_t *prefix_mask, const uint64_t *col_mask, int32_t n_rows,
+                     int32_t *out, int32_t path);
 /* Complement rows, for maximality checks over many survivors.  As
  * tfim_count_nodes_submit with col_off required (col_ids may be NULL when
  * col_off[n_rows] == 0), plus n_compl complement rows: compl_rows lists their
@@ -202,10 +222,12 @@ int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *i
  *                  occupancy rule scaled by a tuner; 1 no split, one word
  *                  range per tile (the reference's BMMA_SPLITK_OFF ablation,
  *                  kernel_bmma.cu:351-368)
- *   "graphs"       1 (default): each wave is replayed from an instantiated
- *                  HIP graph (descriptor copy, output fill, count kernel,
- *                  output copy) whose node parameters are updated per wave;
- *                  0: the same operations issued one by one
+ *   "graphs"       1: each wave's kernels are replayed from an instantiated
+ *                  HIP graph (descriptor stage, count kernels, output drain)
+ *                  whose node parameters are updated per wave; 0 (default):
+ *                  the same kernels launched one by one (on MI355X cheaper
+ *                  on the host, 12 against 22 us per wave, and without the
+ *                  graph's gaps between nodes on the device)
  * Unknown keys and out-of-range values return TFIM_E_INVALID. */
 int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
 

@@ -53,6 +53,8 @@ _SIGS = {
     "tfim_count_nodes_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                             ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]),
+    "tfim_count_masks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.c_int32]),
     "tfim_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int64]),
     "tfim_wave_stats": (ctypes.c_int, [ctypes.c_void_p, _p(ctypes.c_int64), _p(ctypes.c_int64),
                                         _p(ctypes.c_int64), _p(ctypes.c_double)]),
@@ -287,6 +289,25 @@ class Context:
             else:
                 res.append(out[col_off[i]:col_off[i + 1]])
         return res
+
+    def count_masks(self, prefixes, columns, path="auto"):
+        """Batched path with rows as bit masks (at most 64 dense ids loaded):
+        prefixes and columns are lists of id lists; row i's counts come back
+        for its columns in ascending id order."""
+        n = len(prefixes)
+        pm = np.zeros(max(1, n), dtype=np.uint64)
+        cm = np.zeros(max(1, n), dtype=np.uint64)
+        for i in range(n):
+            for x in prefixes[i]:
+                pm[i] |= np.uint64(1) << np.uint64(x)
+            for x in columns[i]:
+                cm[i] |= np.uint64(1) << np.uint64(x)
+        cnt = [bin(int(cm[i])).count("1") for i in range(n)]
+        off = np.zeros(n + 1, dtype=np.int64)
+        off[1:] = np.cumsum(cnt)
+        out = np.empty(max(1, int(off[-1])), dtype=np.int32)
+        self._check(self._lib.tfim_count_masks(self._h, _ptr(pm), _ptr(cm), n, _ptr(out), _PATHS[path]))
+        return [out[off[i]:off[i + 1]] for i in range(n)]
 
     def config(self, key, value):
         self._check(self._lib.tfim_config(self._h, key.encode(), int(value)))

@@ -56,6 +56,7 @@ int main(int argc, char **argv) {
         else if (a == "--gate") opt.gate = next();
         else if (a == "--compl") opt.compl_rows = next();
         else if (a == "--spec") opt.spec = next();
+        else if (a == "--rows") opt.rows = next();
         else if (a == "--workers") opt.workers = atoi(next().c_str());
         else if (a == "--host-threads") opt.host_threads = atoi(next().c_str());
         else if (a == "--flights") opt.flights = atoi(next().c_str());
@@ -110,6 +111,11 @@ int main(int argc, char **argv) {
     }
     if (opt.spec != "on" && opt.spec != "off") {
         fprintf(stderr, "--spec must be on or off\n");
+        tfim_close(ctx);
+        return 1;
+    }
+    if (opt.rows != "auto" && opt.rows != "lists") {
+        fprintf(stderr, "--rows must be auto or lists\n");
         tfim_close(ctx);
         return 1;
     }

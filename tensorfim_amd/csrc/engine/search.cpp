@@ -1,5 +1,7 @@
 #include "search.h"
 
+#include <sched.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -38,6 +40,37 @@ class WaveBatch {
         compl_of_.clear();
         excl_.clear();
         compl_cols_ = 0;
+        pm_.clear();
+        cm_.clear();
+        mask_off_.assign(1, 0);
+        ord_kind_.clear();
+        ord_at_.clear();
+        ord_.clear();
+        mask_ = false;
+    }
+    // A row as bit masks (databases of at most 64 dense ids; a wave holds
+    // rows of one form): prefix ids pm, columns cm.  The library returns the
+    // counts in ascending column order; result() gives them in the row's own
+    // order: `lead` (a column of cm) first and then the others ascending, or
+    // the order of the ids in `order` (n_order of them, exactly the ids of
+    // cm), or ascending when neither is given.
+    int add_mask(uint64_t pm, uint64_t cm, int lead = -1, const int32_t *order = nullptr, int n_order = 0) {
+        mask_ = true;
+        pm_.push_back(pm);
+        cm_.push_back(cm);
+        mask_off_.push_back(mask_off_.back() + __builtin_popcountll(cm));
+        if (order) {
+            ord_kind_.push_back(2);
+            ord_at_.push_back((int32_t)ord_.size());
+            ord_.insert(ord_.end(), order, order + n_order);
+        } else if (lead >= 0) {
+            ord_kind_.push_back(1);
+            ord_at_.push_back(lead);
+        } else {
+            ord_kind_.push_back(0);
+            ord_at_.push_back(0);
+        }
+        return rows() - 1;
     }
     int add(const int32_t *prefix, int k, const int32_t *cols, int nc) {
         item_ids_.insert(item_ids_.end(), prefix, prefix + k);
@@ -61,19 +94,20 @@ class WaveBatch {
         compl_cols_ += live;
         return rows() - 1;
     }
-    int rows() const { return (int)item_off_.size() - 1; }
-    int64_t cols_total() const { return (int64_t)col_ids_.size() + compl_cols_; }
+    int rows() const { return mask_ ? (int)pm_.size() : (int)item_off_.size() - 1; }
+    int64_t cols_total() const { return mask_ ? (int64_t)mask_off_.back() : (int64_t)col_ids_.size() + compl_cols_; }
     // Launches the wave asynchronously (tfim_count_nodes_submit, or the _ex
     // form when it holds complement rows); wait() makes its counts
     // available.  A wave with no columns submits nothing.
     int submit(tfim_ctx *ctx, SearchStats &st, int64_t n_bits, int32_t min_count, std::string &err) {
-        out_.resize(std::max<size_t>(1, col_ids_.size() + 2 * compl_rows_.size()));
+        out_.resize(std::max<size_t>(1, mask_ ? (size_t)mask_off_.back() : col_ids_.size() + 2 * compl_rows_.size()));
         st.nodes += rows();
         ticket_ = -1;
-        if (col_ids_.empty() && compl_rows_.empty()) return TFIM_OK;   // nothing to count
+        if (mask_ ? mask_off_.back() == 0 : col_ids_.empty() && compl_rows_.empty()) return TFIM_OK;   // nothing to count
         const double t0 = now();
         const int32_t one = 0;
-        const int rc = compl_rows_.empty()
+        const int rc = mask_ ? tfim_count_masks_submit(ctx, pm_.data(), cm_.data(), rows(), TFIM_PATH_AUTO, &ticket_)
+                       : compl_rows_.empty()
                            ? tfim_count_nodes_submit(ctx, item_ids_.data(), item_off_.data(), rows(), col_ids_.data(),
                                                      col_off_.data(), TFIM_PATH_AUTO, &ticket_)
                            : tfim_count_nodes_ex_submit(ctx, item_ids_.data(), item_off_.data(), rows(),
@@ -95,18 +129,51 @@ class WaveBatch {
         const int rc = tfim_count_nodes_wait(ctx, ticket_, out_.data());
         st.count_seconds += now() - t0;
         ticket_ = -1;
-        if (rc != TFIM_OK) err = std::string("tfim_count_nodes_wait: ") + tfim_last_error(ctx);
+        if (rc != TFIM_OK) {
+            err = std::string("tfim_count_nodes_wait: ") + tfim_last_error(ctx);
+            return rc;
+        }
+        if (mask_) order_masks();
         return rc;
     }
     // Counts of row r (CSR-aligned with its columns), or for a complement
     // row its two results.
     const int32_t *result(int r) const {
+        if (mask_) return ord_kind_[r] ? perm_.data() + perm_at_[r] : out_.data() + mask_off_[r];
         const int32_t j = compl_of_[r];
         return j < 0 ? out_.data() + col_off_[r] : out_.data() + col_ids_.size() + 2 * (size_t)j;
     }
-    int ncols(int r) const { return col_off_[r + 1] - col_off_[r]; }
+    int ncols(int r) const { return mask_ ? mask_off_[r + 1] - mask_off_[r] : col_off_[r + 1] - col_off_[r]; }
 
    private:
+    // The counts of mask rows with an order of their own, in that order (the
+    // position of column t among a row's ascending columns is the number of
+    // its columns below t).
+    void order_masks() {
+        perm_.clear();
+        perm_at_.assign(pm_.size(), 0);
+        for (size_t r = 0; r < pm_.size(); r++) {
+            if (!ord_kind_[r]) continue;
+            const uint64_t cm = cm_[r];
+            const int32_t *asc = out_.data() + mask_off_[r];
+            const int n = mask_off_[r + 1] - mask_off_[r];
+            perm_at_[r] = (int32_t)perm_.size();
+            auto pos = [&](int32_t t) { return __builtin_popcountll(cm & ((1ull << t) - 1)); };
+            if (ord_kind_[r] == 1) {
+                const int lp = pos(ord_at_[r]);
+                perm_.push_back(asc[lp]);
+                for (int j = 0; j < n; j++)
+                    if (j != lp) perm_.push_back(asc[j]);
+            } else {
+                const int32_t *o = ord_.data() + ord_at_[r];
+                for (int j = 0; j < n; j++) perm_.push_back(asc[pos(o[j])]);
+            }
+        }
+    }
+    bool mask_ = false;
+    std::vector<uint64_t> pm_, cm_;
+    std::vector<int32_t> mask_off_{0}, ord_kind_, ord_at_, ord_, perm_, perm_at_;
+
     std::vector<int32_t> item_ids_, item_off_{0}, col_ids_, col_off_{0}, out_, compl_rows_, compl_of_;
     std::vector<uint64_t> excl_;
     int64_t compl_cols_ = 0;
@@ -668,8 +735,19 @@ class Mafia {
    public:
     Mafia(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, Policy &pol, SearchStats &st)
         : ctx_(ctx), db_(db), opt_(opt), pol_(pol), st_(st), index_(db.n_surv),
-          pool_(opt.host_threads > 0 ? opt.host_threads
-                                     : (int)std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency()))) {}
+          pool_(opt.host_threads > 0 ? opt.host_threads : default_pool_threads()) {}
+
+    // The host pool's default size: the CPUs this process may run on, less
+    // four (the main thread, the packing helper and the runtime's own
+    // threads), at most 12 (measured on the dense replicated workload with a
+    // 16-CPU share: 12 threads 0.111-0.114 s, 8 threads 0.120-0.124 s, 15
+    // threads 0.117-0.123 s).
+    static int default_pool_threads() {
+        cpu_set_t set;
+        int n = (int)std::thread::hardware_concurrency();
+        if (sched_getaffinity(0, sizeof set, &set) == 0) n = CPU_COUNT(&set);
+        return std::max(1, std::min(12, n - 4));
+    }
 
     int run(std::vector<Mfi> &out, std::string &err) {
         const int32_t F = db_.n_surv;
@@ -680,6 +758,8 @@ class Mafia {
         // device when the survivors are too many for listing them per row
         // to be cheap (the same bound as the LDS kernel's)
         compl_ = opt_.compl_rows == "on" || (opt_.compl_rows == "auto" && !lds_shape(db_));
+        // rows as bit masks where every dense id fits one word
+        mask_rows_ = !compl_ && db_.d_count <= 64 && opt_.rows != "lists";
         min_count_ = min_count_of(db_);
         spec_ = opt_.spec != "off";
         excl_words_ = (F + 63) / 64;
@@ -916,6 +996,7 @@ class Mafia {
     int count_wave(Flight &f, std::string &err) {
         WaveBatch &wb_ = f.wb;
         wb_.clear();
+        if (mask_rows_) return count_wave_masks(f, err);
         std::vector<int32_t> &cols = f.cols;
         for (auto &n : f.wave) {
             switch (n.kind) {
@@ -952,6 +1033,48 @@ class Mafia {
                     cols.resize(db_.n_surv);
                     std::iota(cols.begin(), cols.end(), 0);
                     wb_.add(n.head.data(), (int)n.head.size(), cols.data(), (int)cols.size());
+                    f.st.ref_rows++;
+                    break;
+            }
+        }
+        return wb_.submit(ctx_, f.st, db_.n_bits(), min_count_, err);
+    }
+
+    // count_wave for databases of at most 64 dense ids: each row as two bit
+    // masks (its columns the same as the list form's: a maximality row's are
+    // the survivors outside its prefix and its known set, a HUT row's lead
+    // column is its first item).
+    int count_wave_masks(Flight &f, std::string &err) {
+        WaveBatch &wb_ = f.wb;
+        const uint64_t all = db_.n_surv >= 64 ? ~0ull : (1ull << db_.n_surv) - 1;
+        auto bits = [](const std::vector<int32_t> &v) {
+            uint64_t b = 0;
+            for (int32_t i : v) b |= 1ull << i;
+            return b;
+        };
+        for (auto &n : f.wave) {
+            const uint64_t pm = bits(n.head);
+            const uint64_t known = n.known.empty() ? 0ull : n.known[0];
+            switch (n.kind) {
+                case MNode::EXPAND:
+                    wb_.add_mask(pm, bits(n.tail), -1, n.tail.data(), (int)n.tail.size());
+                    f.st.expand_rows++;
+                    break;
+                case MNode::HUT: {
+                    const uint64_t hut = pm | bits(n.tail);
+                    const int32_t x0 = n.head.empty() ? n.tail[0] : n.head[0];
+                    wb_.add_mask(hut, (all & ~(hut | known)) | (1ull << x0), x0);
+                    f.st.hut_rows++;
+                    break;
+                }
+                case MNode::LEAF:
+                case MNode::SPEC:
+                    wb_.add_mask(pm, all & ~(pm | known));
+                    if (n.kind == MNode::LEAF) f.st.leaf_rows++;
+                    else f.st.spec_rows++;
+                    break;
+                case MNode::REF:
+                    wb_.add_mask(pm, all);
                     f.st.ref_rows++;
                     break;
             }
@@ -1457,6 +1580,7 @@ class Mafia {
     std::unique_ptr<WaveSizeTuner> sizer_;   // wave size (unless --wave-cap is given)
     double last_end_ = 0;
     bool compl_ = false;         // HUT, LEAF and SPEC rows are complement rows
+    bool mask_rows_ = false;     // rows counted as bit masks (at most 64 dense ids)
     bool spec_ = true;           // predicted maximal leaves are checked at once (SPEC rows)
     int32_t min_count_ = 0;
     int excl_words_ = 0;

@@ -28,12 +28,13 @@ struct SearchOptions {
     std::string engine = "mafia";
     int32_t wave_cap = 0;         // rows per wave; 0 = automatic
     int32_t workers = 1;          // plain engine: host workers (one stream each)
-    int32_t host_threads = 0;     // mafia: threads for the subsumption checks (0 = up to 8)
+    int32_t host_threads = 0;     // mafia: host pool threads (0 = the CPUs this process may use less four, at most 12)
     int32_t flights = 4;          // mafia: waves in flight (2 to 4)
     bool learned = false;         // mafia: use the MLP ranker/gate
     std::string gate = "indep";   // mafia plain lookahead rule: indep | always
     std::string compl_rows = "auto";   // mafia: HUT/LEAF rows as device-reduced complement rows: auto | on | off
     std::string spec = "on";      // mafia: check nodes predicted to be maximal leaves at once: on | off
+    std::string rows = "auto";    // waves of at most 64 dense ids as bit-mask rows (auto) or as id lists (lists)
     bool rank = false;            // mafia learned: reorder tails with the ranker head
     std::string model_path;       // mafia: load MLP weights (else trained online)
     std::string trace_path;       // mafia: write (features, outcome) records

@@ -691,6 +691,225 @@ bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLd
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged wave kernel for rows given as bit masks (at most 64 dense ids).
+// The scheme of k_count_wave_lds -- workgroup g stages word chunks g, g+G, ...
+// of every id into LDS (slot i = id i), its 16 waves take tiles from a
+// counter in LDS -- with a tile that is (prefix mask, column mask, first
+// output): its record is read one tile ahead with one vector load per lane
+// and broadcast to scalar registers, the ids come from scalar bit scans, and
+// every column group is straight-line code: groups of eight, then one group
+// of four or eight for the rest, padded with the tile's last column (its
+// surplus counts are not added).
+
+// Counts `n` (<= G) columns of a group against the prefix m: ids from the
+// column mask `cm` (consumed), padded with `last`; each column's words are
+// read while the one before it is counted.
+template <int RW, int G>
+__device__ __forceinline__ void mask_group(const uint4 (&m)[RW], const uint4 *lds, uint64_t &cm, int n, int last,
+                                           int lane, uint32_t (&v)[G]) {
+    int s[G];
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+        if (q < n) {
+            s[q] = __builtin_ctzll(cm);
+            cm &= cm - 1;
+        } else {
+            s[q] = last;
+        }
+    }
+    uint4 x[RW];
+    lds_col_load<RW>(x, lds, s[0], lane);
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+        uint4 y[RW];
+        if (q + 1 < G) lds_col_load<RW>(y, lds, s[q + 1], lane);
+        v[q] = lds_col_count<RW>(m, x);
+#pragma unroll
+        for (int qq = 0; qq < RW; qq++) x[qq] = y[qq];
+    }
+}
+
+template <int RW>
+__global__ __launch_bounds__(kLdsThreads) void k_count_wave_mask(const uint4 *__restrict__ bm, int64_t pitch4,
+                                                                 WaveMaskDesc d, int64_t n_chunks) {
+    extern __shared__ uint4 lds[];
+    constexpr int C4 = 64 * RW;   // uint4 per id per chunk
+    int *acc = reinterpret_cast<int *>(lds + (size_t)d.n_items * C4);
+    int *counter = acc + lds_acc_ints(d.n_out);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int n_tiles = d.n_tiles, n_stage = d.n_items * C4;
+    for (int i = tid; i < d.n_out; i += kLdsThreads) acc[i] = 0;
+    auto grab = [&]() {
+        int g = 0;
+        if (lane == 0) g = atomicAdd(counter, 1);
+        return __builtin_amdgcn_readfirstlane(g);
+    };
+    // a tile's record, read by every lane (one cache line) and broadcast
+    auto tile_at = [&](int t) {
+        uint4 a = make_uint4(0, 0, 0, 0), b = make_uint4(0, 0, 0, 0);
+        if (t < n_tiles) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(d.tiles + t);
+            a = p[0];
+            b = p[1];
+        }
+        // (readfirstlane returns a signed int: widen through uint32_t, so a
+        // set bit 31 of a low word does not spread into the high word)
+        auto u = [](uint32_t x) { return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(x); };
+        MaskTile r;
+        r.pm = u(a.x) | (u(a.y) << 32);
+        r.cm = u(a.z) | (u(a.w) << 32);
+        r.ob = __builtin_amdgcn_readfirstlane(b.x);
+        r.k = __builtin_amdgcn_readfirstlane(b.y);
+        r.nc = __builtin_amdgcn_readfirstlane(b.z);
+        r.pad = 0;
+        return r;
+    };
+    for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        __syncthreads();
+        if (tid == 0) *counter = kLdsWaves;   // tiles 0 .. 15 go to waves 0 .. 15
+        const int64_t base = c * C4;
+        for (int i0 = tid; i0 < n_stage; i0 += 4 * kLdsThreads) {
+            uint4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * kLdsThreads;
+                v[u] = make_uint4(0, 0, 0, 0);
+                if (i < n_stage) {
+                    const int64_t g = base + i % C4;
+                    if (g < pitch4) v[u] = bm[(int64_t)(i / C4) * pitch4 + g];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int i = i0 + u * kLdsThreads;
+                if (i < n_stage) lds[i] = v[u];
+            }
+        }
+        __syncthreads();
+        int t = wave;
+        MaskTile tl = tile_at(t);
+        while (t < n_tiles) {
+            const int tn = grab();
+            const MaskTile nx = tile_at(tn);   // in flight while this tile is counted
+            // prefix AND, two ids per step (both reads in flight, one 3-input AND)
+            uint64_t pm = tl.pm;
+            uint4 m[RW];
+            lds_col_load<RW>(m, lds, __builtin_ctzll(pm), lane);
+            pm &= pm - 1;
+            while (pm) {
+                const int a = __builtin_ctzll(pm);
+                pm &= pm - 1;
+                if (pm) {
+                    const int b = __builtin_ctzll(pm);
+                    pm &= pm - 1;
+                    uint4 a0[RW], a1[RW];
+                    lds_col_load<RW>(a0, lds, a, lane);
+                    lds_col_load<RW>(a1, lds, b, lane);
+#pragma unroll
+                    for (int q = 0; q < RW; q++) m[q] = and4(m[q], and4(a0[q], a1[q]));
+                } else {
+                    uint4 a0[RW];
+                    lds_col_load<RW>(a0, lds, a, lane);
+#pragma unroll
+                    for (int q = 0; q < RW; q++) m[q] = and4(m[q], a0[q]);
+                }
+            }
+            uint64_t cm = tl.cm;
+            const int nc = tl.nc, ob = tl.ob;
+            const int last = 63 - __builtin_clzll(cm);
+            int j = 0;
+            for (; j + 8 <= nc; j += 8) {
+                uint32_t v[8];
+                mask_group<RW, 8>(m, lds, cm, 8, last, lane, v);
+                const uint32_t r = row_reduce8_packed(v);
+                if ((lane & 3) == 0) {
+                    const int c0 = ob + j + 2 * ((lane >> 2) & 3);
+                    const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
+                    if (lo) atomicAdd(&acc[c0], lo);
+                    if (hi) atomicAdd(&acc[c0 + 1], hi);
+                }
+            }
+            const int rem = nc - j;   // 0 .. 7, wave-uniform
+            if (rem > 4) {
+                uint32_t v[8];
+                mask_group<RW, 8>(m, lds, cm, rem, last, lane, v);
+                const uint32_t r = row_reduce8_packed(v);
+                if ((lane & 3) == 0) {
+                    const int q0 = 2 * ((lane >> 2) & 3);
+                    const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
+                    if (q0 < rem && lo) atomicAdd(&acc[ob + j + q0], lo);
+                    if (q0 + 1 < rem && hi) atomicAdd(&acc[ob + j + q0 + 1], hi);
+                }
+            } else if (rem > 0) {
+                uint32_t v[4];
+                mask_group<RW, 4>(m, lds, cm, rem, last, lane, v);
+                const uint32_t r = row_reduce4_packed(v);
+                if ((lane & 7) == 0) {
+                    const int q0 = (lane & 8) ? 2 : 0;
+                    const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
+                    if (q0 < rem && lo) atomicAdd(&acc[ob + j + q0], lo);
+                    if (q0 + 1 < rem && hi) atomicAdd(&acc[ob + j + q0 + 1], hi);
+                }
+            }
+            t = tn;
+            tl = nx;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < d.n_out; i += kLdsThreads)
+        if (acc[i]) atomicAdd(&d.out[i], acc[i]);
+}
+
+bool plan_count_wave_mask(const uint64_t *bm, int64_t pitch_words, const WaveMaskDesc &d, int n_cu, bool wide,
+                          KernelLaunch &L) {
+    if (d.n_items < 1 || d.n_items > kMaskMaxItems || d.n_out < 1 || d.n_out > kLdsMaxOut || d.n_tiles < 1)
+        return false;
+    const int64_t pitch4 = pitch_words / 2;
+    const size_t tail_bytes = (size_t)lds_acc_ints(d.n_out) * 4 + 16;
+    auto bytes = [&](int rw) { return (size_t)d.n_items * 64 * rw * 16 + tail_bytes; };
+    constexpr size_t kHalf = 80 * 1024, kFull = 160 * 1024;
+    {
+        static std::mutex mu;
+        static uint64_t done = 0;   // bit per device id below 64
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return false;
+        std::lock_guard<std::mutex> g(mu);
+        if (dev >= 64 || !((done >> dev) & 1)) {
+            const void *fn[4] = {(const void *)k_count_wave_mask<1>, (const void *)k_count_wave_mask<2>,
+                                 (const void *)k_count_wave_mask<3>, (const void *)k_count_wave_mask<4>};
+            for (const void *f : fn)
+                if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFull) != hipSuccess)
+                    return false;
+            if (dev < 64) done |= 1ull << dev;
+        }
+    }
+    // wide: the widest chunk (up to 4 x 128 bits per lane per id) with one
+    // workgroup per CU; narrow: the widest with two per CU
+    const size_t cap = wide ? kFull : kHalf;
+    int rw = 4;
+    while (rw > 1 && bytes(rw) > cap) rw--;
+    if (bytes(rw) > cap) return false;
+    const size_t lds_bytes = bytes(rw);
+    const int64_t n_chunks = (pitch4 + 64 * rw - 1) / (64 * rw);
+    const int per_cu = lds_bytes <= kHalf ? 2 : 1;
+    const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(n_chunks, (int64_t)n_cu * per_cu));
+    L.func = rw == 4   ? (const void *)k_count_wave_mask<4>
+             : rw == 3 ? (const void *)k_count_wave_mask<3>
+             : rw == 2 ? (const void *)k_count_wave_mask<2>
+                       : (const void *)k_count_wave_mask<1>;
+    L.grid = dim3((unsigned)grid);
+    L.block = dim3(kLdsThreads);
+    L.shmem = (uint32_t)lds_bytes;
+    L.reset();
+    L.arg(reinterpret_cast<const uint4 *>(bm));
+    L.arg(pitch4);
+    L.arg(d);
+    L.arg(n_chunks);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 __global__ void k_zero_i32(int32_t *p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -705,6 +924,49 @@ void plan_zero_i32(int32_t *p, int64_t n, KernelLaunch &L) {
     L.reset();
     L.arg(p);
     L.arg(n);
+}
+
+// A wave's descriptor block from pinned host memory (mapped, uncached on the
+// device) into device memory, 16 bytes per thread: the first node of a wave,
+// so no copy call and no second stream are involved.
+__global__ void k_stage16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+void plan_stage16(const void *src, void *dst, int64_t bytes, KernelLaunch &L) {
+    const int64_t n16 = (bytes + 15) / 16;
+    L.func = (const void *)k_stage16;
+    L.grid = dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n16 + 255) / 256, 256)));
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.reset();
+    L.arg(reinterpret_cast<const uint4 *>(src));
+    L.arg(reinterpret_cast<uint4 *>(dst));
+    L.arg(n16);
+}
+
+// A wave's counts out to pinned host memory (mapped), and its device output
+// region back to zero for the slot's next wave: the last node of a wave.
+__global__ void k_drain_i32(int32_t *__restrict__ dev, int32_t *__restrict__ host, int64_t n_copy, int64_t n_zero) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_zero; i += (int64_t)gridDim.x * blockDim.x) {
+        if (i < n_copy) host[i] = dev[i];
+        dev[i] = 0;
+    }
+    __threadfence_system();
+}
+
+void plan_drain_i32(int32_t *dev, int32_t *host, int64_t n_copy, int64_t n_zero, KernelLaunch &L) {
+    const int64_t n = std::max<int64_t>(1, std::max(n_copy, n_zero));
+    L.func = (const void *)k_drain_i32;
+    L.grid = dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024)));
+    L.block = dim3(256);
+    L.shmem = 0;
+    L.reset();
+    L.arg(dev);
+    L.arg(host);
+    L.arg(n_copy);
+    L.arg(std::max(n_copy, n_zero));
 }
 
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) {

@@ -154,6 +154,30 @@ bool launch_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLd
 bool plan_count_wave_lds(const uint64_t *bm, int64_t pitch_words, const WaveLdsDesc &d, int n_cu, bool wide,
                          KernelLaunch &L);
 
+// LDS-staged wave with rows given as bit masks (databases of at most 64 dense
+// ids): slot i is dense id i, every id is staged; a tile holds its prefix and
+// its columns as masks, so neither the host nor the kernel handles id lists.
+// Outputs of a tile: out[ob + j] for its j-th column in ascending id order.
+struct MaskTile {
+    uint64_t pm;     // prefix ids (at least one)
+    uint64_t cm;     // column ids (at least one, all < n_cols)
+    int32_t ob;      // output index of the first column
+    int32_t k, nc;   // popcount(pm), popcount(cm)
+    int32_t pad;
+};
+static_assert(sizeof(MaskTile) == 32, "mask tiles are 32-byte records");
+struct WaveMaskDesc {
+    const MaskTile *tiles;
+    int32_t *out;        // [n_out], zeroed
+    int32_t n_items;     // ids staged: 0 .. n_items - 1 (<= 64)
+    int32_t n_tiles, n_out;
+};
+constexpr int kMaskMaxItems = 64;
+// Returns false if the shape does not fit LDS with the requested staging
+// (wide: one workgroup per CU; narrow: two).
+bool plan_count_wave_mask(const uint64_t *bm, int64_t pitch_words, const WaveMaskDesc &d, int n_cu, bool wide,
+                          KernelLaunch &L);
+
 // Device preprocessing (kernels_prep.hip).
 void launch_prep_hist(const int32_t *ids, int64_t n, int32_t nbins, int32_t *freq, int32_t *bad, int n_cu,
                       hipStream_t s);
@@ -168,6 +192,11 @@ void launch_prep_build(const int32_t *ids, const int64_t *off, const int32_t *ke
 // Zero `n` int32 with a fill kernel (graph-capture safe).
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s);
 void plan_zero_i32(int32_t *p, int64_t n, KernelLaunch &L);
+// A wave's first and last nodes: its descriptor block from pinned host memory
+// to the device, and its counts from the device to pinned host memory with
+// the device region [0, max(n_copy, n_zero)) zeroed again.
+void plan_stage16(const void *src, void *dst, int64_t bytes, KernelLaunch &L);
+void plan_drain_i32(int32_t *dev, int32_t *host, int64_t n_copy, int64_t n_zero, KernelLaunch &L);
 
 // Matrix AND-popcount, packed (VALU) path.  C must be zeroed when split_k>1.
 void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,

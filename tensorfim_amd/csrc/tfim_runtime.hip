@@ -49,6 +49,54 @@ struct SplitTuner {
     }
 };
 
+// Online choice among N arms by measured cost per unit of work: every arm
+// once; then up to three samples of the arms within 1.5x of the best cost;
+// then the best, another arm being re-tried every 16th call in turn (one
+// measured at more than 1.5x the best only every 8th time its turn comes,
+// beyond 3x only every 32nd).  Counts the arms issued, since a sample
+// arrives only when its wave is waited on.
+template <int N>
+struct ArmTuner {
+    double cost[N] = {};
+    int64_t samples[N] = {}, issued[N] = {};
+    int64_t calls = 0;
+    int pick(const bool *allowed) {
+        calls++;
+        int a = -1;
+        for (int i = 0; i < N && a < 0; i++)
+            if (allowed[i] && issued[i] == 0) a = i;
+        int best = -1;
+        for (int i = 0; i < N; i++)
+            if (allowed[i] && samples[i] > 0 && (best < 0 || cost[i] < cost[best])) best = i;
+        if (best < 0)
+            for (int i = 0; i < N && best < 0; i++)
+                if (allowed[i]) best = i;
+        for (int i = 0; i < N && a < 0; i++)
+            if (allowed[i] && samples[i] > 0 && issued[i] < 3 && cost[i] <= 1.5 * cost[best]) a = i;
+        if (a < 0) {
+            a = best;
+            if (calls % 16 == 0) {
+                const int64_t turn = calls / 16;
+                const int cand = (best + 1 + (int)(turn % (N - 1))) % N;
+                const double ratio = cost[cand] / std::max(1e-30, cost[best]);
+                const int64_t period = ratio <= 1.5 ? 1 : ratio <= 3.0 ? 8 : 32;
+                if (allowed[cand] && (turn / (N - 1)) % period == 0) a = cand;
+            }
+        }
+        issued[a]++;
+        return a;
+    }
+    void record(int a, double per) {
+        cost[a] = samples[a] == 0 ? per : 0.8 * cost[a] + 0.2 * per;
+        samples[a]++;
+    }
+};
+
+// Arms of the mask kernel: (columns per tile, wide staging).
+constexpr int kMaskArms = 6;
+constexpr int kMaskArmTile[kMaskArms] = {64, 32, 16, 64, 32, 16};
+constexpr bool kMaskArmWide[kMaskArms] = {true, true, true, false, false, false};
+
 // Online tuner arms for waves that both batched kernels accept:
 // (kernel 1 = L2-resident, 2 = LDS-staged, 5 = fused matrix-core; columns per
 // work tile; LDS staging wide = one workgroup per CU with up to 4 x 128 bits
@@ -94,7 +142,7 @@ struct tfim_ctx {
         int64_t desc_cap = 0;
         int32_t *d_out = nullptr, *h_out = nullptr;
         int64_t out_cap = 0;
-        hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the wave's nodes; ev1 also marks it done
         bool busy = false;
         int64_t ticket = -1, n_out = 0, n_rows = 0;
         int tuned = -1;
@@ -103,9 +151,10 @@ struct tfim_ctx {
         double mf_work = 0;
         int sk_arm = -1;       // third tuner's arm (matrix-core waves)
         bool sk_fused = false; // ... of the fused path's split tuner
+        int mask_arm = -1;     // mask kernel tuner's arm (-1: not tuned)
+        double mask_work = 0;
         double sk_work = 0;
-        // one executable graph per kernel sequence: H2D descriptor copy ->
-        // fill -> ev0 -> kernels -> ev1 -> D2H copy
+        // one executable graph per kernel sequence: stage -> [fill] -> kernels -> drain
         uint64_t *d_mat = nullptr;   // matrix-core path: prefix masks, then the count matrix
         int64_t mat_cap = 0;
         struct Graph {
@@ -114,7 +163,8 @@ struct tfim_ctx {
             int nk = 0;
             hipGraph_t g = nullptr;
             hipGraphExec_t x = nullptr;
-            hipGraphNode_t h2d = nullptr, fill = nullptr, e0 = nullptr, e1 = nullptr, d2h = nullptr;
+            bool has_fill = false;
+            hipGraphNode_t stage = nullptr, fill = nullptr, drain = nullptr;
             hipGraphNode_t kern[kMaxKernels] = {};
         };
         std::vector<Graph> graphs;
@@ -136,19 +186,27 @@ struct tfim_ctx {
     // third tuner: each matrix-core path's word split, as 1/2, 1 or 2 times
     // the occupancy rule's split; EMA of milliseconds per unit of work
     SplitTuner sk, fk;
+    ArmTuner<kMaskArms> mask_tune;   // mask rows: columns per tile and staging
+    std::vector<MaskTile> mask_tiles;
+    int64_t wave_mask_launches = 0;
     // wave tuner: EMA of kernel milliseconds per (row prefix + column) word,
     // per arm (kernel, columns per tile)
     double tune_cost[kTuneArms] = {};
-    int64_t tune_samples[kTuneArms] = {};
+    int64_t tune_samples[kTuneArms] = {}, tune_issued[kTuneArms] = {};
     int64_t tune_calls = 0;
     int tile_cols = 0;      // 0: chosen by the tuner (32 when it does not run)
     int mfma_tile = 0;      // 0 auto (= 256), 128 or 256
     int mfma_split = 0;     // matrix-core waves: 0 split by the occupancy rule and tuner, 1 no split
     int node_shape = 0;     // per-node kernel: 0 auto, else R * 100 + columns per block
     int node_spread = 0;    // per-node counts: 0 auto, 1 one copy, 8 spread over 8 copies
-    int use_graphs = 1;     // waves replayed as instantiated graphs
+    // waves replayed as instantiated graphs (off by default: on MI355X with
+    // ROCm 7.2 a wave's three to five kernels launched one by one cost about
+    // 12 us of host time against 22 us for a graph replay, and run back to
+    // back on the device where the graph leaves ~11 us between nodes)
+    int use_graphs = 0;
     std::vector<int32_t> slot_of;   // item -> LDS slot of the current wave (-1 = none)
     std::vector<int32_t> fslot_of;  // item -> slot of the fused matrix-core wave being packed
+    std::vector<int32_t> tile_bucket;   // counting sort of the LDS kernel's tiles by cost
     const char *fused_nofit = "";   // the limit the last wave refused by the fused kernel exceeded
     std::map<std::string, std::pair<int, int32_t>> popmat_tuned;   // shape -> (tile, split_k)
 
@@ -185,8 +243,11 @@ int fail(tfim_ctx *c, int code, const char *fmt, ...) {
     } while (0)
 
 // grow-only device + pinned host pair
+// (host_flags: of the pinned part; zero: the device part starts zeroed,
+// cleared on the ctx stream ahead of the work that follows)
 template <typename T>
-int ensure_pair(tfim_ctx *c, T **d, T **h, int64_t *cap, int64_t need) {
+int ensure_pair(tfim_ctx *c, T **d, T **h, int64_t *cap, int64_t need, unsigned host_flags = hipHostMallocDefault,
+                bool zero = false) {
     if (need <= *cap) return TFIM_OK;
     int64_t n = std::max<int64_t>(need, std::max<int64_t>(*cap * 2, 1024));
     if (*d) hipFree(*d);
@@ -196,7 +257,9 @@ int ensure_pair(tfim_ctx *c, T **d, T **h, int64_t *cap, int64_t need) {
     *cap = 0;
     if (hipMalloc((void **)d, (size_t)n * sizeof(T)) != hipSuccess)
         return fail(c, TFIM_E_NOMEM, "device scratch allocation of %lld bytes failed", (long long)(n * sizeof(T)));
-    if (h && hipHostMalloc((void **)h, (size_t)n * sizeof(T), hipHostMallocDefault) != hipSuccess)
+    if (zero && hipMemsetAsync(*d, 0, (size_t)n * sizeof(T), c->stream) != hipSuccess)
+        return fail(c, TFIM_E_HIP, "clearing %lld bytes of device scratch failed", (long long)(n * sizeof(T)));
+    if (h && hipHostMalloc((void **)h, (size_t)n * sizeof(T), host_flags) != hipSuccess)
         return fail(c, TFIM_E_NOMEM, "pinned scratch allocation of %lld bytes failed", (long long)(n * sizeof(T)));
     *cap = n;
     return TFIM_OK;
@@ -205,6 +268,20 @@ int ensure_pair(tfim_ctx *c, T **d, T **h, int64_t *cap, int64_t need) {
 int set_device(tfim_ctx *c) {
     HIPCHK(c, hipSetDevice(c->device));
     return TFIM_OK;
+}
+
+// A wave slot's descriptor block (read by the wave's stage kernel straight
+// from pinned host memory, so that memory is mapped coherent, and whole
+// 16-byte units are read) and its output block (the device part is zero
+// whenever the slot is idle: cleared when allocated, and by each wave's
+// drain kernel after it copies the counts to the coherent pinned part).
+template <typename Slot>
+int ensure_desc(tfim_ctx *c, Slot &sl, int64_t bytes) {
+    return ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, (bytes + 15) / 16 * 16, hipHostMallocCoherent);
+}
+template <typename Slot>
+int ensure_out(tfim_ctx *c, Slot &sl, int64_t n) {
+    return ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, std::max<int64_t>(1, n), hipHostMallocCoherent, true);
 }
 
 }  // namespace
@@ -267,8 +344,7 @@ int tfim_open(int device, tfim_ctx **out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     for (auto &sl : c->slots)
-        if (hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
+        if (hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     *out = c;
     return TFIM_OK;
 }
@@ -286,7 +362,6 @@ void tfim_close(tfim_ctx *c) {
         hipFree(sl.d_mat);
         if (sl.ev0) hipEventDestroy(sl.ev0);
         if (sl.ev1) hipEventDestroy(sl.ev1);
-        if (sl.done) hipEventDestroy(sl.done);
         for (auto &g : sl.graphs) {
             if (g.x) hipGraphExecDestroy(g.x);
             if (g.g) hipGraphDestroy(g.g);
@@ -533,74 +608,83 @@ hipKernelNodeParams node_params(KernelLaunch &L) {
     return p;
 }
 
-// Issues one wave on a slot: descriptor copy, fill of the accumulated
-// buffer, the wave's kernels (1 to 3: count kernel, or count kernel and
-// complement reduction, or masks, matrix count and gather) between the
-// slot's timing events, output copy.  With graphs on, these form one graph
-// per (slot, kernel sequence), instantiated on first use and replayed with
-// this wave's node parameters; the slot is idle (its last wave waited on)
-// whenever it is issued again.
+// Issues one wave on a slot, every step a kernel on the ctx stream (no copy
+// calls, no second stream, no cross-stream waits, which cost more host time
+// per wave than the kernels' own launches):
+//   stage: the descriptor block from pinned host memory to the device;
+//   fill (optional): zero a buffer the wave accumulates into other than the
+//         output block (the matrix-core path's count matrix);
+//   the wave's kernels (1 to 3: count kernel, or count kernel and complement
+//         reduction, or masks, matrix count and gather);
+//   drain: the first n_out counts to pinned host memory, and the output
+//         block's [0, max(n_out, n_zero)) back to zero.
+// between the slot's timing events (ev1 also marks the wave done).  With
+// graphs on, these form one graph per (slot, fill or not, kernel sequence),
+// instantiated on first use and replayed with this wave's node parameters.
+// The slot is idle (its last wave waited on, so its buffers are free and
+// its device output block is zero) whenever it is issued again.
 int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fill, int64_t n_fill, int64_t n_out,
-               KernelLaunch *const *ks, int nk) {
-    KernelLaunch Z;
-    plan_zero_i32(fill, n_fill, Z);
+               int64_t n_zero, KernelLaunch *const *ks, int nk) {
     for (int i = 0; i < nk; i++)
         if (ks[i]->overflow || !ks[i]->func) return fail(c, TFIM_E_INVALID, "wave kernel %d: bad launch description", i);
-    const size_t out_bytes = (size_t)n_out * 4;
+    KernelLaunch S, Z, D;
+    plan_stage16(sl.h_desc, sl.d_desc, desc_bytes, S);
+    const bool has_fill = fill && n_fill > 0;
+    if (has_fill) plan_zero_i32(fill, n_fill, Z);
+    plan_drain_i32(sl.d_out, sl.h_out, n_out, n_zero, D);
+    HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
     if (!c->use_graphs) {
-        HIPCHK(c, hipMemcpyAsync(sl.d_desc, sl.h_desc, (size_t)desc_bytes, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, Z.launch(c->stream));
-        HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
+        HIPCHK(c, S.launch(c->stream));
+        if (has_fill) HIPCHK(c, Z.launch(c->stream));
         for (int i = 0; i < nk; i++) HIPCHK(c, ks[i]->launch(c->stream));
-        HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
-        HIPCHK(c, hipMemcpyAsync(sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost, c->stream));
-        return TFIM_OK;
-    }
-    hipKernelNodeParams zp = node_params(Z), kp[tfim_ctx::Slot::Graph::kMaxKernels];
-    for (int i = 0; i < nk; i++) kp[i] = node_params(*ks[i]);
-    tfim_ctx::Slot::Graph *G = nullptr;
-    for (auto &g : sl.graphs) {
-        bool same = g.nk == nk;
-        for (int i = 0; i < nk && same; i++) same = g.funcs[i] == ks[i]->func;
-        if (same) G = &g;
-    }
-    if (!G) {
-        // build and instantiate; only a complete graph is kept
-        tfim_ctx::Slot::Graph g;
-        g.nk = nk;
-        for (int i = 0; i < nk; i++) g.funcs[i] = ks[i]->func;
-        hipError_t e = hipGraphCreate(&g.g, 0);
-        if (e == hipSuccess)
-            e = hipGraphAddMemcpyNode1D(&g.h2d, g.g, nullptr, 0, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
-                                        hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipGraphAddKernelNode(&g.fill, g.g, &g.h2d, 1, &zp);
-        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e0, g.g, &g.fill, 1, sl.ev0);
-        hipGraphNode_t prev = g.e0;
-        for (int i = 0; i < nk && e == hipSuccess; i++) {
-            e = hipGraphAddKernelNode(&g.kern[i], g.g, &prev, 1, &kp[i]);
-            prev = g.kern[i];
-        }
-        if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e1, g.g, &prev, 1, sl.ev1);
-        if (e == hipSuccess)
-            e = hipGraphAddMemcpyNode1D(&g.d2h, g.g, &g.e1, 1, sl.h_out, sl.d_out, out_bytes, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipGraphInstantiate(&g.x, g.g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            if (g.g) hipGraphDestroy(g.g);
-            return fail(c, TFIM_E_HIP, "wave graph construction failed: %s", hipGetErrorString(e));
-        }
-        sl.graphs.push_back(g);
-        G = &sl.graphs.back();
-        c->wave_graphs++;
+        HIPCHK(c, D.launch(c->stream));
     } else {
-        HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->h2d, sl.d_desc, sl.h_desc, (size_t)desc_bytes,
-                                                    hipMemcpyHostToDevice));
-        HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->fill, &zp));
-        for (int i = 0; i < nk; i++) HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->kern[i], &kp[i]));
-        HIPCHK(c, hipGraphExecMemcpyNodeSetParams1D(G->x, G->d2h, sl.h_out, sl.d_out, out_bytes,
-                                                    hipMemcpyDeviceToHost));
+        hipKernelNodeParams sp = node_params(S), zp{}, dp = node_params(D);
+        if (has_fill) zp = node_params(Z);
+        hipKernelNodeParams kp[tfim_ctx::Slot::Graph::kMaxKernels];
+        for (int i = 0; i < nk; i++) kp[i] = node_params(*ks[i]);
+        tfim_ctx::Slot::Graph *G = nullptr;
+        for (auto &g : sl.graphs) {
+            bool same = g.nk == nk && g.has_fill == has_fill;
+            for (int i = 0; i < nk && same; i++) same = g.funcs[i] == ks[i]->func;
+            if (same) G = &g;
+        }
+        if (!G) {
+            // build and instantiate; only a complete graph is kept
+            tfim_ctx::Slot::Graph g;
+            g.nk = nk;
+            g.has_fill = has_fill;
+            for (int i = 0; i < nk; i++) g.funcs[i] = ks[i]->func;
+            hipError_t e = hipGraphCreate(&g.g, 0);
+            if (e == hipSuccess) e = hipGraphAddKernelNode(&g.stage, g.g, nullptr, 0, &sp);
+            hipGraphNode_t prev = g.stage;
+            if (e == hipSuccess && has_fill) {
+                e = hipGraphAddKernelNode(&g.fill, g.g, &prev, 1, &zp);
+                prev = g.fill;
+            }
+            for (int i = 0; i < nk && e == hipSuccess; i++) {
+                e = hipGraphAddKernelNode(&g.kern[i], g.g, &prev, 1, &kp[i]);
+                prev = g.kern[i];
+            }
+            if (e == hipSuccess) e = hipGraphAddKernelNode(&g.drain, g.g, &prev, 1, &dp);
+            if (e == hipSuccess) e = hipGraphInstantiate(&g.x, g.g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                if (g.g) hipGraphDestroy(g.g);
+                return fail(c, TFIM_E_HIP, "wave graph construction failed: %s", hipGetErrorString(e));
+            }
+            sl.graphs.push_back(g);
+            G = &sl.graphs.back();
+            c->wave_graphs++;
+        } else {
+            HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->stage, &sp));
+            if (has_fill) HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->fill, &zp));
+            for (int i = 0; i < nk; i++) HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->kern[i], &kp[i]));
+            HIPCHK(c, hipGraphExecKernelNodeSetParams(G->x, G->drain, &dp));
+        }
+        HIPCHK(c, hipGraphLaunch(G->x, c->stream));
+        c->wave_graph_replays++;
     }
-    HIPCHK(c, hipGraphLaunch(G->x, c->stream));
-    c->wave_graph_replays++;
+    HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
     return TFIM_OK;
 }
 
@@ -625,8 +709,8 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     const int64_t o_excl = align16(o_cids + n_exp * 4);
     const int64_t o_rc = align16(o_excl + (int64_t)n_compl * excl_words * 8);
     const int64_t total = o_rc + (int64_t)n_rows * 4;
-    if (int r = ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, total)) return r;
-    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, std::max<int64_t>(1, n_out))) return r;
+    if (int r = ensure_desc(c, sl, total)) return r;
+    if (int r = ensure_out(c, sl, n_out)) return r;
     const int64_t mask_words = (int64_t)n_rows * c->pitch;
     const int64_t c_ints = (int64_t)n_rows * c->n_cols;
     const int64_t c_off = (mask_words + 31) / 32 * 32;   // 256-byte aligned
@@ -685,13 +769,13 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     plan_wave_gather(d, C, n_rows, c->n_cols, reinterpret_cast<const int32_t *>(sl.d_desc + o_rc), min_count,
                      sl.d_out + n_exp, max_row_cols, G);
     KernelLaunch *ks[3] = {&M, &P, &G};
-    if (int r = issue_wave(c, sl, total, C, zs > 1 ? c_ints : 0, n_out, ks, 3)) return r;
-    HIPCHK(c, hipEventRecord(sl.done, c->stream));
+    if (int r = issue_wave(c, sl, total, zs > 1 ? C : nullptr, zs > 1 ? c_ints : 0, n_out, n_out, ks, 3)) return r;
     sl.busy = true;
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
     sl.n_rows = n_rows;
     sl.sk_arm = sk_arm;
+    sl.mask_arm = -1;
     sl.sk_fused = false;
     sl.sk_work = (double)((n_rows + ra - 1) / ra * ra) * (double)((c->n_cols + 127) / 128 * 128) * (double)c->pitch;
     c->wave_mfma_launches++;
@@ -764,8 +848,8 @@ int wave_submit_fused(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, 
     const int64_t n_dev = n_compl ? o_arr + tiles_m : n_out;
     c->fused_nofit = "outputs within int32 indices";
     if (n_dev > INT32_MAX) return kFusedNoFit;
-    if (int r = ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, total)) return r;
-    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, std::max<int64_t>(1, n_dev))) return r;
+    if (int r = ensure_desc(c, sl, total)) return r;
+    if (int r = ensure_out(c, sl, n_dev)) return r;
     char *h = sl.h_desc;
     memcpy(h + o_slots, slot_rows.data(), slot_rows.size() * 4);
     memcpy(h + o_ioff, item_off, (size_t)(n_rows + 1) * 4);
@@ -846,13 +930,13 @@ int wave_submit_fused(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, 
     c->fused_nofit = "a launch the kernel accepts (tile, slots, LDS attribute)";
     if (!plan_wave_mfma(c->d_bm, c->pitch, d, ti, tj, tiles_m, (int)zs, lds, K)) return kFusedNoFit;
     KernelLaunch *ks[1] = {&K};
-    if (int r = issue_wave(c, sl, total, sl.d_out, n_dev, n_out, ks, 1)) return r;
-    HIPCHK(c, hipEventRecord(sl.done, c->stream));
+    if (int r = issue_wave(c, sl, total, nullptr, 0, n_out, n_dev, ks, 1)) return r;
     sl.busy = true;
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
     sl.n_rows = n_rows;
     sl.sk_arm = sk_arm;
+    sl.mask_arm = -1;
     sl.sk_fused = true;
     sl.sk_work = (double)tiles_m * RA * (double)tiles_n * RB * (double)c->pitch;
     c->wave_fused_launches++;
@@ -975,22 +1059,33 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     int tuned = -1;
     if (lds && wave_kernel == 0) {
         c->tune_calls++;
+        // every arm once; then up to three samples of the arms within 1.5x
+        // of the best one's cost
+        // of the best one's cost (counting the waves issued, whose timings
+        // arrive when they are waited on, and waiting for the first ones)
         for (int a = 0; a < kTuneArms && tuned < 0; a++)
-            if (c->tune_samples[a] < 3) tuned = a;
+            if (c->tune_issued[a] == 0) tuned = a;
+        int best = -1;
+        for (int a = 0; a < kTuneArms; a++)
+            if (c->tune_samples[a] > 0 && (best < 0 || c->tune_cost[a] < c->tune_cost[best])) best = a;
+        if (best < 0) best = 2;   // nothing measured yet: an LDS arm
+        for (int a = 0; a < kTuneArms && tuned < 0; a++)
+            if (c->tune_samples[a] > 0 && c->tune_issued[a] < 3 && c->tune_cost[a] <= 1.5 * c->tune_cost[best])
+                tuned = a;
         if (tuned < 0) {
-            int best = 0;
-            for (int a = 1; a < kTuneArms; a++)
-                if (c->tune_cost[a] < c->tune_cost[best]) best = a;
             tuned = best;
             if (c->tune_calls % 16 == 0) {
-                // round robin over the other arms; an arm measured at more
-                // than twice the best one's cost is re-tried only every
-                // eighth time its turn comes
+                // round robin over the other arms; one measured at more than
+                // 1.5x the best one's cost is re-tried only every 8th time
+                // its turn comes, beyond 3x only every 32nd
                 const int64_t turn = c->tune_calls / 16;
                 const int cand = (best + 1 + (int)(turn % (kTuneArms - 1))) % kTuneArms;
-                if (c->tune_cost[cand] <= 2.0 * c->tune_cost[best] || (turn / (kTuneArms - 1)) % 8 == 0) tuned = cand;
+                const double ratio = c->tune_cost[cand] / std::max(1e-30, c->tune_cost[best]);
+                const int64_t period = ratio <= 1.5 ? 1 : ratio <= 3.0 ? 8 : 32;
+                if ((turn / (kTuneArms - 1)) % period == 0) tuned = cand;
             }
         }
+        c->tune_issued[tuned]++;
         if (kArmKernel[tuned] == 1) {
             lds = false;
             for (int32_t it : slot_rows) c->slot_of[it] = -1;
@@ -1159,8 +1254,8 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     const int64_t o_excl = align16(o_tiles + (int64_t)tiles.size() * (int64_t)sizeof(WaveTile));
     const int64_t o_crows = align16(o_excl + (int64_t)n_compl * excl_words * 8);
     const int64_t total = o_crows + (int64_t)n_compl * 4;
-    if (int r = ensure_pair(c, &sl.d_desc, &sl.h_desc, &sl.desc_cap, total)) return r;
-    if (int r = ensure_pair(c, &sl.d_out, &sl.h_out, &sl.out_cap, n_dev)) return r;
+    if (int r = ensure_desc(c, sl, total)) return r;
+    if (int r = ensure_out(c, sl, n_dev)) return r;
     char *h = sl.h_desc;
     int32_t *hi = reinterpret_cast<int32_t *>(h + o_items);
     int32_t *hc = reinterpret_cast<int32_t *>(h + o_cids);
@@ -1193,21 +1288,28 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     if (lds) {
         // the LDS kernel takes tiles resolved to prefix and output ranges
         static_assert(sizeof(LdsTile) == sizeof(WaveTile), "tile records share the descriptor layout");
+        // by falling cost 2 * columns + prefix items (a column reads and
+        // counts its words, a prefix item only reads and ANDs them), so that
+        // the waves of a workgroup, taking tiles in this order, finish a
+        // chunk together: a stable counting sort on the cost, so equal costs
+        // keep the ascending output order they are generated in
         LdsTile *lt = reinterpret_cast<LdsTile *>(h + o_tiles);
-        for (size_t i = 0; i < tiles.size(); i++) {
-            const WaveTile &w = tiles[i];
-            lt[i].ib = item_off[w.row];
-            lt[i].k = item_off[w.row + 1] - item_off[w.row];
-            lt[i].cb = coff[w.row] + w.col_begin;
-            lt[i].nc = w.col_end - w.col_begin;
+        std::vector<int32_t> &start = c->tile_bucket;
+        int32_t max_cost = 0;
+        for (const WaveTile &w : tiles)
+            max_cost = std::max(max_cost, 2 * (w.col_end - w.col_begin) + item_off[w.row + 1] - item_off[w.row]);
+        start.assign((size_t)max_cost + 2, 0);
+        for (const WaveTile &w : tiles)
+            start[(size_t)(max_cost - (2 * (w.col_end - w.col_begin) + item_off[w.row + 1] - item_off[w.row])) + 1]++;
+        for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
+        for (const WaveTile &w : tiles) {
+            const int32_t k = item_off[w.row + 1] - item_off[w.row], nc = w.col_end - w.col_begin;
+            LdsTile &t = lt[start[(size_t)(max_cost - (2 * nc + k))]++];
+            t.ib = item_off[w.row];
+            t.k = k;
+            t.cb = coff[w.row] + w.col_begin;
+            t.nc = nc;
         }
-        // by falling cost (a column reads and counts its words, a prefix
-        // item only reads and ANDs them), so that the waves of a workgroup,
-        // taking tiles in this order, finish a chunk together
-        std::sort(lt, lt + tiles.size(), [](const LdsTile &a, const LdsTile &b) {
-            const int ca = 2 * a.nc + a.k, cb = 2 * b.nc + b.k;
-            return ca != cb ? ca > cb : a.cb < b.cb;
-        });
     } else {
         memcpy(h + o_tiles, tiles.data(), tiles.size() * sizeof(WaveTile));
     }
@@ -1252,8 +1354,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
                               min_count, sl.d_out + n_exp, R);
     }
     KernelLaunch *ks[2] = {&K, &R};
-    if (int r = issue_wave(c, sl, total, sl.d_out, n_dev, n_out, ks, n_compl ? 2 : 1)) return r;
-    HIPCHK(c, hipEventRecord(sl.done, c->stream));
+    if (int r = issue_wave(c, sl, total, nullptr, 0, n_out, n_dev, ks, n_compl ? 2 : 1)) return r;
     sl.busy = true;
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
@@ -1263,6 +1364,134 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     sl.mf_arm = mf_arm;
     sl.mf_work = mf_work;
     sl.sk_arm = -1;
+    sl.mask_arm = -1;
+    *ticket = sl.ticket;
+    return TFIM_OK;
+}
+
+// Mask rows (tfim_count_masks_submit).  Waves the mask kernel does not take
+// (more than kLdsMaxOut outputs, the matrix-core path asked for, or a wave
+// kernel forced other than the LDS one) are turned into id lists (ascending,
+// so the outputs keep the mask order) and counted by wave_submit.
+int wave_submit_masks(tfim_ctx *c, const uint64_t *pmask, const uint64_t *cmask, int32_t n_rows, int32_t path,
+                      int64_t *ticket) {
+    if (!c || !pmask || !cmask || !ticket || n_rows < 0) return fail(c, TFIM_E_INVALID, "count_masks: null argument");
+    *ticket = -1;
+    if (!c->d_bm) return fail(c, TFIM_E_STATE, "count_masks: no bitmaps loaded");
+    if (c->n_items > kMaskMaxItems)
+        return fail(c, TFIM_E_RANGE, "count_masks: %d dense ids loaded, masks cover at most %d", c->n_items,
+                    kMaskMaxItems);
+    if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED && path != TFIM_PATH_MFMA_I8)
+        return fail(c, TFIM_E_INVALID, "count_masks: unknown path %d", path);
+    const uint64_t ids_ok = c->n_items >= 64 ? ~0ull : (1ull << c->n_items) - 1;
+    const uint64_t cols_ok = c->n_cols >= 64 ? ~0ull : (1ull << c->n_cols) - 1;
+    int64_t n_out = 0, n_pref = 0;
+    for (int32_t r = 0; r < n_rows; r++) {
+        if (!pmask[r] || (pmask[r] & ~ids_ok))
+            return fail(c, TFIM_E_INVALID, "count_masks: row %d prefix mask is empty or names an id >= %d", r,
+                        c->n_items);
+        if (cmask[r] & ~cols_ok)
+            return fail(c, TFIM_E_INVALID, "count_masks: row %d column mask names an id >= %d", r, c->n_cols);
+        n_out += __builtin_popcountll(cmask[r]);
+        n_pref += __builtin_popcountll(pmask[r]);
+    }
+    if (n_out > INT32_MAX) return fail(c, TFIM_E_RANGE, "count_masks: output too large");
+    if (n_rows == 0 || n_out == 0) return TFIM_OK;
+    const bool mask_kernel = n_out <= kLdsMaxOut && path != TFIM_PATH_MFMA_I8 &&
+                             (c->wave_kernel == 0 || c->wave_kernel == 2 || c->wave_kernel == 3);
+    if (!mask_kernel) {
+        std::vector<int32_t> ids, ioff(1, 0), cols, coff(1, 0);
+        ids.reserve((size_t)n_pref);
+        cols.reserve((size_t)n_out);
+        for (int32_t r = 0; r < n_rows; r++) {
+            for (uint64_t b = pmask[r]; b; b &= b - 1) ids.push_back(__builtin_ctzll(b));
+            for (uint64_t b = cmask[r]; b; b &= b - 1) cols.push_back(__builtin_ctzll(b));
+            ioff.push_back((int32_t)ids.size());
+            coff.push_back((int32_t)cols.size());
+        }
+        return wave_submit(c, ids.data(), ioff.data(), n_rows, cols.data(), coff.data(), nullptr, 0, nullptr, 0, path,
+                           ticket);
+    }
+    std::lock_guard<std::mutex> wave_lock(c->wave_mu);
+    tfim_ctx::Slot &sl = c->slots[c->next_ticket % tfim_ctx::kWaveSlots];
+    if (sl.busy)
+        return fail(c, TFIM_E_STATE, "count_masks_submit: %d waves in flight; wait on a ticket first",
+                    tfim_ctx::kWaveSlots);
+    if (int r = set_device(c)) return r;
+    // arm: columns per tile and staging (two workgroups per CU only where a
+    // chunk of every id and the accumulators fit half the LDS)
+    bool allowed[kMaskArms];
+    const bool narrow_fits = (size_t)c->n_items * 1024 + (size_t)(n_out + 3) / 4 * 16 + 16 <= 80 * 1024;
+    for (int a = 0; a < kMaskArms; a++) allowed[a] = kMaskArmWide[a] || narrow_fits;
+    int arm;
+    if (c->wave_kernel == 2 || c->wave_kernel == 3) {
+        arm = c->wave_kernel == 3 || !narrow_fits ? 0 : 3;
+        if (c->tile_cols) arm += c->tile_cols >= 64 ? 0 : c->tile_cols >= 32 ? 1 : 2;
+    } else {
+        arm = c->mask_tune.pick(allowed);
+    }
+    const int tile_cols = c->tile_cols ? c->tile_cols : kMaskArmTile[arm];
+    // tiles of <= tile_cols columns (the lowest ids first), outputs CSR by row
+    std::vector<MaskTile> &tiles = c->mask_tiles;
+    tiles.clear();
+    int32_t ob = 0, max_cost = 0;
+    for (int32_t r = 0; r < n_rows; r++) {
+        const uint64_t pm = pmask[r];
+        const int32_t k = __builtin_popcountll(pm);
+        uint64_t cm = cmask[r];
+        while (cm) {
+            uint64_t part = cm;
+            int32_t n = __builtin_popcountll(cm);
+            if (n > tile_cols) {
+                part = 0;
+                for (int i = 0; i < tile_cols; i++) {
+                    const uint64_t low = cm & (~cm + 1);
+                    part |= low;
+                    cm ^= low;
+                }
+                n = tile_cols;
+            } else {
+                cm = 0;
+            }
+            tiles.push_back({pm, part, ob, k, n, 0});
+            ob += n;
+            max_cost = std::max(max_cost, 2 * n + k);
+        }
+    }
+    const int64_t n_tiles = (int64_t)tiles.size();
+    const int64_t desc_bytes = n_tiles * (int64_t)sizeof(MaskTile);
+    if (int r = ensure_desc(c, sl, desc_bytes)) return r;
+    if (int r = ensure_out(c, sl, n_out)) return r;
+    // by falling cost 2 * columns + prefix ids, stable (see the list path)
+    MaskTile *ht = reinterpret_cast<MaskTile *>(sl.h_desc);
+    std::vector<int32_t> &start = c->tile_bucket;
+    start.assign((size_t)max_cost + 2, 0);
+    for (const MaskTile &t : tiles) start[(size_t)(max_cost - (2 * t.nc + t.k)) + 1]++;
+    for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
+    for (const MaskTile &t : tiles) ht[start[(size_t)(max_cost - (2 * t.nc + t.k))]++] = t;
+    WaveMaskDesc d;
+    d.tiles = reinterpret_cast<const MaskTile *>(sl.d_desc);
+    d.out = sl.d_out;
+    d.n_items = c->n_items;
+    d.n_tiles = (int32_t)n_tiles;
+    d.n_out = (int32_t)n_out;
+    KernelLaunch K;
+    if (!plan_count_wave_mask(c->d_bm, c->pitch, d, c->n_cu, kMaskArmWide[arm], K))
+        return fail(c, TFIM_E_RANGE, "count_masks: the mask kernel rejected a wave of %lld outputs over %d ids",
+                    (long long)n_out, c->n_items);
+    KernelLaunch *ks[1] = {&K};
+    if (int r = issue_wave(c, sl, desc_bytes, nullptr, 0, n_out, n_out, ks, 1)) return r;
+    sl.busy = true;
+    sl.ticket = c->next_ticket++;
+    sl.n_out = n_out;
+    sl.n_rows = n_rows;
+    sl.tuned = -1;
+    sl.mf_arm = -1;
+    sl.sk_arm = -1;
+    sl.mask_arm = (c->wave_kernel == 2 || c->wave_kernel == 3) ? -1 : arm;
+    sl.mask_work = (double)(n_pref + n_out) * (double)c->pitch;
+    c->wave_lds_launches++;
+    c->wave_mask_launches++;
     *ticket = sl.ticket;
     return TFIM_OK;
 }
@@ -1299,6 +1528,19 @@ int tfim_count_nodes_ex(tfim_ctx *c, const int32_t *item_ids, const int32_t *ite
     return tfim_count_nodes_wait(c, ticket, out);
 }
 
+int tfim_count_masks_submit(tfim_ctx *c, const uint64_t *prefix_mask, const uint64_t *col_mask, int32_t n_rows,
+                            int32_t path, int64_t *ticket) {
+    return wave_submit_masks(c, prefix_mask, col_mask, n_rows, path, ticket);
+}
+
+int tfim_count_masks(tfim_ctx *c, const uint64_t *prefix_mask, const uint64_t *col_mask, int32_t n_rows,
+                     int32_t *out, int32_t path) {
+    if (!out) return fail(c, TFIM_E_INVALID, "count_masks: null output");
+    int64_t ticket = -1;
+    if (int r = tfim_count_masks_submit(c, prefix_mask, col_mask, n_rows, path, &ticket)) return r;
+    return tfim_count_nodes_wait(c, ticket, out);
+}
+
 int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     if (!c) return TFIM_E_INVALID;
     if (ticket < 0) return TFIM_OK;   // nothing was submitted (no rows or no columns)
@@ -1321,7 +1563,7 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     } release{c, sl};
     // poll rather than block in the runtime, so that a submit from another
     // thread is not held up behind this wait
-    for (hipError_t q; (q = hipEventQuery(sl.done)) != hipSuccess;) {
+    for (hipError_t q; (q = hipEventQuery(sl.ev1)) != hipSuccess;) {
         if (q != hipErrorNotReady) HIPCHK(c, q);
         std::this_thread::yield();
     }
@@ -1333,6 +1575,7 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     c->wave_launches++;
     c->wave_rows += sl.n_rows;
     if (sl.sk_arm >= 0 && sl.sk_work > 0) (sl.sk_fused ? c->fk : c->sk).record(sl.sk_arm, (double)ms / sl.sk_work);
+    if (sl.mask_arm >= 0 && sl.mask_work > 0) c->mask_tune.record(sl.mask_arm, (double)ms / sl.mask_work);
     if (sl.mf_arm >= 0 && sl.mf_work > 0) {
         const double per = (double)ms / sl.mf_work;
         double &cst = c->mf_cost[sl.mf_arm];

@@ -60,6 +60,15 @@ int tfim_count_nodes_ex(tfim_ctx *c, const int32_t *ids, const int32_t *io, int3
 int tfim_count_nodes_ex_submit(tfim_ctx *c, const int32_t *ids, const int32_t *io, int32_t n, const int32_t *cids, const int32_t *co,
                                const int32_t *cr, int32_t nc, const uint64_t *ex, int32_t mc, int32_t p, int64_t *t) {
   int64_t tk=g_next++; auto &v=g_pending[tk%3]; v.assign((size_t)co[n]+2*nc,0); tfim_count_nodes_ex(c,ids,io,n,cids,co,cr,nc,ex,mc,v.data(),p); *t=tk; return 0; }
+int tfim_count_masks_submit(tfim_ctx *c, const uint64_t *pm, const uint64_t *cm, int32_t n, int32_t, int64_t *t) {
+  if (c->n_items > 64) return -4;
+  int64_t tk=g_next++; auto &v=g_pending[tk%3]; v.clear(); std::vector<int32_t> it, cols, o;
+  for (int r=0;r<n;r++){ it.clear(); cols.clear(); for(uint64_t b=pm[r];b;b&=b-1) it.push_back(__builtin_ctzll(b)); if(it.empty()) return -2;
+    for(uint64_t b=cm[r];b;b&=b-1) cols.push_back(__builtin_ctzll(b)); o.assign(cols.size(),0);
+    if(!cols.empty()) cnt(c,it.data(),(int)it.size(),cols.data(),(int)cols.size(),o.data()); v.insert(v.end(),o.begin(),o.end()); }
+  *t=tk; return 0; }
+int tfim_count_masks(tfim_ctx *c, const uint64_t *pm, const uint64_t *cm, int32_t n, int32_t *out, int32_t p) {
+  int64_t t; int rc=tfim_count_masks_submit(c,pm,cm,n,p,&t); if(rc) return rc; auto &v=g_pending[t%3]; memcpy(out,v.data(),v.size()*4); return 0; }
 int tfim_count_nodes_wait(tfim_ctx *, int64_t t, int32_t *out) { if(t<0) return 0; auto &v=g_pending[t%3]; memcpy(out,v.data(),v.size()*4); return 0; }
 int tfim_wave_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *r, double *ms) { if(a)*a=0; if(b)*b=0; if(r)*r=0; if(ms)*ms=0; return 0; }
 int tfim_wave_graph_stats(tfim_ctx *, int64_t *g, int64_t *l) { if(g)*g=0; if(l)*l=0; return 0; }

@@ -21,8 +21,9 @@ FIX = os.path.join(HERE, "golden", "synthetic")
 RUNS = {
     "clickstream": [("mafia", []), ("mafia", ["--learned"]), ("mafia", ["--compl", "off"]), ("wave", []),
                     ("plain", ["--workers", "8"])],
-    "dense-replicated": [("mafia", []), ("mafia", ["--learned"]), ("wave", []), ("wave", ["--wave-kernel", "4"]),
-                         ("plain", ["--workers", "8"])],
+    "dense-replicated": [("mafia", []), ("mafia", ["--learned"]), ("mafia", ["--rows", "lists"]),
+                         ("mafia", ["--graphs", "1"]), ("wave", []),
+                         ("wave", ["--wave-kernel", "4"]), ("plain", ["--workers", "8"])],
     "gene": [("mafia", []), ("mafia", ["--learned"]), ("mafia", ["--compl", "off"])],
     "dense-small": [("mafia", []), ("mafia", ["--learned"]), ("plain", [])],
 }

@@ -18,7 +18,8 @@ from conftest import GOLD_PAIRS, gold_path
 pytestmark = pytest.mark.gpu
 
 ENGINES = [("plain", []), ("wave", []), ("mafia", []), ("mafia", ["--learned"]), ("mafia", ["--compl", "on"]),
-           ("mafia", ["--compl", "off"]), ("mafia", ["--spec", "off"])]
+           ("mafia", ["--compl", "off"]), ("mafia", ["--spec", "off"]), ("mafia", ["--rows", "lists"]),
+           ("mafia", ["--graphs", "1"])]
 HEADER_KEYS = ["transNum", "dimension", "itemIDmax", "dimensionReduced", "transNumReduced", "MFIsNumber",
                "transLengthMean", "transLengthReducedMean"]
 

@@ -28,7 +28,7 @@ def mock_engine(tmp_path_factory):
 ENGINES = [["--engine", "mafia"], ["--engine", "mafia", "--gate", "always"], ["--engine", "wave"],
            ["--engine", "plain", "--workers", "3"], ["--engine", "mafia", "--compl", "on"],
            ["--engine", "mafia", "--gate", "always", "--compl", "on"], ["--engine", "mafia", "--spec", "off"],
-           ["--engine", "mafia", "--compl", "on", "--spec", "off"]]
+           ["--engine", "mafia", "--compl", "on", "--spec", "off"], ["--engine", "mafia", "--rows", "lists"]]
 
 
 @pytest.mark.parametrize("ds,thr", GOLD_PAIRS[:5])

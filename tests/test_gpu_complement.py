@@ -123,7 +123,7 @@ def test_complement_waves_replayed(ctx, graphs):
         st = ctx.wave_stats()
         print("paths: l2 %d mfma %d" % (st["l2_launches"], st["mfma_launches"]), flush=True)
     finally:
-        ctx.config("graphs", 1)
+        ctx.config("graphs", 0)
 
 
 def test_complement_rows_rejected_when_malformed(ctx):

@@ -194,7 +194,7 @@ def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
             np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols))
     finally:
         ctx.config("wave_kernel", 0)
-        ctx.config("graphs", 1)
+        ctx.config("graphs", 0)
 
 
 def test_wave_graph_replays(ctx):
@@ -204,6 +204,7 @@ def test_wave_graph_replays(ctx):
     n_bits, n_items = 50_001, 40
     rows = rand_rows(rng, n_items, n_bits, 0.6)
     ctx.load_bitmaps(rows, n_bits, n_items)
+    ctx.config("graphs", 1)
     before = ctx.wave_stats()
     for rep, n in enumerate([50, 3, 200, 1, 120, 7, 60]):
         prefixes = [rng.choice(n_items, size=int(rng.integers(1, 6)), replace=False) for _ in range(n)]
@@ -214,6 +215,7 @@ def test_wave_graph_replays(ctx):
         # eager work between replays
         np.testing.assert_array_equal(ctx.count_node(prefixes[0]), oracle_lib.node_counts(rows, prefixes[0], n_items))
     after = ctx.wave_stats()
+    ctx.config("graphs", 0)
     assert after["graph_launches"] - before["graph_launches"] >= 7
     assert after["graphs"] - before["graphs"] <= 8   # instantiated once per (slot, kernel)
 
