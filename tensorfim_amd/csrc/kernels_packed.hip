@@ -948,8 +948,27 @@ void plan_stage16(const void *src, void *dst, int64_t bytes, KernelLaunch &L) {
 
 // A wave's counts out to pinned host memory (mapped), and its device output
 // region back to zero for the slot's next wave: the last node of a wave.
+// 16-byte loads and stores (the buffers are 256-byte aligned), so each wave
+// of the kernel writes 1 KB runs to the host; every element is read and
+// zeroed by the same thread.
 __global__ void k_drain_i32(int32_t *__restrict__ dev, int32_t *__restrict__ host, int64_t n_copy, int64_t n_zero) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_zero; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x, t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint4 *d4 = reinterpret_cast<uint4 *>(dev);
+    uint4 *h4 = reinterpret_cast<uint4 *>(host);
+    const int64_t c4 = n_copy / 4, z4 = n_zero / 4;   // n_copy <= n_zero
+    for (int64_t i = t0; i < z4; i += stride) {
+        const uint4 v = d4[i];
+        if (i < c4) {
+            h4[i] = v;
+        } else if (i == c4) {   // the copy's last partial 16 bytes
+            const int64_t b = 4 * i;
+            if (b < n_copy) host[b] = (int32_t)v.x;
+            if (b + 1 < n_copy) host[b + 1] = (int32_t)v.y;
+            if (b + 2 < n_copy) host[b + 2] = (int32_t)v.z;
+        }
+        d4[i] = make_uint4(0, 0, 0, 0);
+    }
+    for (int64_t i = 4 * z4 + t0; i < n_zero; i += stride) {
         if (i < n_copy) host[i] = dev[i];
         dev[i] = 0;
     }
@@ -957,16 +976,17 @@ __global__ void k_drain_i32(int32_t *__restrict__ dev, int32_t *__restrict__ hos
 }
 
 void plan_drain_i32(int32_t *dev, int32_t *host, int64_t n_copy, int64_t n_zero, KernelLaunch &L) {
-    const int64_t n = std::max<int64_t>(1, std::max(n_copy, n_zero));
+    const int64_t nz = std::max(n_copy, n_zero);
+    const int64_t n4 = std::max<int64_t>(1, (nz + 3) / 4);
     L.func = (const void *)k_drain_i32;
-    L.grid = dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024)));
+    L.grid = dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n4 + 255) / 256, 1024)));
     L.block = dim3(256);
     L.shmem = 0;
     L.reset();
     L.arg(dev);
     L.arg(host);
     L.arg(n_copy);
-    L.arg(std::max(n_copy, n_zero));
+    L.arg(nz);
 }
 
 void launch_zero_i32(int32_t *p, int64_t n, hipStream_t s) {

@@ -142,7 +142,8 @@ struct tfim_ctx {
         int64_t desc_cap = 0;
         int32_t *d_out = nullptr, *h_out = nullptr;
         int64_t out_cap = 0;
-        hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the wave's nodes; ev1 also marks it done
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the wave's count kernels
+        hipEvent_t done = nullptr;                 // after its drain
         bool busy = false;
         int64_t ticket = -1, n_out = 0, n_rows = 0;
         int tuned = -1;
@@ -154,7 +155,8 @@ struct tfim_ctx {
         int mask_arm = -1;     // mask kernel tuner's arm (-1: not tuned)
         double mask_work = 0;
         double sk_work = 0;
-        // one executable graph per kernel sequence: stage -> [fill] -> kernels -> drain
+        // one executable graph per kernel sequence: stage -> [fill] -> ev0 ->
+        // kernels -> ev1 -> drain
         uint64_t *d_mat = nullptr;   // matrix-core path: prefix masks, then the count matrix
         int64_t mat_cap = 0;
         struct Graph {
@@ -164,7 +166,7 @@ struct tfim_ctx {
             hipGraph_t g = nullptr;
             hipGraphExec_t x = nullptr;
             bool has_fill = false;
-            hipGraphNode_t stage = nullptr, fill = nullptr, drain = nullptr;
+            hipGraphNode_t stage = nullptr, fill = nullptr, e0 = nullptr, e1 = nullptr, drain = nullptr;
             hipGraphNode_t kern[kMaxKernels] = {};
         };
         std::vector<Graph> graphs;
@@ -344,7 +346,8 @@ int tfim_open(int device, tfim_ctx **out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     for (auto &sl : c->slots)
-        if (hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
+        if (hipEventCreate(&sl.ev0) != hipSuccess || hipEventCreate(&sl.ev1) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) { tfim_close(c); return TFIM_E_HIP; }
     *out = c;
     return TFIM_OK;
 }
@@ -362,6 +365,7 @@ void tfim_close(tfim_ctx *c) {
         hipFree(sl.d_mat);
         if (sl.ev0) hipEventDestroy(sl.ev0);
         if (sl.ev1) hipEventDestroy(sl.ev1);
+        if (sl.done) hipEventDestroy(sl.done);
         for (auto &g : sl.graphs) {
             if (g.x) hipGraphExecDestroy(g.x);
             if (g.g) hipGraphDestroy(g.g);
@@ -615,14 +619,15 @@ hipKernelNodeParams node_params(KernelLaunch &L) {
 //   fill (optional): zero a buffer the wave accumulates into other than the
 //         output block (the matrix-core path's count matrix);
 //   the wave's kernels (1 to 3: count kernel, or count kernel and complement
-//         reduction, or masks, matrix count and gather);
+//         reduction, or masks, matrix count and gather), between the slot's
+//         timing events (so the tuners see the count kernels alone);
 //   drain: the first n_out counts to pinned host memory, and the output
-//         block's [0, max(n_out, n_zero)) back to zero.
-// between the slot's timing events (ev1 also marks the wave done).  With
-// graphs on, these form one graph per (slot, fill or not, kernel sequence),
-// instantiated on first use and replayed with this wave's node parameters.
-// The slot is idle (its last wave waited on, so its buffers are free and
-// its device output block is zero) whenever it is issued again.
+//         block's [0, max(n_out, n_zero)) back to zero;
+// then the slot's `done` event.  With graphs on, stage to drain form one
+// graph per (slot, fill or not, kernel sequence), instantiated on first use
+// and replayed with this wave's node parameters.  The slot is idle (its last
+// wave waited on, so its buffers are free and its device output block is
+// zero) whenever it is issued again.
 int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fill, int64_t n_fill, int64_t n_out,
                int64_t n_zero, KernelLaunch *const *ks, int nk) {
     for (int i = 0; i < nk; i++)
@@ -632,11 +637,12 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fil
     const bool has_fill = fill && n_fill > 0;
     if (has_fill) plan_zero_i32(fill, n_fill, Z);
     plan_drain_i32(sl.d_out, sl.h_out, n_out, n_zero, D);
-    HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
     if (!c->use_graphs) {
         HIPCHK(c, S.launch(c->stream));
         if (has_fill) HIPCHK(c, Z.launch(c->stream));
+        HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
         for (int i = 0; i < nk; i++) HIPCHK(c, ks[i]->launch(c->stream));
+        HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
         HIPCHK(c, D.launch(c->stream));
     } else {
         hipKernelNodeParams sp = node_params(S), zp{}, dp = node_params(D);
@@ -662,11 +668,14 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fil
                 e = hipGraphAddKernelNode(&g.fill, g.g, &prev, 1, &zp);
                 prev = g.fill;
             }
+            if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e0, g.g, &prev, 1, sl.ev0);
+            prev = g.e0;
             for (int i = 0; i < nk && e == hipSuccess; i++) {
                 e = hipGraphAddKernelNode(&g.kern[i], g.g, &prev, 1, &kp[i]);
                 prev = g.kern[i];
             }
-            if (e == hipSuccess) e = hipGraphAddKernelNode(&g.drain, g.g, &prev, 1, &dp);
+            if (e == hipSuccess) e = hipGraphAddEventRecordNode(&g.e1, g.g, &prev, 1, sl.ev1);
+            if (e == hipSuccess) e = hipGraphAddKernelNode(&g.drain, g.g, &g.e1, 1, &dp);
             if (e == hipSuccess) e = hipGraphInstantiate(&g.x, g.g, nullptr, nullptr, 0);
             if (e != hipSuccess) {
                 if (g.g) hipGraphDestroy(g.g);
@@ -684,7 +693,7 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fil
         HIPCHK(c, hipGraphLaunch(G->x, c->stream));
         c->wave_graph_replays++;
     }
-    HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
+    HIPCHK(c, hipEventRecord(sl.done, c->stream));
     return TFIM_OK;
 }
 
@@ -1563,7 +1572,7 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     } release{c, sl};
     // poll rather than block in the runtime, so that a submit from another
     // thread is not held up behind this wait
-    for (hipError_t q; (q = hipEventQuery(sl.ev1)) != hipSuccess;) {
+    for (hipError_t q; (q = hipEventQuery(sl.done)) != hipSuccess;) {
         if (q != hipErrorNotReady) HIPCHK(c, q);
         std::this_thread::yield();
     }
