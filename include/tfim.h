@@ -329,7 +329,10 @@ int tfim_prep_finish(tfim_ctx *ctx, const int32_t *dense_of, int32_t n_items, in
 #define TFIM_MLP_PARAMS 5250
 typedef struct tfim_mlp tfim_mlp;
 
-/* He-uniform weights from SplitMix64(seed), zero biases. */
+/* He-uniform weights from SplitMix64(seed), zero biases; the device scratch
+ * and pinned staging for batches and requests of up to 8,192 rows are
+ * allocated here, so training and inference on such batches allocate
+ * nothing. */
 int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out);
 void tfim_mlp_destroy(tfim_mlp *mlp);
 int tfim_mlp_get_params(tfim_mlp *mlp, float *params);
@@ -350,6 +353,10 @@ int tfim_mlp_gradients(tfim_mlp *mlp, const float *x, const float *labels, const
  * call returns once the inputs are copied (later MLP calls order after it). */
 int tfim_mlp_train_step(tfim_mlp *mlp, const float *x, const float *labels, const float *mask, int32_t n,
                         float lr, float *loss);
+/* `steps` such Adam steps on one batch, staged to the device once; with loss
+ * != NULL it waits and returns the loss the last step saw. */
+int tfim_mlp_train(tfim_mlp *mlp, const float *x, const float *labels, const float *mask, int32_t n, float lr,
+                   int32_t steps, float *loss);
 
 /* Waits for all work on the ctx stream. */
 int tfim_sync(tfim_ctx *ctx);

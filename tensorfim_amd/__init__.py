@@ -100,6 +100,8 @@ _SIGS = {
                                            ctypes.c_int32, ctypes.c_void_p, _p(ctypes.c_float)]),
     "tfim_mlp_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_int32, ctypes.c_float, _p(ctypes.c_float)]),
+    "tfim_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_int32, ctypes.c_float, ctypes.c_int32, _p(ctypes.c_float)]),
 }
 
 EXPORTED_SYMBOLS = sorted(_SIGS)
@@ -411,6 +413,14 @@ class Mlp:
         loss = ctypes.c_float()
         self.ctx._check(self.ctx._lib.tfim_mlp_train_step(self._h, _ptr(x), _ptr(y), _ptr(m), x.shape[0],
                                                           ctypes.c_float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def train(self, x, y, m, lr=1e-3, steps=1):
+        """`steps` Adam steps on one batch staged once; the last step's loss."""
+        x, y, m = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, m))
+        loss = ctypes.c_float()
+        self.ctx._check(self.ctx._lib.tfim_mlp_train(self._h, _ptr(x), _ptr(y), _ptr(m), x.shape[0],
+                                                     ctypes.c_float(lr), int(steps), ctypes.byref(loss)))
         return loss.value
 
 

@@ -20,6 +20,7 @@
 
 #include "dataset.h"
 #include "results.h"
+#include "policy.h"
 #include "search.h"
 #include "tfim.h"
 
@@ -154,8 +155,19 @@ int main(int argc, char **argv) {
 
     std::vector<Mfi> mfis;
     SearchStats st;
+    // the MAFIA engine's policy, made before the search (a learned policy
+    // creates its network here: device set-up, like loading the bitmaps)
+    std::unique_ptr<Policy> pol;
+    if (opt.engine == "mafia" && !db.duplicates && db.d_count > 0 && db.base_bits > 0) {
+        pol = make_policy(ctx, opt, err);
+        if (!pol) {
+            fprintf(stderr, "search failed: %s\n", err.c_str());
+            tfim_close(ctx);
+            return 1;
+        }
+    }
     const double c0 = now();
-    if (db.d_count > 0 && db.base_bits > 0) rc = run_search(ctx, db, opt, mfis, st, err);
+    if (db.d_count > 0 && db.base_bits > 0) rc = run_search(ctx, db, opt, mfis, st, err, pol.get());
     const double c1 = now();
     if (rc != TFIM_OK) {
         fprintf(stderr, "search failed: %s\n", err.c_str());
@@ -226,8 +238,10 @@ int main(int argc, char **argv) {
     printf("hut_success: %lld\nhut_skipped: %lld\npep_moves: %lld\nhutmfi_pruned: %lld\nleaf_subsumed: %lld\n",
            (long long)st.hut_success, (long long)st.hut_skipped, (long long)st.pep_moves,
            (long long)st.hutmfi_pruned, (long long)st.leaf_subsumed);
-    printf("counted_bitops: %.6g\nlearn_time(s): %g\nmlp_train_steps: %lld\nmodel_lookaheads: %lld\n",
-           st.counted_bitops, st.learn_seconds, (long long)st.mlp_train_steps, (long long)st.model_lookaheads);
+    printf("counted_bitops: %.6g\nlearn_time(s): %g\nmodel_setup_time(s): %g\nmlp_train_steps: %lld\n"
+           "model_lookaheads: %lld\n",
+           st.counted_bitops, st.learn_seconds, st.model_setup_seconds, (long long)st.mlp_train_steps,
+           (long long)st.model_lookaheads);
     printf("model_lookahead_hits: %lld\ngate_threshold: %g\nexplore_share: %g\nmodel_failures: %lld\n",
            (long long)st.model_lookahead_hits, st.gate_threshold, st.explore_share, (long long)st.model_failures);
     tfim_close(ctx);

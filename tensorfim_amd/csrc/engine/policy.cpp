@@ -64,7 +64,7 @@ class LearnedPolicy : public Policy {
             if (f) fclose(f);
         }
         if (!opt.trace_path.empty()) trace_ = fopen(opt.trace_path.c_str(), "wb");
-        extra_seconds_ += secs_since(t0);
+        setup_seconds_ = secs_since(t0);
     }
     ~LearnedPolicy() override {
         if (mlp_) tfim_mlp_destroy(mlp_);
@@ -75,11 +75,18 @@ class LearnedPolicy : public Policy {
         N_ = (double)std::max(1, db.trans_num_reduced);
         thr_ = std::max(1.0, db.thr_abs);
         freq_ = &db.freq_dense;
-        plain_.decide(ds, db);   // plain order and the independence-estimate rule
-        // features: one gate row per node with a tail of two or more (kept as
-        // a training record when its lookahead runs); the model is asked only
-        // about nodes the rule declines with tails of at most kMaxGateTail,
-        // plus the ranker rows of tails it reorders
+        // plain order and the independence-estimate rule, node by node
+        run_parallel((int64_t)ds.size(), [&](int64_t q) {
+            thread_local std::vector<Decision *> one(1);
+            one[0] = ds[q];
+            plain_.decide(one, db);
+        });
+        // Which nodes get feature rows: one gate row per node with a tail of
+        // two or more (kept as a training record when its lookahead runs);
+        // the model is asked only about nodes the rule declines with tails of
+        // at most kMaxGateTail, plus the ranker rows of tails it reorders.
+        // The draws are made here in node order; the rows are then filled in
+        // parallel (each node writes only its own rows).
         feats_.clear();
         infer_.clear();
         node_row_.assign(ds.size(), -1);
@@ -89,6 +96,7 @@ class LearnedPolicy : public Policy {
         const bool use_model = trained_ && mlp_;
         // gate-only: the model's verdicts are settled one wave later (resolve)
         const bool deferred = use_model && !opt_.rank;
+        int64_t rows = 0, n_infer = 0;
         for (size_t q = 0; q < ds.size(); q++) {
             Decision *d = ds[q];
             if (d->head.empty() || d->tail.size() < 2) continue;
@@ -108,25 +116,37 @@ class LearnedPolicy : public Policy {
             const bool ask = !d->lookahead && d->tail.size() <= kMaxGateTail && (ask_p_ >= 1.0f || rand01() < ask_p_);
             if (!need) need = d->lookahead ? rand01() < kRecordAccepted : ask;
             if (!need) continue;
-            node_features(*d);
-            node_row_[q] = (int64_t)(feats_.size() / NF) - 1;
-            if (use_model && ask) {
-                infer_row_[q] = (int64_t)(infer_.size() / NF);
-                infer_.insert(infer_.end(), feats_.end() - NF, feats_.end());
-            }
+            node_row_[q] = rows++;
+            if (use_model && ask) infer_row_[q] = n_infer++;
             if (items) {
-                // rank of each item in the plain (increasing support) order
-                rank_.assign(d->tail.size(), 0);
-                for (size_t j = 0; j < d->order.size(); j++) rank_[d->order[j]] = (int32_t)j;
-                item_row0_[q] = (int64_t)(feats_.size() / NF);
-                for (size_t i = 0; i < d->tail.size(); i++) item_features(*d, i);
+                item_row0_[q] = rows;
+                rows += (int64_t)d->tail.size();
                 if (use_model && opt_.rank) {
-                    const int64_t first = (int64_t)(infer_.size() / NF);
-                    infer_.insert(infer_.end(), feats_.end() - (ptrdiff_t)(d->tail.size() * NF), feats_.end());
-                    item_infer0_[q] = first;
+                    item_infer0_[q] = n_infer;
+                    n_infer += (int64_t)d->tail.size();
                 }
             }
         }
+        feats_.resize((size_t)(rows * NF));
+        infer_.resize((size_t)(n_infer * NF));
+        run_parallel((int64_t)ds.size(), [&](int64_t q) {
+            if (node_row_[q] < 0) return;
+            const Decision &d = *ds[q];
+            float *nf = &feats_[(size_t)(node_row_[q] * NF)];
+            const NodeAgg a = node_features(d, nf);
+            if (infer_row_[q] >= 0) std::copy_n(nf, NF, &infer_[(size_t)(infer_row_[q] * NF)]);
+            if (item_row0_[q] >= 0) {
+                // rank of each item in the plain (increasing support) order
+                thread_local std::vector<int32_t> rank;
+                rank.assign(d.tail.size(), 0);
+                for (size_t j = 0; j < d.order.size(); j++) rank[(size_t)d.order[j]] = (int32_t)j;
+                for (size_t i = 0; i < d.tail.size(); i++) {
+                    float *f = &feats_[(size_t)((item_row0_[q] + (int64_t)i) * NF)];
+                    item_features(d, a, rank[i], i, f);
+                    if (item_infer0_[q] >= 0) std::copy_n(f, NF, &infer_[(size_t)((item_infer0_[q] + (int64_t)i) * NF)]);
+                }
+            }
+        });
         if (feats_.empty()) return;
         const int64_t n = (int64_t)(infer_.size() / NF);
         if (deferred) {
@@ -192,6 +212,8 @@ class LearnedPolicy : public Policy {
         }
     }
 
+    void set_parallel(ParallelFor pf) override { par_ = std::move(pf); }
+
     void resolve(std::vector<Decision *> &ds) override {
         // ds: the decisions decide() left pending, in the order it saw them;
         // their feature rows are the rows of the queued inference
@@ -234,10 +256,11 @@ class LearnedPolicy : public Policy {
         st.gate_threshold = gate_p_;
         st.explore_share = explore_p_;
         st.model_failures = failures_;
-        st.learn_seconds += extra_seconds_;   // set-up and training (decide() is timed by the engine)
+        st.learn_seconds += extra_seconds_;   // training (decide() is timed by the engine)
+        st.model_setup_seconds = setup_seconds_;
         if (opt_.verbose)
-            fprintf(stderr, "learned policy: set-up+training %.4f s (%lld steps), forward %.4f s in %lld calls (%lld rows)\n",
-                    extra_seconds_, (long long)steps_, forward_seconds_, (long long)forward_calls_,
+            fprintf(stderr, "learned policy: set-up %.4f s, training %.4f s (%lld steps), forward %.4f s in %lld calls (%lld rows)\n",
+                    setup_seconds_, extra_seconds_, (long long)steps_, forward_seconds_, (long long)forward_calls_,
                     (long long)forward_rows_);
         if (mlp_ && !opt_.model_path.empty() && opt_.learned && trained_) {
             // keep what was learned for the next run on the same data
@@ -299,29 +322,38 @@ class LearnedPolicy : public Policy {
         a.log_min_s = std::log(std::max(1.0, min_s) / thr_);
         return a;
     }
-    void node_features(const Decision &d) {
+    // The gate features of a node into f[0 .. NF); returns the tail
+    // aggregates its item rows use.
+    NodeAgg node_features(const Decision &d, float *f) const {
         const NodeAgg a = agg(d);
         const double margin = std::log(a.s / thr_);
-        const float f[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
+        const float v[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
                              (float)(d.tail.size() / 16.0), (float)(10 * margin), (float)a.sum_log_r,
                              (float)(10 * (margin + a.sum_log_r)), (float)a.min_r, (float)a.mean_r, (float)a.max_r,
                              (float)a.frac_hi, (float)(d.depth / 16.0), (float)a.mean_freq, 0.0f};
-        feats_.insert(feats_.end(), f, f + NF);
-        cur_ = a;
+        std::copy_n(v, NF, f);
+        return a;
     }
-    void item_features(const Decision &d, size_t i) {
-        const NodeAgg &a = cur_;
+    // The ranker features of tail item i (rank: its place in the plain
+    // order) into f[0 .. NF).
+    void item_features(const Decision &d, const NodeAgg &a, int rank, size_t i, float *f) const {
         const double margin = std::log(a.s / thr_);
         const double r = std::max(1, d.tail_sup[i]) / a.s;
-        const int rank = rank_[i];
-        const float f[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
+        const float v[NF] = {(float)(a.s / N_), (float)(thr_ / a.s), (float)(d.head.size() / 16.0),
                              (float)(d.tail.size() / 16.0), (float)(10 * margin), (float)(10 * std::log(r)),
                              (float)(10 * std::log(std::max(1, d.tail_sup[i]) / thr_)),
                              (float)(rank / std::max<double>(1, d.tail.size() - 1)),
                              (float)((double)(*freq_)[d.tail[i]] / N_), (float)(r - a.mean_r),
                              (float)(10 * (margin + a.sum_log_r)), (float)(d.depth / 16.0),
                              (float)(10 * (margin + a.sum_log_r - std::log(r))), 1.0f};
-        feats_.insert(feats_.end(), f, f + NF);
+        std::copy_n(v, NF, f);
+    }
+    void run_parallel(int64_t n, const std::function<void(int64_t)> &f) {
+        if (par_) {
+            par_(n, f);
+        } else {
+            for (int64_t i = 0; i < n; i++) f(i);
+        }
     }
 
     // Records live in a ring of kRing slots; a label that arrives after its
@@ -435,15 +467,14 @@ class LearnedPolicy : public Policy {
         const int32_t take = (int32_t)(tx_.size() / NF);
         if (take == 0) return;
         float loss = 0;
-        for (int e = 0; e < kEpochs; e++) {
-            // the last step reports the loss (and waits); the others stay queued
-            float *lp = (e == kEpochs - 1 && opt_.verbose) ? &loss : nullptr;
-            if (tfim_mlp_train_step(mlp_, tx_.data(), ty_.data(), tm_.data(), take, 2e-3f, lp) != TFIM_OK) {
-                model_failed("training");
-                return;
-            }
-            steps_++;
+        // kEpochs steps on the batch, staged once and queued on the model's
+        // stream (with --verbose the last one reports its loss and waits)
+        if (tfim_mlp_train(mlp_, tx_.data(), ty_.data(), tm_.data(), take, 2e-3f, kEpochs,
+                           opt_.verbose ? &loss : nullptr) != TFIM_OK) {
+            model_failed("training");
+            return;
         }
+        steps_ += kEpochs;
         if (opt_.verbose) fprintf(stderr, "mlp train: %d records, loss %.4f\n", take, loss);
         trained_ = !disabled_;
     }
@@ -485,8 +516,7 @@ class LearnedPolicy : public Policy {
     int64_t failures_ = 0;
     double N_ = 1, thr_ = 1;
     const std::vector<int32_t> *freq_ = nullptr;
-    NodeAgg cur_{};
-    std::vector<int32_t> rank_;
+    ParallelFor par_;
     std::vector<int64_t> node_row_, item_row0_, infer_row_, item_infer0_;
     std::vector<float> infer_, pend_feats_;
     int64_t pend_n_ = 0;   // rows of the inference queued for pending decisions
@@ -499,7 +529,7 @@ class LearnedPolicy : public Policy {
     float ask_p_ = kAskMin;          // share of declined nodes the model is asked about (grows while it pays)
     int64_t expl_taken_ = 0, expl_success_ = 0;
     uint8_t gate_src_ = kSrcRule;
-    double extra_seconds_ = 0, forward_seconds_ = 0;
+    double extra_seconds_ = 0, forward_seconds_ = 0, setup_seconds_ = 0;
     int64_t forward_calls_ = 0, forward_rows_ = 0;
     std::vector<float> feats_, logits_;
     std::vector<float> rec_x_ = std::vector<float>((size_t)kRing * NF);

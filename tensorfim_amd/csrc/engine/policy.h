@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -51,6 +52,11 @@ class Policy {
     // true when decide() of one node depends on that node alone (no state,
     // no batch): the engine may then call it per node from several threads
     virtual bool decides_alone() const { return false; }
+    // A parallel loop the engine lends a batch policy: pf(n, f) calls f(i)
+    // for every i < n on the engine's host threads and returns when all are
+    // done; each call may touch only its own outputs.
+    using ParallelFor = std::function<void(int64_t, const std::function<void(int64_t)> &)>;
+    virtual void set_parallel(ParallelFor pf) { (void)pf; }
 };
 
 // MAFIA's plain policy: tail by increasing support (ties: ascending id).

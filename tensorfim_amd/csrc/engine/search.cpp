@@ -751,6 +751,7 @@ class Mafia {
 
     int run(std::vector<Mfi> &out, std::string &err) {
         const int32_t F = db_.n_surv;
+        pol_.set_parallel([this](int64_t n, const std::function<void(int64_t)> &f) { pool_.run(n, f); });
         cap_ = opt_.wave_cap > 0 ? opt_.wave_cap : auto_wave_cap(db_);
         flights_ = std::max(2, std::min(kFlights, opt_.flights));
         if (opt_.wave_cap <= 0) sizer_.reset(new WaveSizeTuner(cap_));
@@ -1589,7 +1590,7 @@ class Mafia {
 }  // namespace
 
 int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std::vector<Mfi> &out,
-               SearchStats &st, std::string &err) {
+               SearchStats &st, std::string &err, Policy *given) {
     if (opt.engine == "plain") return run_plain(ctx, db, opt.workers, out, st, err, opt.node_log);
     if (opt.engine == "wave") {
         const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
@@ -1605,11 +1606,15 @@ int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std:
             const int32_t cap = opt.wave_cap > 0 ? opt.wave_cap : auto_wave_cap(db);
             return run_wave(ctx, db, cap, out, st, err);
         }
-        std::unique_ptr<Policy> pol = make_policy(ctx, opt, err);
-        if (!pol) return TFIM_E_STATE;
-        Mafia m(ctx, db, opt, *pol, st);
+        std::unique_ptr<Policy> own;
+        if (!given) {
+            own = make_policy(ctx, opt, err);
+            if (!own) return TFIM_E_STATE;
+        }
+        Policy &pol = given ? *given : *own;
+        Mafia m(ctx, db, opt, pol, st);
         const int rc = m.run(out, err);
-        pol->finish(st);
+        pol.finish(st);
         tfim_wave_stats(ctx, nullptr, &st.lds_waves, nullptr, &st.kernel_ms);
         return rc;
     }

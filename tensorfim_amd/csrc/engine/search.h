@@ -65,7 +65,8 @@ struct SearchStats {
     double kernel_ms = 0;         // device time of count kernels (HIP events)
     int64_t lds_waves = 0;        // waves counted by the LDS-staged kernel
     int32_t wave_size = 0;        // mafia: rows per wave the wave-size tuner settled on (or the fixed cap)
-    double learn_seconds = 0;     // host time in the learned part
+    double learn_seconds = 0;     // host time in the learned part while mining
+    double model_setup_seconds = 0;   // creating the learned part's network (before mining starts)
     int64_t mlp_train_steps = 0;
     int64_t model_lookaheads = 0;  // lookaheads run because the model (not the rule) asked
     int64_t model_lookahead_hits = 0;   // of those (exploration excluded), the ones whose HUT was frequent
@@ -74,8 +75,13 @@ struct SearchStats {
     int64_t model_failures = 0;         // model calls that failed (the rule decided from then on)
 };
 
+class Policy;
+
 // Runs the engine; returns MFIs (dense ids, insertion order) with frequencies.
+// The MAFIA engine's search policy is `pol` when given (made beforehand with
+// make_policy, so that setting up a learned policy's network is not part of
+// the search), else one it makes itself.
 int run_search(tfim_ctx *ctx, const Database &db, const SearchOptions &opt, std::vector<Mfi> &out,
-               SearchStats &stats, std::string &err);
+               SearchStats &stats, std::string &err, Policy *pol = nullptr);
 
 }  // namespace tfim

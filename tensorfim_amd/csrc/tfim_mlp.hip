@@ -77,15 +77,16 @@ int ensure(tfim_mlp *m, int64_t n, Scratch &sc) {
         if (e_ != hipSuccess) return ctx_fail((m)->ctx, TFIM_E_HIP, hipGetErrorString(e_)); \
     } while (0)
 
-// forward+backward into scratch; returns the loss and leaves G filled
-int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float *loss, Scratch &sc) {
+// Stages a training batch in pinned memory and queues its copy to the
+// device scratch (so a queued step never reads the caller's buffers; the
+// previous batch must have left the staging first).  Returns 1/count of the
+// masked outputs in *inv.
+int stage_batch(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, Scratch &sc, float *inv) {
     if (int r = ensure(m, n, sc)) return r;
     hipStream_t s = m->stream;
     double cnt = 0;
     for (int64_t i = 0; i < (int64_t)n * OUT; i++) cnt += mask[i];
-    const float inv = (float)(1.0 / std::max(1.0, cnt));
-    // stage the batch in pinned memory so that a queued step never reads the
-    // caller's buffers; the previous batch must have left the staging first
+    *inv = (float)(1.0 / std::max(1.0, cnt));
     const int64_t per = IN + 2 * OUT;
     if (m->t_cap > 0) MCHK(m, hipEventSynchronize(m->staged));
     if (n > m->t_cap) {
@@ -103,6 +104,13 @@ int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int3
     MCHK(m, hipMemcpyAsync(sc.Y, m->h_t + (size_t)n * IN, (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
     MCHK(m, hipMemcpyAsync(sc.M, m->h_t + (size_t)n * (IN + OUT), (size_t)n * OUT * 4, hipMemcpyHostToDevice, s));
     MCHK(m, hipEventRecord(m->staged, s));
+    return TFIM_OK;
+}
+
+// Forward+backward of the staged batch into scratch: leaves G filled; with
+// `loss`, waits and returns the batch's summed loss.
+int fwd_bwd_staged(tfim_mlp *m, int32_t n, float inv, float *loss, Scratch &sc) {
+    hipStream_t s = m->stream;
     launch_mlp_fwd_bwd(m->P, sc.X, sc.Y, sc.M, n, inv, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, sc.L, s);
     launch_mlp_grad(sc.X, sc.H1, sc.H2, sc.D1, sc.D2, sc.DY, n, sc.part, m->G, s);
     MCHK(m, hipGetLastError());
@@ -113,6 +121,39 @@ int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int3
         double t = 0;
         for (float v : lv) t += v;
         *loss = (float)t;
+    }
+    return TFIM_OK;
+}
+
+// forward+backward into scratch; returns the loss and leaves G filled
+int fwd_bwd(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float *loss, Scratch &sc) {
+    float inv = 0;
+    if (int r = stage_batch(m, x, y, mask, n, sc, &inv)) return r;
+    return fwd_bwd_staged(m, n, inv, loss, sc);
+}
+
+// Room for batches and inference requests of n rows: device scratch and
+// pinned staging, allocated once up front.
+int reserve(tfim_mlp *m, int64_t n) {
+    Scratch sc;
+    if (int r = ensure(m, n, sc)) return r;
+    if (n > m->t_cap) {
+        hipHostFree(m->h_t);
+        m->h_t = nullptr;
+        m->t_cap = 0;
+        if (hipHostMalloc(&m->h_t, (size_t)n * (IN + 2 * OUT) * 4) != hipSuccess)
+            return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp pinned staging allocation failed");
+        m->t_cap = n;
+    }
+    if (n > m->h_cap) {
+        hipHostFree(m->h_x);
+        hipHostFree(m->h_o);
+        m->h_x = m->h_o = nullptr;
+        m->h_cap = 0;
+        if (hipHostMalloc(&m->h_x, (size_t)n * IN * 4) != hipSuccess ||
+            hipHostMalloc(&m->h_o, (size_t)n * OUT * 4) != hipSuccess)
+            return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp pinned staging allocation failed");
+        m->h_cap = n;
     }
     return TFIM_OK;
 }
@@ -147,6 +188,12 @@ int tfim_mlp_create(tfim_ctx *ctx, uint64_t seed, tfim_mlp **out) {
     init(H * IN + H, H, H);               // W2
     init(H * IN + H + H * H + H, OUT, H); // W3
     if (int r = tfim_mlp_set_params(m, p.data())) {
+        tfim_mlp_destroy(m);
+        return r;
+    }
+    // the engine's batches and inference requests (up to 8,192 rows) then
+    // allocate nothing while it mines
+    if (int r = reserve(m, 8192)) {
         tfim_mlp_destroy(m);
         return r;
     }
@@ -241,20 +288,30 @@ int tfim_mlp_gradients(tfim_mlp *m, const float *x, const float *y, const float 
     return TFIM_OK;
 }
 
+int tfim_mlp_train(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float lr,
+                   int32_t steps, float *loss) {
+    if (!m || !x || !y || !mask || n < 1 || steps < 1) return TFIM_E_INVALID;
+    Scratch sc;
+    float inv = 0;
+    if (int r = stage_batch(m, x, y, mask, n, sc, &inv)) return r;
+    hipStream_t s = m->stream;
+    for (int32_t k = 0; k < steps; k++) {
+        // the last step reports the loss when asked (before its update)
+        if (int r = fwd_bwd_staged(m, n, inv, k + 1 == steps ? loss : nullptr, sc)) return r;
+        m->step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = (float)(1.0 - std::pow(0.9, (double)m->step));
+        const float c2 = (float)(1.0 - std::pow(0.999, (double)m->step));
+        launch_mlp_adam(m->P, m->G, m->Mo, m->Vo, m->Pb, lr, b1, b2, 1e-8f, c1, c2, s);
+        MCHK(m, hipGetLastError());
+    }
+    if (loss) MCHK(m, hipStreamSynchronize(s));   // without a loss request the steps stay queued
+    return TFIM_OK;
+}
+
 int tfim_mlp_train_step(tfim_mlp *m, const float *x, const float *y, const float *mask, int32_t n, float lr,
                         float *loss) {
-    if (!m || !x || !y || !mask || n < 1) return TFIM_E_INVALID;
-    Scratch sc;
-    if (int r = fwd_bwd(m, x, y, mask, n, loss, sc)) return r;
-    m->step++;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float c1 = (float)(1.0 - std::pow(0.9, (double)m->step));
-    const float c2 = (float)(1.0 - std::pow(0.999, (double)m->step));
-    hipStream_t s = m->stream;
-    launch_mlp_adam(m->P, m->G, m->Mo, m->Vo, m->Pb, lr, b1, b2, 1e-8f, c1, c2, s);
-    MCHK(m, hipGetLastError());
-    if (loss) MCHK(m, hipStreamSynchronize(s));   // without a loss request the step stays queued
-    return TFIM_OK;
+    return tfim_mlp_train(m, x, y, mask, n, lr, 1, loss);
 }
 
 }  // extern "C"

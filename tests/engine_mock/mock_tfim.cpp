@@ -81,4 +81,5 @@ int tfim_mlp_forward(tfim_mlp *, const float *, int32_t, float *, int32_t) { ret
 int tfim_mlp_forward_submit(tfim_mlp *, const float *, int32_t, int32_t) { return -2; }
 int tfim_mlp_forward_wait(tfim_mlp *, float *) { return -2; }
 int tfim_mlp_train_step(tfim_mlp *, const float *, const float *, const float *, int32_t, float, float *) { return -2; }
+int tfim_mlp_train(tfim_mlp *, const float *, const float *, const float *, int32_t, float, int32_t, float *) { return -2; }
 }

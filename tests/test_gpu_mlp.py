@@ -93,3 +93,19 @@ def test_forward_submit_wait_orders_with_training(ctx):
     with pytest.raises(tf.TfimError):
         mlp.forward_wait()                            # nothing outstanding
     mlp.close()
+
+
+def test_multi_step_training_equals_single_steps(ctx):
+    """tfim_mlp_train(steps=4), which stages the batch once, leaves the same
+    parameters as four tfim_mlp_train_step calls on that batch (same kernels
+    in the same order), and reports the loss the last step saw."""
+    rng = np.random.default_rng(21)
+    X, Y, M = _batch(rng, 3000)
+    a, b = tf.Mlp(ctx, seed=4), tf.Mlp(ctx, seed=4)
+    for _ in range(4):
+        la = a.train_step(X, Y, M, lr=5e-3)
+    lb = b.train(X, Y, M, lr=5e-3, steps=4)
+    np.testing.assert_array_equal(a.params(), b.params())
+    assert la == lb
+    a.close()
+    b.close()
