@@ -729,6 +729,39 @@ __device__ __forceinline__ void mask_group(const uint4 (&m)[RW], const uint4 *ld
     }
 }
 
+// The last G (1 .. 7) columns of a tile: counted exactly, reduced with the
+// 4- or 8-column row reduction over zeros, one LDS atomic per (row of lanes,
+// column) into out[0 .. G).
+template <int RW, int G>
+__device__ __forceinline__ void mask_tail(const uint4 (&m)[RW], const uint4 *lds, uint64_t &cm, int lane,
+                                          int *out) {
+    uint32_t v[G];
+    mask_group<RW, G>(m, lds, cm, G, 0, lane, v);
+    if (G <= 4) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int q = 0; q < G; q++) w[q] = v[q];
+        const uint32_t r = row_reduce4_packed(w);
+        if ((lane & 7) == 0) {
+            const int q0 = (lane & 8) ? 2 : 0;
+            const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
+            if (q0 < G && lo) atomicAdd(&out[q0], lo);
+            if (q0 + 1 < G && hi) atomicAdd(&out[q0 + 1], hi);
+        }
+    } else {
+        uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int q = 0; q < G; q++) w[q] = v[q];
+        const uint32_t r = row_reduce8_packed(w);
+        if ((lane & 3) == 0) {
+            const int q0 = 2 * ((lane >> 2) & 3);
+            const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
+            if (q0 < G && lo) atomicAdd(&out[q0], lo);
+            if (q0 + 1 < G && hi) atomicAdd(&out[q0 + 1], hi);
+        }
+    }
+}
+
 template <int RW>
 __global__ __launch_bounds__(kLdsThreads) void k_count_wave_mask(const uint4 *__restrict__ bm, int64_t pitch4,
                                                                  WaveMaskDesc d, int64_t n_chunks) {
@@ -830,27 +863,17 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_mask(const uint4 *__
                     if (hi) atomicAdd(&acc[c0 + 1], hi);
                 }
             }
-            const int rem = nc - j;   // 0 .. 7, wave-uniform
-            if (rem > 4) {
-                uint32_t v[8];
-                mask_group<RW, 8>(m, lds, cm, rem, last, lane, v);
-                const uint32_t r = row_reduce8_packed(v);
-                if ((lane & 3) == 0) {
-                    const int q0 = 2 * ((lane >> 2) & 3);
-                    const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
-                    if (q0 < rem && lo) atomicAdd(&acc[ob + j + q0], lo);
-                    if (q0 + 1 < rem && hi) atomicAdd(&acc[ob + j + q0 + 1], hi);
-                }
-            } else if (rem > 0) {
-                uint32_t v[4];
-                mask_group<RW, 4>(m, lds, cm, rem, last, lane, v);
-                const uint32_t r = row_reduce4_packed(v);
-                if ((lane & 7) == 0) {
-                    const int q0 = (lane & 8) ? 2 : 0;
-                    const int lo = (int)(r & 0xFFFFu), hi = (int)(r >> 16);
-                    if (q0 < rem && lo) atomicAdd(&acc[ob + j + q0], lo);
-                    if (q0 + 1 < rem && hi) atomicAdd(&acc[ob + j + q0 + 1], hi);
-                }
+            // the last 1 .. 7 columns: a group of exactly that many (no
+            // padding columns counted), reduced as a group of 4 or 8
+            switch (nc - j) {
+                case 1: mask_tail<RW, 1>(m, lds, cm, lane, acc + ob + j); break;
+                case 2: mask_tail<RW, 2>(m, lds, cm, lane, acc + ob + j); break;
+                case 3: mask_tail<RW, 3>(m, lds, cm, lane, acc + ob + j); break;
+                case 4: mask_tail<RW, 4>(m, lds, cm, lane, acc + ob + j); break;
+                case 5: mask_tail<RW, 5>(m, lds, cm, lane, acc + ob + j); break;
+                case 6: mask_tail<RW, 6>(m, lds, cm, lane, acc + ob + j); break;
+                case 7: mask_tail<RW, 7>(m, lds, cm, lane, acc + ob + j); break;
+                default: break;
             }
             t = tn;
             tl = nx;

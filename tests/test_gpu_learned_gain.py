@@ -1,18 +1,28 @@
-"""What the trained network gains over the search without it.
+"""What the trained network changes against the engine it is added to.
 
-The MAFIA-style engine without a model runs the classical head-union-tail
-lookahead at every node with a tail of two or more (``--gate always``).  With
-``--learned`` the MLP is trained online on the GPU from the engine's own
-lookahead outcomes and decides which lookaheads to run (an analytic
-independence estimate proposes, the model adds, a small exploration share
-keeps labels coming; the gate threshold and the exploration share calibrate
-themselves from the outcomes of the lookaheads they add).  Every decision is confirmed by exact counts, so the MFI set
-must not change; what changes is the number of counted rows, mostly the
-lookaheads that fail.
+The MAFIA-style engine decides, per node, whether to spend a head-union-tail
+(HUT) lookahead row.  Without the network an analytic rule decides
+(``--gate indep``: the support head+tail would have if its items were
+independent given the head); ``--gate always`` is MAFIA's classical
+lookahead at every node with a tail of two or more.  With ``--learned`` the
+MLP is trained online on the GPU from the engine's own lookahead outcomes
+and adds the lookaheads it expects to succeed to the rule's (a small
+exploration share keeps labels coming; the threshold and the share calibrate
+themselves).  Every decision is confirmed by exact counts, so the MFI set
+never changes; only the counted rows do.
 
-The dense replicated workload (49,046 base transactions x 16, support 0.85)
-is the measured case.  The analytic rule alone (``--gate indep``) is run
-beside it and reported, so the model's share of the gain is visible.
+Measured result, stated plainly: the network does not beat the rule.  On
+the dense replicated workload every lookahead the rule takes succeeds, and
+``--gate always`` -- every lookahead there is -- finds only about 3 % more
+successes than the rule at the cost of ~40,000 failed rows; a perfect gate
+could therefore save at most those few successes' subtrees, about 1 % of the
+rows.  The network's additions mostly fail (it calibrates its threshold to
+the ceiling) and it ends within a few tenths of a percent of the rule's rows.
+The tests below assert exactly that: the output is unchanged, the network is
+trained on the GPU, the headroom any gate has over the rule is that small,
+and the learned engine stays within 1 % of the rule's rows.  Against
+``--gate always`` it runs far fewer failing lookaheads, which is the rule's
+saving that it keeps, not a gain of the network.
 """
 import subprocess
 
@@ -32,7 +42,7 @@ def _run(data, tmp_path, name, extra):
     return tf.load_results(path), st
 
 
-def test_learned_gate_gain_dense_replicated(tmp_path):
+def test_learned_gate_against_the_rule_dense_replicated(tmp_path):
     data = tmp_path / "dense.txt"
     subprocess.run([tf.binary("tfim_gen"), "dense-replicated", str(data)], check=True)
     classic, sc = _run(data, tmp_path, "always", ["--gate", "always"])
@@ -40,21 +50,29 @@ def test_learned_gate_gain_dense_replicated(tmp_path):
     rule, sr = _run(data, tmp_path, "indep", ["--gate", "indep"])
     assert learned == classic == rule
     assert int(sl["mlp_train_steps"]) > 0, "the network was never trained"
+    rows = lambda st: int(st["rows"])
+    ok = lambda st: int(st["hut_success"])
     failed = lambda st: int(st["hut_rows"]) - int(st["hut_success"])
-    # the trained gate runs far fewer lookaheads that fail ...
+    # the engine the network is added to: the rule's lookaheads all succeed,
+    # and no gate can find many more successes than it does (a bound on what
+    # any learned gate could save) ...
+    assert failed(sr) == 0
+    assert ok(sc) - ok(sr) <= 0.05 * ok(sr), (ok(sc), ok(sr))
+    # ... and the learned engine stays within 1 % of the rule's rows
+    assert rows(sl) <= 1.01 * rows(sr), (rows(sl), rows(sr))
+    # against the classical lookahead it keeps the rule's saving: far fewer
+    # failed lookaheads, nearly every success, fewer rows
     assert failed(sl) * 4 <= failed(sc), (failed(sl), failed(sc))
-    # ... keeps nearly every lookahead that succeeds ...
-    assert int(sl["hut_success"]) >= 0.9 * int(sc["hut_success"])
-    # ... and counts fewer rows in total
-    assert int(sl["rows"]) < int(sc["rows"])
+    assert ok(sl) >= 0.9 * ok(sc)
+    assert rows(sl) < rows(sc)
 
 
 def test_learned_policy_trains_on_gene(tmp_path):
     """The wide gene-expression-like workload (1,100 x 20,000, support 0.12)
-    has few lookahead decisions and none of them succeeds: the network must
-    still be trained online (gate labels from the warm-up sample, ranker
-    labels from sampled nodes' children), and the output must equal the
-    search without it."""
+    has few lookahead decisions and none of them succeeds under any gate: the
+    network must still be trained online (gate labels from the warm-up
+    sample, ranker labels from sampled nodes' children), and the output must
+    equal the search without it."""
     data = tmp_path / "gene.txt"
     subprocess.run([tf.binary("tfim_gen"), "gene", str(data)], check=True)
     res = {}
@@ -66,6 +84,7 @@ def test_learned_policy_trains_on_gene(tmp_path):
         res[name] = (tf.load_results(path), st)
     assert res["learned"][0] == res["indep"][0] == res["always"][0]
     assert int(res["learned"][1]["mlp_train_steps"]) > 0, "the network was never trained"
+    assert int(res["always"][1]["hut_success"]) == 0   # no lookahead here can succeed
     # the trained gate looks ahead far less than the classical rule, whose
     # lookaheads all fail here
     assert int(res["learned"][1]["hut_rows"]) * 2 <= int(res["always"][1]["hut_rows"])
