@@ -347,9 +347,12 @@ def bench_e2e(args):
                      "rows": int(st["rows"]), "waves": int(st["waves"]), "mfis": int(st["MFIsNumber"]),
                      "kernel_ms": float(st.get("kernel_ms", 0)), "counted_bitops": float(st["counted_bitops"])}
         for k in ("hut_rows", "hut_success", "model_lookaheads", "mlp_train_steps", "graph_launches", "lds_waves",
-                  "l2_waves", "mfma_waves", "wave_size", "spec_hits"):
+                  "l2_waves", "mfma_waves", "fused_waves", "wave_size", "spec_hits"):
             if k in st:
                 out[name][k] = int(st[k])
+        for k in ("learn_time(s)", "model_setup_time(s)", "process_time(s)", "build_time(s)"):
+            if k in st:
+                out[name][k.replace("(s)", "_s")] = float(st[k])
         if out[name]["kernel_ms"] > 0:
             # the batched count kernels against the AND + popcount VALU roof
             tb = out[name]["counted_bitops"] / (out[name]["kernel_ms"] * 1e-3) / 1e12
@@ -408,9 +411,13 @@ def bench_e2e_wide(args):
                         "waves": int(st["waves"]), "mfis": len(got), "kernel_ms": float(st.get("kernel_ms", 0)),
                         "counted_bitops": float(st["counted_bitops"]),
                         "matches_golden": mfi_digest(got) == fx["digest"]}
-            for k in ("l2_waves", "mfma_waves", "wave_size", "mlp_train_steps"):
+            for k in ("l2_waves", "mfma_waves", "fused_waves", "wave_size", "mlp_train_steps", "hut_rows",
+                      "hut_success"):
                 if k in st:
                     out[run][k] = int(st[k])
+            for k in ("learn_time(s)", "model_setup_time(s)", "process_time(s)"):
+                if k in st:
+                    out[run][k.replace("(s)", "_s")] = float(st[k])
             if out[run]["kernel_ms"] > 0:
                 # listed (row, column) pairs' bit-ops over the count kernels' time
                 out[run]["count_kernels_tbitops"] = out[run]["counted_bitops"] / (out[run]["kernel_ms"] * 1e-3) / 1e12

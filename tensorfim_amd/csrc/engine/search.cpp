@@ -1275,6 +1275,9 @@ class Mafia {
             }
             pre_kept_[r] = kept;
             pre_pep_[r] = pep;
+            // a SPEC row whose tail died is settled by its own counts below
+            // (maximal, or not: no check against the found MFIs is needed)
+            if (spec && d.tail.empty()) return;
             // head + frequent tail: the leaf head when the tail is empty,
             // else the node's head-union-tail
             thread_local std::vector<int32_t> q;
