@@ -545,19 +545,32 @@ class MfiIndex {
         return any_and(sl.data(), sl.size(), (size_t)(n0 >> 6), words_);
     }
     void add(const std::vector<int32_t> &x) {
-        if (n_ % 64 == 0) {
-            words_++;
-            for (auto &b : bits_) b.push_back(0ull);
-        }
+        grow();
         for (int32_t i : x) {
             bits_[i][n_ >> 6] |= 1ull << (n_ & 63);
             cnt_[i]++;
         }
         n_++;
     }
+    // The same for a set of items < 64 given as a mask.
+    void add_mask(uint64_t m) {
+        grow();
+        for (; m; m &= m - 1) {
+            const int i = __builtin_ctzll(m);
+            bits_[(size_t)i][n_ >> 6] |= 1ull << (n_ & 63);
+            cnt_[(size_t)i]++;
+        }
+        n_++;
+    }
     int64_t size() const { return n_; }
 
    private:
+    void grow() {
+        if (n_ % 64 == 0) {
+            words_++;
+            for (auto &b : bits_) b.push_back(0ull);
+        }
+    }
     std::vector<std::vector<uint64_t>> bits_;
     std::vector<int64_t> cnt_;   // MFIs containing each item
     size_t words_ = 0;
@@ -843,6 +856,19 @@ class Mafia {
                 if (int rc = launch(fl_[cur], false, err)) return rc;
         }
         st_.wave_size = sizer_ ? sizer_->chosen() : cap_;
+        // the MFIs kept as masks, as id lists (largest id first), in parallel
+        const size_t o0 = out.size();
+        out.resize(o0 + out_masks_.size());
+        pool_.run((int64_t)out_masks_.size(), [&](int64_t i) {
+            Mfi &m = out[o0 + (size_t)i];
+            for (uint64_t b = out_masks_[(size_t)i].first; b;) {
+                const int32_t top = 63 - __builtin_clzll(b);
+                m.items.push_back(top);
+                b &= ~(1ull << top);
+            }
+            m.frequency = out_masks_[(size_t)i].second;
+        });
+        out_masks_.clear();
         return TFIM_OK;
     }
 
@@ -1084,14 +1110,21 @@ class Mafia {
     }
 
     void emit(std::vector<int32_t> &&items, int32_t sup, std::vector<Mfi> &out) {
-        std::sort(items.begin(), items.end(), std::greater<int32_t>());   // insertion order: root first
-        if (db_.n_surv <= 64) {   // the set itself as a 64-bit key
+        if (db_.n_surv <= 64) {
+            // the set as a 64-bit mask: deduplicated, indexed and kept as is;
+            // its id list (largest id first) is made once, after the search
             uint64_t key = 0;
             for (int32_t i : items) key |= 1ull << i;
             if (!seen64_.insert(key)) return;
-        } else if (!seen_.insert(items).second) {
+            index_.add_mask(key);
+            st_.mfi_total++;
+            const int32_t top = 63 - __builtin_clzll(key);
+            if (db_.is_root[top])
+                out_masks_.push_back({key, __builtin_popcountll(key) == 1 ? db_.freq_dense[top] : sup});
             return;
         }
+        std::sort(items.begin(), items.end(), std::greater<int32_t>());   // insertion order: root first
+        if (!seen_.insert(items).second) return;
         index_.add(items);
         st_.mfi_total++;
         if (db_.is_root[items[0]]) {
@@ -1539,6 +1572,7 @@ class Mafia {
     std::vector<int32_t> spec_other_;
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
     KeySet64 seen64_;
+    std::vector<std::pair<uint64_t, int32_t>> out_masks_;   // reported MFIs of <= 64 survivors (mask, frequency)
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;
     std::vector<Decision> dec_;   // decisions of the wave being processed (first n_dec_)
