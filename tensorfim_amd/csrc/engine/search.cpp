@@ -552,15 +552,30 @@ class MfiIndex {
         }
         n_++;
     }
-    // The same for a set of items < 64 given as a mask.
-    void add_mask(uint64_t m) {
-        grow();
-        for (; m; m &= m - 1) {
-            const int i = __builtin_ctzll(m);
-            bits_[(size_t)i][n_ >> 6] |= 1ull << (n_ & 63);
-            cnt_[(size_t)i]++;
+    // The sets keys[0 .. n) of items < 64 given as masks, numbered in that
+    // order.  The bits of the MFIs that share an index word are gathered per
+    // item first, so each item's word is written once per 64 MFIs.
+    void add_masks(const uint64_t *keys, size_t n) {
+        uint64_t col[64];
+        for (size_t j = 0; j < n;) {
+            grow();
+            const size_t w = (size_t)(n_ >> 6);
+            const size_t take = std::min(n - j, (size_t)(64 - (n_ & 63)));
+            uint64_t any = 0;
+            for (size_t q = 0; q < take; q++) any |= keys[j + q];
+            for (uint64_t m = any; m; m &= m - 1) col[__builtin_ctzll(m)] = 0;
+            for (size_t q = 0; q < take; q++) {
+                const uint64_t bit = 1ull << ((n_ + (int64_t)q) & 63);
+                for (uint64_t m = keys[j + q]; m; m &= m - 1) col[__builtin_ctzll(m)] |= bit;
+            }
+            for (uint64_t m = any; m; m &= m - 1) {
+                const int i = __builtin_ctzll(m);
+                bits_[(size_t)i][w] |= col[i];
+                cnt_[(size_t)i] += __builtin_popcountll(col[i]);
+            }
+            n_ += (int64_t)take;
+            j += take;
         }
-        n_++;
     }
     int64_t size() const { return n_; }
 
@@ -718,6 +733,26 @@ class KeySet64 {
         n_++;
         return true;
     }
+    // Room for `more` keys beyond those inserted (at most half full then),
+    // so that insert_shared never has to grow the table.
+    void reserve(size_t more) {
+        while ((n_ + more) * 2 > slots_.size()) grow();
+    }
+    // insert() for several threads at once, after reserve(): an empty slot
+    // is claimed by compare-and-swap.  The caller reports how many keys it
+    // inserted with added() once the threads are done.
+    bool insert_shared(uint64_t key) {
+        const size_t mask = slots_.size() - 1;
+        for (size_t i = mix(key) & mask;; i = (i + 1) & mask) {
+            uint64_t cur = __atomic_load_n(&slots_[i], __ATOMIC_ACQUIRE);
+            if (cur == 0 &&
+                __atomic_compare_exchange_n(&slots_[i], &cur, key, false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE))
+                return true;
+            if (cur == key) return false;   // there, or just put there by another thread
+        }
+    }
+    void added(size_t k) { n_ += k; }
+    size_t size() const { return n_; }
 
    private:
     static uint64_t mix(uint64_t z) {
@@ -774,6 +809,7 @@ class Mafia {
         compl_ = opt_.compl_rows == "on" || (opt_.compl_rows == "auto" && !lds_shape(db_));
         // rows as bit masks where every dense id fits one word
         mask_rows_ = !compl_ && db_.d_count <= 64 && opt_.rows != "lists";
+        keys64_ = F <= 64;
         min_count_ = min_count_of(db_);
         spec_ = opt_.spec != "off";
         excl_words_ = (F + 63) / 64;
@@ -857,16 +893,24 @@ class Mafia {
         }
         st_.wave_size = sizer_ ? sizer_->chosen() : cap_;
         // the MFIs kept as masks, as id lists (largest id first), in parallel
+        // (chunks of 1,024 sets: one id list costs about one allocation)
+        constexpr int64_t kChunk = 1024;
         const size_t o0 = out.size();
-        out.resize(o0 + out_masks_.size());
-        pool_.run((int64_t)out_masks_.size(), [&](int64_t i) {
-            Mfi &m = out[o0 + (size_t)i];
-            for (uint64_t b = out_masks_[(size_t)i].first; b;) {
-                const int32_t top = 63 - __builtin_clzll(b);
-                m.items.push_back(top);
-                b &= ~(1ull << top);
+        const int64_t n_masks = (int64_t)out_masks_.size();
+        out.resize(o0 + (size_t)n_masks);
+        pool_.run((n_masks + kChunk - 1) / kChunk, [&](int64_t c) {
+            for (int64_t i = c * kChunk; i < std::min(n_masks, (c + 1) * kChunk); i++) {
+                Mfi &m = out[o0 + (size_t)i];
+                const uint64_t key = out_masks_[(size_t)i].first;
+                m.items.resize((size_t)__builtin_popcountll(key));
+                size_t k = 0;
+                for (uint64_t b = key; b;) {
+                    const int32_t top = 63 - __builtin_clzll(b);
+                    m.items[k++] = top;
+                    b &= ~(1ull << top);
+                }
+                m.frequency = out_masks_[(size_t)i].second;
             }
-            m.frequency = out_masks_[(size_t)i].second;
         });
         out_masks_.clear();
         return TFIM_OK;
@@ -1109,14 +1153,17 @@ class Mafia {
         return wb_.submit(ctx_, f.st, db_.n_bits(), min_count_, err);
     }
 
-    void emit(std::vector<int32_t> &&items, int32_t sup, std::vector<Mfi> &out) {
-        if (db_.n_surv <= 64) {
-            // the set as a 64-bit mask: deduplicated, indexed and kept as is;
-            // its id list (largest id first) is made once, after the search
-            uint64_t key = 0;
-            for (int32_t i : items) key |= 1ull << i;
-            if (!seen64_.insert(key)) return;
-            index_.add_mask(key);
+    // An MFI found by wave row r: with at most 64 survivors the set is the
+    // 64-bit key the row claimed on the host pool (none when another row
+    // found the same set first), kept as is and indexed with the wave's
+    // others after the row loop; its id list (largest id first) is made
+    // once, after the search.  Else the id list is deduplicated and indexed
+    // here.
+    void emit(int r, std::vector<int32_t> &&items, int32_t sup, std::vector<Mfi> &out) {
+        if (keys64_) {
+            const uint64_t key = emit_key_[(size_t)r];
+            if (!key) return;
+            new_keys_.push_back(key);
             st_.mfi_total++;
             const int32_t top = 63 - __builtin_clzll(key);
             if (db_.is_root[top])
@@ -1140,16 +1187,17 @@ class Mafia {
     // answers (and checks any MFI found after them).
     void check_leaves(std::vector<Decision *> &ds) {
         const int64_t n_now = index_.size();
-        pool_.run((int64_t)ds.size(), [&](int64_t i) {
-            Decision &d = *ds[i];
-            d.leaf_sub = -1;
-            if (d.order.empty() || (d.lookahead && d.tail.size() >= 2)) return;   // a HUT row goes first
-            thread_local std::vector<int32_t> leaf;
-            leaf.assign(d.head.begin(), d.head.end());
-            leaf.push_back(d.tail[d.order.back()]);
-            d.leaf_sub = index_.subsumed(leaf) ? 1 : 0;
-            d.leaf_n = n_now;
-        });
+        pool_.run((int64_t)ds.size(), [&](int64_t i) { check_leaf(*ds[i], n_now); });
+    }
+    // One decision's leaf check against the n_now MFIs found so far.
+    void check_leaf(Decision &d, int64_t n_now) const {
+        d.leaf_sub = -1;
+        if (d.order.empty() || (d.lookahead && d.tail.size() >= 2)) return;   // a HUT row goes first
+        thread_local std::vector<int32_t> leaf;
+        leaf.assign(d.head.begin(), d.head.end());
+        leaf.push_back(d.tail[d.order.back()]);
+        d.leaf_sub = index_.subsumed(leaf) ? 1 : 0;
+        d.leaf_n = n_now;
     }
 
     // The children of d (child i: head + t_i with tail t_{i+1..}), in stacking
@@ -1238,42 +1286,81 @@ class Mafia {
         for (size_t i = 0; i < m; i++) stack_.push_back(std::move(kids_[i]));
     }
 
+    // Outcomes of a row that settles itself, found on the host pool from its
+    // own counts: it found an MFI; a HUT row is frequent.
+    enum : uint8_t { kFound = 1, kHutFreq = 2 };
+
     int process_wave(Flight &f, std::vector<Mfi> &out) {
         settle_undecided();
         std::vector<MNode> &wave_ = f.wave;
         WaveBatch &wb_ = f.wb;
-        // this wave's decisions: dec_[0 .. n_dec_), slots reused across waves
-        // (their vectors keep their storage)
-        n_dec_ = 0;
-        // Subsumption checks of the EXPAND rows, in parallel against the MFIs
-        // found so far.  An EXPAND row checks head + every frequent tail item
-        // (the leaf head after PEP when no other item stays frequent, else
-        // its HUT); the loop below adds a check against the MFIs that rows
-        // before it emit, so decisions equal a row-by-row pass.
+        const size_t rows = wave_.size();
         // SPEC rows counted against explicit complement columns: the tail's
         // counts read out of the complement's, how many frequent columns lie
         // outside the tail, and the infrequent ones (known for the subtree)
-        spec_tres_.resize(wave_.size());
-        spec_other_.assign(wave_.size(), 0);
+        spec_tres_.resize(rows);
+        spec_other_.assign(rows, 0);
         const int64_t n_start = index_.size();
-        pre_sub_.assign(wave_.size(), 0);
-        // Per row, in parallel: the SPEC preparation above (the row's columns
-        // are the survivors outside head and known, ascending, so a column's
-        // index is a prefix popcount of that live mask), then the
-        // subsumption check of the rows that expand.
-        // The rows that expand also build their decision here (frequent
-        // tail, PEP moves into the head, infrequent items into known): it
-        // depends on the row alone.  What depends on the order of the rows
-        // (MFIs emitted by earlier rows, the stack, the policy) is done by
-        // the serial loop below.
-        if (pre_dec_.size() < wave_.size()) pre_dec_.resize(wave_.size());
-        pre_kept_.assign(wave_.size(), 0);
-        pre_pep_.assign(wave_.size(), 0);
-        pool_.run((int64_t)wave_.size(), [&](int64_t r) {
+        pre_sub_.assign(rows, 0);
+        row_flag_.assign(rows, 0);
+        emit_key_.assign(rows, 0ull);
+        if (keys64_) seen64_.reserve(rows);
+        if (pre_dec_.size() < rows) pre_dec_.resize(rows);
+        pre_kept_.assign(rows, 0);
+        pre_pep_.assign(rows, 0);
+        auto expands = [&](const MNode &n) { return n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_); };
+        auto tail_n = [&](int r) { return wave_[r].kind == MNode::SPEC ? (int)spec_tres_[r].size() : wb_.ncols(r); };
+        auto key_of = [](const std::vector<int32_t> &v) {
+            uint64_t b = 0;
+            for (int32_t i : v) b |= 1ull << i;
+            return b;
+        };
+        // a found set's key, claimed at once so that a set two rows find is
+        // reported by one of them
+        auto claim = [&](int64_t r, uint64_t key) { emit_key_[(size_t)r] = seen64_.insert_shared(key) ? key : 0ull; };
+        // Per row, in parallel, what depends on the row alone:
+        //  - a row that settles itself (HUT, LEAF, complement SPEC): whether
+        //    it found an MFI (with at most 64 survivors its key is claimed
+        //    here) and whether a HUT row is frequent;
+        //  - SPEC preparation (the row's columns are the survivors outside
+        //    head and known, ascending, so a column's index is a prefix
+        //    popcount of that live mask);
+        //  - a row that expands: its decision (frequent tail, PEP moves into
+        //    the head, infrequent items into known) and its subsumption check
+        //    against the MFIs found before the wave.  An EXPAND row checks
+        //    head + every frequent tail item (the leaf head after PEP when no
+        //    other item stays frequent, else its HUT); the checks against the
+        //    MFIs this wave finds follow the row loop, so decisions equal a
+        //    row-by-row pass.
+        // What depends on the order of the rows (the stack, the policy's
+        // observations, the MFI index) is done by the serial loop below.
+        pool_.run((int64_t)rows, [&](int64_t r) {
             MNode &n = wave_[r];
-            if (n.kind == MNode::SPEC && !compl_) spec_prepare((int)r, n, wb_.result((int)r));
-            if (!(n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_))) return;
+            if (!expands(n)) {
+                const int32_t *res = wb_.result((int)r);
+                const int nc = wb_.ncols((int)r);
+                uint8_t fl = 0;
+                if (n.kind == MNode::SPEC) {
+                    // complement row: res = {sup(head), frequent extensions};
+                    // none means its whole tail dropped out
+                    if (res[1] == 0) fl = kFound;
+                } else if (n.kind == MNode::HUT || n.kind == MNode::LEAF) {
+                    // explicit: res = {[sup(HUT),] counts of the complement};
+                    // complement row: res = {sup(row), frequent extensions}
+                    const bool hut = n.kind == MNode::HUT;
+                    if (!hut || ok_(res[0])) {
+                        bool maximal = !compl_ || res[1] == 0;
+                        for (int j = hut ? 1 : 0; j < nc && maximal; j++) maximal = !ok_(res[j]);
+                        fl = (uint8_t)((hut ? kHutFreq : 0) | (maximal ? kFound : 0));
+                    }
+                }
+                row_flag_[(size_t)r] = fl;
+                if ((fl & kFound) && keys64_)
+                    claim(r, n.kind == MNode::HUT ? key_of(n.head) | key_of(n.tail) : key_of(n.head));
+                return;
+            }
             const bool spec = n.kind == MNode::SPEC;
+            if (spec) spec_prepare((int)r, n, wb_.result((int)r));
             const int32_t *res = spec ? spec_tres_[r].data() : wb_.result((int)r);
             const int nc = spec ? (int)spec_tres_[r].size() : wb_.ncols((int)r);
             Decision &d = pre_dec_[r];
@@ -1308,9 +1395,16 @@ class Mafia {
             }
             pre_kept_[r] = kept;
             pre_pep_[r] = pep;
-            // a SPEC row whose tail died is settled by its own counts below
-            // (maximal, or not: no check against the found MFIs is needed)
-            if (spec && d.tail.empty()) return;
+            // a SPEC row whose tail died is settled by its own counts: a
+            // maximal leaf when no frequent column lies outside the
+            // (equivalent) tail items, else nothing
+            if (spec && d.tail.empty()) {
+                if (spec_other_[r] == 0) {
+                    row_flag_[(size_t)r] = kFound;
+                    if (keys64_) claim(r, key_of(d.head));
+                }
+                return;
+            }
             // head + frequent tail: the leaf head when the tail is empty,
             // else the node's head-union-tail
             thread_local std::vector<int32_t> q;
@@ -1318,27 +1412,23 @@ class Mafia {
             q.insert(q.end(), d.tail.begin(), d.tail.end());
             pre_sub_[r] = index_.subsumed(q);
         });
-        auto expands = [&](const MNode &n) { return n.kind == MNode::EXPAND || (n.kind == MNode::SPEC && !compl_); };
-        auto tail_n = [&](int r) { return wave_[r].kind == MNode::SPEC ? (int)spec_tres_[r].size() : wb_.ncols(r); };
         // Rows that expand into a leaf or a decision are checked against the
         // MFIs found before the wave (pre_sub_) and against those this wave
-        // emits: the first pass below emits and notes these rows (a leaf gets
-        // its place among the pending nodes), the checks then run on the host
-        // pool against every MFI the wave emitted, and a last pass in row
-        // order settles them.  (Pruning by any found MFI is sound, so the
-        // checks need not see only the MFIs of earlier rows.)
+        // finds: the row loop below reports the MFIs and notes these rows (a
+        // leaf gets its place among the pending nodes), the checks then run
+        // on the host pool against every MFI the wave found, and a last pass
+        // in row order settles them.  (Pruning by any found MFI is sound, so
+        // the checks need not see only the MFIs of earlier rows.)
         chk_.clear();
-        for (int r = 0; r < (int)wave_.size(); r++) {
+        new_keys_.clear();
+        for (int r = 0; r < (int)rows; r++) {
             MNode &n = wave_[r];
-            const int32_t *res = wb_.result(r);
-            const int nc = wb_.ncols(r);
+            const uint8_t fl = row_flag_[(size_t)r];
             if (n.kind == MNode::SPEC && compl_) {
-                // complement row: res = {sup(head), frequent extensions}
-                if (res[1] == 0) {
+                if (fl & kFound) {
                     st_.spec_hits++;
-                    // no survivor extends it: its whole tail dropped out
                     if (n.item_record >= 0) pol_.observe_value(n.item_record, 0.0f);
-                    emit(std::move(n.head), n.sup, out);
+                    emit(r, std::move(n.head), n.sup, out);
                 } else {
                     st_.spec_misses++;
                     n.kind = MNode::EXPAND;   // lives on: count its tail
@@ -1352,11 +1442,9 @@ class Mafia {
                 if (n.item_record >= 0) pol_.observe_value(n.item_record, nc > 0 ? (float)pre_kept_[r] / nc : 0.0f);
                 st_.pep_moves += pre_pep_[r];
                 if (d.tail.empty() && spec) {
-                    // the complement's counts already decide the leaf: no
-                    // frequent column outside the (equivalent) tail items
-                    if (spec_other_[r] == 0) {
+                    if (fl & kFound) {
                         st_.spec_hits++;
-                        emit(std::move(d.head), d.sup, out);
+                        emit(r, std::move(d.head), d.sup, out);
                     }
                     continue;
                 }
@@ -1369,18 +1457,18 @@ class Mafia {
                 }
                 chk_.push_back({r, -1, false});   // a decision unless a found MFI holds head + tail
             } else if (n.kind == MNode::HUT) {
-                // explicit: res = {sup(HUT), counts of the complement};
-                // complement row: res = {sup(HUT), frequent extensions}
-                const bool hut_freq = ok_(res[0]);
+                const bool hut_freq = (fl & kHutFreq) != 0;
                 pol_.observe(n.decision, hut_freq);
                 if (hut_freq) {
                     st_.hut_success++;
-                    bool maximal = true;
-                    if (compl_) maximal = res[1] == 0;
-                    for (int j = 1; j < nc && maximal; j++) maximal = !ok_(res[j]);
-                    std::vector<int32_t> hut = n.head;
-                    hut.insert(hut.end(), n.tail.begin(), n.tail.end());
-                    if (maximal) emit(std::move(hut), res[0], out);
+                    if (fl & kFound) {
+                        std::vector<int32_t> hut;
+                        if (!keys64_) {
+                            hut = n.head;
+                            hut.insert(hut.end(), n.tail.begin(), n.tail.end());
+                        }
+                        emit(r, std::move(hut), wb_.result(r)[0], out);
+                    }
                 } else {
                     Decision d;
                     d.head = n.head;
@@ -1394,11 +1482,10 @@ class Mafia {
                     hut_children_.push_back(std::move(d));
                 }
             } else if (n.kind == MNode::LEAF) {
-                bool maximal = true;
-                if (compl_) maximal = res[1] == 0;
-                for (int j = 0; j < nc && maximal; j++) maximal = !ok_(res[j]);
-                if (maximal) emit(std::move(n.head), n.sup, out);
+                if (fl & kFound) emit(r, std::move(n.head), n.sup, out);
             } else {   // REF: reference rule, own output path
+                const int32_t *res = wb_.result(r);
+                const int nc = wb_.ncols(r);
                 int32_t cnt = 0;
                 for (int j = 0; j < nc; j++) cnt += ok_(res[j]);
                 if (cnt == (int32_t)n.head.size()) {
@@ -1418,9 +1505,19 @@ class Mafia {
                 }
             }
         }
+        if (keys64_) {
+            index_.add_masks(new_keys_.data(), new_keys_.size());
+            seen64_.added(new_keys_.size());
+        }
+        // The checks against the MFIs found so far; a decision that survives
+        // them is decided here too when the policy decides each node alone,
+        // and its leaf child checked (no MFI is found before the children are
+        // built, so the leaf check sees every MFI there is then).
+        const bool alone = pol_.decides_alone();
+        const int64_t n_now = index_.size();
         pool_.run((int64_t)chk_.size(), [&](int64_t i) {
             Check &c = chk_[i];
-            const Decision &d = pre_dec_[c.row];
+            Decision &d = pre_dec_[c.row];
             if (pre_sub_[c.row]) {
                 c.sub = true;
             } else if (c.pend >= 0) {
@@ -1431,8 +1528,15 @@ class Mafia {
                 hut.insert(hut.end(), d.tail.begin(), d.tail.end());
                 c.sub = index_.subsumed_from(hut, n_start);
             }
+            if (alone && c.pend < 0 && !c.sub) {
+                thread_local std::vector<Decision *> one(1);
+                one[0] = &d;
+                pol_.decide(one, db_);
+                check_leaf(d, n_now);
+            }
         });
         pend_drop_.assign(pending_.size(), 0);
+        decs_.clear();
         for (const Check &c : chk_) {
             Decision &d = pre_dec_[c.row];
             if (c.pend >= 0) {
@@ -1450,25 +1554,14 @@ class Mafia {
             } else if (c.sub) {
                 st_.hutmfi_pruned++;
             } else {
-                if (n_dec_ == dec_.size()) dec_.emplace_back();
-                std::swap(dec_[n_dec_++], d);
+                decs_.push_back(&d);
             }
         }
-        // policy: order tails, choose lookaheads (batched over the wave; a
-        // policy that decides each node alone runs on the host pool)
-        std::vector<Decision *> ptrs;
-        for (size_t q = 0; q < n_dec_; q++) ptrs.push_back(&dec_[q]);
-        if (!ptrs.empty()) {
+        // a policy that decides the wave's nodes together (order tails,
+        // choose lookaheads) runs here
+        if (!alone && !decs_.empty()) {
             const double t0 = now();
-            if (pol_.decides_alone()) {
-                pool_.run((int64_t)ptrs.size(), [&](int64_t i) {
-                    thread_local std::vector<Decision *> one(1);
-                    one[0] = ptrs[i];
-                    pol_.decide(one, db_);
-                });
-            } else {
-                pol_.decide(ptrs, db_);
-            }
+            pol_.decide(decs_, db_);
             st_.learn_seconds += now() - t0;
         }
         // push in reverse so the first decision's work is on top
@@ -1476,13 +1569,14 @@ class Mafia {
             if (!pend_drop_[i]) stack_.push_back(std::move(pending_[i]));
         pending_.clear();
         {
-            // pending decisions too: their leaf check then needs no pass of
-            // its own when they are settled (push_children checks the MFIs
-            // found in between; a lookahead taken after all leaves it unused)
+            // the leaf checks not made above, pending decisions too: their
+            // leaf check then needs no pass of its own when they are settled
+            // (push_children checks the MFIs found in between; a lookahead
+            // taken after all leaves it unused)
             std::vector<Decision *> leaves;
             for (auto &d : hut_children_) leaves.push_back(&d);
-            for (size_t q = 0; q < n_dec_; q++) leaves.push_back(&dec_[q]);
-            check_leaves(leaves);
+            if (!alone) leaves.insert(leaves.end(), decs_.begin(), decs_.end());
+            if (!leaves.empty()) check_leaves(leaves);
         }
         for (auto &d : hut_children_) push_children(d);
         hut_children_.clear();
@@ -1491,8 +1585,8 @@ class Mafia {
         // then stacked here in that order
         work_.clear();
         size_t total = 0;
-        for (size_t q = n_dec_; q-- > 0;) {
-            Decision &d = dec_[q];
+        for (size_t q = decs_.size(); q-- > 0;) {
+            Decision &d = *decs_[q];
             if (d.pending) {
                 undecided_.push_back(std::move(d));   // settled after the next wave
                 continue;
@@ -1507,10 +1601,10 @@ class Mafia {
             if (!work_[i].hut) kid_work_.push_back(i);
         pool_.run((int64_t)kid_work_.size(), [&](int64_t i) {
             Work &w = work_[kid_work_[i]];
-            w.n = build_children(dec_[w.dec], kids_.data() + w.off, &w.subsumed);
+            w.n = build_children(*decs_[w.dec], kids_.data() + w.off, &w.subsumed);
         });
         for (const Work &w : work_) {
-            Decision &d = dec_[w.dec];
+            Decision &d = *decs_[w.dec];
             if (w.hut) {
                 apply(d);   // its head-union-tail row
                 continue;
@@ -1573,10 +1667,12 @@ class Mafia {
     std::unordered_set<std::vector<int32_t>, VecHash> seen_;
     KeySet64 seen64_;
     std::vector<std::pair<uint64_t, int32_t>> out_masks_;   // reported MFIs of <= 64 survivors (mask, frequency)
+    std::vector<uint8_t> row_flag_;   // per wave row: kFound, kHutFreq
+    std::vector<uint64_t> emit_key_;  // per wave row: key of the MFI it found (<= 64 survivors), 0 = none
+    std::vector<uint64_t> new_keys_;  // keys the wave found, in row order
     std::vector<MNode> stack_, pending_;
     std::vector<Decision> hut_children_, undecided_;
-    std::vector<Decision> dec_;   // decisions of the wave being processed (first n_dec_)
-    size_t n_dec_ = 0;
+    std::vector<Decision *> decs_;   // decisions of the wave being processed (in pre_dec_)
     // a decision's stacking work: its HUT row, or its children (n kept of
     // those built at kids_[off ..), `subsumed` leaves dropped)
     struct Work {
@@ -1598,7 +1694,9 @@ class Mafia {
     std::vector<size_t> kid_work_;
     std::vector<MNode> kids_;   // children under construction
     static constexpr int kFlights = 4;   // at most; flights_ in use
-    static constexpr size_t kMaxFree = 1 << 15;
+    // spent nodes kept for reuse, at most: a wave's rows and children need a
+    // few thousand; more would only be freed at the end of the search
+    static constexpr size_t kMaxFree = 4096;
     Flight fl_[kFlights];
     int flights_ = 4;
     AsyncWorker helper_;
@@ -1619,6 +1717,7 @@ class Mafia {
     double last_end_ = 0;
     bool compl_ = false;         // HUT, LEAF and SPEC rows are complement rows
     bool mask_rows_ = false;     // rows counted as bit masks (at most 64 dense ids)
+    bool keys64_ = false;        // found MFIs kept as 64-bit keys (at most 64 survivors)
     bool spec_ = true;           // predicted maximal leaves are checked at once (SPEC rows)
     int32_t min_count_ = 0;
     int excl_words_ = 0;
