@@ -232,7 +232,9 @@ int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *i
 int tfim_config(tfim_ctx *ctx, const char *key, int64_t value);
 
 /* Statistics of the batched path since tfim_open: launches, how many used
- * the LDS-staged kernel, rows, summed kernel milliseconds (HIP events).  In
+ * the LDS-staged kernel, rows, summed kernel milliseconds (HIP events around
+ * a wave's count kernels; a mask wave's kernel stamps its own span, first
+ * workgroup start to last workgroup end, on the device wall clock).  In
  * TFIM_PATH_AUTO the library's wave tuner picks, for waves that fit both
  * kernels, the kernel and the columns per work tile (L2 kernel with 32; LDS
  * kernel with 16, 32 or 64, narrow or wide staging): each arm is timed

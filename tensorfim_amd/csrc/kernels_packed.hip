@@ -772,6 +772,7 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_mask(const uint4 *__
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int n_tiles = d.n_tiles, n_stage = d.n_items * C4;
+    if (tid == 0) atomicMax(&d.stamp[0], ~(unsigned long long)wall_clock64());
     for (int i = tid; i < d.n_out; i += kLdsThreads) acc[i] = 0;
     auto grab = [&]() {
         int g = 0;
@@ -882,6 +883,8 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_mask(const uint4 *__
     __syncthreads();
     for (int i = tid; i < d.n_out; i += kLdsThreads)
         if (acc[i]) atomicAdd(&d.out[i], acc[i]);
+    __syncthreads();
+    if (tid == 0) atomicMax(&d.stamp[1], (unsigned long long)wall_clock64());
 }
 
 bool plan_count_wave_mask(const uint64_t *bm, int64_t pitch_words, const WaveMaskDesc &d, int n_cu, bool wide,

@@ -169,6 +169,9 @@ static_assert(sizeof(MaskTile) == 32, "mask tiles are 32-byte records");
 struct WaveMaskDesc {
     const MaskTile *tiles;
     int32_t *out;        // [n_out], zeroed
+    // [2], zeroed: the kernel's span on the device wall clock, as the max of
+    // the complement of each workgroup's start and the max of its end
+    unsigned long long *stamp;
     int32_t n_items;     // ids staged: 0 .. n_items - 1 (<= 64)
     int32_t n_tiles, n_out;
 };

@@ -117,6 +117,7 @@ struct tfim_ctx {
     char name[256] = {0};
     int64_t hbm = 0;
     int clock_mhz = 0;
+    int wall_khz = 0;   // rate of the device wall clock the mask kernel stamps its span with
     std::string err;
     std::mutex err_mu;
     // wave state (slots, tuner, counters): one thread may submit a wave while
@@ -144,6 +145,9 @@ struct tfim_ctx {
         int64_t out_cap = 0;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the wave's count kernels
         hipEvent_t done = nullptr;                 // after its drain
+        // a mask wave's span, stamped by its kernel at output int stamp_at
+        // (-1: timed by ev0/ev1)
+        int64_t stamp_at = -1;
         bool busy = false;
         int64_t ticket = -1, n_out = 0, n_rows = 0;
         int tuned = -1;
@@ -340,6 +344,10 @@ int tfim_open(int device, tfim_ctx **out) {
     else snprintf(c->name, sizeof c->name, "%s, %d CUs", p.gcnArchName, c->n_cu);   // the runtime gave no marketing name
     c->hbm = (int64_t)p.totalGlobalMem;
     c->clock_mhz = p.clockRate / 1000;
+    if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || c->wall_khz <= 0) {
+        delete c;
+        return TFIM_E_HIP;
+    }
     // device address of the no-allocation entry point's scratch (once per device)
     if (device < kMaxDevices && scratch_setup(device, &c->d_scratch) != TFIM_OK) { delete c; return TFIM_E_HIP; }
     // from here on a failure releases what was created (tfim_close skips null handles)
@@ -629,7 +637,7 @@ hipKernelNodeParams node_params(KernelLaunch &L) {
 // wave waited on, so its buffers are free and its device output block is
 // zero) whenever it is issued again.
 int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fill, int64_t n_fill, int64_t n_out,
-               int64_t n_zero, KernelLaunch *const *ks, int nk) {
+               int64_t n_zero, KernelLaunch *const *ks, int nk, bool events = true) {
     for (int i = 0; i < nk; i++)
         if (ks[i]->overflow || !ks[i]->func) return fail(c, TFIM_E_INVALID, "wave kernel %d: bad launch description", i);
     KernelLaunch S, Z, D;
@@ -640,9 +648,11 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fil
     if (!c->use_graphs) {
         HIPCHK(c, S.launch(c->stream));
         if (has_fill) HIPCHK(c, Z.launch(c->stream));
-        HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
+        // (a kernel that stamps its own span needs no events around it:
+        // each costs about 10 us of the gap between two waves' kernels)
+        if (events) HIPCHK(c, hipEventRecord(sl.ev0, c->stream));
         for (int i = 0; i < nk; i++) HIPCHK(c, ks[i]->launch(c->stream));
-        HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
+        if (events) HIPCHK(c, hipEventRecord(sl.ev1, c->stream));
         HIPCHK(c, D.launch(c->stream));
     } else {
         hipKernelNodeParams sp = node_params(S), zp{}, dp = node_params(D);
@@ -780,6 +790,7 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
     KernelLaunch *ks[3] = {&M, &P, &G};
     if (int r = issue_wave(c, sl, total, zs > 1 ? C : nullptr, zs > 1 ? c_ints : 0, n_out, n_out, ks, 3)) return r;
     sl.busy = true;
+    sl.stamp_at = -1;
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
     sl.n_rows = n_rows;
@@ -941,6 +952,7 @@ int wave_submit_fused(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, 
     KernelLaunch *ks[1] = {&K};
     if (int r = issue_wave(c, sl, total, nullptr, 0, n_out, n_dev, ks, 1)) return r;
     sl.busy = true;
+    sl.stamp_at = -1;
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
     sl.n_rows = n_rows;
@@ -1365,6 +1377,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
     KernelLaunch *ks[2] = {&K, &R};
     if (int r = issue_wave(c, sl, total, nullptr, 0, n_out, n_dev, ks, n_compl ? 2 : 1)) return r;
     sl.busy = true;
+    sl.stamp_at = -1;
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
     sl.n_rows = n_rows;
@@ -1470,7 +1483,9 @@ int wave_submit_masks(tfim_ctx *c, const uint64_t *pmask, const uint64_t *cmask,
     const int64_t n_tiles = (int64_t)tiles.size();
     const int64_t desc_bytes = n_tiles * (int64_t)sizeof(MaskTile);
     if (int r = ensure_desc(c, sl, desc_bytes)) return r;
-    if (int r = ensure_out(c, sl, n_out)) return r;
+    // outputs, then (16-byte aligned) the kernel's two wall-clock stamps
+    const int64_t stamp_at = (n_out + 3) / 4 * 4, n_block = stamp_at + 4;
+    if (int r = ensure_out(c, sl, n_block)) return r;
     // by falling cost 2 * columns + prefix ids, stable (see the list path)
     MaskTile *ht = reinterpret_cast<MaskTile *>(sl.h_desc);
     std::vector<int32_t> &start = c->tile_bucket;
@@ -1481,6 +1496,7 @@ int wave_submit_masks(tfim_ctx *c, const uint64_t *pmask, const uint64_t *cmask,
     WaveMaskDesc d;
     d.tiles = reinterpret_cast<const MaskTile *>(sl.d_desc);
     d.out = sl.d_out;
+    d.stamp = reinterpret_cast<unsigned long long *>(sl.d_out + stamp_at);
     d.n_items = c->n_items;
     d.n_tiles = (int32_t)n_tiles;
     d.n_out = (int32_t)n_out;
@@ -1489,8 +1505,9 @@ int wave_submit_masks(tfim_ctx *c, const uint64_t *pmask, const uint64_t *cmask,
         return fail(c, TFIM_E_RANGE, "count_masks: the mask kernel rejected a wave of %lld outputs over %d ids",
                     (long long)n_out, c->n_items);
     KernelLaunch *ks[1] = {&K};
-    if (int r = issue_wave(c, sl, desc_bytes, nullptr, 0, n_out, n_out, ks, 1)) return r;
+    if (int r = issue_wave(c, sl, desc_bytes, nullptr, 0, n_block, n_block, ks, 1, false)) return r;
     sl.busy = true;
+    sl.stamp_at = stamp_at;
     sl.ticket = c->next_ticket++;
     sl.n_out = n_out;
     sl.n_rows = n_rows;
@@ -1579,7 +1596,16 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     memcpy(out, sl.h_out, (size_t)sl.n_out * 4);
     std::lock_guard<std::mutex> g(c->wave_mu);
     float ms = 0;
-    hipEventElapsedTime(&ms, sl.ev0, sl.ev1);
+    if (sl.stamp_at >= 0) {
+        // first workgroup start (kept as its complement under a max) to
+        // last workgroup end, in wall-clock ticks
+        uint64_t st[2];
+        memcpy(st, sl.h_out + sl.stamp_at, sizeof st);
+        const uint64_t t0 = ~st[0], t1 = st[1];
+        ms = st[0] && t1 > t0 ? (float)((double)(t1 - t0) / c->wall_khz) : 0.0f;
+    } else {
+        hipEventElapsedTime(&ms, sl.ev0, sl.ev1);
+    }
     c->wave_ms += ms;
     c->wave_launches++;
     c->wave_rows += sl.n_rows;
