@@ -75,12 +75,12 @@ class LearnedPolicy : public Policy {
         N_ = (double)std::max(1, db.trans_num_reduced);
         thr_ = std::max(1.0, db.thr_abs);
         freq_ = &db.freq_dense;
-        // plain order and the independence-estimate rule, node by node
-        run_parallel((int64_t)ds.size(), [&](int64_t q) {
-            thread_local std::vector<Decision *> one(1);
-            one[0] = ds[q];
-            plain_.decide(one, db);
-        });
+        // plain order and the independence-estimate rule, node by node (the
+        // engine may have made it already: prepare)
+        if (!std::all_of(ds.begin(), ds.end(), [](const Decision *d) { return d->prepared; }))
+            run_parallel((int64_t)ds.size(), [&](int64_t q) {
+                if (!ds[q]->prepared) prepare(*ds[q], db);
+            });
         // Which nodes get feature rows: one gate row per node with a tail of
         // two or more (kept as a training record when its lookahead runs);
         // the model is asked only about nodes the rule declines with tails of
@@ -213,6 +213,14 @@ class LearnedPolicy : public Policy {
     }
 
     void set_parallel(ParallelFor pf) override { par_ = std::move(pf); }
+    void prepare(Decision &d, const Database &db) override {
+        thread_local std::vector<Decision *> one(1);
+        one[0] = &d;
+        plain_.decide(one, db);
+        d.prepared = true;
+    }
+    // the ranker reorders tails; the gate only adds lookaheads to the rule's
+    bool keeps_order() const override { return !opt_.rank; }
 
     void resolve(std::vector<Decision *> &ds) override {
         // ds: the decisions decide() left pending, in the order it saw them;

@@ -34,6 +34,7 @@ struct Decision {
     std::vector<uint64_t> known;    // survivors e with sup(head + e) < thr (bitset), engine-owned
     int8_t leaf_sub = -1;           // engine: last child's leaf subsumed (1) or not (0) by the
     int64_t leaf_n = 0;             //   MFIs found before leaf_n; -1 = not yet checked
+    bool prepared = false;          // prepare() has run on it (engine resets it per node)
 };
 
 class Policy {
@@ -52,6 +53,14 @@ class Policy {
     // true when decide() of one node depends on that node alone (no state,
     // no batch): the engine may then call it per node from several threads
     virtual bool decides_alone() const { return false; }
+    // For a policy that decides a wave's nodes together: the part of
+    // decide() that depends on one node alone, which the engine may call per
+    // node from several threads before decide() (which then skips it for
+    // nodes marked `prepared`).
+    virtual void prepare(Decision &d, const Database &db) { (void)d; (void)db; }
+    // true when decide() keeps the tail order prepare() made and takes back
+    // no lookahead prepare() chose (the engine then checks leaves before it)
+    virtual bool keeps_order() const { return false; }
     // A parallel loop the engine lends a batch policy: pf(n, f) calls f(i)
     // for every i < n on the engine's host threads and returns when all are
     // done; each call may touch only its own outputs.

@@ -1377,6 +1377,7 @@ class Mafia {
             d.record = -1;
             d.leaf_sub = -1;
             d.leaf_n = 0;
+            d.prepared = false;
             int kept = 0, pep = 0;
             for (int j = 0; j < nc; j++) {
                 const int32_t t = n.tail[j], sv = res[j];
@@ -1510,10 +1511,14 @@ class Mafia {
             seen64_.added(new_keys_.size());
         }
         // The checks against the MFIs found so far; a decision that survives
-        // them is decided here too when the policy decides each node alone,
+        // them is decided here too when the policy decides each node alone
+        // (else the part of its decision that depends on it alone is made),
         // and its leaf child checked (no MFI is found before the children are
-        // built, so the leaf check sees every MFI there is then).
+        // built, so the leaf check sees every MFI there is then; a policy
+        // that decides the wave together must keep the tail order, and any
+        // lookahead, made here).
         const bool alone = pol_.decides_alone();
+        const bool early_leaf = alone || pol_.keeps_order();
         const int64_t n_now = index_.size();
         pool_.run((int64_t)chk_.size(), [&](int64_t i) {
             Check &c = chk_[i];
@@ -1528,11 +1533,15 @@ class Mafia {
                 hut.insert(hut.end(), d.tail.begin(), d.tail.end());
                 c.sub = index_.subsumed_from(hut, n_start);
             }
-            if (alone && c.pend < 0 && !c.sub) {
-                thread_local std::vector<Decision *> one(1);
-                one[0] = &d;
-                pol_.decide(one, db_);
-                check_leaf(d, n_now);
+            if (c.pend < 0 && !c.sub) {
+                if (alone) {
+                    thread_local std::vector<Decision *> one(1);
+                    one[0] = &d;
+                    pol_.decide(one, db_);
+                } else {
+                    pol_.prepare(d, db_);
+                }
+                if (early_leaf) check_leaf(d, n_now);
             }
         });
         pend_drop_.assign(pending_.size(), 0);
@@ -1575,7 +1584,7 @@ class Mafia {
             // taken after all leaves it unused)
             std::vector<Decision *> leaves;
             for (auto &d : hut_children_) leaves.push_back(&d);
-            if (!alone) leaves.insert(leaves.end(), decs_.begin(), decs_.end());
+            if (!early_leaf) leaves.insert(leaves.end(), decs_.begin(), decs_.end());
             if (!leaves.empty()) check_leaves(leaves);
         }
         for (auto &d : hut_children_) push_children(d);

@@ -47,6 +47,14 @@ __device__ __forceinline__ uint4 and4(const uint4 a, const uint4 b) {
     return make_uint4(a.x & b.x, a.y & b.y, a.z & b.z, a.w & b.w);
 }
 
+// a AND b AND c in one v_bitop3_b32 per dword (gfx950's three-input logic
+// op; truth table 0x80: only the all-ones input gives one), where two
+// v_and_b32 would be needed.
+__device__ __forceinline__ uint4 and3_4(const uint4 a, const uint4 b, const uint4 c) {
+    return make_uint4(__builtin_amdgcn_bitop3_b32(a.x, b.x, c.x, 0x80), __builtin_amdgcn_bitop3_b32(a.y, b.y, c.y, 0x80),
+                      __builtin_amdgcn_bitop3_b32(a.z, b.z, c.z, 0x80), __builtin_amdgcn_bitop3_b32(a.w, b.w, c.w, 0x80));
+}
+
 // Reduce-scatter of 8 per-lane values over a 64-lane wave in 10 shuffles:
 // afterwards lane L with (L & 7) == 0 holds the wave total of v[(L >> 3) & 7].
 __device__ __forceinline__ int wave_reduce_scatter8(const int v[8]) {
@@ -326,7 +334,7 @@ __global__ __launch_bounds__(256) void k_count_wave(const uint4 *__restrict__ bm
 #pragma unroll
             for (int q = 0; q < RW; q++) {
                 const int64_t idx = base + q * 64;
-                if (idx < pitch4) m[q] = and4(m[q], and4(r0[idx], r1[idx]));
+                if (idx < pitch4) m[q] = and3_4(m[q], r0[idx], r1[idx]);
             }
         }
         for (; i < ie; i++) {
@@ -560,7 +568,7 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_lds(const uint4 *__r
                 lds_col_load<RW>(a0, lds, __builtin_amdgcn_readlane(pslot, i), lane);
                 lds_col_load<RW>(a1, lds, __builtin_amdgcn_readlane(pslot, i + 1), lane);
 #pragma unroll
-                for (int q = 0; q < RW; q++) m[q] = and4(m[q], and4(a0[q], a1[q]));
+                for (int q = 0; q < RW; q++) m[q] = and3_4(m[q], a0[q], a1[q]);
             }
             for (; i < k; i++) {
                 const int sl = i < 64 ? __builtin_amdgcn_readlane(pslot, i) : d.item_slots[tl.ib + i];
@@ -841,7 +849,7 @@ __global__ __launch_bounds__(kLdsThreads) void k_count_wave_mask(const uint4 *__
                     lds_col_load<RW>(a0, lds, a, lane);
                     lds_col_load<RW>(a1, lds, b, lane);
 #pragma unroll
-                    for (int q = 0; q < RW; q++) m[q] = and4(m[q], and4(a0[q], a1[q]));
+                    for (int q = 0; q < RW; q++) m[q] = and3_4(m[q], a0[q], a1[q]);
                 } else {
                     uint4 a0[RW];
                     lds_col_load<RW>(a0, lds, a, lane);
