@@ -8,8 +8,10 @@ A step is one pass of the hot path over one batch: the isolated count
 C[i][j] = popcount(A_i AND B_j) for 65,536 prefix bitmaps x
 512 item bitmaps over 2^22 transactions (i.i.d. Bernoulli(0.3) bits, seed 1),
 resident in HBM before timing.  `value` is support-count Tbit-op/s of the
-int8-MFMA path (bit-ops = rows x items x transactions).  Also reported in the
-same run: the packed AND+v_bcnt path on the same data, the CPU baseline (the
+first of --paths (default: the fp4 matrix-core path, each bit an exact e2m1
+value summed in fp32; bit-ops = rows x items x transactions).  Also reported
+in the same run: the int8-MFMA path and the packed AND+v_bcnt path on the
+same data, the CPU baseline (the
 oracle's popcount on a 512-row sample, host threads stated), the plain
 per-node count kernel's HBM fraction on two shapes (unless --no-per-node), and unless
 --no-e2e the end-to-end mining time of the dense replicated workload (49,046-row base x256
@@ -20,7 +22,7 @@ two wide workloads (clickstream: 990,000 transactions at support 0.1 %; gene:
 1,100 x 20,000 at support 0.12), checked against their golden MFI digests.
 Unless --no-sweep: the reference microbench's eight shapes
 (code/microbench/bmma_bench.cu:438-447) on both paths beside its published
-RTX 3060 Ti figures.  --sustain-seconds S (default 30, 0 = off): the int8
+RTX 3060 Ti figures.  --sustain-seconds S (default 30, 0 = off): the headline
 path run back to back on the headline shape for S seconds, per-launch times
 before and after the first 3 s (bmma_bench.cu:216-289), counts re-checked
 afterwards.
@@ -43,6 +45,7 @@ import tensorfim_amd as tf  # noqa: E402
 METRIC = "support-count Tbit-op/s; end-to-end MFI seconds, batched+learned vs plain GPU"
 HBM_PEAK_GBS = 8000.0
 I8_PEAK_TOPS = 5000.0      # dense int8 MFMA, MI355X
+FP4_PEAK_TFLOPS = 10100.0  # dense fp4 (e2m1) MFMA, MI355X
 VALU_PEAK_TOPS = 39.3      # 256 CU x 64 lanes x 2.4 GHz int32 ops
 # AND + popcount of 32 bit-pairs per lane takes two VALU ops: 16 bit-ops per op
 VALU_BITOP_PEAK_T = VALU_PEAK_TOPS * 16
@@ -58,6 +61,19 @@ PACKED_TRAFFIC_BYTES = 141876016.75 * 1024.0
 # kernel); profiles/popmat_mfma_rp_65536x512x4M_traffic.md holds the readings.
 # Read 44,790,465 KiB + written 1,048,576 KiB (mean over dispatches).
 TRAFFIC_KERNEL = "k_popmat_mfma_rp"
+# The same readings for the fp4 matrix-core kernel (`bench.py --no-e2e --no-cpu
+# --paths fp4,mfma`, profiles/popmat_fp4_65536x512x4M_kernel_stats.md): its
+# loads are 128-byte row segments (16 words per row per stage), for which
+# FETCH_SIZE reads half (the factor 2.00 of the plain copy kernel's 128-byte
+# requests); WRITE_SIZE exact.  Read 19,644,622 KiB x 2 + written 1,048,576 KiB.
+FP4_TRAFFIC_KERNEL = "k_popmat_fp4_256"
+FP4_TRAFFIC_BYTES = (19644622 * 2 + 1048576) * 1024.0
+HEAD_DESC = {
+    "fp4": "fp4 matrix-core path (each bit an exact e2m1 value, v_mfma_f32_32x32x64_f8f6f4, fp32 sums exact; "
+           "256x256 tiles, stages fetched straight into LDS, operands expanded in registers)",
+    "mfma": "int8-MFMA path (256x128 tiles, operand bytes expanded in registers from packed bits in LDS)",
+    "packed": "packed AND + v_bcnt path (VALU)",
+}
 TRAFFIC_BYTES = (44790465 + 1048576) * 1024.0
 
 
@@ -150,7 +166,7 @@ def bench_isolated(ctx, args):
     C_exp = oracle_lib.popcount_matrix(A_sample, B_host, threads=host_cpu_share())
     log("oracle counts of %d sampled rows x %d items in %.1f s" % (len(sample_rows), d, time.perf_counter() - t0))
     for path in args.paths:
-        split = 8 if path == "mfma" else 1
+        split = 8 if path in ("mfma", "fp4") else 1
         ms = ctx.time_popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=split, warmup=args.warmup,
                                       reps=args.steps)
         # exactness on the sampled rows, all 512 columns each: both 256-row
@@ -170,6 +186,10 @@ def bench_isolated(ctx, args):
                                      "frac": 2 * tbit / I8_PEAK_TOPS,
                                      "memory_gbs": TRAFFIC_BYTES / (ms * 1e-3) / 1e9,
                                      "memory_frac": TRAFFIC_BYTES / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        elif path == "fp4":
+            # one multiply-add of two e2m1 values per bit-op: 2 flops
+            res[path]["roofline"] = {"bound": "mfma", "achieved_tflops": 2 * tbit, "peak_tflops": FP4_PEAK_TFLOPS,
+                                     "frac": 2 * tbit / FP4_PEAK_TFLOPS}
         else:
             res[path]["roofline"] = {"bound": "valu", "achieved_tbitops": tbit, "peak_tbitops": VALU_BITOP_PEAK_T,
                                      "frac": tbit / VALU_BITOP_PEAK_T,
@@ -187,8 +207,9 @@ def bench_isolated(ctx, args):
             res[path]["tuned"] = {"tile": tile, "split_k": tsplit, "kernel_ms": tms,
                                   "tbitops": bitops / (tms * 1e-3) / 1e12, "exact_on_sample": bool((got == C_exp).all())}
             log("%-6s tuned: tile %d split %d  %.3f ms" % (path, tile, tsplit, tms))
-    if args.sustain_seconds > 0 and "mfma" in args.paths:
-        res["mfma"]["sustain"] = sustain(ctx, A, B, C, b, d, pitch, args.sustain_seconds, sample_rows, C_exp)
+    if args.sustain_seconds > 0 and args.paths[0] in ("fp4", "mfma"):
+        res[args.paths[0]]["sustain"] = sustain(ctx, A, B, C, b, d, pitch, args.sustain_seconds, sample_rows, C_exp,
+                                                args.paths[0])
     for x in (A, B, C):
         x.free()
     # the CPU baseline's sample: the 512 distinct rows of the two edge tiles
@@ -196,12 +217,12 @@ def bench_isolated(ctx, args):
     return res, bitops, alg_bytes, B_host, A_sample[edge], n_bits
 
 
-def sustain(ctx, A, B, C, b, d, pitch, seconds, sample_rows, C_exp):
-    """The int8 path (split-k 8) back to back for `seconds` of wall time,
+def sustain(ctx, A, B, C, b, d, pitch, seconds, sample_rows, C_exp, path):
+    """The headline matrix-core path (split-k 8) back to back for `seconds` of wall time,
     each launch timed with HIP events; mean time in the first 3 s against
     the rest, as the reference's throttle probe reports it
     (bmma_bench.cu:216-289).  The counts are re-checked afterwards."""
-    log("sustain: int8 path, %.0f s ..." % seconds)
+    log("sustain: %s path, %.0f s ..." % (path, seconds))
     bitops = float(b) * d * pitch * 64
     ms_list, walls = [], []
     t0 = time.perf_counter()
@@ -210,7 +231,7 @@ def sustain(ctx, A, B, C, b, d, pitch, seconds, sample_rows, C_exp):
         wall = time.perf_counter() - t0
         if wall >= seconds:
             break
-        ms_list.append(ctx.time_popcount_matrix(A, B, C, b, d, pitch, path="mfma", split_k=8, warmup=0, reps=1))
+        ms_list.append(ctx.time_popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=8, warmup=0, reps=1))
         walls.append(time.perf_counter() - t0)
         if walls[-1] >= next_log:
             log("  %.0f s: %d launches, last %.2f ms" % (walls[-1], len(ms_list), ms_list[-1]))
@@ -264,7 +285,7 @@ def bench_sweep(ctx, args):
         ent = {"transactions": n_bits, "items": d, "rows": b}
         if pub:
             ent["published_rtx3060ti_tbitops"] = {"bmma_naive": pub[0], "bmma_smem": pub[1], "bitmap": pub[2]}
-        for path in ("mfma", "packed"):
+        for path in ("fp4", "mfma", "packed"):
             tile, split = ctx.popcount_matrix_plan(A, B, C, b, d, pitch, path=path)
             ms = ctx.time_popcount_matrix(A, B, C, b, d, pitch, path=path, split_k=0, warmup=1, reps=max(3, args.steps // 2))
             got = np.stack([C.read(np.int32, d, offset=r * d * 4)[cols] for r in rows])
@@ -273,10 +294,11 @@ def bench_sweep(ctx, args):
                          "exact_on_sample": ok}
             if not ok:
                 raise SystemExit("sweep %s: %s path differs from the oracle" % (ent, path))
-        if "mfma" in ent:
-            ent["mfma"]["i8_frac"] = 2 * ent["mfma"]["tbitops"] / I8_PEAK_TOPS
-        log("sweep N=%d d=%d b=%d: mfma %.0f Tbit-op/s (tile %d split %d), packed %.0f" % (
-            n_bits, d, b, ent["mfma"]["tbitops"], ent["mfma"]["tile"], ent["mfma"]["split_k"], ent["packed"]["tbitops"]))
+        ent["mfma"]["i8_frac"] = 2 * ent["mfma"]["tbitops"] / I8_PEAK_TOPS
+        ent["fp4"]["fp4_frac"] = 2 * ent["fp4"]["tbitops"] / FP4_PEAK_TFLOPS
+        log("sweep N=%d d=%d b=%d: fp4 %.0f Tbit-op/s (split %d), int8 %.0f (tile %d split %d), packed %.0f" % (
+            n_bits, d, b, ent["fp4"]["tbitops"], ent["fp4"]["split_k"], ent["mfma"]["tbitops"], ent["mfma"]["tile"],
+            ent["mfma"]["split_k"], ent["packed"]["tbitops"]))
         out.append(ent)
         for x in (A, B, C):
             x.free()
@@ -488,7 +510,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-e2e", action="store_true")
-    ap.add_argument("--paths", default="mfma,packed")
+    ap.add_argument("--paths", default="fp4,mfma,packed")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-node", action="store_true")
@@ -512,8 +534,8 @@ def main():
     per_node = None if args.no_per_node else bench_per_node(ctx, args)
     sweep = None if args.no_sweep else bench_sweep(ctx, args)
     ctx.close()
-    head = res.get("mfma") or res[args.paths[0]]
-    head_path = "mfma" if "mfma" in res else args.paths[0]
+    head_path = args.paths[0]
+    head = res[head_path]
 
     threads = args.cpu_threads or host_cpu_share()
     cpu = None
@@ -532,7 +554,13 @@ def main():
     e2e_wide = None if args.no_e2e else bench_e2e_wide(args)
 
     kernel_s = head["kernel_ms"] * 1e-3
-    if head_path == "mfma":
+    if head_path == "fp4":
+        # one e2m1 multiply-add per bit-op = 2 flops
+        achieved = 2.0 * bitops / kernel_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": FP4_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP4_PEAK_TFLOPS, "traffic": FP4_TRAFFIC_BYTES, "kernel": FP4_TRAFFIC_KERNEL,
+                "algorithmic_bytes": alg_bytes}
+    elif head_path == "mfma":
         achieved = 2.0 * bitops / kernel_s / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": I8_PEAK_TOPS, "unit": "TFLOP/s",
                 "frac": achieved / I8_PEAK_TOPS, "traffic": TRAFFIC_BYTES, "kernel": TRAFFIC_KERNEL,
@@ -552,11 +580,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i8" if head_path == "mfma" else "u32",
+        "dtype": {"fp4": "fp4 e2m1 x e2m1 -> fp32 (operands 0, 0.5, 1, 2; every sum an exact integer)",
+                  "mfma": "i8"}.get(head_path, "u32"),
         "data": "synthetic (Bernoulli(0.3) bits generated on the device, seed 1)",
         "config": {"workload": "isolated support count: 65,536 prefix bitmaps x 512 item bitmaps x 2^22 "
-                               "transactions, int8-MFMA path (256x128 tiles, operand bytes expanded "
-                               "in registers from packed bits in LDS), split-k 8",
+                               "transactions, split-k 8, %s" % HEAD_DESC[head_path],
                    "rows": 65536, "items": 512, "transactions": 1 << 22},
         "roofline": roof,
         "cpu_baseline": cpu,

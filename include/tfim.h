@@ -41,7 +41,10 @@ enum {
 enum {
     TFIM_PATH_AUTO = 0,     /* library chooses (autotuned per shape) */
     TFIM_PATH_PACKED = 1,   /* 128-bit loads, AND, v_bcnt, wave reductions */
-    TFIM_PATH_MFMA_I8 = 2   /* bits unpacked to 0/1 int8 in LDS, v_mfma_i32_*_i8 */
+    TFIM_PATH_MFMA_I8 = 2,  /* bits unpacked to 0/1 int8 in LDS, v_mfma_i32_*_i8 */
+    TFIM_PATH_MFMA_FP4 = 3  /* matrix count only: bits as exact e2m1 values (0, 1.0,
+                               2.0), v_mfma_f32_32x32x64_f8f6f4, twice the int8
+                               rate; fp32 sums stay exact (at most 2^25 per split) */
 };
 
 /* ---- context ---------------------------------------------------------- */

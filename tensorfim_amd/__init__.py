@@ -19,8 +19,9 @@ LIB_PATH = os.path.join(_PKG, "lib", "libtfim.so")
 BIN_DIR = os.path.join(_PKG, "bin")
 
 TFIM_OK = 0
-PATH_AUTO, PATH_PACKED, PATH_MFMA_I8 = 0, 1, 2
-_PATHS = {"auto": PATH_AUTO, "packed": PATH_PACKED, "mfma": PATH_MFMA_I8, "mfma_i8": PATH_MFMA_I8}
+PATH_AUTO, PATH_PACKED, PATH_MFMA_I8, PATH_MFMA_FP4 = 0, 1, 2, 3
+_PATHS = {"auto": PATH_AUTO, "packed": PATH_PACKED, "mfma": PATH_MFMA_I8, "mfma_i8": PATH_MFMA_I8,
+          "fp4": PATH_MFMA_FP4, "mfma_fp4": PATH_MFMA_FP4}
 
 _lib = None
 

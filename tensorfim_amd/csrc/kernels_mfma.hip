@@ -195,7 +195,48 @@ __device__ __forceinline__ v4i rp_expand_b(uint32_t w, int parity) {
     return r;
 }
 
-template <int TI, int TJ>
+// ---------------------------------------------------------------------------
+// The same kernel on the fp4 matrix cores (FMT = kFmtFp4).  CDNA4's
+// v_mfma_f32_32x32x64_f8f6f4 with both operands in e2m1 takes 32 k-values of
+// 4 bits per lane and runs at twice the int8 rate.  A 0/1 bit becomes one of
+// the exact e2m1 values 0, 0.5, 1.0 or 2.0 (nibble 0b0001 = 0.5, the one
+// subnormal; 0b0010 = 1.0; 0b0100 = 2.0; the sign bit is never set): a 32-bit
+// word w gives four operand dwords, dword g holding the bits 4n + g (n =
+// 0..7) in nibble n.  Side A keeps bits 4n, 4n+1, 4n+2 in place (0.5, 1.0,
+// 2.0; one AND each) and moves 4n+3 to 4n+2 (2.0); side B places each at the
+// reciprocal value (2.0, 1.0, 0.5, 0.5), so every matching pair contributes
+// exactly 1.0 and the fp32 accumulator holds the popcount itself, an integer
+// of at most 2^24 (a split holds at most 2^24 bits): nothing rounds.
+//
+// K map.  One MFMA step consumes a whole 32-bit word per lane: at step s
+// (0..7) of a stage lane half h supplies word 8h + s of the row, the same
+// word in A and B, so every bit of the stage is used exactly once.
+constexpr int kFmtI8 = 0, kFmtFp4 = 1;
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ v8i fp4_expand_a(uint32_t w) {
+    v8i r = {0, 0, 0, 0, 0, 0, 0, 0};
+    r[0] = (int)(w & 0x11111111u);          // bit 4n   -> 0.5
+    r[1] = (int)(w & 0x22222222u);          // bit 4n+1 -> 1.0
+    r[2] = (int)(w & 0x44444444u);          // bit 4n+2 -> 2.0
+    r[3] = (int)((w >> 1) & 0x44444444u);   // bit 4n+3 -> 2.0
+    return r;
+}
+
+__device__ __forceinline__ v8i fp4_expand_b(uint32_t w) {
+    v8i r = {0, 0, 0, 0, 0, 0, 0, 0};
+    r[0] = (int)((w << 2) & 0x44444444u);   // bit 4n   -> 2.0
+    r[1] = (int)(w & 0x22222222u);          // bit 4n+1 -> 1.0
+    r[2] = (int)((w >> 2) & 0x11111111u);   // bit 4n+2 -> 0.5
+    r[3] = (int)((w >> 3) & 0x11111111u);   // bit 4n+3 -> 0.5
+    return r;
+}
+
+template <int FMT> struct RpAcc { typedef v16i type; };
+template <> struct RpAcc<kFmtFp4> { typedef v16f type; };
+
+template <int TI, int TJ, int FMT = kFmtI8>
 __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__restrict__ A,
                                                         const uint64_t *__restrict__ B,
                                                         int32_t *__restrict__ C, int64_t b, int64_t d,
@@ -216,22 +257,18 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
     const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
 
     // loader: load j of thread t is uint4 part (idx & 3) of row (idx >> 2), idx = 256 j + t.
-    // Rows past the edge load the last row and zero the value: a load that is
-    // always from the global buffer stays a global load (a choice between the
+    // Rows past the edge load the last row again: an output depends only on
+    // its own A row and B row, and the outputs of those rows are never
+    // written, so their LDS copies need no zeroing.  A load that is always
+    // from the global buffer stays a global load (a choice between the
     // buffer and a zero constant would make it a flat load through a private
     // copy of the constant, and waits on LDS traffic would then also wait for
-    // the loads in flight).
-    // The edge mask is applied when the stage is written to LDS, so nothing
-    // reads the loaded registers before then and the loads stay in flight
-    // during the MFMAs of the current stage.
+    // the loads in flight); nothing reads the loaded registers before the
+    // stage is written to LDS, so the loads stay in flight during the MFMAs
+    // of the current stage.
     typedef uint32_t u4v __attribute__((ext_vector_type(4)));
     typedef const __attribute__((address_space(1))) u4v *gptr4;
     u4v ra[LA], rb[LB];
-    uint32_t ma[LA], mb[LB];
-#pragma unroll
-    for (int j = 0; j < LA; j++) ma[j] = row0 + ((j * 256 + tid) >> 2) < b ? ~0u : 0u;
-#pragma unroll
-    for (int j = 0; j < LB; j++) mb[j] = col0 + ((j * 256 + tid) >> 2) < d ? ~0u : 0u;
     auto gload = [&](int64_t kw) {
 #pragma unroll
         for (int j = 0; j < LA; j++) {
@@ -248,16 +285,16 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
 #pragma unroll
         for (int j = 0; j < LA; j++) {
             const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
-            *reinterpret_cast<u4v *>(&As[buf][r * RP_STRIDE + 4 * part]) = ra[j] & ma[j];
+            *reinterpret_cast<u4v *>(&As[buf][r * RP_STRIDE + 4 * part]) = ra[j];
         }
 #pragma unroll
         for (int j = 0; j < LB; j++) {
             const int idx = j * 256 + tid, r = idx >> 2, part = idx & 3;
-            *reinterpret_cast<u4v *>(&Bs[buf][r * RP_STRIDE + 4 * part]) = rb[j] & mb[j];
+            *reinterpret_cast<u4v *>(&Bs[buf][r * RP_STRIDE + 4 * part]) = rb[j];
         }
     };
 
-    v16i acc[TI][TJ];
+    typename RpAcc<FMT>::type acc[TI][TJ];
 #pragma unroll
     for (int i = 0; i < TI; i++)
 #pragma unroll
@@ -288,25 +325,46 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
                 const int r = wc * (32 * TJ) + j * 32 + l32;
                 pb[j] = *reinterpret_cast<const uint4 *>(&Bs[buf][r * RP_STRIDE + 8 * half + 4 * hk]);
             }
+            if constexpr (FMT == kFmtI8) {
 #pragma unroll
-            for (int s2 = 0; s2 < 8; s2++) {
-                const int wi = s2 >> 1, par = s2 & 1;
-                v4i ea[TI], eb[TJ];
+                for (int s2 = 0; s2 < 8; s2++) {
+                    const int wi = s2 >> 1, par = s2 & 1;
+                    v4i ea[TI], eb[TJ];
 #pragma unroll
-                for (int i = 0; i < TI; i++) {
-                    const uint32_t w = wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w;
-                    ea[i] = rp_expand_a(w, par);
+                    for (int i = 0; i < TI; i++) {
+                        const uint32_t w = wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w;
+                        ea[i] = rp_expand_a(w, par);
+                    }
+#pragma unroll
+                    for (int j = 0; j < TJ; j++) {
+                        const uint32_t w = wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w;
+                        eb[j] = rp_expand_b(w, par);
+                    }
+#pragma unroll
+                    for (int i = 0; i < TI; i++)
+#pragma unroll
+                        for (int j = 0; j < TJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[i], eb[j], acc[i][j], 0, 0, 0);
                 }
+            } else {
 #pragma unroll
-                for (int j = 0; j < TJ; j++) {
-                    const uint32_t w = wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w;
-                    eb[j] = rp_expand_b(w, par);
-                }
+                for (int wi = 0; wi < 4; wi++) {
+                    v8i ea[TI], eb[TJ];
 #pragma unroll
-                for (int i = 0; i < TI; i++)
+                    for (int i = 0; i < TI; i++)
+                        ea[i] = fp4_expand_a(wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w);
 #pragma unroll
                     for (int j = 0; j < TJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[i], eb[j], acc[i][j], 0, 0, 0);
+                        eb[j] = fp4_expand_b(wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w);
+                    // cbsz = blgp = 4: both operands e2m1; zero scale operands
+                    // select the unscaled instruction
+#pragma unroll
+                    for (int i = 0; i < TI; i++)
+#pragma unroll
+                        for (int j = 0; j < TJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[i], eb[j], acc[i][j], 4,
+                                                                                        4, 0, 0, 0, 0);
+                }
             }
         }
         // keep the stores of the next stage after this stage's LDS reads, so
@@ -326,7 +384,162 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
                 const int64_t r = row0 + wr * (32 * TI) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
                 const int64_t c = col0 + wc * (32 * TJ) + j * 32 + l32;
                 if (r < b && c < d) {
-                    const int32_t v = acc[i][j][q] >> 6;
+                    int32_t v;
+                    if constexpr (FMT == kFmtI8) v = acc[i][j][q] >> 6;
+                    else v = (int32_t)acc[i][j][q];
+                    if (atomic) atomicAdd(&C[r * d + c], v);
+                    else C[r * d + c] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// fp4 matrix count with 256 x 256 outputs per workgroup: waves in a 2 x 2
+// grid, each owning 4 x 4 fragments of 32 x 32 (256 fp32 accumulators per
+// lane, one wave per SIMD, one workgroup per CU).  Per 64 k-values a wave
+// expands 8 operand words for 16 MFMAs (3 VALU ops per MFMA, against 4.25
+// with 4 x 2 fragments).  Operand expansion as in k_popmat_mfma_rp.
+//
+// With one workgroup per CU nothing else hides memory latency, so stages of
+// SW words per row (SW = 16: 1,024 bits) are fetched straight into LDS by
+// global_load_lds_dwordx4 (no registers hold data in flight), double-
+// buffered: per stage, wait for this thread's loads, one barrier, read the
+// whole stage from LDS into registers, then issue the next stage's loads
+// into the other buffer (every wave read it before this barrier) and run
+// the stage's MFMAs from registers while those loads travel.  Reading
+// before issuing also keeps the compiler's own waits on the LDS loads from
+// covering the next stage.
+//
+// K map: at step s (0..SW/2-1) of a stage, lane half h supplies word
+// (SW/2) h + s of the row, the same word in A and B.
+//
+// LDS layout: a buffer holds 256 rows x 8 SW bytes per side, unpadded (one
+// load instruction of a wave fills 1 KB contiguously, lane l at byte 16 l).
+// A row has NC = SW / 2 chunks of 16 bytes; chunk c of row r sits in slot
+// c ^ f(r), f(r) = (r / (32 / SW)) mod NC, so any 16 consecutive rows of one
+// ds_read_b128 pass cover all 64 banks once.  Rows past the edge load the
+// last row again (their outputs are never written).
+constexpr int kF4StageWords = 16;
+
+template <int SW>
+__global__ __launch_bounds__(256, 1) void k_popmat_fp4_256(const uint64_t *__restrict__ A,
+                                                           const uint64_t *__restrict__ B, int32_t *__restrict__ C,
+                                                           int64_t b, int64_t d, int64_t pitch, int64_t k_per_split,
+                                                           int atomic, int tiles_n) {
+    constexpr int TF = 4, T = 256;
+    constexpr int RB = SW * 8;               // bytes per row per stage
+    constexpr int NC = SW / 2;               // 16-byte chunks per row
+    constexpr int RPI = 1024 / RB;           // rows per load instruction (1 KB)
+    constexpr int NI = T / RPI / 4;          // load instructions per wave per side
+    constexpr int FS = SW == 16 ? 1 : 2;     // f(r) = (r >> FS) & (NC - 1)
+    constexpr int SIDE = T * RB;             // bytes per side per buffer
+    constexpr int HW = NC / 2;               // uint4 per fragment per lane per stage
+    __shared__ __attribute__((aligned(1024))) unsigned char lds[2 * 2 * SIDE];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
+    const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
+    const int64_t row0 = (lin / tiles_n) * T, col0 = (lin % tiles_n) * T;
+    const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
+    const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
+    const int nst = (int)((kend - kbeg) / SW);   // kend - kbeg is a multiple of SW
+
+    // byte offsets of this lane's loads from the tile's first row (256 rows
+    // x pitch x 8 bytes < 2^32 for pitch < 2^21 words)
+    const char *atile = reinterpret_cast<const char *>(A + row0 * pitch);
+    const char *btile = reinterpret_cast<const char *>(B + col0 * pitch);
+    uint32_t aoff[NI], boff[NI];
+#pragma unroll
+    for (int j = 0; j < NI; j++) {
+        const int r = (4 * j + wave) * RPI + lane / NC;
+        const int chunk = (lane % NC) ^ ((r >> FS) & (NC - 1));
+        aoff[j] = (uint32_t)((std::min<int64_t>(row0 + r, b - 1) - row0) * pitch * 8 + 16 * chunk);
+        boff[j] = (uint32_t)((std::min<int64_t>(col0 + r, d - 1) - col0) * pitch * 8 + 16 * chunk);
+    }
+    typedef const __attribute__((address_space(1))) void *gvoid;
+    typedef __attribute__((address_space(3))) void *lvoid;
+    auto issue = [&](int st) {
+        const int64_t kb = (kbeg + SW * (int64_t)st) * 8;
+        unsigned char *ba = lds + (st & 1) * 2 * SIDE, *bb = ba + SIDE;
+#pragma unroll
+        for (int j = 0; j < NI; j++)
+            __builtin_amdgcn_global_load_lds((gvoid)(atile + kb + aoff[j]), (lvoid)(ba + (4 * j + wave) * 1024), 16, 0,
+                                             0);
+#pragma unroll
+        for (int j = 0; j < NI; j++)
+            __builtin_amdgcn_global_load_lds((gvoid)(btile + kb + boff[j]), (lvoid)(bb + (4 * j + wave) * 1024), 16, 0,
+                                             0);
+    };
+
+    v16f acc[TF][TF];
+#pragma unroll
+    for (int i = 0; i < TF; i++)
+#pragma unroll
+        for (int j = 0; j < TF; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[i][j][q] = 0.f;
+
+    // this lane's rows r = 128 wr + 32 i + l32 (A), 128 wc + 32 i + l32 (B)
+    // all have f(r) = f(l32)
+    const int swz = (l32 >> FS) & (NC - 1);
+    const int arow = (wr * 128 + l32) * RB, brow = (wc * 128 + l32) * RB;
+
+    if (nst > 0) issue(0);
+    for (int st = 0; st < nst; st++) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this thread's loads of stage st have landed
+        __syncthreads();
+        const unsigned char *ba = lds + (st & 1) * 2 * SIDE, *bb = ba + SIDE;
+        uint4 pa[HW][TF], pb[HW][TF];
+#pragma unroll
+        for (int hk = 0; hk < HW; hk++) {
+            const int slot = ((HW * half + hk) ^ swz) * 16;
+#pragma unroll
+            for (int i = 0; i < TF; i++) {
+                pa[hk][i] = *reinterpret_cast<const uint4 *>(ba + arow + i * 32 * RB + slot);
+                pb[hk][i] = *reinterpret_cast<const uint4 *>(bb + brow + i * 32 * RB + slot);
+            }
+        }
+        if (st + 1 < nst) issue(st + 1);
+#pragma unroll
+        for (int hk = 0; hk < HW; hk++) {
+#pragma unroll
+            for (int wi = 0; wi < 4; wi++) {
+                v8i ea[TF], eb[TF];
+#pragma unroll
+                for (int i = 0; i < TF; i++) {
+                    const uint4 &qa = pa[hk][i], &qb = pb[hk][i];
+                    ea[i] = fp4_expand_a(wi == 0 ? qa.x : wi == 1 ? qa.y : wi == 2 ? qa.z : qa.w);
+                    eb[i] = fp4_expand_b(wi == 0 ? qb.x : wi == 1 ? qb.y : wi == 2 ? qb.z : qb.w);
+                }
+#pragma unroll
+                for (int i = 0; i < TF; i++)
+#pragma unroll
+                    for (int j = 0; j < TF; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[i], eb[j], acc[i][j], 4, 4, 0,
+                                                                                    0, 0, 0);
+            }
+        }
+        // One wave per SIMD issues in order, so the operands of the next
+        // step are built between this step's MFMAs (one MFMA, then three
+        // VALU ops), not after them while the matrix core idles.
+#pragma unroll
+        for (int m = 0; m < HW * 4 * TF * TF; m++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < TF; i++)
+#pragma unroll
+        for (int j = 0; j < TF; j++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int64_t r = row0 + wr * (32 * TF) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
+                const int64_t c = col0 + wc * (32 * TF) + j * 32 + l32;
+                if (r < b && c < d) {
+                    const int32_t v = (int32_t)acc[i][j][q];
                     if (atomic) atomicAdd(&C[r * d + c], v);
                     else C[r * d + c] = v;
                 }
@@ -722,18 +935,25 @@ bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc 
 int rp_rows_per_tile(int64_t b) { return b <= 64 ? 64 : b <= 128 ? 128 : 256; }
 
 int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                            int64_t pitch_words, int split_k, KernelLaunch &L) {
+                            int64_t pitch_words, int split_k, KernelLaunch &L, bool fp4) {
     if (split_k < 1) split_k = 1;
     int64_t kps = (pitch_words + split_k - 1) / split_k;
-    kps = (kps + 7) / 8 * 8;                                     // whole 512-bit stages
+    kps = (kps + 15) / 16 * 16;                                  // whole 1,024-bit stages
     kps = std::min<int64_t>(kps, (int64_t)1 << 18);              // <= 2^24 bits per split
     const int64_t zs = (pitch_words + kps - 1) / kps;
     // row tile by the row count: 64, 128 or 256 rows x 128 columns (waves of
     // the engine often have few rows; a 256-row tile would count padding)
-    const int RA = rp_rows_per_tile(b), RB = 128;
+    const int RA = rp_rows_per_tile(b), RB = fp4 && RA == 256 && d > 128 ? 256 : 128;
     const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
-    L.func = RA == 64 ? (const void *)k_popmat_mfma_rp<1, 2>
-                      : RA == 128 ? (const void *)k_popmat_mfma_rp<2, 2> : (const void *)k_popmat_mfma_rp<4, 2>;
+    if (fp4 && RA == 256 && d > 128)
+        L.func = (const void *)k_popmat_fp4_256<kF4StageWords>;
+    else if (fp4)
+        L.func = RA == 64    ? (const void *)k_popmat_mfma_rp<1, 2, kFmtFp4>
+                 : RA == 128 ? (const void *)k_popmat_mfma_rp<2, 2, kFmtFp4>
+                             : (const void *)k_popmat_mfma_rp<4, 2, kFmtFp4>;
+    else
+        L.func = RA == 64 ? (const void *)k_popmat_mfma_rp<1, 2>
+                          : RA == 128 ? (const void *)k_popmat_mfma_rp<2, 2> : (const void *)k_popmat_mfma_rp<4, 2>;
     L.grid = dim3((unsigned)(tm * tn), (unsigned)zs);
     L.block = dim3(256);
     L.shmem = 0;
@@ -846,12 +1066,12 @@ void plan_wave_gather(const WaveDesc &d, const int32_t *C, int32_t n_rows, int32
     L.arg(res);
 }
 
-void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                           int64_t pitch_words, int split_k, hipStream_t s) {
+hipError_t launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                                 int64_t pitch_words, int split_k, hipStream_t s, bool fp4) {
     KernelLaunch L;
-    const int64_t zs = plan_popmat_mfma_rp(A, B, C, b, d, pitch_words, split_k, L);
+    const int64_t zs = plan_popmat_mfma_rp(A, B, C, b, d, pitch_words, split_k, L, fp4);
     if (zs > 1 && split_k <= 1) launch_zero_i32(C, b * d, s);   // the caller zeroes C only for split_k > 1
-    L.launch(s);
+    return L.launch(s);
 }
 
 void launch_popmat_mfma(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,

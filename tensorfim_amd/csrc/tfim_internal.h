@@ -204,15 +204,16 @@ void plan_drain_i32(int32_t *dev, int32_t *host, int64_t n_copy, int64_t n_zero,
 // Matrix AND-popcount, packed (VALU) path.  C must be zeroed when split_k>1.
 void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                           int64_t pitch_words, int split_k, hipStream_t s);
-// Matrix AND-popcount, int8 MFMA path, 256x128 tiles with the operand bytes
-// expanded in registers from packed LDS.  C must be zeroed when split_k>1.
-void launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                           int64_t pitch_words, int split_k, hipStream_t s);
+// Matrix AND-popcount on the matrix cores, 256x128 tiles with the operands
+// expanded in registers from packed LDS: int8 MFMA, or (fp4) e2m1 MFMA at
+// twice the int8 rate.  C must be zeroed when split_k>1.
+hipError_t launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
+                                 int64_t pitch_words, int split_k, hipStream_t s, bool fp4 = false);
 // The same as a launch description (graph node), with a row tile of 64, 128
 // or 256 rows (rp_rows_per_tile(b)) x 128 columns; C must be zeroed when the
 // returned number of word splits is above 1.
 int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                            int64_t pitch_words, int split_k, KernelLaunch &L);
+                            int64_t pitch_words, int split_k, KernelLaunch &L, bool fp4 = false);
 int rp_rows_per_tile(int64_t b);
 
 // Matrix-core wave path: the prefix masks of a wave's rows are built in

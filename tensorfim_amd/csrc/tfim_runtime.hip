@@ -1757,23 +1757,29 @@ static int popmat_check(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_
     if (b * pitch * 8 > A->bytes || d * pitch * 8 > B->bytes || b * d * 4 > C->bytes)
         return fail(c, TFIM_E_INVALID, "popcount_matrix: buffer too small");
     if (pitch * 64 > (int64_t)INT32_MAX) return fail(c, TFIM_E_RANGE, "popcount_matrix: rows too long for int32 counts");
-    if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED && path != TFIM_PATH_MFMA_I8)
+    if (path != TFIM_PATH_AUTO && path != TFIM_PATH_PACKED && path != TFIM_PATH_MFMA_I8 && path != TFIM_PATH_MFMA_FP4)
         return fail(c, TFIM_E_INVALID, "popcount_matrix: unknown path %d", path);
     return TFIM_OK;
 }
 
-static void popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
-                          int64_t d, int64_t pitch, int32_t path, int32_t split_k, int tile) {
+static hipError_t popmat_launch(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_buf *C, int64_t b,
+                                int64_t d, int64_t pitch, int32_t path, int32_t split_k, int tile) {
     if (split_k <= 0) split_k = 1;
     if (split_k > 1) launch_zero_i32((int32_t *)C->ptr, b * d, c->stream);
     const uint64_t *a = (const uint64_t *)A->ptr, *bb = (const uint64_t *)B->ptr;
     int32_t *cc = (int32_t *)C->ptr;
     if (path == TFIM_PATH_PACKED) {
         launch_popmat_packed(a, bb, cc, b, d, pitch, split_k, c->stream);
+    } else if (path == TFIM_PATH_MFMA_FP4) {
+        if (hipError_t e = launch_popmat_mfma_rp(a, bb, cc, b, d, pitch, split_k, c->stream, true)) return e;
     } else {
-        if (tile == 256) launch_popmat_mfma_rp(a, bb, cc, b, d, pitch, split_k, c->stream);
-        else launch_popmat_mfma(a, bb, cc, b, d, pitch, split_k, c->stream);
+        if (tile == 256) {
+            if (hipError_t e = launch_popmat_mfma_rp(a, bb, cc, b, d, pitch, split_k, c->stream)) return e;
+        } else {
+            launch_popmat_mfma(a, bb, cc, b, d, pitch, split_k, c->stream);
+        }
     }
+    return hipGetLastError();
 }
 
 // Tile and split-k of a matrix count.  split_k >= 1 and a configured
@@ -1794,15 +1800,14 @@ static int popmat_plan(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_b
         return TFIM_OK;
     }
     std::vector<int> tiles;
-    if (path == TFIM_PATH_PACKED) tiles = {0};
+    if (path == TFIM_PATH_PACKED || path == TFIM_PATH_MFMA_FP4) tiles = {0};
     else if (c->mfma_tile) tiles = {c->mfma_tile};
     else tiles = {256, 128};
     double best = 1e300;
     for (int t : tiles)
         for (int32_t sk : {1, 2, 4, 8, 16}) {
             HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-            popmat_launch(c, A, B, C, b, d, pitch, path, sk, t);
-            HIPCHK(c, hipGetLastError());
+            HIPCHK(c, popmat_launch(c, A, B, C, b, d, pitch, path, sk, t));
             HIPCHK(c, hipEventRecord(c->ev1, c->stream));
             HIPCHK(c, hipEventSynchronize(c->ev1));
             float ms = 0;
@@ -1824,8 +1829,7 @@ int tfim_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim
     int tile;
     int32_t split;
     if (int r = popmat_plan(c, A, B, C, b, d, pitch, path, split_k, &tile, &split)) return r;
-    popmat_launch(c, A, B, C, b, d, pitch, path, split, tile);
-    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, popmat_launch(c, A, B, C, b, d, pitch, path, split, tile));
     return TFIM_OK;
 }
 
@@ -1837,7 +1841,7 @@ int tfim_popcount_matrix_plan(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B,
     int t;
     int32_t sk;
     if (int r = popmat_plan(c, A, B, C, b, d, pitch, path, 0, &t, &sk)) return r;
-    *tile = path == TFIM_PATH_PACKED ? 64 : t;
+    *tile = path == TFIM_PATH_PACKED ? 64 : path == TFIM_PATH_MFMA_FP4 ? 256 : t;
     *split_k = sk;
     return TFIM_OK;
 }
@@ -1851,11 +1855,9 @@ int tfim_time_popcount_matrix(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B,
     int tile;
     int32_t split;
     if (int r = popmat_plan(c, A, B, C, b, d, pitch, path, split_k, &tile, &split)) return r;
-    for (int i = 0; i < warmup; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split, tile);
-    HIPCHK(c, hipGetLastError());
+    for (int i = 0; i < warmup; i++) HIPCHK(c, popmat_launch(c, A, B, C, b, d, pitch, path, split, tile));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int i = 0; i < reps; i++) popmat_launch(c, A, B, C, b, d, pitch, path, split, tile);
-    HIPCHK(c, hipGetLastError());
+    for (int i = 0; i < reps; i++) HIPCHK(c, popmat_launch(c, A, B, C, b, d, pitch, path, split, tile));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0;

@@ -136,7 +136,7 @@ def test_fill_bernoulli_matches_oracle(ctx):
     buf.free()
 
 
-@pytest.mark.parametrize("path", ["packed", "mfma", "mfma128", "mfma256"])
+@pytest.mark.parametrize("path", ["packed", "mfma", "mfma128", "mfma256", "fp4"])
 @pytest.mark.parametrize("b,d,n_bits,split", [(130, 70, 4096 + 65, 1), (256, 256, 65_536, 1),
                                                (200, 513, 40_000, 3), (64, 33, 2048 * 9, 8),
                                                (1, 1, 64, 1), (515, 300, 2048 * 3 + 1, 2),
@@ -155,11 +155,39 @@ def test_popcount_matrix_matches_oracle(ctx, path, b, d, n_bits, split):
     tile = {"mfma128": 128, "mfma256": 256}.get(path, 0)
     ctx.config("mfma_tile", tile)
     try:
-        ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path="packed" if path == "packed" else "mfma", split_k=split)
+        ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path=path if path in ("packed", "fp4") else "mfma",
+                            split_k=split)
     finally:
         ctx.config("mfma_tile", 0)
     got = dC.read(np.int32, b * d).reshape(b, d)
     np.testing.assert_array_equal(got, oracle_lib.popcount_matrix(A, B))
+    for x in (dA, dB, dC):
+        x.free()
+
+
+@pytest.mark.parametrize("path", ["mfma", "fp4", "packed"])
+def test_popcount_matrix_all_ones_largest_split(ctx, path):
+    """Rows of ones over 2^24 + 2,048 + 1 bits: the first word split holds the
+    largest number of bits a split may hold (2^24), so the matrix-core
+    accumulators reach their extreme (int8: 64 x 2^24 = 2^30; fp4: 2 x 2^24 =
+    2^25 in fp32, still exact); every count must be the row length, and a row
+    of zeros must give 0."""
+    n_bits = (1 << 24) + 2048 + 1
+    pitch = tf.pitch_words(n_bits)
+    b, d = 3, 2
+    ones = rand_rows(np.random.default_rng(0), 1, n_bits, 1.0)[0]
+    A = np.zeros((b, pitch), np.uint64)
+    B = np.zeros((d, pitch), np.uint64)
+    w = (n_bits + 63) // 64
+    A[0, :w] = ones
+    A[2, :w] = ones
+    B[:, :w] = ones
+    dA, dB, dC = ctx.buffer(A.nbytes), ctx.buffer(B.nbytes), ctx.buffer(b * d * 4)
+    dA.write(A)
+    dB.write(B)
+    ctx.popcount_matrix(dA, dB, dC, b, d, pitch, path=path, split_k=1)
+    got = dC.read(np.int32, b * d).reshape(b, d)
+    np.testing.assert_array_equal(got, np.array([[n_bits] * d, [0] * d, [n_bits] * d], np.int32))
     for x in (dA, dB, dC):
         x.free()
 
@@ -220,7 +248,7 @@ def test_wave_graph_replays(ctx):
     assert after["graphs"] - before["graphs"] <= 8   # instantiated once per (slot, kernel)
 
 
-@pytest.mark.parametrize("path", ["mfma", "packed"])
+@pytest.mark.parametrize("path", ["mfma", "packed", "fp4"])
 def test_popcount_matrix_tuned_split(ctx, path):
     """split_k = 0: the tuner times the candidates on first use of the shape,
     keeps its choice, and the result stays exact."""
@@ -241,7 +269,7 @@ def test_popcount_matrix_tuned_split(ctx, path):
         np.testing.assert_array_equal(dC.read(np.int32, b * d).reshape(b, d), exp)
     tile, split = ctx.popcount_matrix_plan(dA, dB, dC, b, d, pitch, path=path)
     assert split in (1, 2, 4, 8, 16)
-    assert tile in ((64,) if path == "packed" else (128, 256))
+    assert tile in {"packed": (64,), "fp4": (256,)}.get(path, (128, 256))
     for x in (dA, dB, dC):
         x.free()
 

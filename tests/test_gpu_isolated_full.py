@@ -76,7 +76,8 @@ def test_device_generator_matches_oracle(full):
                - 0.3) < 0.01
 
 
-@pytest.mark.parametrize("path,split", [("mfma", 8), ("packed", 1), ("mfma", 1), ("packed", 8)])
+@pytest.mark.parametrize("path,split", [("mfma", 8), ("fp4", 8), ("packed", 1), ("mfma", 1), ("fp4", 1),
+                                        ("packed", 8)])
 def test_full_size_counts_match_oracle(full, path, split):
     ctx, A, B, C, pitch, rows, _, _, expected = full
     print("path %s split_k %d" % (path, split), flush=True)
