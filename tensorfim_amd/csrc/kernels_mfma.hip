@@ -935,7 +935,8 @@ bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc 
 int rp_rows_per_tile(int64_t b) { return b <= 64 ? 64 : b <= 128 ? 128 : 256; }
 
 int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                            int64_t pitch_words, int split_k, KernelLaunch &L, bool fp4) {
+                            int64_t pitch_words, int split_k, KernelLaunch &L, bool fp4, bool wide) {
+    wide = wide && fp4;
     if (split_k < 1) split_k = 1;
     int64_t kps = (pitch_words + split_k - 1) / split_k;
     kps = (kps + 15) / 16 * 16;                                  // whole 1,024-bit stages
@@ -943,9 +944,9 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
     const int64_t zs = (pitch_words + kps - 1) / kps;
     // row tile by the row count: 64, 128 or 256 rows x 128 columns (waves of
     // the engine often have few rows; a 256-row tile would count padding)
-    const int RA = rp_rows_per_tile(b), RB = fp4 && RA == 256 && d > 128 ? 256 : 128;
+    const int RA = rp_rows_per_tile(b), RB = wide && RA == 256 && d > 128 ? 256 : 128;
     const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
-    if (fp4 && RA == 256 && d > 128)
+    if (RB == 256)
         L.func = (const void *)k_popmat_fp4_256<kF4StageWords>;
     else if (fp4)
         L.func = RA == 64    ? (const void *)k_popmat_mfma_rp<1, 2, kFmtFp4>

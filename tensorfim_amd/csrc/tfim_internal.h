@@ -210,10 +210,11 @@ void launch_popmat_packed(const uint64_t *A, const uint64_t *B, int32_t *C, int6
 hipError_t launch_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
                                  int64_t pitch_words, int split_k, hipStream_t s, bool fp4 = false);
 // The same as a launch description (graph node), with a row tile of 64, 128
-// or 256 rows (rp_rows_per_tile(b)) x 128 columns; C must be zeroed when the
-// returned number of word splits is above 1.
+// or 256 rows (rp_rows_per_tile(b)) x 128 columns, or for fp4 with `wide` and
+// more than 128 rows and columns 256 x 256 (one workgroup per CU); C must be
+// zeroed when the returned number of word splits is above 1.
 int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, int64_t b, int64_t d,
-                            int64_t pitch_words, int split_k, KernelLaunch &L, bool fp4 = false);
+                            int64_t pitch_words, int split_k, KernelLaunch &L, bool fp4 = false, bool wide = true);
 int rp_rows_per_tile(int64_t b);
 
 // Matrix-core wave path: the prefix masks of a wave's rows are built in

@@ -774,7 +774,9 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
         sk_arm = c->sk.pick();
         split = SplitTuner::split_of(sk_arm, base, stages);
     }
-    const int64_t zs = plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P);
+    // fp4 matrix cores (twice the int8 rate, same counts) with the 128-column
+    // tiles the occupancy rule above assumes
+    const int64_t zs = plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P, true, false);
     WaveDesc d{};
     d.item_ids = d_items;
     d.item_off = d_ioff;

@@ -73,13 +73,27 @@ int tfim_count_nodes_wait(tfim_ctx *, int64_t t, int32_t *out) { if(t<0) return 
 int tfim_wave_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *r, double *ms) { if(a)*a=0; if(b)*b=0; if(r)*r=0; if(ms)*ms=0; return 0; }
 int tfim_wave_graph_stats(tfim_ctx *, int64_t *g, int64_t *l) { if(g)*g=0; if(l)*l=0; return 0; }
 int tfim_wave_path_stats(tfim_ctx *, int64_t *a, int64_t *b, int64_t *m, int64_t *f) { if(a)*a=0; if(b)*b=0; if(m)*m=0; if(f)*f=0; return 0; }
-int tfim_mlp_create(tfim_ctx *, uint64_t, tfim_mlp **) { return -2; }
-void tfim_mlp_destroy(tfim_mlp *) {}
-int tfim_mlp_get_params(tfim_mlp *, float *) { return -2; }
-int tfim_mlp_set_params(tfim_mlp *, const float *) { return -2; }
-int tfim_mlp_forward(tfim_mlp *, const float *, int32_t, float *, int32_t) { return -2; }
-int tfim_mlp_forward_submit(tfim_mlp *, const float *, int32_t, int32_t) { return -2; }
-int tfim_mlp_forward_wait(tfim_mlp *, float *) { return -2; }
-int tfim_mlp_train_step(tfim_mlp *, const float *, const float *, const float *, int32_t, float, float *) { return -2; }
-int tfim_mlp_train(tfim_mlp *, const float *, const float *, const float *, int32_t, float, int32_t, float *) { return -2; }
+// A stand-in network: logits are a fixed hash of each feature row, so the
+// learned policy's verdicts (lookahead gate, tail order) land on arbitrary
+// nodes and the search logic around them is exercised; training does nothing.
+}
+struct tfim_mlp { std::vector<float> q; int32_t n = 0; };
+static float mock_logit(const float *x, int h) {
+  uint64_t z = 0x9E3779B97F4A7C15ull * (uint64_t)(h + 1);
+  for (int i = 0; i < TFIM_MLP_INPUTS; i++) { uint32_t b; memcpy(&b, x + i, 4); z = (z ^ b) * 0xBF58476D1CE4E5B9ull; z ^= z >> 29; }
+  return (float)((int64_t)(z >> 40) % 601 - 300) / 100.0f;
+}
+extern "C" {
+int tfim_mlp_create(tfim_ctx *, uint64_t, tfim_mlp **o) { *o = new tfim_mlp(); return 0; }
+void tfim_mlp_destroy(tfim_mlp *m) { delete m; }
+int tfim_mlp_get_params(tfim_mlp *, float *p) { memset(p, 0, TFIM_MLP_PARAMS * 4); return 0; }
+int tfim_mlp_set_params(tfim_mlp *, const float *) { return 0; }
+int tfim_mlp_forward(tfim_mlp *, const float *x, int32_t n, float *y, int32_t) {
+  for (int32_t i = 0; i < n; i++) for (int h = 0; h < 2; h++) y[i * 2 + h] = mock_logit(x + (size_t)i * TFIM_MLP_INPUTS, h);
+  return 0; }
+int tfim_mlp_forward_submit(tfim_mlp *m, const float *x, int32_t n, int32_t) {
+  m->q.assign(x, x + (size_t)n * TFIM_MLP_INPUTS); m->n = n; return 0; }
+int tfim_mlp_forward_wait(tfim_mlp *m, float *y) { return tfim_mlp_forward(m, m->q.data(), m->n, y, 1); }
+int tfim_mlp_train_step(tfim_mlp *, const float *, const float *, const float *, int32_t, float, float *l) { if (l) *l = 0; return 0; }
+int tfim_mlp_train(tfim_mlp *, const float *, const float *, const float *, int32_t, float, int32_t, float *l) { if (l) *l = 0; return 0; }
 }

@@ -28,7 +28,9 @@ def mock_engine(tmp_path_factory):
 ENGINES = [["--engine", "mafia"], ["--engine", "mafia", "--gate", "always"], ["--engine", "wave"],
            ["--engine", "plain", "--workers", "3"], ["--engine", "mafia", "--compl", "on"],
            ["--engine", "mafia", "--gate", "always", "--compl", "on"], ["--engine", "mafia", "--spec", "off"],
-           ["--engine", "mafia", "--compl", "on", "--spec", "off"], ["--engine", "mafia", "--rows", "lists"]]
+           ["--engine", "mafia", "--compl", "on", "--spec", "off"], ["--engine", "mafia", "--rows", "lists"],
+           ["--engine", "mafia", "--learned"], ["--engine", "mafia", "--learned", "--compl", "on"],
+           ["--engine", "mafia", "--learned", "--rank"]]
 
 
 @pytest.mark.parametrize("ds,thr", GOLD_PAIRS[:5])
