@@ -394,9 +394,9 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
 }
 
 // ---------------------------------------------------------------------------
-// fp4 matrix count with 256 x 256 outputs per workgroup: waves in a 2 x 2
-// grid, each owning 4 x 4 fragments of 32 x 32 (256 fp32 accumulators per
-// lane, one wave per SIMD, one workgroup per CU).  Per 64 k-values a wave
+// fp4 matrix count with 256 x 256 outputs per workgroup (TJ = 4): waves in a
+// 2 x 2 grid, each owning 4 x TJ fragments of 32 x 32 (256 fp32 accumulators
+// per lane, one wave per SIMD, one workgroup per CU).  Per 64 k-values a wave
 // expands 8 operand words for 16 MFMAs (3 VALU ops per MFMA, against 4.25
 // with 4 x 2 fragments).  Operand expansion as in k_popmat_mfma_rp.
 //
@@ -421,27 +421,28 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
 // last row again (their outputs are never written).
 constexpr int kF4StageWords = 16;
 
-template <int SW>
-__global__ __launch_bounds__(256, 1) void k_popmat_fp4_256(const uint64_t *__restrict__ A,
-                                                           const uint64_t *__restrict__ B, int32_t *__restrict__ C,
-                                                           int64_t b, int64_t d, int64_t pitch, int64_t k_per_split,
-                                                           int atomic, int tiles_n) {
-    constexpr int TF = 4, T = 256;
+template <int SW, int TJ>
+__global__ __launch_bounds__(256, TJ == 4 ? 1 : 2) void k_popmat_fp4_256(const uint64_t *__restrict__ A,
+                                                                        const uint64_t *__restrict__ B,
+                                                                        int32_t *__restrict__ C, int64_t b, int64_t d,
+                                                                        int64_t pitch, int64_t k_per_split, int atomic,
+                                                                        int tiles_n) {
+    constexpr int TI = 4, TA = 256, TB = 64 * TJ;   // fragments per wave in rows; rows of A and B per tile
     constexpr int RB = SW * 8;               // bytes per row per stage
     constexpr int NC = SW / 2;               // 16-byte chunks per row
     constexpr int RPI = 1024 / RB;           // rows per load instruction (1 KB)
-    constexpr int NI = T / RPI / 4;          // load instructions per wave per side
+    constexpr int NIA = TA / RPI / 4, NIB = TB / RPI / 4;   // load instructions per wave per side
     constexpr int FS = SW == 16 ? 1 : 2;     // f(r) = (r >> FS) & (NC - 1)
-    constexpr int SIDE = T * RB;             // bytes per side per buffer
+    constexpr int SIDEA = TA * RB, SIDEB = TB * RB;   // bytes per side per buffer
     constexpr int HW = NC / 2;               // uint4 per fragment per lane per stage
-    __shared__ __attribute__((aligned(1024))) unsigned char lds[2 * 2 * SIDE];
+    __shared__ __attribute__((aligned(1024))) unsigned char lds[2 * (SIDEA + SIDEB)];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, l32 = lane & 31;
     const int wr = wave >> 1, wc = wave & 1;
     const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
     const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
-    const int64_t row0 = (lin / tiles_n) * T, col0 = (lin % tiles_n) * T;
+    const int64_t row0 = (lin / tiles_n) * TA, col0 = (lin % tiles_n) * TB;
     const int64_t kbeg = (int64_t)blockIdx.y * k_per_split;
     const int64_t kend = std::min<int64_t>(pitch, kbeg + k_per_split);
     const int nst = (int)((kend - kbeg) / SW);   // kend - kbeg is a multiple of SW
@@ -450,73 +451,83 @@ __global__ __launch_bounds__(256, 1) void k_popmat_fp4_256(const uint64_t *__res
     // x pitch x 8 bytes < 2^32 for pitch < 2^21 words)
     const char *atile = reinterpret_cast<const char *>(A + row0 * pitch);
     const char *btile = reinterpret_cast<const char *>(B + col0 * pitch);
-    uint32_t aoff[NI], boff[NI];
+    uint32_t aoff[NIA], boff[NIB];
 #pragma unroll
-    for (int j = 0; j < NI; j++) {
+    for (int j = 0; j < NIA; j++) {
         const int r = (4 * j + wave) * RPI + lane / NC;
         const int chunk = (lane % NC) ^ ((r >> FS) & (NC - 1));
         aoff[j] = (uint32_t)((std::min<int64_t>(row0 + r, b - 1) - row0) * pitch * 8 + 16 * chunk);
+    }
+#pragma unroll
+    for (int j = 0; j < NIB; j++) {
+        const int r = (4 * j + wave) * RPI + lane / NC;
+        const int chunk = (lane % NC) ^ ((r >> FS) & (NC - 1));
         boff[j] = (uint32_t)((std::min<int64_t>(col0 + r, d - 1) - col0) * pitch * 8 + 16 * chunk);
     }
     typedef const __attribute__((address_space(1))) void *gvoid;
     typedef __attribute__((address_space(3))) void *lvoid;
     auto issue = [&](int st) {
         const int64_t kb = (kbeg + SW * (int64_t)st) * 8;
-        unsigned char *ba = lds + (st & 1) * 2 * SIDE, *bb = ba + SIDE;
+        unsigned char *ba = lds + (st & 1) * (SIDEA + SIDEB), *bb = ba + SIDEA;
 #pragma unroll
-        for (int j = 0; j < NI; j++)
+        for (int j = 0; j < NIA; j++)
             __builtin_amdgcn_global_load_lds((gvoid)(atile + kb + aoff[j]), (lvoid)(ba + (4 * j + wave) * 1024), 16, 0,
                                              0);
 #pragma unroll
-        for (int j = 0; j < NI; j++)
+        for (int j = 0; j < NIB; j++)
             __builtin_amdgcn_global_load_lds((gvoid)(btile + kb + boff[j]), (lvoid)(bb + (4 * j + wave) * 1024), 16, 0,
                                              0);
     };
 
-    v16f acc[TF][TF];
+    v16f acc[TI][TJ];
 #pragma unroll
-    for (int i = 0; i < TF; i++)
+    for (int i = 0; i < TI; i++)
 #pragma unroll
-        for (int j = 0; j < TF; j++)
+        for (int j = 0; j < TJ; j++)
 #pragma unroll
             for (int q = 0; q < 16; q++) acc[i][j][q] = 0.f;
 
-    // this lane's rows r = 128 wr + 32 i + l32 (A), 128 wc + 32 i + l32 (B)
-    // all have f(r) = f(l32)
+    // this lane's rows r = 128 wr + 32 i + l32 (A), 32 TJ wc + 32 j + l32
+    // (B) all have f(r) = f(l32)
     const int swz = (l32 >> FS) & (NC - 1);
-    const int arow = (wr * 128 + l32) * RB, brow = (wc * 128 + l32) * RB;
+    const int arow = (wr * 128 + l32) * RB, brow = (wc * 32 * TJ + l32) * RB;
 
     if (nst > 0) issue(0);
     for (int st = 0; st < nst; st++) {
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");   // this thread's loads of stage st have landed
         __syncthreads();
-        const unsigned char *ba = lds + (st & 1) * 2 * SIDE, *bb = ba + SIDE;
-        uint4 pa[HW][TF], pb[HW][TF];
+        const unsigned char *ba = lds + (st & 1) * (SIDEA + SIDEB), *bb = ba + SIDEA;
+        uint4 pa[HW][TI], pb[HW][TJ];
 #pragma unroll
         for (int hk = 0; hk < HW; hk++) {
             const int slot = ((HW * half + hk) ^ swz) * 16;
 #pragma unroll
-            for (int i = 0; i < TF; i++) {
+            for (int i = 0; i < TI; i++)
                 pa[hk][i] = *reinterpret_cast<const uint4 *>(ba + arow + i * 32 * RB + slot);
-                pb[hk][i] = *reinterpret_cast<const uint4 *>(bb + brow + i * 32 * RB + slot);
-            }
+#pragma unroll
+            for (int j = 0; j < TJ; j++)
+                pb[hk][j] = *reinterpret_cast<const uint4 *>(bb + brow + j * 32 * RB + slot);
         }
         if (st + 1 < nst) issue(st + 1);
 #pragma unroll
         for (int hk = 0; hk < HW; hk++) {
 #pragma unroll
             for (int wi = 0; wi < 4; wi++) {
-                v8i ea[TF], eb[TF];
+                v8i ea[TI], eb[TJ];
 #pragma unroll
-                for (int i = 0; i < TF; i++) {
-                    const uint4 &qa = pa[hk][i], &qb = pb[hk][i];
+                for (int i = 0; i < TI; i++) {
+                    const uint4 &qa = pa[hk][i];
                     ea[i] = fp4_expand_a(wi == 0 ? qa.x : wi == 1 ? qa.y : wi == 2 ? qa.z : qa.w);
-                    eb[i] = fp4_expand_b(wi == 0 ? qb.x : wi == 1 ? qb.y : wi == 2 ? qb.z : qb.w);
                 }
 #pragma unroll
-                for (int i = 0; i < TF; i++)
+                for (int j = 0; j < TJ; j++) {
+                    const uint4 &qb = pb[hk][j];
+                    eb[j] = fp4_expand_b(wi == 0 ? qb.x : wi == 1 ? qb.y : wi == 2 ? qb.z : qb.w);
+                }
 #pragma unroll
-                    for (int j = 0; j < TF; j++)
+                for (int i = 0; i < TI; i++)
+#pragma unroll
+                    for (int j = 0; j < TJ; j++)
                         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[i], eb[j], acc[i][j], 4, 4, 0,
                                                                                     0, 0, 0);
             }
@@ -524,20 +535,22 @@ __global__ __launch_bounds__(256, 1) void k_popmat_fp4_256(const uint64_t *__res
         // One wave per SIMD issues in order, so the operands of the next
         // step are built between this step's MFMAs (one MFMA, then three
         // VALU ops), not after them while the matrix core idles.
+        if constexpr (TJ == 4) {
 #pragma unroll
-        for (int m = 0; m < HW * 4 * TF * TF; m++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+            for (int m = 0; m < HW * 4 * TI * TJ; m++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+            }
         }
     }
 #pragma unroll
-    for (int i = 0; i < TF; i++)
+    for (int i = 0; i < TI; i++)
 #pragma unroll
-        for (int j = 0; j < TF; j++)
+        for (int j = 0; j < TJ; j++)
 #pragma unroll
             for (int q = 0; q < 16; q++) {
-                const int64_t r = row0 + wr * (32 * TF) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
-                const int64_t c = col0 + wc * (32 * TF) + j * 32 + l32;
+                const int64_t r = row0 + wr * (32 * TI) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
+                const int64_t c = col0 + wc * (32 * TJ) + j * 32 + l32;
                 if (r < b && c < d) {
                     const int32_t v = (int32_t)acc[i][j][q];
                     if (atomic) atomicAdd(&C[r * d + c], v);
@@ -944,10 +957,12 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
     const int64_t zs = (pitch_words + kps - 1) / kps;
     // row tile by the row count: 64, 128 or 256 rows x 128 columns (waves of
     // the engine often have few rows; a 256-row tile would count padding)
+    // (a 256 x 128 form of k_popmat_fp4_256 at two workgroups per CU, 512-bit
+    // stages, measured 42.5 ms against 41.0 ms on the 65,536 x 512 x 2^22 count)
     const int RA = rp_rows_per_tile(b), RB = wide && RA == 256 && d > 128 ? 256 : 128;
     const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
     if (RB == 256)
-        L.func = (const void *)k_popmat_fp4_256<kF4StageWords>;
+        L.func = (const void *)k_popmat_fp4_256<kF4StageWords, 4>;
     else if (fp4)
         L.func = RA == 64    ? (const void *)k_popmat_mfma_rp<1, 2, kFmtFp4>
                  : RA == 128 ? (const void *)k_popmat_mfma_rp<2, 2, kFmtFp4>
