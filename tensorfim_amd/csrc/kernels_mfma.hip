@@ -959,7 +959,10 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
     // the engine often have few rows; a 256-row tile would count padding)
     // (a 256 x 128 form of k_popmat_fp4_256 at two workgroups per CU, 512-bit
     // stages, measured 42.5 ms against 41.0 ms on the 65,536 x 512 x 2^22 count)
-    const int RA = rp_rows_per_tile(b), RB = wide && RA == 256 && d > 128 ? 256 : 128;
+    // (k_popmat_fp4_256 addresses a tile with 32-bit byte offsets: 256 rows of
+    // at most 2^21 words)
+    const int RA = rp_rows_per_tile(b),
+              RB = wide && RA == 256 && d > 128 && pitch_words < ((int64_t)1 << 21) ? 256 : 128;
     const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
     if (RB == 256)
         L.func = (const void *)k_popmat_fp4_256<kF4StageWords, 4>;
