@@ -572,8 +572,9 @@ __global__ __launch_bounds__(256, TJ == 4 ? 1 : 2) void k_popmat_fp4_256(const u
 //   - each row's prefix AND is built from Ps into the A operand block (As),
 //     one stage ahead, so row masks never leave LDS;
 //   - the survivor tile's words are staged as the B operand (Bs);
-//   - the MFMA loop of k_popmat_mfma_rp (register-expanded int8 operands,
-//     64 x popcount per accumulator).
+//   - the MFMA loop of k_popmat_mfma_rp in its fp4 form (register-expanded
+//     e2m1 operands on v_mfma_f32_32x32x64_f8f6f4, the fp32 accumulator
+//     holding the popcount itself).
 // One barrier per stage.  Epilogue: the tile's counts go to LDS, then
 //   - explicit rows: the wave's listed outputs that fall in this tile
 //     (entries grouped by column tile and row, resolved on the host) are
@@ -723,7 +724,7 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
         }
     };
 
-    v16i acc[TI][TJ];
+    v16f acc[TI][TJ];   // fp4 matrix cores: acc = the count itself
 #pragma unroll
     for (int i = 0; i < TI; i++)
 #pragma unroll
@@ -768,24 +769,20 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
                 pb[j] = *reinterpret_cast<const uint4 *>(&Bs[(cur * RB + r) * RP_STRIDE + 8 * half + 4 * hk]);
             }
 #pragma unroll
-            for (int s2 = 0; s2 < 8; s2++) {
-                const int wi = s2 >> 1, par = s2 & 1;
-                v4i ea[TI], eb[TJ];
+            for (int wi = 0; wi < 4; wi++) {
+                v8i ea[TI], eb[TJ];
 #pragma unroll
-                for (int i = 0; i < TI; i++) {
-                    const uint32_t w = wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w;
-                    ea[i] = rp_expand_a(w, par);
-                }
+                for (int i = 0; i < TI; i++)
+                    ea[i] = fp4_expand_a(wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w);
 #pragma unroll
-                for (int j = 0; j < TJ; j++) {
-                    const uint32_t w = wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w;
-                    eb[j] = rp_expand_b(w, par);
-                }
+                for (int j = 0; j < TJ; j++)
+                    eb[j] = fp4_expand_b(wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w);
 #pragma unroll
                 for (int i = 0; i < TI; i++)
 #pragma unroll
                     for (int j = 0; j < TJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[i], eb[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[i], eb[j], acc[i][j], 4, 4, 0,
+                                                                                    0, 0, 0);
             }
         }
         // the next stage's masks from its staged slot words (Ps[cur ^ 1],
@@ -809,7 +806,7 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
             for (int q = 0; q < 16; q++) {
                 const int r = wr * (32 * TI) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
                 const int c = wc * (32 * TJ) + j * 32 + l32;
-                T[r * TS + c] = acc[i][j][q] >> 6;
+                T[r * TS + c] = (int32_t)acc[i][j][q];
             }
     __syncthreads();
     const bool split = gridDim.y > 1;

@@ -217,6 +217,11 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
                             int64_t pitch_words, int split_k, KernelLaunch &L, bool fp4 = false, bool wide = true);
 int rp_rows_per_tile(int64_t b);
 
+// A training round of a learned policy's network was queued on stream s of
+// ctx's device: waves issued before it ends share the device with it, and
+// their times are kept out of the wave tuners.
+void ctx_model_training(tfim_ctx *c, hipStream_t s);
+
 // Matrix-core wave path: the prefix masks of a wave's rows are built in
 // `masks` ([n_rows][pitch] u64), C = masks AND-popcount survivors is counted
 // by k_popmat_mfma_rp ([n_rows][n_cols] int32), and k_wave_gather writes the

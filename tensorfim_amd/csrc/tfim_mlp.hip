@@ -305,6 +305,7 @@ int tfim_mlp_train(tfim_mlp *m, const float *x, const float *y, const float *mas
         launch_mlp_adam(m->P, m->G, m->Mo, m->Vo, m->Pb, lr, b1, b2, 1e-8f, c1, c2, s);
         MCHK(m, hipGetLastError());
     }
+    tfim::ctx_model_training(m->ctx, s);
     if (loss) MCHK(m, hipStreamSynchronize(s));   // without a loss request the steps stay queued
     return TFIM_OK;
 }

@@ -105,6 +105,12 @@ constexpr int kTuneArms = 8;
 constexpr int kArmKernel[kTuneArms] = {1, 2, 2, 2, 2, 2, 2, 5};
 constexpr int kArmTile[kTuneArms] = {32, 16, 32, 64, 16, 32, 64, 0};
 constexpr bool kArmWide[kTuneArms] = {false, false, false, false, true, true, true, false};
+// The matrix cores are taken unless the L2 kernel is predicted at less than
+// 1/kMfmaBias of their time: the per-unit costs are averages over waves of
+// different shapes, and the L2 kernel's time varies more with the shape
+// (clickstream, support 0.1 %: 25 ms of count kernels with the plain
+// comparison, 16 ms with every wave on the matrix cores).
+constexpr double kMfmaBias = 2.0;
 // A fused matrix-core wave that does not fit the kernel's limits (the caller
 // takes another path); not an error.
 constexpr int kFusedNoFit = 1;
@@ -159,6 +165,10 @@ struct tfim_ctx {
         int mask_arm = -1;     // mask kernel tuner's arm (-1: not tuned)
         double mask_work = 0;
         double sk_work = 0;
+        // the model was training when the wave was issued, or a training
+        // round was queued before it ended: its time is kept out of the tuners
+        bool contended = false;
+        int64_t train_seq = 0;
         // one executable graph per kernel sequence: stage -> [fill] -> ev0 ->
         // kernels -> ev1 -> drain
         uint64_t *d_mat = nullptr;   // matrix-core path: prefix masks, then the count matrix
@@ -177,16 +187,23 @@ struct tfim_ctx {
     };
     static constexpr int kWaveSlots = 4;
     Slot slots[kWaveSlots];
+    // Training rounds of a learned policy's network run on its own stream and
+    // share the device with the count kernels; `train_ev` follows the last one
+    // queued (guarded by wave_mu).
+    hipEvent_t train_ev = nullptr;
+    bool train_ev_set = false;
+    int64_t train_seq = 0;
     int64_t next_ticket = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     int64_t wave_launches = 0, wave_graph_replays = 0, wave_graphs = 0, wave_rows = 0, wave_lds_launches = 0;
+    int64_t wave_contended = 0;   // waves kept out of the tuners (model training alongside)
     double wave_ms = 0;
     int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel, 3 LDS kernel with wide staging, 4 matrix cores
     // second tuner, for waves outside the LDS kernel: arm 0 = L2 kernel, arm 1
     // = matrix cores in three launches, arm 2 = fused matrix-core launch; EMA
     // of kernel milliseconds per unit of each arm's work
-    double mf_cost[3] = {64.0 / 300e12 * 1e3, 64.0 / 1500e12 * 1e3, 64.0 / 1500e12 * 1e3};   // ms per u64 word (priors)
+    double mf_cost[3] = {64.0 / 300e12 * 1e3, 64.0 / 3000e12 * 1e3, 64.0 / 1500e12 * 1e3};   // ms per u64 word (priors)
     int64_t mf_samples[3] = {};
     int64_t mf_calls = 0, wave_mfma_launches = 0, wave_fused_launches = 0;
     // third tuner: each matrix-core path's word split, as 1/2, 1 or 2 times
@@ -360,6 +377,17 @@ int tfim_open(int device, tfim_ctx **out) {
     return TFIM_OK;
 }
 
+extern "C++" {
+namespace tfim {
+void ctx_model_training(tfim_ctx *c, hipStream_t s) {
+    std::lock_guard<std::mutex> g(c->wave_mu);
+    if (!c->train_ev && hipEventCreateWithFlags(&c->train_ev, hipEventDisableTiming) != hipSuccess) c->train_ev = nullptr;
+    c->train_ev_set = c->train_ev && hipEventRecord(c->train_ev, s) == hipSuccess;
+    c->train_seq++;
+}
+}  // namespace tfim
+}
+
 void tfim_close(tfim_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
@@ -380,6 +408,7 @@ void tfim_close(tfim_ctx *c) {
         }
     }
     if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->train_ev) hipEventDestroy(c->train_ev);
     if (c->ev1) hipEventDestroy(c->ev1);
     hipFree(c->p_ids); hipFree(c->p_off);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -640,6 +669,9 @@ int issue_wave(tfim_ctx *c, tfim_ctx::Slot &sl, int64_t desc_bytes, int32_t *fil
                int64_t n_zero, KernelLaunch *const *ks, int nk, bool events = true) {
     for (int i = 0; i < nk; i++)
         if (ks[i]->overflow || !ks[i]->func) return fail(c, TFIM_E_INVALID, "wave kernel %d: bad launch description", i);
+    // (the callers hold wave_mu)
+    sl.train_seq = c->train_seq;
+    sl.contended = c->train_ev_set && hipEventQuery(c->train_ev) == hipErrorNotReady;
     KernelLaunch S, Z, D;
     plan_stage16(sl.h_desc, sl.d_desc, desc_bytes, S);
     const bool has_fill = fill && n_fill > 0;
@@ -924,7 +956,7 @@ int wave_submit_fused(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, 
     }
     int64_t kps = (c->pitch + split - 1) / split;
     kps = (kps + 7) / 8 * 8;
-    kps = std::min<int64_t>(kps, (int64_t)1 << 18);   // <= 2^24 bits per split: accumulators stay below 2^30
+    kps = std::min<int64_t>(kps, (int64_t)1 << 18);   // <= 2^24 bits per split: the fp32 accumulators stay exact
     const int64_t zs = (c->pitch + kps - 1) / kps;
     WaveMfmaDesc d{};
     const char *dd = sl.d_desc;
@@ -1188,20 +1220,25 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             int arm = 1;
             if (wave_kernel == 0) {
                 c->mf_calls++;
-                // the costs start from priors (the L2 kernel's and the matrix
-                // cores' rates on MI355X, about 300 and 1,500 Tbit-op/s over
-                // their units of work) and become measured EMAs; another arm
-                // is tried on every 16th wave
-                arm = 0;
-                for (int a = 1; a < 3; a++)
-                    if ((a != 1 || fits) && w[a] * c->mf_cost[a] < w[arm] * c->mf_cost[arm]) arm = a;
-                if (c->mf_calls % 16 == 0) {
-                    // another arm; one predicted at more than twice the
-                    // chosen one's time only every eighth turn
+                // the costs start from priors (the L2 kernel's and the fp4
+                // matrix cores' rates on MI355X, about 300 and 3,000
+                // Tbit-op/s over their units of work) and become measured
+                // EMAs; the other arm is tried on every 16th wave (one
+                // predicted at more than twice the chosen one's time only
+                // every eighth turn, unless it has fewer than three samples:
+                // one wave of an unusual shape, or timed beside other device
+                // work, would otherwise shut it out for the rest of a run).
+                // The fused matrix-core kernel is not among the automatic
+                // arms: its prefix build from LDS made it the slowest arm on
+                // every workload measured (clickstream: 39 fused waves took
+                // 49 ms of count kernels where the three launches took 16;
+                // gene 5.9 ms against 1.8), so it runs only when asked for.
+                arm = fits && w[1] * c->mf_cost[1] < kMfmaBias * w[0] * c->mf_cost[0] ? 1 : 0;
+                if (c->mf_calls % 16 == 0 && fits) {
                     const int64_t turn = c->mf_calls / 16;
-                    int cand = (arm + 1 + (int)(turn % 2)) % 3;
-                    if (cand == 1 && !fits) cand = 2;
-                    if (cand != arm && (w[cand] * c->mf_cost[cand] <= 2.0 * w[arm] * c->mf_cost[arm] || (turn / 2) % 8 == 0))
+                    const int cand = 1 - arm;
+                    if (c->mf_samples[cand] < 3 || w[cand] * c->mf_cost[cand] <= 2.0 * w[arm] * c->mf_cost[arm] ||
+                        turn % 8 == 0)
                         arm = cand;
                 }
             }
@@ -1611,6 +1648,10 @@ int tfim_count_nodes_wait(tfim_ctx *c, int64_t ticket, int32_t *out) {
     c->wave_ms += ms;
     c->wave_launches++;
     c->wave_rows += sl.n_rows;
+    if (sl.contended || sl.train_seq != c->train_seq) {
+        c->wave_contended++;   // timed while the model trained: not a sample of its kernel's cost
+        return TFIM_OK;
+    }
     if (sl.sk_arm >= 0 && sl.sk_work > 0) (sl.sk_fused ? c->fk : c->sk).record(sl.sk_arm, (double)ms / sl.sk_work);
     if (sl.mask_arm >= 0 && sl.mask_work > 0) c->mask_tune.record(sl.mask_arm, (double)ms / sl.mask_work);
     if (sl.mf_arm >= 0 && sl.mf_work > 0) {
