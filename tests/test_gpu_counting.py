@@ -197,7 +197,7 @@ def test_popcount_matrix_all_ones_largest_split(ctx, path):
 @pytest.mark.parametrize("n_bits,n_items", [(1100, 30), (3200, 75), (400_003, 90), (2_000_000, 38)])
 def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
     """Every batched path (1 = L2-resident kernel, 2 = LDS-staged, 3 =
-    LDS-staged with wide staging, 4 = matrix cores: masks, int8-MFMA count of
+    LDS-staged with wide staging, 4 = matrix cores: masks, fp4 matrix count of
     all (row, survivor) pairs, gather; 5 = the fused matrix-core kernel) on
     the same waves,
     replayed from graphs or issued directly: uneven prefixes, uneven and empty
@@ -223,6 +223,31 @@ def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
     finally:
         ctx.config("wave_kernel", 0)
         ctx.config("graphs", 0)
+
+
+@pytest.mark.parametrize("n_rows", [250, 129])
+def test_wave_matrix_cores_large_waves(ctx, n_rows):
+    """Matrix-core waves (fp4 count) of more than 128 rows against 130
+    survivors (two column tiles) over a row length off every multiple of 2,048
+    bits, with uneven prefixes and tails and full-column rows, against the
+    oracle."""
+    rng = np.random.default_rng(n_rows)
+    n_bits, n_items = 3 * 2048 * 7 + 77, 131
+    rows = rand_rows(rng, n_items, n_bits, 0.7)
+    n_cols = n_items - 1
+    ctx.load_bitmaps(rows, n_bits, n_cols)
+    ctx.config("wave_kernel", 4)
+    try:
+        prefixes = [rng.choice(n_items, size=int(rng.integers(1, 7)), replace=False) for _ in range(n_rows)]
+        tails = [np.sort(rng.choice(n_cols, size=int(rng.integers(0, 60)), replace=False)) for _ in range(n_rows)]
+        got = ctx.count_nodes(prefixes, tails)
+        for p, t, g in zip(prefixes, tails, got):
+            np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols)[t])
+        full = ctx.count_nodes(prefixes)
+        for p, g in zip(prefixes, full):
+            np.testing.assert_array_equal(g, oracle_lib.node_counts(rows, p, n_cols))
+    finally:
+        ctx.config("wave_kernel", 0)
 
 
 def test_wave_graph_replays(ctx):
