@@ -127,8 +127,9 @@ int tfim_time_count_node(tfim_ctx *ctx, const int32_t *items, int32_t k, int32_t
  * synchronous.  path: TFIM_PATH_AUTO lets the library choose the kernel per
  * wave (its tuners); TFIM_PATH_PACKED keeps the wave on the packed (AND +
  * v_bcnt) kernels; TFIM_PATH_MFMA_I8 counts it on the matrix cores (the fused
- * kernel; the three-launch form -- prefix masks, int8-MFMA count with split-k,
- * gather -- for waves outside its limits). */
+ * kernel; the three-launch form -- prefix masks, fp4 matrix count with
+ * split-k, gather -- for waves outside its limits; both count on the fp4
+ * matrix cores, whose sums are the exact integer counts). */
 int tfim_count_nodes(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *item_off,
                      int32_t n_rows, const int32_t *col_ids, const int32_t *col_off,
                      int32_t *out, int32_t path);
@@ -204,17 +205,18 @@ int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *i
  *   "wave_kernel"  0 auto, 1 L2-resident kernel, 2 LDS-staged kernel (two
  *                  workgroups per CU), 3 LDS-staged with wide staging (one
  *                  workgroup per CU, up to 512 bits per lane per column),
- *                  4 matrix cores in three launches (prefix masks, int8-MFMA
+ *                  4 matrix cores in three launches (prefix masks, fp4 matrix
  *                  count of every (row, survivor) pair, gather; waves of up
  *                  to 8 GiB of masks), 5 fused matrix-core kernel (one
  *                  launch: prefix masks built in LDS from the wave's staged
- *                  item words, int8-MFMA count, split-k reduced in the
+ *                  item words, fp4 matrix count, split-k reduced in the
  *                  epilogue; waves of at most 256 distinct prefix items).
  *                  In auto mode the wave tuner also tries the fused kernel on
  *                  waves the LDS kernel takes, and waves outside the LDS
- *                  kernel go to the L2 kernel, the three-launch or the fused
+ *                  kernel go to the L2 kernel or the three-launch
  *                  matrix-core form by a second tuner that predicts each
- *                  one's time from its measured cost per unit of work
+ *                  one's time from its measured cost per unit of work (the
+ *                  fused kernel there only when forced)
  *   "tile_cols"    columns per work tile of the batched path: 8..64, or 0
  *                  (default) to let the wave tuner choose
  *   "mfma_tile"    workgroup tile of the int8-MFMA matrix count: 0 auto
@@ -247,7 +249,7 @@ int tfim_wave_stats(tfim_ctx *ctx, int64_t *launches, int64_t *lds_launches,
                     int64_t *rows, double *kernel_ms);
 
 /* Waves since tfim_open by kernel: the L2-resident kernel, the LDS-staged
- * kernel, the matrix-core path in three launches (prefix masks, int8-MFMA
+ * kernel, the matrix-core path in three launches (prefix masks, fp4 matrix
  * count of every (row, survivor) pair with split-k over word ranges, gather
  * of the wave's outputs) and the fused matrix-core kernel (the same in one
  * launch, masks built in LDS, split-k reduced in the epilogue) -- the forms

@@ -1163,7 +1163,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
         }
     }
     // Waves outside the LDS kernel go to the L2 kernel (counts only the
-    // listed pairs) or to the matrix cores (masks + int8 MFMA count of every
+    // listed pairs) or to the matrix cores (masks + fp4 matrix count of every
     // (row, survivor) pair + gather).  wave_kernel 4 forces the second; in
     // auto mode a second tuner predicts each arm's time as its work times its
     // measured cost per unit (priors from the two paths' rates on MI355X
@@ -1203,7 +1203,7 @@ int wave_submit(tfim_ctx *c, const int32_t *item_ids, const int32_t *item_off, i
             }
             const double w_l2 = ((double)n_item_ids + pairs) * (double)c->pitch;
             // padded matrix pairs, plus the mask build's traffic weighted by
-            // the ratio of int8-MFMA word-pair rate to HBM word rate (~39)
+            // the ratio of the matrix cores' word-pair rate to HBM word rate (~39)
             const int ra = rp_rows_per_tile(n_rows);
             const double w_mf = ((double)((n_rows + ra - 1) / ra * ra) * (double)((c->n_cols + 127) / 128 * 128) +
                                  39.0 * (double)(n_item_ids + n_rows)) *
