@@ -63,14 +63,14 @@ PACKED_TRAFFIC_BYTES = 141876016.75 * 1024.0
 TRAFFIC_KERNEL = "k_popmat_mfma_rp"
 # The same readings for the fp4 matrix-core kernel (`bench.py --no-e2e --no-cpu
 # --paths fp4,mfma`, profiles/popmat_fp4_65536x512x4M_kernel_stats.md): its
-# loads are 128-byte row segments (16 words per row per stage), for which
-# FETCH_SIZE reads half (the factor 2.00 of the plain copy kernel's 128-byte
-# requests); WRITE_SIZE exact.  Read 19,644,622 KiB x 2 + written 1,048,576 KiB.
+# loads are 64-byte row segments (8 words per row per stage), read exactly by
+# FETCH_SIZE (factor 1.00, as for the int8 kernel); WRITE_SIZE exact.
+# Read 36,051,761 KiB + written 1,048,576 KiB (mean over dispatches).
 FP4_TRAFFIC_KERNEL = "k_popmat_fp4_256"
-FP4_TRAFFIC_BYTES = (19644622 * 2 + 1048576) * 1024.0
+FP4_TRAFFIC_BYTES = (36051761 + 1048576) * 1024.0
 HEAD_DESC = {
     "fp4": "fp4 matrix-core path (each bit an exact e2m1 value, v_mfma_f32_32x32x64_f8f6f4, fp32 sums exact; "
-           "256x256 tiles, stages fetched straight into LDS, operands expanded in registers)",
+           "256x256 tiles, eight waves, stages fetched straight into LDS, operands expanded in registers)",
     "mfma": "int8-MFMA path (256x128 tiles, operand bytes expanded in registers from packed bits in LDS)",
     "packed": "packed AND + v_bcnt path (VALU)",
 }

@@ -394,14 +394,17 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
 }
 
 // ---------------------------------------------------------------------------
-// fp4 matrix count with 256 x 256 outputs per workgroup (TJ = 4): waves in a
-// 2 x 2 grid, each owning 4 x TJ fragments of 32 x 32 (256 fp32 accumulators
-// per lane, one wave per SIMD, one workgroup per CU).  Per 64 k-values a wave
-// expands 8 operand words for 16 MFMAs (3 VALU ops per MFMA, against 4.25
-// with 4 x 2 fragments).  Operand expansion as in k_popmat_mfma_rp.
+// fp4 matrix count with 256 x 256 outputs per workgroup (one workgroup per
+// CU): NW waves in a 2 x (NW / 2) grid, each owning 4 x TJ fragments of
+// 32 x 32.  The form in use is NW = 8, TJ = 2: two waves per SIMD, 128 fp32
+// accumulators per lane, 4.25 VALU ops of operand expansion per MFMA that the
+// other wave on the SIMD overlaps with its MFMAs.  (NW = 4, TJ = 4 -- one
+// wave per SIMD, 256 accumulators, 3 VALU ops per MFMA that its own MFMAs do
+// not hide -- measured 1.5 % slower.)  Operand expansion as in
+// k_popmat_mfma_rp.
 //
-// With one workgroup per CU nothing else hides memory latency, so stages of
-// SW words per row (SW = 16: 1,024 bits) are fetched straight into LDS by
+// With one workgroup per CU no other workgroup hides memory latency, so
+// stages of SW words per row (SW = 8: 512 bits) are fetched straight into LDS by
 // global_load_lds_dwordx4 (no registers hold data in flight), double-
 // buffered: per stage, wait for this thread's loads, one barrier, read the
 // whole stage from LDS into registers, then issue the next stage's loads
@@ -419,19 +422,17 @@ __global__ __launch_bounds__(256, 2) void k_popmat_mfma_rp(const uint64_t *__res
 // c ^ f(r), f(r) = (r / (32 / SW)) mod NC, so any 16 consecutive rows of one
 // ds_read_b128 pass cover all 64 banks once.  Rows past the edge load the
 // last row again (their outputs are never written).
-constexpr int kF4StageWords = 16;
-
-template <int SW, int TJ>
-__global__ __launch_bounds__(256, TJ == 4 ? 1 : 2) void k_popmat_fp4_256(const uint64_t *__restrict__ A,
-                                                                        const uint64_t *__restrict__ B,
-                                                                        int32_t *__restrict__ C, int64_t b, int64_t d,
-                                                                        int64_t pitch, int64_t k_per_split, int atomic,
-                                                                        int tiles_n) {
-    constexpr int TI = 4, TA = 256, TB = 64 * TJ;   // fragments per wave in rows; rows of A and B per tile
+template <int SW, int TJ, int NW = 4>
+__global__ __launch_bounds__(64 * NW, (TJ == 4 || NW == 8) ? 1 : 2) void k_popmat_fp4_256(
+    const uint64_t *__restrict__ A, const uint64_t *__restrict__ B, int32_t *__restrict__ C, int64_t b, int64_t d,
+    int64_t pitch, int64_t k_per_split, int atomic, int tiles_n) {
+    // NW waves in a 2 x (NW / 2) grid, each owning 4 x TJ fragments
+    constexpr int WC = NW / 2;
+    constexpr int TI = 4, TA = 256, TB = 32 * TJ * WC;   // fragments per wave in rows; rows of A and B per tile
     constexpr int RB = SW * 8;               // bytes per row per stage
     constexpr int NC = SW / 2;               // 16-byte chunks per row
     constexpr int RPI = 1024 / RB;           // rows per load instruction (1 KB)
-    constexpr int NIA = TA / RPI / 4, NIB = TB / RPI / 4;   // load instructions per wave per side
+    constexpr int NIA = TA / RPI / NW, NIB = TB / RPI / NW;   // load instructions per wave per side
     constexpr int FS = SW == 16 ? 1 : 2;     // f(r) = (r >> FS) & (NC - 1)
     constexpr int SIDEA = TA * RB, SIDEB = TB * RB;   // bytes per side per buffer
     constexpr int HW = NC / 2;               // uint4 per fragment per lane per stage
@@ -439,7 +440,7 @@ __global__ __launch_bounds__(256, TJ == 4 ? 1 : 2) void k_popmat_fp4_256(const u
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int half = lane >> 5, l32 = lane & 31;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave / WC, wc = wave % WC;
     const int64_t nblk = (int64_t)gridDim.x, bid = blockIdx.x;
     const int64_t lin = (nblk % 8 == 0) ? (bid % 8) * (nblk / 8) + bid / 8 : bid;
     const int64_t row0 = (lin / tiles_n) * TA, col0 = (lin % tiles_n) * TB;
@@ -454,13 +455,13 @@ __global__ __launch_bounds__(256, TJ == 4 ? 1 : 2) void k_popmat_fp4_256(const u
     uint32_t aoff[NIA], boff[NIB];
 #pragma unroll
     for (int j = 0; j < NIA; j++) {
-        const int r = (4 * j + wave) * RPI + lane / NC;
+        const int r = (NW * j + wave) * RPI + lane / NC;
         const int chunk = (lane % NC) ^ ((r >> FS) & (NC - 1));
         aoff[j] = (uint32_t)((std::min<int64_t>(row0 + r, b - 1) - row0) * pitch * 8 + 16 * chunk);
     }
 #pragma unroll
     for (int j = 0; j < NIB; j++) {
-        const int r = (4 * j + wave) * RPI + lane / NC;
+        const int r = (NW * j + wave) * RPI + lane / NC;
         const int chunk = (lane % NC) ^ ((r >> FS) & (NC - 1));
         boff[j] = (uint32_t)((std::min<int64_t>(col0 + r, d - 1) - col0) * pitch * 8 + 16 * chunk);
     }
@@ -471,11 +472,11 @@ __global__ __launch_bounds__(256, TJ == 4 ? 1 : 2) void k_popmat_fp4_256(const u
         unsigned char *ba = lds + (st & 1) * (SIDEA + SIDEB), *bb = ba + SIDEA;
 #pragma unroll
         for (int j = 0; j < NIA; j++)
-            __builtin_amdgcn_global_load_lds((gvoid)(atile + kb + aoff[j]), (lvoid)(ba + (4 * j + wave) * 1024), 16, 0,
+            __builtin_amdgcn_global_load_lds((gvoid)(atile + kb + aoff[j]), (lvoid)(ba + (NW * j + wave) * 1024), 16, 0,
                                              0);
 #pragma unroll
         for (int j = 0; j < NIB; j++)
-            __builtin_amdgcn_global_load_lds((gvoid)(btile + kb + boff[j]), (lvoid)(bb + (4 * j + wave) * 1024), 16, 0,
+            __builtin_amdgcn_global_load_lds((gvoid)(btile + kb + boff[j]), (lvoid)(bb + (NW * j + wave) * 1024), 16, 0,
                                              0);
     };
 
@@ -961,8 +962,12 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
     const int RA = rp_rows_per_tile(b),
               RB = wide && RA == 256 && d > 128 && pitch_words < ((int64_t)1 << 21) ? 256 : 128;
     const int64_t tm = (b + RA - 1) / RA, tn = (d + RB - 1) / RB;
+    // 256 x 256 tiles: eight waves (two per SIMD) of 4 x 2 fragments with
+    // 512-bit stages, 39.6 ms on the 65,536 x 512 x 2^22 count, against 40.2
+    // ms for four waves (one per SIMD) of 4 x 4 fragments with 1,024-bit
+    // stages (the other wave on a SIMD hides the operand expansion)
     if (RB == 256)
-        L.func = (const void *)k_popmat_fp4_256<kF4StageWords, 4>;
+        L.func = (const void *)k_popmat_fp4_256<8, 2, 8>;
     else if (fp4)
         L.func = RA == 64    ? (const void *)k_popmat_mfma_rp<1, 2, kFmtFp4>
                  : RA == 128 ? (const void *)k_popmat_mfma_rp<2, 2, kFmtFp4>
@@ -971,7 +976,7 @@ int64_t plan_popmat_mfma_rp(const uint64_t *A, const uint64_t *B, int32_t *C, in
         L.func = RA == 64 ? (const void *)k_popmat_mfma_rp<1, 2>
                           : RA == 128 ? (const void *)k_popmat_mfma_rp<2, 2> : (const void *)k_popmat_mfma_rp<4, 2>;
     L.grid = dim3((unsigned)(tm * tn), (unsigned)zs);
-    L.block = dim3(256);
+    L.block = dim3(RB == 256 ? 512 : 256);
     L.shmem = 0;
     L.reset();
     L.arg(A);
