@@ -353,12 +353,14 @@ def bench_e2e(args):
     sets = {}
     # batched+learned and batched MAFIA; the reference TensorFIM engine's
     # enumeration in waves, autotuned and with every wave on the matrix cores
-    # (its int8-MFMA split-k form); the plain per-node engine
+    # (its split-k matrix form, on the fp4 and on the int8 matrix cores); the
+    # plain per-node engine
     # (the reference's split-k ablation, BMMA_SPLITK_OFF, kernel_bmma.cu:351-368
     # and README.md:80: the same matrix-core waves with one word range per tile)
     nodes = os.path.join(tmp, "dense_replicated.nodes")
     runs = [("batched_learned", "mafia", ["--learned"]), ("batched", "mafia", []),
             ("wave_autotuned", "wave", []), ("wave_mfma_splitk", "wave", ["--wave-kernel", "4"]),
+            ("wave_mfma_int8_splitk", "wave", ["--wave-kernel", "4", "--mfma-format", "int8"]),
             ("wave_mfma_splitk_off", "wave", ["--wave-kernel", "4", "--split-k", "off"]),
             ("plain_w8", "plain", ["--workers", "8", "--dump-nodes", nodes]), ("plain", "plain", [])]
     for name, engine, extra in runs:

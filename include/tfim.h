@@ -223,6 +223,10 @@ int tfim_count_nodes_ex(tfim_ctx *ctx, const int32_t *item_ids, const int32_t *i
  *                  (= 256), 256 (256 x 128 outputs, operand bytes expanded
  *                  in registers from packed bits in LDS) or 128 (128 x 128,
  *                  bytes expanded into LDS)
+ *   "mfma_format"  matrix-core waves (wave_kernel 4 and 5 and their automatic
+ *                  use): 0 (default) the fp4 matrix cores, each bit an exact
+ *                  e2m1 value; 1 the int8 matrix cores (bits as 0/1 bytes,
+ *                  int32 sums).  Both give the same exact counts.
  *   "mfma_split"   matrix-core waves: 0 (default) split the word axis by an
  *                  occupancy rule scaled by a tuner; 1 no split, one word
  *                  range per tile (the reference's BMMA_SPLITK_OFF ablation,

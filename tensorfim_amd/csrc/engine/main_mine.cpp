@@ -4,7 +4,8 @@
 //             [--replicate K] [--out PATH] [--wave-cap N] [--device D]
 //             [--workers N] [--host-threads N] [--learned] [--rank] [--gate indep|always]
 //             [--compl auto|on|off] [--spec on|off]
-//             [--wave-kernel 0|1|2|3] [--graphs 0|1] [--parse-threads N]
+//             [--wave-kernel 0|1|2|3|4|5] [--mfma-format fp4|int8] [--split-k auto|off]
+//             [--graphs 0|1] [--parse-threads N]
 //             [--prep gpu|host] [--model PATH] [--trace PATH] [--verbose]
 //
 // Writes "<transactionFile>-<%f support>=Results.txt" (or PATH) in the
@@ -40,6 +41,7 @@ int main(int argc, char **argv) {
     int replicate = 0, device = 0;
     std::string dump_nodes;        // plain engine: write every counted node (original ids), one per line
     std::string split_k = "auto";  // matrix-core waves: auto | off (one word range per tile)
+    std::string mfma_format = "fp4";   // matrix-core waves: fp4 | int8 matrix cores
     std::vector<std::vector<int32_t>> node_log;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; i++) {
@@ -71,6 +73,7 @@ int main(int argc, char **argv) {
         else if (a == "--verbose") opt.verbose = true;
         else if (a == "--dump-nodes") dump_nodes = next();
         else if (a == "--split-k") split_k = next();
+        else if (a == "--mfma-format") mfma_format = next();
         else pos.push_back(a);
     }
     if (pos.size() < 2) {
@@ -93,6 +96,12 @@ int main(int argc, char **argv) {
     }
     if (wave_kernel && tfim_config(ctx, "wave_kernel", wave_kernel) != TFIM_OK) {
         fprintf(stderr, "%s\n", tfim_last_error(ctx));
+        tfim_close(ctx);
+        return 1;
+    }
+    if ((mfma_format != "fp4" && mfma_format != "int8") ||
+        tfim_config(ctx, "mfma_format", mfma_format == "int8" ? 1 : 0) != TFIM_OK) {
+        fprintf(stderr, "--mfma-format must be fp4 or int8\n");
         tfim_close(ctx);
         return 1;
     }

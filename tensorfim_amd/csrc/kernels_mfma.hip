@@ -590,7 +590,7 @@ constexpr int kFusedRegSlots = 16;   // prefix slots per row held in registers
 // once read 16 different slots from disjoint LDS banks
 constexpr int PS_STRIDE = 20;
 
-template <int TI, int TJ>
+template <int TI, int TJ, int FMT = kFmtFp4>
 __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict__ bm, int64_t pitch,
                                                       WaveMfmaDesc d) {
     constexpr int RA = 64 * TI, RB = 64 * TJ, TS = RB + 1;
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
         }
     };
 
-    v16f acc[TI][TJ];   // fp4 matrix cores: acc = the count itself
+    typename RpAcc<FMT>::type acc[TI][TJ];   // fp4: the count itself; int8: 64 x the count
 #pragma unroll
     for (int i = 0; i < TI; i++)
 #pragma unroll
@@ -769,21 +769,40 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
                 const int r = wc * (32 * TJ) + j * 32 + l32;
                 pb[j] = *reinterpret_cast<const uint4 *>(&Bs[(cur * RB + r) * RP_STRIDE + 8 * half + 4 * hk]);
             }
+            if constexpr (FMT == kFmtFp4) {
 #pragma unroll
-            for (int wi = 0; wi < 4; wi++) {
-                v8i ea[TI], eb[TJ];
+                for (int wi = 0; wi < 4; wi++) {
+                    v8i ea[TI], eb[TJ];
 #pragma unroll
-                for (int i = 0; i < TI; i++)
-                    ea[i] = fp4_expand_a(wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w);
-#pragma unroll
-                for (int j = 0; j < TJ; j++)
-                    eb[j] = fp4_expand_b(wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w);
-#pragma unroll
-                for (int i = 0; i < TI; i++)
+                    for (int i = 0; i < TI; i++)
+                        ea[i] = fp4_expand_a(wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w);
 #pragma unroll
                     for (int j = 0; j < TJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[i], eb[j], acc[i][j], 4, 4, 0,
-                                                                                    0, 0, 0);
+                        eb[j] = fp4_expand_b(wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w);
+#pragma unroll
+                    for (int i = 0; i < TI; i++)
+#pragma unroll
+                        for (int j = 0; j < TJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ea[i], eb[j], acc[i][j], 4, 4,
+                                                                                        0, 0, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int s2 = 0; s2 < 8; s2++) {
+                    const int wi = s2 >> 1, par = s2 & 1;
+                    v4i ea[TI], eb[TJ];
+#pragma unroll
+                    for (int i = 0; i < TI; i++)
+                        ea[i] = rp_expand_a(wi == 0 ? pa[i].x : wi == 1 ? pa[i].y : wi == 2 ? pa[i].z : pa[i].w, par);
+#pragma unroll
+                    for (int j = 0; j < TJ; j++)
+                        eb[j] = rp_expand_b(wi == 0 ? pb[j].x : wi == 1 ? pb[j].y : wi == 2 ? pb[j].z : pb[j].w, par);
+#pragma unroll
+                    for (int i = 0; i < TI; i++)
+#pragma unroll
+                        for (int j = 0; j < TJ; j++)
+                            acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ea[i], eb[j], acc[i][j], 0, 0, 0);
+                }
             }
         }
         // the next stage's masks from its staged slot words (Ps[cur ^ 1],
@@ -807,7 +826,8 @@ __global__ __launch_bounds__(256, 2) void k_wave_mfma(const uint64_t *__restrict
             for (int q = 0; q < 16; q++) {
                 const int r = wr * (32 * TI) + i * 32 + 8 * (q >> 2) + 4 * half + (q & 3);
                 const int c = wc * (32 * TJ) + j * 32 + l32;
-                T[r * TS + c] = (int32_t)acc[i][j][q];
+                if constexpr (FMT == kFmtFp4) T[r * TS + c] = (int32_t)acc[i][j][q];
+                else T[r * TS + c] = acc[i][j][q] >> 6;
             }
     __syncthreads();
     const bool split = gridDim.y > 1;
@@ -899,11 +919,17 @@ size_t wave_mfma_lds_bytes(int ti, int tj, int n_slots, int max_tile_list) {
 }
 
 bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc &d, int ti, int tj, int tiles_m,
-                    int splits, size_t lds_bytes, KernelLaunch &L) {
+                    int splits, size_t lds_bytes, KernelLaunch &L, bool int8) {
     const void *f = nullptr;
-    if (tj == 1) f = ti == 1 ? (const void *)k_wave_mfma<1, 1> : ti == 2 ? (const void *)k_wave_mfma<2, 1>
-                                                                          : (const void *)k_wave_mfma<4, 1>;
-    else f = ti == 1 ? (const void *)k_wave_mfma<1, 2> : (const void *)k_wave_mfma<2, 2>;
+    if (int8) {
+        if (tj == 1) f = ti == 1 ? (const void *)k_wave_mfma<1, 1, kFmtI8> : ti == 2 ? (const void *)k_wave_mfma<2, 1, kFmtI8>
+                                                                                      : (const void *)k_wave_mfma<4, 1, kFmtI8>;
+        else f = ti == 1 ? (const void *)k_wave_mfma<1, 2, kFmtI8> : (const void *)k_wave_mfma<2, 2, kFmtI8>;
+    } else {
+        if (tj == 1) f = ti == 1 ? (const void *)k_wave_mfma<1, 1> : ti == 2 ? (const void *)k_wave_mfma<2, 1>
+                                                                              : (const void *)k_wave_mfma<4, 1>;
+        else f = ti == 1 ? (const void *)k_wave_mfma<1, 2> : (const void *)k_wave_mfma<2, 2>;
+    }
     if ((tj == 2 && ti == 4) || d.n_slots < 1 || d.n_slots > 64 * kFusedSlotLoads || lds_bytes > kFusedMaxLds)
         return false;
     {
@@ -914,9 +940,11 @@ bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc 
         if (hipGetDevice(&dev) != hipSuccess) return false;
         std::lock_guard<std::mutex> g(mu);
         if (dev >= 64 || !((done >> dev) & 1)) {
-            const void *fn[5] = {(const void *)k_wave_mfma<1, 1>, (const void *)k_wave_mfma<2, 1>,
-                                 (const void *)k_wave_mfma<4, 1>, (const void *)k_wave_mfma<1, 2>,
-                                 (const void *)k_wave_mfma<2, 2>};
+            const void *fn[10] = {(const void *)k_wave_mfma<1, 1>, (const void *)k_wave_mfma<2, 1>,
+                                  (const void *)k_wave_mfma<4, 1>, (const void *)k_wave_mfma<1, 2>,
+                                  (const void *)k_wave_mfma<2, 2>, (const void *)k_wave_mfma<1, 1, kFmtI8>,
+                                  (const void *)k_wave_mfma<2, 1, kFmtI8>, (const void *)k_wave_mfma<4, 1, kFmtI8>,
+                                  (const void *)k_wave_mfma<1, 2, kFmtI8>, (const void *)k_wave_mfma<2, 2, kFmtI8>};
             for (const void *fp : fn) {
                 // the dynamic limit is the CU's LDS less the kernel's static shared variables
                 hipFuncAttributes fa{};

@@ -259,7 +259,7 @@ size_t wave_mfma_lds_bytes(int ti, int tj, int n_slots, int max_tile_list);
 // Launch description; false if the shape is outside the kernel's limits
 // (more than 256 slots, more LDS than a CU has, an unsupported tile).
 bool plan_wave_mfma(const uint64_t *bm, int64_t pitch_words, const WaveMfmaDesc &d, int ti, int tj, int tiles_m,
-                    int splits, size_t lds_bytes, KernelLaunch &L);
+                    int splits, size_t lds_bytes, KernelLaunch &L, bool int8 = false);
 constexpr int kFusedMaxSlots = 256;
 // Dynamic LDS limit of k_wave_mfma: a CU's 160 KB less room for its static
 // shared variable.

@@ -200,6 +200,7 @@ struct tfim_ctx {
     int64_t wave_contended = 0;   // waves kept out of the tuners (model training alongside)
     double wave_ms = 0;
     int wave_kernel = 0;    // 0 auto, 1 L2 kernel, 2 LDS kernel, 3 LDS kernel with wide staging, 4 matrix cores
+    bool mfma_int8 = false; // wave matrix-core paths on the int8 matrix cores (default: fp4)
     // second tuner, for waves outside the LDS kernel: arm 0 = L2 kernel, arm 1
     // = matrix cores in three launches, arm 2 = fused matrix-core launch; EMA
     // of kernel milliseconds per unit of each arm's work
@@ -806,9 +807,10 @@ int wave_submit_mfma(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, c
         sk_arm = c->sk.pick();
         split = SplitTuner::split_of(sk_arm, base, stages);
     }
-    // fp4 matrix cores (twice the int8 rate, same counts) with the 128-column
-    // tiles the occupancy rule above assumes
-    const int64_t zs = plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P, true, false);
+    // fp4 matrix cores (twice the int8 rate, same counts; int8 if configured)
+    // with the 128-column tiles the occupancy rule above assumes
+    const int64_t zs =
+        plan_popmat_mfma_rp(masks, c->d_bm, C, n_rows, c->n_cols, c->pitch, (int)split, P, !c->mfma_int8, false);
     WaveDesc d{};
     d.item_ids = d_items;
     d.item_off = d_ioff;
@@ -982,7 +984,7 @@ int wave_submit_fused(tfim_ctx *c, tfim_ctx::Slot &sl, const int32_t *item_ids, 
     d.dense_out = csr ? 0 : 1;
     KernelLaunch K;
     c->fused_nofit = "a launch the kernel accepts (tile, slots, LDS attribute)";
-    if (!plan_wave_mfma(c->d_bm, c->pitch, d, ti, tj, tiles_m, (int)zs, lds, K)) return kFusedNoFit;
+    if (!plan_wave_mfma(c->d_bm, c->pitch, d, ti, tj, tiles_m, (int)zs, lds, K, c->mfma_int8)) return kFusedNoFit;
     KernelLaunch *ks[1] = {&K};
     if (int r = issue_wave(c, sl, total, nullptr, 0, n_out, n_dev, ks, 1)) return r;
     sl.busy = true;
@@ -1696,6 +1698,10 @@ int tfim_config(tfim_ctx *c, const char *key, int64_t value) {
     }
     if (k == "mfma_tile" && (value == 0 || value == 128 || value == 256)) {
         c->mfma_tile = (int)value;
+        return TFIM_OK;
+    }
+    if (k == "mfma_format" && (value == 0 || value == 1)) {
+        c->mfma_int8 = value == 1;
         return TFIM_OK;
     }
     if (k == "mfma_split" && (value == 0 || value == 1)) {

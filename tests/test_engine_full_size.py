@@ -23,7 +23,8 @@ RUNS = {
                     ("plain", ["--workers", "8"])],
     "dense-replicated": [("mafia", []), ("mafia", ["--learned"]), ("mafia", ["--rows", "lists"]),
                          ("mafia", ["--graphs", "1"]), ("wave", []),
-                         ("wave", ["--wave-kernel", "4"]), ("plain", ["--workers", "8"])],
+                         ("wave", ["--wave-kernel", "4"]), ("wave", ["--wave-kernel", "4", "--mfma-format", "int8"]),
+                         ("plain", ["--workers", "8"])],
     "gene": [("mafia", []), ("mafia", ["--learned"]), ("mafia", ["--compl", "off"])],
     "dense-small": [("mafia", []), ("mafia", ["--learned"]), ("plain", [])],
 }

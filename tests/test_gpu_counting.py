@@ -193,21 +193,25 @@ def test_popcount_matrix_all_ones_largest_split(ctx, path):
 
 
 @pytest.mark.parametrize("graphs", [1, 0])
-@pytest.mark.parametrize("kernel", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("kernel", [1, 2, 3, 4, 5, "4-int8", "5-int8"])
 @pytest.mark.parametrize("n_bits,n_items", [(1100, 30), (3200, 75), (400_003, 90), (2_000_000, 38)])
 def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
     """Every batched path (1 = L2-resident kernel, 2 = LDS-staged, 3 =
     LDS-staged with wide staging, 4 = matrix cores: masks, fp4 matrix count of
-    all (row, survivor) pairs, gather; 5 = the fused matrix-core kernel) on
+    all (row, survivor) pairs, gather; 5 = the fused matrix-core kernel; both
+    matrix-core forms also on the int8 matrix cores) on
     the same waves,
     replayed from graphs or issued directly: uneven prefixes, uneven and empty
     tails, full-column rows."""
-    rng = np.random.default_rng(kernel * 100 + n_items)
+    int8 = isinstance(kernel, str)   # the matrix-core paths on the int8 matrix cores
+    kernel = int(str(kernel)[0])
+    rng = np.random.default_rng(kernel * 100 + n_items + 7 * int8)
     rows = rand_rows(rng, n_items, n_bits, 0.65)
     n_cols = n_items - 1
     ctx.load_bitmaps(rows, n_bits, n_cols)
     ctx.config("wave_kernel", kernel)
     ctx.config("graphs", graphs)
+    ctx.config("mfma_format", 1 if int8 else 0)
     try:
         prefixes, tails = [], []
         for i in range(120):
@@ -223,6 +227,7 @@ def test_wave_kernels_forced(ctx, kernel, graphs, n_bits, n_items):
     finally:
         ctx.config("wave_kernel", 0)
         ctx.config("graphs", 0)
+        ctx.config("mfma_format", 0)
 
 
 @pytest.mark.parametrize("n_rows", [250, 129])
