@@ -150,8 +150,10 @@ int reserve(tfim_mlp *m, int64_t n) {
         hipHostFree(m->h_o);
         m->h_x = m->h_o = nullptr;
         m->h_cap = 0;
-        if (hipHostMalloc(&m->h_x, (size_t)n * IN * 4) != hipSuccess ||
-            hipHostMalloc(&m->h_o, (size_t)n * OUT * 4) != hipSuccess)
+        // coherent: the inference kernel reads its rows from and writes its
+        // logits to this memory directly (no copies around it)
+        if (hipHostMalloc(&m->h_x, (size_t)n * IN * 4, hipHostMallocCoherent) != hipSuccess ||
+            hipHostMalloc(&m->h_o, (size_t)n * OUT * 4, hipHostMallocCoherent) != hipSuccess)
             return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp pinned staging allocation failed");
         m->h_cap = n;
     }
@@ -244,17 +246,19 @@ int tfim_mlp_forward_submit(tfim_mlp *m, const float *x, int32_t n, int32_t bf16
         hipHostFree(m->h_o);
         m->h_x = m->h_o = nullptr;
         m->h_cap = 0;
-        if (hipHostMalloc(&m->h_x, (size_t)n * IN * 4) != hipSuccess ||
-            hipHostMalloc(&m->h_o, (size_t)n * OUT * 4) != hipSuccess)
+        if (hipHostMalloc(&m->h_x, (size_t)n * IN * 4, hipHostMallocCoherent) != hipSuccess ||
+            hipHostMalloc(&m->h_o, (size_t)n * OUT * 4, hipHostMallocCoherent) != hipSuccess)
             return ctx_fail(m->ctx, TFIM_E_NOMEM, "mlp pinned staging allocation failed");
         m->h_cap = n;
     }
     std::memcpy(m->h_x, x, (size_t)n * IN * 4);
     hipStream_t s = m->stream;
-    MCHK(m, hipMemcpyAsync(sc.X, m->h_x, (size_t)n * IN * 4, hipMemcpyHostToDevice, s));
-    launch_mlp_infer(m->P, bf16 ? m->Pb : nullptr, sc.X, n, sc.O, s);
+    // one launch: the kernel reads the rows from and writes the logits to the
+    // coherent pinned staging (a request is tens of rows; two copies around
+    // the kernel cost more host time than the kernel)
+    (void)sc;
+    launch_mlp_infer(m->P, bf16 ? m->Pb : nullptr, m->h_x, n, m->h_o, s);
     MCHK(m, hipGetLastError());
-    MCHK(m, hipMemcpyAsync(m->h_o, sc.O, (size_t)n * OUT * 4, hipMemcpyDeviceToHost, s));
     MCHK(m, hipEventRecord(m->fwd_done, s));
     m->fwd_n = n;
     return TFIM_OK;
