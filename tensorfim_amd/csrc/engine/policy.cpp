@@ -129,8 +129,13 @@ class LearnedPolicy : public Policy {
         }
         feats_.resize((size_t)(rows * NF));
         infer_.resize((size_t)(n_infer * NF));
-        run_parallel((int64_t)ds.size(), [&](int64_t q) {
-            if (node_row_[q] < 0) return;
+        // the nodes with rows; a few rows are filled here (handing them to
+        // the host pool costs more than computing them)
+        with_rows_.clear();
+        for (size_t q = 0; q < ds.size(); q++)
+            if (node_row_[q] >= 0) with_rows_.push_back((int64_t)q);
+        const auto fill = [&](int64_t w) {
+            const int64_t q = with_rows_[(size_t)w];
             const Decision &d = *ds[q];
             float *nf = &feats_[(size_t)(node_row_[q] * NF)];
             const NodeAgg a = node_features(d, nf);
@@ -146,7 +151,12 @@ class LearnedPolicy : public Policy {
                     if (item_infer0_[q] >= 0) std::copy_n(f, NF, &infer_[(size_t)((item_infer0_[q] + (int64_t)i) * NF)]);
                 }
             }
-        });
+        };
+        if (rows <= kSerialRows) {
+            for (size_t w = 0; w < with_rows_.size(); w++) fill((int64_t)w);
+        } else {
+            run_parallel((int64_t)with_rows_.size(), fill);
+        }
         if (feats_.empty()) return;
         const int64_t n = (int64_t)(infer_.size() / NF);
         if (deferred) {
@@ -508,6 +518,7 @@ class LearnedPolicy : public Policy {
     static constexpr float kRecordAccepted = 0.25f;  // share of the rule's lookaheads kept as training records
     static constexpr float kRankRecord = 0.125f;     // without --rank: share of nodes whose items are recorded
     static constexpr int64_t kRankRecordMax = 8192;  // ... until this many item records exist
+    static constexpr int64_t kSerialRows = 256;      // feature rows of one wave filled without the host pool
 
     tfim_ctx *ctx_;
     const SearchOptions &opt_;
@@ -525,7 +536,7 @@ class LearnedPolicy : public Policy {
     double N_ = 1, thr_ = 1;
     const std::vector<int32_t> *freq_ = nullptr;
     ParallelFor par_;
-    std::vector<int64_t> node_row_, item_row0_, infer_row_, item_infer0_;
+    std::vector<int64_t> node_row_, item_row0_, infer_row_, item_infer0_, with_rows_;
     std::vector<float> infer_, pend_feats_;
     int64_t pend_n_ = 0;   // rows of the inference queued for pending decisions
     int64_t model_gated_ = 0, model_success_ = 0;
