@@ -1,7 +1,7 @@
 """Benchmark: exact support counting on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-e2e] [--no-cpu] [--no-per-node] [--tuned]
-                    [--paths mfma,packed] [--no-sweep] [--sustain-seconds S]
+                    [--paths fp4,mfma,packed] [--no-sweep] [--sustain-seconds S]
     python bench.py --copy-calibration MIB     (a plain device copy, for counter scaling)
 
 A step is one pass of the hot path over one batch: the isolated count
