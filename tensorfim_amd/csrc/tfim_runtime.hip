@@ -1852,9 +1852,13 @@ static int popmat_plan(tfim_ctx *c, const tfim_buf *A, const tfim_buf *B, tfim_b
     if (path == TFIM_PATH_PACKED || path == TFIM_PATH_MFMA_FP4) tiles = {0};
     else if (c->mfma_tile) tiles = {c->mfma_tile};
     else tiles = {256, 128};
+    // splits 1, 2, 4, ... while a split keeps at least 64 words: shapes of few
+    // tiles (128 rows x 256 items) need far more than 16 to fill the CUs
+    std::vector<int32_t> splits;
+    for (int32_t sk = 1; sk <= 512 && (sk == 1 || pitch / sk >= 64); sk *= 2) splits.push_back(sk);
     double best = 1e300;
     for (int t : tiles)
-        for (int32_t sk : {1, 2, 4, 8, 16}) {
+        for (int32_t sk : splits) {
             HIPCHK(c, hipEventRecord(c->ev0, c->stream));
             HIPCHK(c, popmat_launch(c, A, B, C, b, d, pitch, path, sk, t));
             HIPCHK(c, hipEventRecord(c->ev1, c->stream));
